@@ -1,0 +1,163 @@
+"""ctypes binding of libopx.so (declared in include/opx.h).
+
+The library is required: if it is missing or fails to load, every kernel call
+raises ``RuntimeError`` (there is no CPU fallback).  Build it with
+``python -m offline_particles_amd._build`` or ``__graft_entry__.build()``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _build
+
+LIB_PATH = _build.LIB_PATH
+
+# ---- enums (opx.h) ----
+OPX_F64, OPX_F32, OPX_BF16 = 0, 1, 2
+OPX_OK, OPX_ERR_INVALID, OPX_ERR_HIP, OPX_ERR_UNSUPPORTED = 0, -1, -2, -3
+
+
+class OpxError(RuntimeError):
+    """An error returned by libopx."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+class OpxInvalidArgument(OpxError, ValueError):
+    """libopx rejected an argument (OPX_ERR_INVALID)."""
+
+
+_vp = ctypes.c_void_p
+
+
+class opx_field(ctypes.Structure):
+    _fields_ = [
+        ("prev", _vp), ("next", _vp),
+        ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
+        ("shape", ctypes.c_int64 * 3),
+        ("offsets", ctypes.c_double * 3),
+        ("ft", ctypes.c_double), ("gt", ctypes.c_double),
+    ]
+
+
+ROMS_FIELD_NAMES = ("u", "v", "w", "dx", "dy", "h", "C", "zeta")
+
+
+class opx_roms_fields(ctypes.Structure):
+    _fields_ = [(n, opx_field) for n in ROMS_FIELD_NAMES]
+
+
+class opx_scalars(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("alpha", ctypes.c_double), ("hc", ctypes.c_double),
+        ("NZ", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("zidx_min", ctypes.c_double), ("zidx_max", ctypes.c_double),
+        ("yidx_min", ctypes.c_double), ("yidx_max", ctypes.c_double),
+        ("xidx_min", ctypes.c_double), ("xidx_max", ctypes.c_double),
+    ]
+
+
+PARTICLE_PTR_NAMES = ("zidx", "yidx", "xidx", "z", "dz0", "dz1", "dz2", "dyidx0", "dyidx1", "dyidx2",
+                      "dxidx0", "dxidx1", "dxidx2")
+
+
+class opx_particles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("status", _vp)] + [(n, _vp) for n in PARTICLE_PTR_NAMES]
+
+
+# name -> argtypes; every entry point returns int
+_SIGNATURES = {
+    "opx_finite_indices": [ctypes.POINTER(opx_particles), _vp],
+    "opx_domain_bounds": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars), _vp],
+    "opx_validate": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars), _vp],
+    "opx_rk2_step_1": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars),
+                       ctypes.POINTER(opx_roms_fields), _vp],
+    "opx_rk2_step_2": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars),
+                       ctypes.POINTER(opx_roms_fields), _vp],
+    "opx_rk2_update": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars),
+                       ctypes.POINTER(opx_roms_fields), _vp],
+    "opx_ab3_w_advection": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars),
+                            ctypes.POINTER(opx_roms_fields), _vp],
+    "opx_ab3_post_w_advection": [ctypes.POINTER(opx_particles), ctypes.POINTER(opx_scalars),
+                                 ctypes.POINTER(opx_roms_fields), _vp],
+}
+
+# extra signatures registered by other modules (fused engine)
+EXTRA_SIGNATURES: dict = {}
+
+_lib = None
+_load_error: Exception | None = None
+
+
+def load():
+    """Load libopx.so once; raise RuntimeError if it is missing."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libopx.so not found at {LIB_PATH}; build it with `python -m offline_particles_amd._build` "
+            "(there is no CPU fallback)")
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+    except OSError as e:  # pragma: no cover - depends on the machine
+        _load_error = e
+        raise RuntimeError(f"failed to load {LIB_PATH}: {e}") from e
+    L.opx_last_error.restype = ctypes.c_char_p
+    L.opx_last_error.argtypes = []
+    L.opx_abi_version.restype = ctypes.c_int
+    for name, args in {**_SIGNATURES, **EXTRA_SIGNATURES}.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def register(name: str, argtypes, restype=ctypes.c_int):
+    """Declare an additional entry point's signature (applied at load)."""
+    EXTRA_SIGNATURES[name] = argtypes
+    if _lib is not None:
+        fn = getattr(_lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point; raise OpxError on a non-zero return code."""
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != OPX_OK:
+        msg = L.opx_last_error().decode(errors="replace")
+        if rc == OPX_ERR_INVALID:
+            raise OpxInvalidArgument(name, rc, msg)
+        raise OpxError(name, rc, msg)
+
+
+def declared_symbols(header: str | None = None) -> list[str]:
+    """Function names declared in include/opx.h (used by the export test)."""
+    import re
+
+    header = header or os.path.join(_build.INCLUDE, "opx.h")
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(opx_[a-z0-9_]+)\s*\(", text)))
+
+
+def storage_code(dtype) -> int:
+    """opx_dtype for a torch or numpy dtype."""
+    import torch
+
+    if dtype in (torch.float64, np.float64) or dtype == np.dtype(np.float64):
+        return OPX_F64
+    if dtype in (torch.float32, np.float32) or dtype == np.dtype(np.float32):
+        return OPX_F32
+    if dtype == torch.bfloat16:
+        return OPX_BF16
+    raise TypeError(f"unsupported field dtype {dtype}")

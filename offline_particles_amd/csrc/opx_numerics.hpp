@@ -1,0 +1,178 @@
+// Device numerics shared by the libopx kernels: index truncation, stencil
+// loads with the per-value time blend, linear/bilinear/trilinear interpolation
+// and the ROMS Vtransform-2 vertical coordinate.
+//
+// The FP64 versions keep the reference's operation order (paths relative to
+// src/offline_particles/ of the reference):
+//   truncate_index / linear / bilinear / trilinear   kernels/_interpolation/linear.pxd:3-65
+//   compute_z / compute_zidx                         kernels/roms/_vertical_coordinate.pxd:19-103
+//   per-value blend prev*gt + next*ft                fields.py:423-428
+// Translation units that need bit-exact FP64 results are compiled with
+// -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "opx.h"
+
+namespace opx {
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// `<Py_ssize_t>idx` then clamp to [0, max_idx] (linear.pxd:3-9).  NaN and
+// values outside the int64 range convert to INT64_MIN on x86-64 and clamp to
+// 0; that behaviour is reproduced here.
+__device__ __forceinline__ int64_t truncate_index(double idx, int64_t max_idx)
+{
+    if (!(idx > -9223372036854775808.0 && idx < 9223372036854775808.0))
+        return 0;
+    int64_t i = (int64_t)idx;
+    return i < 0 ? 0 : (i > max_idx ? max_idx : i);
+}
+
+// Stencil value i of a field, blended in its storage type and widened to FP64
+// (fields.py:423-428 then np.ascontiguousarray(float64), _core.pyx:5-24).
+__device__ __forceinline__ double load_blend(const opx_field &f, int64_t i)
+{
+    if (f.dtype == OPX_F64) {
+        const double *p = static_cast<const double *>(f.prev);
+        if (f.next == nullptr)
+            return p[i];
+        double a = p[i] * f.gt;
+        double b = static_cast<const double *>(f.next)[i] * f.ft;
+        return a + b;
+    } else if (f.dtype == OPX_F32) {
+        const float *p = static_cast<const float *>(f.prev);
+        if (f.next == nullptr)
+            return (double)p[i];
+        float a = p[i] * (float)f.gt;
+        float b = static_cast<const float *>(f.next)[i] * (float)f.ft;
+        return (double)(a + b);
+    } else {
+        const uint16_t *p = static_cast<const uint16_t *>(f.prev);
+        if (f.next == nullptr)
+            return (double)bf16_bits_to_f32(p[i]);
+        float a = bf16_bits_to_f32(p[i]) * (float)f.gt;
+        float b = bf16_bits_to_f32(static_cast<const uint16_t *>(f.next)[i]) * (float)f.ft;
+        return (double)(a + b);
+    }
+}
+
+// linear_interpolation (linear.pxd:11-17) on a 1-D field at rho index idx.
+__device__ __forceinline__ double linear_f64(const opx_field &f, double idx)
+{
+    idx += f.offsets[0];
+    int64_t I0 = truncate_index(idx, f.shape[0] - 2);
+    double f0 = idx - (double)I0;
+    double g0 = 1.0 - f0;
+    return g0 * load_blend(f, I0) + f0 * load_blend(f, I0 + 1);
+}
+
+// bilinear_interpolation (linear.pxd:19-33) on a 2-D field at rho (y, x).
+__device__ __forceinline__ double bilinear_f64(const opx_field &f, double y, double x)
+{
+    double idx0 = y + f.offsets[0];
+    double idx1 = x + f.offsets[1];
+    const int64_t n1 = f.shape[1];
+    int64_t I0 = truncate_index(idx0, f.shape[0] - 2);
+    int64_t I1 = truncate_index(idx1, n1 - 2);
+    double f0 = idx0 - (double)I0;
+    double f1 = idx1 - (double)I1;
+    double g0 = 1.0 - f0;
+    double g1 = 1.0 - f1;
+    int64_t b = I0 * n1 + I1;
+    double v00 = load_blend(f, b);
+    double v01 = load_blend(f, b + 1);
+    double v10 = load_blend(f, b + n1);
+    double v11 = load_blend(f, b + n1 + 1);
+    return g0 * g1 * v00 + g0 * f1 * v01 + f0 * g1 * v10 + f0 * f1 * v11;
+}
+
+// trilinear_interpolation (linear.pxd:35-65), sum order of :58-65.
+__device__ __forceinline__ double trilinear_f64(const opx_field &f, double z, double y, double x)
+{
+    double idx0 = z + f.offsets[0];
+    double idx1 = y + f.offsets[1];
+    double idx2 = x + f.offsets[2];
+    const int64_t n1 = f.shape[1], n2 = f.shape[2];
+    int64_t I0 = truncate_index(idx0, f.shape[0] - 2);
+    int64_t I1 = truncate_index(idx1, n1 - 2);
+    int64_t I2 = truncate_index(idx2, n2 - 2);
+    double f0 = idx0 - (double)I0;
+    double f1 = idx1 - (double)I1;
+    double f2 = idx2 - (double)I2;
+    double g0 = 1.0 - f0;
+    double g1 = 1.0 - f1;
+    double g2 = 1.0 - f2;
+    int64_t b00 = (I0 * n1 + I1) * n2 + I2;
+    int64_t b01 = b00 + n2;
+    int64_t b10 = b00 + n1 * n2;
+    int64_t b11 = b10 + n2;
+    double v000 = load_blend(f, b00), v001 = load_blend(f, b00 + 1);
+    double v010 = load_blend(f, b01), v011 = load_blend(f, b01 + 1);
+    double v100 = load_blend(f, b10), v101 = load_blend(f, b10 + 1);
+    double v110 = load_blend(f, b11), v111 = load_blend(f, b11 + 1);
+    return (g0 * g1 * g2 * v000 +
+            g0 * g1 * f2 * v001 +
+            g0 * f1 * g2 * v010 +
+            g0 * f1 * f2 * v011 +
+            f0 * g1 * g2 * v100 +
+            f0 * g1 * f2 * v101 +
+            f0 * f1 * g2 * v110 +
+            f0 * f1 * f2 * v111);
+}
+
+// ---- ROMS Vtransform = 2 vertical coordinate (_vertical_coordinate.pxd) ----
+
+__device__ __forceinline__ double sigma_coordinate(double zidx, int NZ) { return (zidx + 0.5) / NZ - 1.0; }
+
+__device__ __forceinline__ double S_coordinate(double hc, double sigma, double h, double C)
+{
+    return (hc * sigma + h * C) / (hc + h);
+}
+
+// compute_z (:31-35)
+__device__ __forceinline__ double compute_z(double zidx, int NZ, double hc, double h, double C, double zeta)
+{
+    double sigma = sigma_coordinate(zidx, NZ);
+    double S = S_coordinate(hc, sigma, h, C);
+    return zeta + (zeta + h) * S;
+}
+
+// C[k] of the 1-D stretching field (always a static field).
+__device__ __forceinline__ double C_at(const opx_field &C, int64_t k) { return load_blend(C, k); }
+
+// compute_zidx (:98-103) with _compute_Cidx_from_S's edge returns and binary
+// search (:50-84) and _zidx_from_S's linear inverse (:86-96).
+__device__ __forceinline__ double compute_zidx(double z, double h, double zeta, double hc, int NZ,
+                                               const opx_field &C)
+{
+    const double C_offset = C.offsets[0];
+    const int64_t C_size = C.shape[0];
+    double S = (z - zeta) / (zeta + h);
+    int64_t Cidx;
+    if (S <= S_coordinate(hc, sigma_coordinate(0.0 - C_offset, NZ), h, C_at(C, 0))) {
+        Cidx = 0;
+    } else if (S >= S_coordinate(hc, sigma_coordinate((double)(C_size - 2) - C_offset, NZ), h,
+                                 C_at(C, C_size - 2))) {
+        Cidx = C_size - 2;
+    } else {
+        int64_t lo = 0, hi = C_size - 2;
+        while (lo < hi - 1) {
+            int64_t mid = (lo + hi) / 2;
+            double S_mid = S_coordinate(hc, sigma_coordinate((double)mid - C_offset, NZ), h, C_at(C, mid));
+            if (S_mid <= S)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        Cidx = lo;
+    }
+    double S_low = S_coordinate(hc, sigma_coordinate((double)Cidx - C_offset, NZ), h, C_at(C, Cidx));
+    double S_high = S_coordinate(hc, sigma_coordinate((double)(Cidx + 1) - C_offset, NZ), h, C_at(C, Cidx + 1));
+    double f = (S - S_low) / (S_high - S_low);
+    return (double)Cidx - C_offset + f;
+}
+
+} // namespace opx

@@ -1,0 +1,61 @@
+// Host-side helpers shared by the libopx translation units: thread-local error
+// message, argument checks and launch-error reporting.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "opx.h"
+
+namespace opx {
+
+// Defined in opx_plain.hip.
+void set_error(const char *fmt, ...);
+
+#define OPX_REQUIRE(cond, ...)                 \
+    do {                                       \
+        if (!(cond)) {                         \
+            ::opx::set_error(__VA_ARGS__);     \
+            return OPX_ERR_INVALID;            \
+        }                                      \
+    } while (0)
+
+inline int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s: HIP launch failed: %s", what, hipGetErrorString(e));
+        return OPX_ERR_HIP;
+    }
+    return OPX_OK;
+}
+
+inline int dtype_size(int dtype)
+{
+    switch (dtype) {
+    case OPX_F64: return 8;
+    case OPX_F32: return 4;
+    case OPX_BF16: return 2;
+    default: return 0;
+    }
+}
+
+// Validates one field descriptor: data pointer, dtype, ndim and sizes >= 2
+// (the interpolation clamps base indices to [0, size-2]).
+inline int check_field(const opx_field &f, int ndim, const char *name, bool allow_time)
+{
+    OPX_REQUIRE(f.prev != nullptr, "field '%s': null data pointer", name);
+    OPX_REQUIRE(dtype_size(f.dtype) != 0, "field '%s': unknown dtype %d", name, f.dtype);
+    OPX_REQUIRE(f.ndim == ndim, "field '%s': expected %d active dims, got %d", name, ndim, f.ndim);
+    for (int d = 0; d < ndim; ++d)
+        OPX_REQUIRE(f.shape[d] >= 2, "field '%s': dimension %d has size %lld < 2", name, d,
+                    (long long)f.shape[d]);
+    OPX_REQUIRE(allow_time || f.next == nullptr, "field '%s' must be static", name);
+    return OPX_OK;
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+} // namespace opx

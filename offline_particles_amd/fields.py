@@ -1,0 +1,275 @@
+"""Fields resident in HBM (mirrors the reference's fields.py).
+
+``StaticField`` holds one device array.  ``TimeDependentField`` holds its
+snapshots in device memory and keeps the reference's two-snapshot window
+(``set_time_index``/``increment_time``/``decrement_time``, fields.py:340-392).
+
+The one deliberate difference is where the time blend happens.  The
+reference's ``get_field_data`` (fields.py:394-430) blends the *whole* window
+into an FP64 array on every launch.  Here ``get_field_data`` returns a
+``BlendWindow`` (the two snapshots plus the rounded weights) and the kernels
+blend each stencil value they read, ``prev*gt + next*ft`` in the storage
+type, which is the same arithmetic on the same operands: the values a kernel
+interpolates are bit-identical to the reference's blended array, without the
+O(grid) pass.  ``BlendWindow.materialize()`` produces the full blended array
+when a caller needs it.
+"""
+
+from __future__ import annotations
+
+import collections
+
+import numpy as np
+import torch
+
+from ._lib import OPX_BF16, OPX_F32, OPX_F64, storage_code
+from .spatial_arrays import BBox, DeviceArray, SpatialArray, Stagger, as_device_tensor
+
+FieldData = collections.namedtuple("FieldData", ["array", "offsets"])
+
+
+class BlendWindow:
+    """Snapshots It and It+1 of a time-dependent field plus blend weights.
+
+    ``ft``/``gt`` are the weights rounded to the storage type as
+    fields.py:423-424 does (``dtype.type(ft)``, ``dtype.type(1 - ft)``);
+    bfloat16 storage blends in float32 with float32 weights.
+    """
+
+    __slots__ = ("prev", "next", "ft", "gt", "It")
+
+    def __init__(self, prev: torch.Tensor, next: torch.Tensor, ft: float, gt: float, It: int = 0):
+        self.prev = prev
+        self.next = next
+        self.ft = float(ft)
+        self.gt = float(gt)
+        self.It = It
+
+    @property
+    def shape(self) -> tuple:
+        return tuple(self.prev.shape)
+
+    @property
+    def dtype(self):
+        return self.prev.dtype
+
+    @property
+    def ndim(self) -> int:
+        return self.prev.dim()
+
+    def materialize(self) -> torch.Tensor:
+        """The blended FP64 array the reference would hand to a kernel."""
+        if self.prev.dtype == torch.float64:
+            a = self.prev * self.gt
+            b = self.next * self.ft
+            return a + b
+        work = torch.float32
+        a = self.prev.to(work) * torch.tensor(self.gt, dtype=work, device=self.prev.device)
+        b = self.next.to(work) * torch.tensor(self.ft, dtype=work, device=self.prev.device)
+        return (a + b).to(torch.float64)
+
+
+def blend_weights(time_index: float, dtype) -> tuple[int, float, float]:
+    """(It, ft, gt) exactly as fields.py:409-424 computes them."""
+    It, ft = divmod(time_index, 1)
+    It = int(It)
+    code = storage_code(dtype)
+    if code == OPX_F64:
+        ft = np.float64(ft)
+        gt = np.float64(1 - ft)
+    else:  # float32 and bfloat16 storage blend in float32
+        ft = np.float32(ft)
+        gt = np.float32(1 - ft)
+    return It, float(ft), float(gt)
+
+
+class Field:
+    """Abstract base of fields (fields.py:15-119)."""
+
+    def __init__(self, z_stagger, y_stagger, x_stagger) -> None:
+        self._z_stagger = Stagger(z_stagger)
+        self._y_stagger = Stagger(y_stagger)
+        self._x_stagger = Stagger(x_stagger)
+        self._z_offset = self._z_stagger.offset
+        self._y_offset = self._y_stagger.offset
+        self._x_offset = self._x_stagger.offset
+
+    z_stagger = property(lambda self: self._z_stagger)
+    y_stagger = property(lambda self: self._y_stagger)
+    x_stagger = property(lambda self: self._x_stagger)
+    z_offset = property(lambda self: self._z_offset)
+    y_offset = property(lambda self: self._y_offset)
+    x_offset = property(lambda self: self._x_offset)
+
+    @property
+    def stagger(self):
+        return (self._z_stagger, self._y_stagger, self._x_stagger)
+
+    @property
+    def all_offsets(self):
+        return (self._z_offset, self._y_offset, self._x_offset)
+
+    @property
+    def active_offsets(self) -> tuple:
+        return tuple(o for o, s in zip(self.all_offsets, self.stagger) if s.is_active)
+
+    @property
+    def dmask(self):
+        return tuple(s.is_active for s in self.stagger)
+
+    @property
+    def nspatial_dims(self) -> int:
+        return len(self.spatial_shape)
+
+    def _expected_spatial(self, simulation_shape):
+        staggered = (
+            self.z_stagger.expected_size(simulation_shape[1]),
+            self.y_stagger.expected_size(simulation_shape[2]),
+            self.x_stagger.expected_size(simulation_shape[3]),
+        )
+        return tuple(s for s in staggered if s is not None)
+
+
+class StaticField(Field):
+    """A field that does not change in time (fields.py:122-231)."""
+
+    def __init__(self, data: SpatialArray):
+        super().__init__(data.z_stagger, data.y_stagger, data.x_stagger)
+        self._data = data
+
+    @property
+    def data(self) -> SpatialArray:
+        return self._data
+
+    @property
+    def dtype(self):
+        return self._data.dtype
+
+    @property
+    def spatial_shape(self) -> tuple:
+        return self._data.shape
+
+    def __repr__(self) -> str:
+        return (f"StaticField(shape={self._data.shape}, z_stagger={self.z_stagger}, "
+                f"y_stagger={self.y_stagger}, x_stagger={self.x_stagger})")
+
+    def validate_shape(self, simulation_shape) -> None:
+        expected = self._expected_spatial(simulation_shape)
+        if tuple(self._data.shape) != expected:
+            raise ValueError(f"Expected shape {expected} but data has shape {tuple(self._data.shape)}")
+
+    def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
+        array, offsets = self._data.get_data_subset(bbox)
+        return FieldData(array, offsets)
+
+    @classmethod
+    def from_numpy(cls, data, z_stagger, y_stagger, x_stagger, *, device=None) -> "StaticField":
+        return cls(DeviceArray(data, Stagger(z_stagger), Stagger(y_stagger), Stagger(x_stagger), device))
+
+    from_array = from_numpy
+
+
+class TimeDependentField(Field):
+    """A field given as snapshots on a time axis, resident in HBM (fields.py:237-458).
+
+    ``data`` is a (T, *spatial) host or device array.  With ``cyclic=True``
+    snapshot ``It`` is ``data[It % T]`` so a short set of periodic snapshots
+    serves an arbitrarily long time axis; otherwise ``It`` must lie in
+    ``[0, T-2]`` as in the reference.
+    """
+
+    def __init__(self, data, z_stagger, y_stagger, x_stagger, spatial_array_factory=None,
+                 output_dtype=np.float64, *, cyclic: bool = False, device=None, storage_dtype=None):
+        super().__init__(z_stagger, y_stagger, x_stagger)
+        if data.ndim < 2:
+            raise ValueError("TimeDependentField requires at least 2 dimensions (time + spatial).")
+        if data.shape[0] < 2 and not cyclic:
+            raise ValueError("TimeDependentField requires at least 2 time steps.")
+        self._snapshots = as_device_tensor(data, device, storage_dtype)
+        self._num_timesteps = int(self._snapshots.shape[0])
+        self._cyclic = bool(cyclic)
+        self._output_dtype = output_dtype
+        self._It = 0
+
+    @property
+    def data(self) -> torch.Tensor:
+        return self._snapshots
+
+    @property
+    def dtype(self):
+        return self._snapshots.dtype
+
+    @property
+    def cyclic(self) -> bool:
+        return self._cyclic
+
+    @property
+    def num_snapshots(self) -> int:
+        return self._num_timesteps
+
+    @property
+    def spatial_shape(self) -> tuple:
+        return tuple(self._snapshots.shape[1:])
+
+    def __repr__(self) -> str:
+        return (f"TimeDependentField(shape={tuple(self._snapshots.shape)}, z_stagger={self.z_stagger}, "
+                f"y_stagger={self.y_stagger}, x_stagger={self.x_stagger}, cyclic={self._cyclic})")
+
+    def validate_shape(self, simulation_shape) -> None:
+        expected = self._expected_spatial(simulation_shape)
+        if self.spatial_shape != expected:
+            raise ValueError(f"Expected spatial shape {expected} but data has shape {self.spatial_shape}")
+        if not self._cyclic and self._num_timesteps != simulation_shape[0]:
+            raise ValueError(f"Expected {simulation_shape[0]} snapshots, data has {self._num_timesteps}")
+
+    def snapshot(self, It: int) -> torch.Tensor:
+        """Device view of snapshot It."""
+        if self._cyclic:
+            return self._snapshots[It % self._num_timesteps]
+        return self._snapshots[It]
+
+    @property
+    def previous_time_slice(self) -> torch.Tensor:
+        return self.snapshot(self._It)
+
+    @property
+    def next_time_slice(self) -> torch.Tensor:
+        return self.snapshot(self._It + 1)
+
+    def increment_time(self) -> None:
+        if not self._cyclic and self._It == self._num_timesteps - 2:
+            raise IndexError("Cannot increment past the penultimate timestep.")
+        self._It += 1
+
+    def decrement_time(self) -> None:
+        if not self._cyclic and self._It == 0:
+            raise IndexError("Cannot decrement past the first timestep.")
+        self._It -= 1
+
+    def set_time_index(self, It: int) -> None:
+        """Move the window to [It, It+1] (fields.py:368-392)."""
+        if It == self._It:
+            return
+        if It == self._It + 1:
+            return self.increment_time()
+        if It == self._It - 1:
+            return self.decrement_time()
+        if not self._cyclic and (It < 0 or It > self._num_timesteps - 2):
+            raise IndexError(f"Valid range of time indices is 0,...,{self._num_timesteps - 2}, got {It}.")
+        self._It = It
+
+    def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
+        It, ft, gt = blend_weights(time_index, self._snapshots.dtype)
+        self.set_time_index(It)
+        window = BlendWindow(self.snapshot(It), self.snapshot(It + 1), ft, gt, It)
+        return FieldData(window, self.active_offsets)
+
+    @classmethod
+    def from_numpy(cls, data, z_stagger, y_stagger, x_stagger, **kw) -> "TimeDependentField":
+        return cls(data, z_stagger, y_stagger, x_stagger, **kw)
+
+    from_array = from_numpy
+
+
+__all__ = ["FieldData", "BlendWindow", "Field", "StaticField", "TimeDependentField", "blend_weights",
+           "OPX_F32", "OPX_F64", "OPX_BF16"]

@@ -1,0 +1,110 @@
+"""Turn particles / scalars / FieldData into the C structs of include/opx.h.
+
+Checks mirror what the reference's typed memoryviews enforce at kernel entry
+(dtype, C-contiguity; kernels/_core.pyx:5-24 and e.g. ab3_w_advection.pyx:22-40)
+plus the device check this path needs.  A missing particle field, scalar or
+field raises ``KeyError`` like the reference's dictionary lookups.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from .. import _lib
+from ..fields import BlendWindow
+
+# opx_particles member -> reference particle field name
+_PARTICLE_NAMES = {
+    "zidx": "zidx", "yidx": "yidx", "xidx": "xidx", "z": "z",
+    "dz0": "_dz0", "dz1": "_dz1", "dz2": "_dz2",
+    "dyidx0": "_dyidx0", "dyidx1": "_dyidx1", "dyidx2": "_dyidx2",
+    "dxidx0": "_dxidx0", "dxidx1": "_dxidx1", "dxidx2": "_dxidx2",
+}
+
+
+def _check_tensor(t: torch.Tensor, name: str, dtype: torch.dtype) -> int:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"particle field '{name}' must be a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"Buffer dtype mismatch for '{name}': expected {dtype}, got {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError(f"particle field '{name}' must be in device memory (got {t.device})")
+    if not t.is_contiguous():
+        raise ValueError(f"particle field '{name}' must be C-contiguous")
+    return t.data_ptr()
+
+
+def particles_struct(particles, required=()) -> _lib.opx_particles:
+    s = _lib.opx_particles()
+    s.n = len(particles)
+    s.status = _check_tensor(particles["status"], "status", torch.uint8)
+    for member, name in _PARTICLE_NAMES.items():
+        if name in particles:
+            setattr(s, member, _check_tensor(particles[name], name, torch.float64))
+        elif name in required:
+            raise KeyError(name)
+    return s
+
+
+def scalars_struct(scalars: dict, need=()) -> _lib.opx_scalars:
+    for k in need:
+        if k not in scalars:
+            raise KeyError(k)
+    g = scalars.get
+    return _lib.opx_scalars(
+        float(g("_dt", 0.0)), float(g("_RK2_alpha", 0.0)), float(g("hc", 0.0)), int(g("NZ", 0)), 0,
+        float(g("zidx_min", 0.0)), float(g("zidx_max", 0.0)),
+        float(g("yidx_min", 0.0)), float(g("yidx_max", 0.0)),
+        float(g("xidx_min", 0.0)), float(g("xidx_max", 0.0)),
+    )
+
+
+def _device_data(t: torch.Tensor, name: str) -> int:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"field '{name}' data must be a device tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"field '{name}' must be in device memory (got {t.device})")
+    if not t.is_contiguous():
+        raise ValueError(f"field '{name}' must be C-contiguous")
+    return t.data_ptr()
+
+
+def field_struct(fd, name: str) -> _lib.opx_field:
+    arr, offsets = fd.array, fd.offsets
+    f = _lib.opx_field()
+    if isinstance(arr, BlendWindow):
+        f.prev = _device_data(arr.prev, name)
+        f.next = _device_data(arr.next, name)
+        if arr.next.shape != arr.prev.shape or arr.next.dtype != arr.prev.dtype:
+            raise ValueError(f"field '{name}': snapshot shapes/dtypes differ")
+        f.ft, f.gt = arr.ft, arr.gt
+        t = arr.prev
+    else:
+        f.prev = _device_data(arr, name)
+        f.next = None
+        t = arr
+    f.dtype = _lib.storage_code(t.dtype)
+    f.ndim = t.dim()
+    if len(offsets) != t.dim():
+        raise ValueError(f"field '{name}': {len(offsets)} offsets for {t.dim()} dims")
+    for d in range(t.dim()):
+        f.shape[d] = int(t.shape[d])
+        f.offsets[d] = float(offsets[d])
+    return f
+
+
+def fields_struct(fielddata: dict, names) -> _lib.opx_roms_fields:
+    s = _lib.opx_roms_fields()
+    for name in names:
+        setattr(s, name, field_struct(fielddata[name], name))
+    return s
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def byref(x):
+    return ctypes.byref(x)

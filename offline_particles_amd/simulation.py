@@ -1,0 +1,227 @@
+"""Simulation (mirrors the reference's simulation.py:27-367 for the stepping path).
+
+``Simulation.step`` runs one timestepper step and then any scheduled events;
+``set_indices`` / ``set_particle_field`` write the device particle arrays.
+Events and schedulers are the reference's minimal host-side plumbing
+(events/_events.py, events/_schedulers.py) so that event kernels launch at the
+same iterations/times; output writers (zarr) are not part of this package.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import itertools
+from typing import Callable
+
+import numpy as np
+import torch
+
+from .kernels import merge_particle_fields
+from .launcher import Launcher
+from .particles import Particles, ParticlesView
+
+
+@dataclasses.dataclass(frozen=True)
+class SimulationState:
+    time: object
+    dt: object
+    tidx: np.float64
+    iteration: int
+    particles: ParticlesView
+
+
+class Event:
+    """A function on the simulation state plus kernels launched before it."""
+
+    def __init__(self, name: str, func: Callable, *kernels) -> None:
+        self._name = name
+        self._func = func
+        self._kernels = kernels
+
+    def __call__(self, state: SimulationState) -> None:
+        self._func(state)
+
+    name = property(lambda self: self._name)
+    kernels = property(lambda self: self._kernels)
+
+
+class IterationScheduler:
+    """Triggers events every N iterations (events/_schedulers.py:13-71)."""
+
+    def __init__(self) -> None:
+        self._next = None
+        self._events: dict = {}
+
+    def _schedule_event(self, iteration, N, event) -> None:
+        self._events.setdefault(iteration, []).append((N, event))
+
+    @property
+    def events(self):
+        for lst in self._events.values():
+            for _, e in lst:
+                yield e
+
+    def register_event(self, first: int, n: int, event: Event) -> None:
+        self._schedule_event(first, n, event)
+        self.set_next()
+
+    def set_next(self) -> None:
+        self._next = min(self._events.keys()) if self._events else None
+
+    def __call__(self, iteration: int) -> list:
+        out = []
+        while self._next is not None and self._next <= iteration:
+            for N, event in self._events.pop(self._next, []):
+                out.append(event)
+                self._schedule_event(self._next + N, N, event)
+            self.set_next()
+        return out
+
+
+class TimeScheduler:
+    """Triggers events every dt of simulated time (events/_schedulers.py:74-140)."""
+
+    def __init__(self, *, forward: bool = True) -> None:
+        self._forward = forward
+        self._next_time = None
+        self._events: dict = {}
+
+    def _schedule_event(self, time, dt, event) -> None:
+        self._events.setdefault(time, []).append((dt, event))
+
+    @property
+    def events(self):
+        for lst in self._events.values():
+            for _, e in lst:
+                yield e
+
+    def register_event(self, first, dt, event: Event) -> None:
+        self._schedule_event(first, dt, event)
+        self.set_next()
+
+    def set_next(self) -> None:
+        if not self._events:
+            self._next_time = None
+        else:
+            self._next_time = min(self._events) if self._forward else max(self._events)
+
+    def __call__(self, time) -> list:
+        out = []
+        while (nt := self._next_time) is not None and (nt <= time if self._forward else nt >= time):
+            for dt, event in self._events.pop(nt, []):
+                out.append(event)
+                self._schedule_event(nt + dt, dt, event)
+            self.set_next()
+        return out
+
+
+class Simulation:
+    """A particle simulation on one device."""
+
+    def __init__(self, nparticles: int, timestepper, fieldset, iteration_scheduler=None, time_scheduler=None,
+                 output_writers=None, *, device=None) -> None:
+        self._timestepper = timestepper
+        self._fieldset = fieldset
+        self._iteration_scheduler = iteration_scheduler or IterationScheduler()
+        self._time_scheduler = time_scheduler or TimeScheduler()
+        self._launcher = Launcher(fieldset)
+        self._launcher.maybe_increase_index_padding(timestepper.index_padding)
+        self._launcher.register_scalar_data_sources_from_object(self._timestepper)
+        for event in itertools.chain(self._iteration_scheduler.events, self._time_scheduler.events):
+            self._launcher.register_scalar_data_sources_from_object(event)
+        kernels = list(self._timestepper.kernels)
+        for event in itertools.chain(self._iteration_scheduler.events, self._time_scheduler.events):
+            kernels.extend(event.kernels)
+        particle_fields = merge_particle_fields(kernels)
+        self._particles = Particles(nparticles, device=device, **particle_fields)
+        self._particles_view = ParticlesView(self._particles)
+        self._invoke_events()
+
+    timestepper = property(lambda self: self._timestepper)
+    fieldset = property(lambda self: self._fieldset)
+    launcher = property(lambda self: self._launcher)
+    time = property(lambda self: self._timestepper.time)
+    iteration = property(lambda self: self._timestepper.iteration)
+    dt = property(lambda self: self._timestepper.dt)
+    tidx = property(lambda self: self._timestepper.tidx)
+    particles = property(lambda self: self._particles_view)
+    iteration_scheduler = property(lambda self: self._iteration_scheduler)
+    time_scheduler = property(lambda self: self._time_scheduler)
+
+    @property
+    def state(self) -> SimulationState:
+        return SimulationState(time=self.time, dt=self.dt, tidx=self.tidx, iteration=self.iteration,
+                               particles=self._particles_view)
+
+    def _invoke_events(self) -> None:
+        for event in itertools.chain(self._iteration_scheduler(self.iteration), self._time_scheduler(self.time)):
+            for kernel in event.kernels:
+                self._launcher.launch_kernel(kernel, self._particles, self.tidx)
+            event(self.state)
+
+    def _as_field(self, name, values):
+        target = self._particles[name]
+        if isinstance(values, torch.Tensor):
+            t = values.to(device=target.device, dtype=target.dtype)
+        else:
+            np_dtype = torch.empty((), dtype=target.dtype).numpy().dtype
+            arr = np.asarray(values, dtype=np_dtype)
+            t = torch.from_numpy(np.ascontiguousarray(arr)).to(target.device)
+        return torch.broadcast_to(t, target.shape)
+
+    def set_indices(self, zidx=None, yidx=None, xidx=None) -> None:
+        """Set particle indices (simulation.py:178-207); all inputs are checked before any write."""
+        vals = {}
+        for name, v in (("zidx", zidx), ("yidx", yidx), ("xidx", xidx)):
+            if v is not None:
+                vals[name] = self._as_field(name, v)
+        for name, t in vals.items():
+            self._particles[name].copy_(t)
+
+    def set_particle_field(self, field_name: str, values) -> None:
+        """Set one particle field (simulation.py:209-223)."""
+        self._particles[field_name].copy_(self._as_field(field_name, values))
+
+    def step(self) -> None:
+        """Advance the particles by one timestep, then run scheduled events."""
+        self._timestepper.timestep_particles(self._particles, self._launcher)
+        self._invoke_events()
+
+
+class SimulationBuilder:
+    """Builds a Simulation with events (simulation.py:232-367; output writers omitted)."""
+
+    def __init__(self, timestepper, fieldset) -> None:
+        self._timestepper = timestepper
+        self._fieldset = fieldset
+        self._iteration_scheduler = IterationScheduler()
+        self._time_scheduler = TimeScheduler()
+
+    def every_n(self, n: int, event: Event, *, first: int | None = None) -> None:
+        if n <= 0:
+            raise ValueError("n must be a positive integer.")
+        self._iteration_scheduler.register_event(0 if first is None else first, n, event)
+
+    def every_dt(self, dt, event: Event, *, first=None) -> None:
+        t = self._timestepper.time
+        if first is None:
+            first = t
+        try:
+            _ = t + dt
+        except TypeError as e:
+            raise TypeError(f"Incompatible dt type {type(dt)} for timestepper time type {type(t)}") from e
+        self._time_scheduler.register_event(first, dt, event)
+
+    def add_event(self, event: Event, *, n=None, dt=None, first=None) -> None:
+        if n is not None and dt is not None:
+            raise ValueError("Cannot specify both n and dt.")
+        if n is not None:
+            self.every_n(n, event, first=first)
+        elif dt is not None:
+            self.every_dt(dt, event, first=first)
+        else:
+            raise ValueError("Either n or dt must be specified.")
+
+    def build_simulation(self, nparticles: int, *, device=None) -> Simulation:
+        return Simulation(nparticles, self._timestepper, self._fieldset, self._iteration_scheduler,
+                          self._time_scheduler, {}, device=device)
