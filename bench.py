@@ -10,6 +10,11 @@ Forms measured in the same run:
 - plain_fp32: the drop-in behind the reference API (Simulation.step ->
   validation, ab3_w_advection, ab3_post_w_advection HIP kernels), FP32 fields,
   FP64 positions and FP64 interpolation.
+- fused_fp32 / fused_bf16: AB3Engine, one fused HIP kernel per step over
+  split positions (int32 cell + FP32 offsets) with u, v, w stored in FP32 or
+  BF16, Morton re-binning every 25 steps (the re-binning is inside the timed
+  region).
+The headline ``value`` is the fastest form; every form is listed in ``forms``.
 
 Prints ONE JSON line.  ``value`` is particle-steps/s of the headline form with
 all inputs resident in HBM; particle-steps count the particles active during
@@ -177,6 +182,77 @@ def run_plain(wl, device, steps, warmup):
     return res
 
 
+def field_bytes_touched_fused(wl, pos, storage_bytes_3d):
+    """Distinct field bytes one fused launch reads: u, v, w (2 snapshots), packed h/dx/dy, zeta (2), C."""
+    from offline_particles_amd.workloads import OFFSETS
+
+    total = 0
+    for name in ("u", "v", "w"):
+        total += 2 * storage_bytes_3d * touched_nodes(wl.fields[name].shape[1:], OFFSETS[name], pos)
+    n2 = touched_nodes(wl.fields["h"].shape, OFFSETS["h"], pos)
+    total += 16 * n2 + 2 * 4 * n2
+    total += wl.fields["C"].nbytes
+    return total
+
+
+# fused AB3 kernel: per active particle it reads status (1), release id (2),
+# cell (4), 3 offsets (12) and 6 history values (24) and writes cell, offsets
+# and history (40 B).
+FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
+
+
+def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow):
+    import torch
+
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.workloads import build_fieldset
+
+    fs = build_fieldset(wl, device=device)
+    eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
+                    capacity=wl.nparticles, rebin_every=rebin_every, xlow=xlow)
+    del fs
+    eng.add_release(*wl.positions, nsteps=steps + warmup + 2)
+    eng.rebin()
+    eng.step(warmup)
+    torch.cuda.synchronize()
+    a0 = eng.active_count()
+    z, y, x = eng.positions()
+    act = eng.status() < 128
+    sb = 2 if storage == torch.bfloat16 else 4
+    fbytes = field_bytes_touched_fused(wl, [z[act], y[act], x[act]], sb)
+    del z, y, x, act
+    eng.timer = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms = float(np.mean([a.elapsed_time(b) for a, b in eng.timer]))
+    eng.timer = None
+    a1 = eng.active_count()
+    eng.check_clock()
+    elapsed = t1 - t0
+    actn = 0.5 * (a0 + a1)
+    algo = actn * FUSED_STATE_BYTES + fbytes
+    name = "fused_bf16" if storage == torch.bfloat16 else "fused_fp32"
+    res = {
+        "form": name,
+        "particle_steps_per_s": actn * steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "active_particles": actn,
+        "kernel": f"k_ab3_fused<{'bf16' if sb == 2 else 'f32'}> (opx_ab3_fused)",
+        "kernel_ms": kms,
+        "algo_bytes": algo,
+        "field_bytes_touched": fbytes,
+        "achieved_GBs": algo / (kms * 1e-3) / 1e9,
+        "rebin_every": rebin_every,
+        "xlow": xlow,
+    }
+    del eng
+    torch.cuda.empty_cache()
+    return res
+
+
 # ---------------------------------------------------------------------------
 # CPU baseline: reference-structured FP64 oracle on the host cores
 # ---------------------------------------------------------------------------
@@ -217,6 +293,55 @@ def cpu_baseline(wl, seconds, max_particles):
     }
 
 
+def accuracy(wl, device, nsub, steps):
+    """Max / RMS position deviation of each form from the FP64 oracle after ``steps`` steps."""
+    import torch
+
+    from oracle import OracleField, OracleSimulation
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.workloads import OFFSETS, TIME_DEPENDENT, build_fieldset, build_simulation
+
+    pos = tuple(a[:nsub] for a in wl.positions)
+    t = time.perf_counter()
+    fields = {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in wl.fields.items()}
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    sim = OracleSimulation(fields, wl.time_array, wl.dt, scheme="ab3", nparticles=nsub, hc=wl.hc, NZ=wl.NZ,
+                           bounds=wl.bounds, threads=threads)
+    for k, a in zip(("zidx", "yidx", "xidx"), pos):
+        sim.p[k][:] = a
+    sim.p["status"][:] = 10
+    sim.run(steps)
+    log(f"accuracy: oracle {nsub} x {steps} steps in {time.perf_counter() - t:.1f}s")
+    ref = {k: sim.p[k] for k in ("zidx", "yidx", "xidx", "status")}
+
+    def report(name, z, y, x, st):
+        both = (st < 128) & (ref["status"] < 128)
+        d = np.stack([np.abs(z - ref["zidx"]), np.abs(y - ref["yidx"]), np.abs(x - ref["xidx"])])[:, both]
+        dm = d.max(axis=0)
+        return {"max": float(dm.max()), "rms": float(np.sqrt((dm**2).mean())),
+                "p99": float(np.quantile(dm, 0.99)),
+                "max_zyx": [float(v) for v in d.max(axis=1)],
+                "status_mismatch": int((st != ref["status"]).sum()), "compared": int(both.sum())}
+
+    out = {"particles": nsub, "steps": steps}
+    g = build_simulation(wl, "ab3", device=device, positions=pos)
+    for _ in range(steps):
+        g.step()
+    out["plain_fp32"] = report("plain", *(g.particles.to_numpy(k) for k in ("zidx", "yidx", "xidx", "status")))
+    del g
+    fs = build_fieldset(wl, device=device)
+    for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
+        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=dt, device=device, capacity=nsub)
+        eng.add_release(*pos, nsteps=steps)
+        eng.step(steps)
+        eng.check_clock()
+        z, y, x = (t_.cpu().numpy() for t_ in eng.positions())
+        out[name] = report(name, z, y, x, eng.status().cpu().numpy())
+        del eng
+    log("accuracy:", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +352,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-particles", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain_fp32,fused_fp32,fused_bf16")
+    ap.add_argument("--rebin-every", type=int, default=25)
+    ap.add_argument("--xlow", type=int, default=2)
+    ap.add_argument("--accuracy", type=int, default=0, metavar="N",
+                    help="also report deviation from the FP64 oracle on the first N particles")
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -247,11 +378,26 @@ def main():
                             dtype=np.float32)
     log(f"workload generated in {time.perf_counter() - t:.1f}s: grid {NX}x{NY}x{NZ}, {args.particles} particles")
 
-    forms = {}
-    forms["plain_fp32"] = run_plain(wl, device, args.steps, args.warmup)
-    log("plain_fp32", forms["plain_fp32"])
+    acc = None
+    if args.accuracy:
+        wl_acc = wl
+        if args.accuracy_steps + 5 > args.steps + args.warmup + 5:
+            wl_acc = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.accuracy,
+                                        steps=args.accuracy_steps + 5, dtype=np.float32)
+        acc = accuracy(wl_acc, device, args.accuracy, args.accuracy_steps)
+        del wl_acc
 
-    head = forms["plain_fp32"]
+    forms = {}
+    wanted = args.forms.split(",")
+    if "plain_fp32" in wanted:
+        forms["plain_fp32"] = run_plain(wl, device, args.steps, args.warmup)
+        log("plain_fp32", forms["plain_fp32"])
+    for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
+        if name in wanted:
+            forms[name] = run_fused(wl, device, args.steps, args.warmup, dt, args.rebin_every, args.xlow)
+            log(name, forms[name])
+
+    head = max(forms.values(), key=lambda f: f["particle_steps_per_s"])
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -269,10 +415,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"plain_fp32": "f64", "fused_fp32": "f32", "fused_bf16": "f32"}[head["form"]],
         "data": "synthetic (seeded turbulent ROMS fields, uniform random particles)",
         "config": {
-            "workload": f"AB3 dt=120s, {args.particles} particles, {NX}x{NY}x{NZ} grid, 8 hourly FP32 snapshots",
+            "workload": f"AB3 dt=120s, {args.particles} particles, {NX}x{NY}x{NZ} grid, 8 hourly snapshots "
+                        f"(FP32 data; the {head['form']} form)",
             "form": head["form"],
             "particles": args.particles,
             "grid": [NX, NY, NZ],
@@ -287,6 +434,7 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "accuracy": acc,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
     }

@@ -139,6 +139,116 @@ int opx_ab3_w_advection(const opx_particles *p, const opx_scalars *s, const opx_
 int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const opx_roms_fields *f,
                              void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Fused form: one kernel per AB3 step over split positions.           */
+/*                                                                     */
+/* The reference's AB3 step is three launches, each with a full-grid   */
+/* time blend (timesteppers.py:296-305: validation_kernel,             */
+/* ab3_w_advection, advance_time, ab3_post_w_advection).  Here one     */
+/* kernel does the pre-step validation, the AB3 stage and update and   */
+/* the post-step zidx at t_{n+1} per particle, with the time blend at  */
+/* the stencil values.  Positions are split into a packed int32 cell   */
+/* and 32-bit fixed-point offsets; u, v, w are FP32 or BF16 snapshot   */
+/* rings; h, dx, dy, zeta FP32; the vertical coordinate, the position  */
+/* update and the clock run in FP64.                                   */
+/* The staggering is the ROMS C-grid's: u (NZ, NY, NX-1), v (NZ, NY-1, */
+/* NX), w (NZ+1, NY, NX), zeta/h/dx/dy (NY, NX), C = Cs_w (NZ+1).      */
+/* ------------------------------------------------------------------ */
+
+/* Split-position particle state, SoA.  Per axis, index = c + q * 2^-32 with
+ * c = floor(index) and q a 32-bit fixed-point offset, so a position converts
+ * to FP64 exactly.  `cell` packs the three axes as
+ * (c_z + 2) << (xbits + ybits) | (c_y + 2) << xbits | (c_x + 2); an axis field
+ * of all ones marks a non-finite index (NaN/inf: flagged NONFINITE by the next
+ * validation, exported as NaN).  hz1..hx2 are the FP32 AB3 derivative history
+ * f_{n-1}, f_{n-2} (the reference's _dz1.._dxidx2).  `rel` (release id per
+ * particle) may be NULL (all release 0); `z` (FP32 output) may be NULL; `pid`
+ * is permuted with the state by opx_rebin and may be NULL when rebinning is
+ * not used. */
+typedef struct opx_split_state {
+    int64_t n;
+    uint8_t *status;
+    int32_t *cell;
+    uint32_t *fz, *fy, *fx;
+    float *hz1, *hy1, *hx1, *hz2, *hy2, *hx2;
+    int32_t *pid;
+    uint16_t *rel;
+    float *z;
+} opx_split_state;
+
+/* Grid sizes (rho points), cell-field widths with (1 << bits) >= N + 5 and
+ * xbits + ybits + zbits <= 31, hc and the inclusive index bounds. */
+typedef struct opx_grid {
+    int32_t NZ, NY, NX;
+    int32_t xbits, ybits, zbits;
+    double hc;
+    double zidx_min, zidx_max, yidx_min, yidx_max, xidx_min, xidx_max;
+} opx_grid;
+
+/* A ring of `nslots` snapshots per time-dependent field, contiguous with the
+ * given element strides between slots; `hdd` packs the static (h, dx, dy, 0)
+ * per rho point as float4. */
+typedef struct opx_window {
+    const void *u, *v, *w;  /* dtype3 storage */
+    const float *zeta;
+    int64_t u_stride, v_stride, w_stride, zeta_stride;
+    int32_t dtype3; /* OPX_F32 or OPX_BF16 */
+    int32_t nslots;
+    const float *hdd; /* (NY, NX, 4) */
+    const double *C;  /* (NZ + 1) */
+} opx_window;
+
+/* Per-release FP64 clock kept on the device.  A release advances at global
+ * steps >= start_step while steps_left > 0. */
+typedef struct opx_release_clock {
+    double t;
+    int32_t start_step;
+    int32_t steps_left;
+} opx_release_clock;
+
+/* Per-release weights of one step, written by opx_clock_advance: u/v/w/zeta
+ * blend at t_n in slots (s0, s1) with (ft, gt); zeta at t_{n+1} in slots
+ * (s0p, s1p) with (ft1, gt1); FP32-rounded like fields.py:423-424. */
+typedef struct opx_release_step {
+    float ft, gt, ft1, gt1;
+    int32_t s0, s1, s0p, s1p;
+    int32_t active;
+    int32_t pad_;
+} opx_release_step;
+
+/* Advance `nrel` release clocks by one step of `dt` and write their step
+ * weights.  The time index restates Timestepper.get_time_index
+ * (timesteppers.py:64-85) on the device: searchsorted over time_array, then
+ * idx + (t - t0)/(t1 - t0), and divmod(tidx, 1) (fields.py:409).  Snapshot It
+ * lives in slot It mod nslots.  A release whose time leaves the time array is
+ * stopped and *err_flag is set to 1 (the reference raises ValueError /
+ * IndexError). */
+int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel,
+                      const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
+                      double dt, int32_t *err_flag, void *stream);
+
+/* One fused AB3 step (validation -> ab3_w_advection -> ab3_post_w_advection)
+ * for particles [0, n) (or [0, *n_active) when n_active is not NULL). */
+int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, const int32_t *n_active, void *stream);
+
+/* Split <-> FP64 index conversion.  opx_split_from_f64 writes the state of
+ * particle i from idx arrays (and status/rel untouched); opx_split_to_f64
+ * writes zidx/yidx/xidx[pid[i]] (or [i] if pid is NULL). */
+int opx_split_from_f64(const opx_split_state *s, const opx_grid *g, const double *zidx, const double *yidx,
+                       const double *xidx, void *stream);
+int opx_split_to_f64(const opx_split_state *s, const opx_grid *g, double *zidx, double *yidx, double *xidx,
+                     void *stream);
+
+/* Re-bin: sort particles by a spatial key (Morton order of the cell with the
+ * low `xlow` x bits kept contiguous; inactive particles last) and permute the
+ * whole state from `src` into `dst` (same n).  Writes the number of active
+ * particles to *n_active (device int).  `work` is caller-owned device memory
+ * of at least opx_rebin_workspace_bytes(n) bytes. */
+int opx_rebin_workspace_bytes(int64_t n, uint64_t *bytes);
+int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+              void *work, uint64_t work_bytes, int32_t *n_active, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

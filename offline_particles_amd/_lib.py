@@ -88,7 +88,52 @@ _SIGNATURES = {
                                  ctypes.POINTER(opx_roms_fields), _vp],
 }
 
-# extra signatures registered by other modules (fused engine)
+
+class opx_split_state(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("status", _vp), ("cell", _vp), ("fz", _vp), ("fy", _vp), ("fx", _vp),
+                ("hz1", _vp), ("hy1", _vp), ("hx1", _vp), ("hz2", _vp), ("hy2", _vp), ("hx2", _vp),
+                ("pid", _vp), ("rel", _vp), ("z", _vp)]
+
+
+class opx_grid(ctypes.Structure):
+    _fields_ = [("NZ", ctypes.c_int32), ("NY", ctypes.c_int32), ("NX", ctypes.c_int32),
+                ("xbits", ctypes.c_int32), ("ybits", ctypes.c_int32), ("zbits", ctypes.c_int32),
+                ("hc", ctypes.c_double),
+                ("zidx_min", ctypes.c_double), ("zidx_max", ctypes.c_double),
+                ("yidx_min", ctypes.c_double), ("yidx_max", ctypes.c_double),
+                ("xidx_min", ctypes.c_double), ("xidx_max", ctypes.c_double)]
+
+
+class opx_window(ctypes.Structure):
+    _fields_ = [("u", _vp), ("v", _vp), ("w", _vp), ("zeta", _vp),
+                ("u_stride", ctypes.c_int64), ("v_stride", ctypes.c_int64), ("w_stride", ctypes.c_int64),
+                ("zeta_stride", ctypes.c_int64), ("dtype3", ctypes.c_int32), ("nslots", ctypes.c_int32),
+                ("hdd", _vp), ("C", _vp)]
+
+
+class opx_release_clock(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("start_step", ctypes.c_int32), ("steps_left", ctypes.c_int32)]
+
+
+class opx_release_step(ctypes.Structure):
+    _fields_ = [("ft", ctypes.c_float), ("gt", ctypes.c_float), ("ft1", ctypes.c_float), ("gt1", ctypes.c_float),
+                ("s0", ctypes.c_int32), ("s1", ctypes.c_int32), ("s0p", ctypes.c_int32), ("s1p", ctypes.c_int32),
+                ("active", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+_P = ctypes.POINTER
+_SIGNATURES.update({
+    "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_double, _vp, _vp],
+    "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, _vp],
+    "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
+    "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
+    "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],
+    "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_uint64,
+                  _vp, _vp],
+})
+
+# extra signatures registered by other modules
 EXTRA_SIGNATURES: dict = {}
 
 _lib = None

@@ -1,0 +1,733 @@
+// Fused AB3 step over split positions, the on-device release clock, and
+// Morton re-binning.
+//
+// One thread per particle does, in registers, what the reference does in
+// three launches per step (timesteppers.py:296-305, paths relative to
+// src/offline_particles/ of the reference):
+//   validation_kernel       kernels/validation.pyx:21-77
+//   ab3_w_advection         kernels/roms/ab3_w_advection.pyx:20-178
+//   ab3_post_w_advection    kernels/roms/ab3_w_advection.pyx:180-229 (at t_{n+1})
+// with the time blend done on the stencil values (fields.py:409-428) instead
+// of over the whole grid.
+//
+// Precision: u, v, w are stored as FP32 or BF16; u and v are interpolated in
+// FP32, w, h and zeta with FP64 weights, and the ROMS vertical transform
+// (kernels/roms/_vertical_coordinate.pxd) runs in FP64.  A position is a
+// packed int32 cell (floor of the index, per axis) plus a 32-bit fixed-point
+// offset (units of 2^-32 cell), so it converts to FP64 exactly and the AB3
+// update is applied in FP64 and rounded once to 2^-32 per step.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdint>
+
+#include "opx.h"
+#include "opx_runtime.hpp"
+
+namespace opx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kXcds = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ float ld3(const float *__restrict__ p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float ld3(const uint16_t *__restrict__ p, uint32_t i)
+{
+    return __uint_as_float((uint32_t)p[i] << 16);
+}
+
+constexpr double kQ = 4294967296.0; // 2^32: one cell in fixed-point offset units
+constexpr int kBias = 2;            // stored cell = floor(index) + 2, range [-2, N+1]
+
+// One axis of a particle position: index = c + q * 2^-32 with c = floor(index)
+// (unclamped, in [-2, N+1]); `ok` is false for a non-finite index.
+struct Axis {
+    int c;
+    uint32_t q;
+    bool ok;
+    __device__ __forceinline__ double pos() const { return (double)c + (double)q * (1.0 / kQ); }
+    __device__ __forceinline__ float frac() const { return (float)q * (1.0f / 4294967296.0f); }
+    __device__ __forceinline__ bool upper_half() const { return q >= 0x80000000u; }
+};
+
+__device__ __forceinline__ Axis decode_axis(uint32_t cell, int shift, int bits)
+{
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint32_t v = (cell >> shift) & mask;
+    Axis a;
+    a.ok = v != mask;
+    a.c = (int)v - kBias;
+    a.q = 0;
+    return a;
+}
+
+// Round an FP64 index to the fixed-point form; non-finite -> marker.  Cells
+// beyond [-2, N+1] are clamped (such a particle is out of the domain either way
+// and the next validation flags it).
+__device__ __forceinline__ Axis encode_axis(double v, int N)
+{
+    Axis a;
+    if (!__builtin_isfinite(v)) {
+        a.ok = false;
+        a.c = 0;
+        a.q = 0;
+        return a;
+    }
+    a.ok = true;
+    double fl = fmin(fmax(floor(v), (double)-kBias), (double)(N + 1));
+    double r = rint((v - fl) * kQ);
+    if (r >= kQ) {
+        if (fl < (double)(N + 1)) {
+            fl += 1.0;
+            r -= kQ;
+        } else {
+            r = kQ - 1.0;
+        }
+    }
+    r = fmax(r, 0.0);
+    a.c = (int)fl;
+    a.q = (uint32_t)r;
+    return a;
+}
+
+__device__ __forceinline__ uint32_t axis_field(const Axis &a, int bits)
+{
+    return a.ok ? (uint32_t)(a.c + kBias) : (1u << bits) - 1u;
+}
+
+__device__ __forceinline__ uint32_t pack_cell(const Axis &z, const Axis &y, const Axis &x, const opx_grid &g)
+{
+    return (axis_field(z, g.zbits) << (g.xbits + g.ybits)) | (axis_field(y, g.ybits) << g.xbits) |
+           axis_field(x, g.xbits);
+}
+
+// One interpolation axis: base index and weight of the upper node.
+struct Ax {
+    int b;
+    float w;
+};
+
+// CENTER stagger (offset 0, size N): truncate(idx) clamped to [0, N-2]
+// (kernels/_interpolation/linear.pxd:3-9), weight unclamped (extrapolation).
+__device__ __forceinline__ Ax ax_center(const Axis &a, int N)
+{
+    int b = max(0, min(a.c, N - 2));
+    return {b, a.frac() + (float)(a.c - b)};
+}
+
+// INNER stagger (offset -0.5, size N-1): truncate(idx - 0.5) clamped to [0, N-3].
+__device__ __forceinline__ Ax ax_inner(const Axis &a, int N)
+{
+    int b = a.c - 1 + (a.upper_half() ? 1 : 0);
+    b = max(0, min(b, N - 3));
+    return {b, a.frac() + ((float)(a.c - b) - 0.5f)};
+}
+
+// OUTER stagger (offset +0.5, size N+1): truncate(idx + 0.5) clamped to [0, N-1].
+__device__ __forceinline__ Ax ax_outer(const Axis &a, int N)
+{
+    int b = a.c + (a.upper_half() ? 1 : 0);
+    b = max(0, min(b, N - 1));
+    return {b, a.frac() + ((float)(a.c - b) + 0.5f)};
+}
+
+// FP64 weight of the upper node for base b and stagger offset o.
+__device__ __forceinline__ double w64(const Axis &a, int b, double o) { return ((double)(a.c - b) + o) + (double)a.q * (1.0 / kQ); }
+
+template <typename T>
+__device__ __forceinline__ float blend(const T *__restrict__ p, const T *__restrict__ q, uint32_t i, float gt,
+                                       float ft)
+{
+    return ld3(p, i) * gt + ld3(q, i) * ft;
+}
+
+// Trilinear interpolation of the blended field (FP32).
+template <typename T>
+__device__ __forceinline__ float trilinear(const T *__restrict__ p, const T *__restrict__ q, float gt, float ft,
+                                           int n1, int n2, Ax a0, Ax a1, Ax a2)
+{
+    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
+    const uint32_t b = ((uint32_t)a0.b * (uint32_t)n1 + (uint32_t)a1.b) * (uint32_t)n2 + (uint32_t)a2.b;
+    float v000 = blend(p, q, b, gt, ft), v001 = blend(p, q, b + 1, gt, ft);
+    float v010 = blend(p, q, b + s1, gt, ft), v011 = blend(p, q, b + s1 + 1, gt, ft);
+    float v100 = blend(p, q, b + s0, gt, ft), v101 = blend(p, q, b + s0 + 1, gt, ft);
+    float v110 = blend(p, q, b + s0 + s1, gt, ft), v111 = blend(p, q, b + s0 + s1 + 1, gt, ft);
+    const float f0 = a0.w, f1 = a1.w, f2 = a2.w;
+    const float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
+    float c00 = v000 * g2 + v001 * f2, c01 = v010 * g2 + v011 * f2;
+    float c10 = v100 * g2 + v101 * f2, c11 = v110 * g2 + v111 * f2;
+    return (c00 * g1 + c01 * f1) * g0 + (c10 * g1 + c11 * f1) * f0;
+}
+
+// Trilinear interpolation in FP64 of the blended field (corner values blended
+// in FP32 like the reference's FP32 blend, then widened).  Used for w: the
+// vertical index moves through layers that can be a fraction of a metre thick.
+template <typename T>
+__device__ __forceinline__ double trilinear64(const T *__restrict__ p, const T *__restrict__ q, float gt, float ft,
+                                              int n1, int n2, int b0, int b1, int b2, double f0, double f1,
+                                              double f2)
+{
+    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
+    const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
+    double v000 = blend(p, q, b, gt, ft), v001 = blend(p, q, b + 1, gt, ft);
+    double v010 = blend(p, q, b + s1, gt, ft), v011 = blend(p, q, b + s1 + 1, gt, ft);
+    double v100 = blend(p, q, b + s0, gt, ft), v101 = blend(p, q, b + s0 + 1, gt, ft);
+    double v110 = blend(p, q, b + s0 + s1, gt, ft), v111 = blend(p, q, b + s0 + s1 + 1, gt, ft);
+    const double g0 = 1.0 - f0, g1 = 1.0 - f1, g2 = 1.0 - f2;
+    double c00 = v000 * g2 + v001 * f2, c01 = v010 * g2 + v011 * f2;
+    double c10 = v100 * g2 + v101 * f2, c11 = v110 * g2 + v111 * f2;
+    return (c00 * g1 + c01 * f1) * g0 + (c10 * g1 + c11 * f1) * f0;
+}
+
+// FP64 bilinear of a 2-D field whose corner values are given.
+__device__ __forceinline__ double bil64(double v00, double v01, double v10, double v11, double fy, double fx)
+{
+    double gy = 1.0 - fy, gx = 1.0 - fx;
+    return gy * gx * v00 + gy * fx * v01 + fy * gx * v10 + fy * fx * v11;
+}
+
+struct Surface {
+    double h, zeta;
+    float dx, dy;
+};
+
+// h (and dx, dy) from the packed static metrics and zeta blended in FP32 then
+// widened, bilinear at rho position (y, x) with FP64 weights.
+__device__ __forceinline__ Surface surface(const float4 *__restrict__ hdd, const float *__restrict__ zp,
+                                           const float *__restrict__ zq, float gt, float ft, int NY, int NX,
+                                           const Axis &y, const Axis &x, bool want_dxdy)
+{
+    const int by = max(0, min(y.c, NY - 2)), bx = max(0, min(x.c, NX - 2));
+    const double wy = w64(y, by, 0.0), wx = w64(x, bx, 0.0);
+    const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
+    float4 m00 = hdd[b], m01 = hdd[b + 1], m10 = hdd[b + NX], m11 = hdd[b + NX + 1];
+    Surface s;
+    s.h = bil64(m00.x, m01.x, m10.x, m11.x, wy, wx);
+    float z00 = zp[b] * gt + zq[b] * ft;
+    float z01 = zp[b + 1] * gt + zq[b + 1] * ft;
+    float z10 = zp[b + NX] * gt + zq[b + NX] * ft;
+    float z11 = zp[b + NX + 1] * gt + zq[b + NX + 1] * ft;
+    s.zeta = bil64(z00, z01, z10, z11, wy, wx);
+    if (want_dxdy) {
+        const float fy = (float)wy, fx = (float)wx;
+        float gy = 1.0f - fy, gx = 1.0f - fx;
+        s.dx = (m00.y * gx + m01.y * fx) * gy + (m10.y * gx + m11.y * fx) * fy;
+        s.dy = (m00.z * gx + m01.z * fx) * gy + (m10.z * gx + m11.z * fx) * fy;
+    }
+    return s;
+}
+
+// XCD-aware block order: XCD k (blocks k, k+8, ...) works on the k-th
+// contiguous eighth of the particle range, so neighbouring (sorted) particles
+// share one L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int64_t xcd_particle_index()
+{
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t lb = (b % kXcds) * (G / kXcds) + b / kXcds;
+    return (int64_t)lb * kBlock + threadIdx.x;
+}
+
+// ---------------------------------------------------------------- clock
+
+// Timestepper.get_time_index (timesteppers.py:64-85): false where the
+// reference raises (time outside the array, or t == t_last).
+__device__ bool time_index(const double *__restrict__ ta, int n, double t, double &tidx)
+{
+    if (!(t >= ta[0] && t <= ta[n - 1]))
+        return false;
+    int lo = 0, hi = n; // searchsorted(side="right"): count of entries <= t
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ta[mid] <= t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    int idx = lo - 1;
+    if (idx + 1 >= n)
+        return false;
+    double t0 = ta[idx], t1 = ta[idx + 1];
+    double fraction = (t - t0) / (t1 - t0);
+    tidx = (double)idx + fraction;
+    return true;
+}
+
+// divmod(tidx, 1) and FP32 weights (fields.py:409-424), slots in the ring.
+__device__ __forceinline__ void weights(double tidx, int nslots, float &ft, float &gt, int &s0, int &s1)
+{
+    double It = floor(tidx);
+    double f = tidx - It;
+    ft = (float)f;
+    gt = 1.0f - ft;
+    long long I = (long long)It;
+    s0 = (int)(I % nslots);
+    s1 = (int)((I + 1) % nslots);
+}
+
+__global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int nrel, const double *ta, int ntime,
+                        int nslots, int step, double dt, int32_t *err)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrel)
+        return;
+    opx_release_clock c = clocks[r];
+    opx_release_step e{};
+    e.active = 0;
+    if (c.steps_left > 0 && step >= c.start_step) {
+        double tidx0, tidx1;
+        double t1 = c.t + dt; // advance_time: time += dt (timesteppers.py:159)
+        if (time_index(ta, ntime, c.t, tidx0) && time_index(ta, ntime, t1, tidx1)) {
+            weights(tidx0, nslots, e.ft, e.gt, e.s0, e.s1);
+            weights(tidx1, nslots, e.ft1, e.gt1, e.s0p, e.s1p);
+            e.active = 1;
+            c.t = t1;
+            c.steps_left -= 1;
+        } else {
+            c.steps_left = 0;
+            if (err)
+                *err = 1;
+        }
+        clocks[r] = c;
+    }
+    table[r] = e;
+}
+
+// ---------------------------------------------------------------- fused AB3
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_ab3_fused(opx_split_state s, opx_grid g, opx_window win, const opx_release_step *__restrict__ table,
+                double dt, const int32_t *__restrict__ n_active)
+{
+    const int64_t i = xcd_particle_index();
+    int64_t n = s.n;
+    if (n_active)
+        n = min(n, (int64_t)*n_active);
+    if (i >= n)
+        return;
+    const uint8_t st = s.status[i];
+    if (st & OPX_INACTIVE)
+        return;
+    const int r = s.rel ? (int)s.rel[i] : 0;
+    const opx_release_step rs = table[r];
+    if (!rs.active)
+        return;
+
+    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    Axis az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+    Axis ay = decode_axis(cell, g.xbits, g.ybits);
+    Axis ax = decode_axis(cell, 0, g.xbits);
+
+    // ---- validation (validation.pyx:21-77), inclusive bounds in FP64
+    if (!(az.ok && ay.ok && ax.ok)) {
+        s.status[i] = OPX_NONFINITE;
+        return;
+    }
+    az.q = s.fz[i];
+    ay.q = s.fy[i];
+    ax.q = s.fx[i];
+    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+    if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
+          xi <= g.xidx_max)) {
+        s.status[i] = OPX_OUT_OF_DOMAIN;
+        return;
+    }
+
+    // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
+    const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v), *W = static_cast<const T *>(win.w);
+    const float gt = rs.gt, ft = rs.ft;
+    const Ax cz = ax_center(az, NZ), cy = ax_center(ay, NY), cx = ax_center(ax, NX);
+    float u = trilinear(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz, cy,
+                        ax_inner(ax, NX));
+    float v = trilinear(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz,
+                        ax_inner(ay, NY), cx);
+    const Ax ow = ax_outer(az, NZ);
+    const double w = trilinear64(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
+                                 cx.b, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    const float4 *hdd = reinterpret_cast<const float4 *>(win.hdd);
+    Surface s0 = surface(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, gt, ft, NY,
+                         NX, ay, ax, true);
+    const float dx0 = u / s0.dx, dy0 = v / s0.dy, dz0 = (float)w;
+
+    // ---- Euler -> AB2 -> AB3 start-up (ab3_w_advection.pyx:121-138)
+    float dx1, dy1, dz1, dx2, dy2, dz2;
+    uint8_t st_new = st;
+    if (st == OPX_MULTISTEP_1) {
+        dx1 = dx2 = dx0;
+        dy1 = dy2 = dy0;
+        dz1 = dz2 = dz0;
+        st_new = OPX_MULTISTEP_2;
+    } else {
+        dx1 = s.hx1[i];
+        dy1 = s.hy1[i];
+        dz1 = s.hz1[i];
+        if (st == OPX_MULTISTEP_2) {
+            dx2 = -dx0 + 2.0f * dx1;
+            dy2 = -dy0 + 2.0f * dy1;
+            dz2 = -dz0 + 2.0f * dz1;
+            st_new = OPX_NORMAL;
+        } else {
+            dx2 = s.hx2[i];
+            dy2 = s.hy2[i];
+            dz2 = s.hz2[i];
+        }
+    }
+
+    // ---- z at x_n, t_n (ab3_w_advection.pyx:144-165), FP64
+    const double hc = g.hc;
+    const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
+    const double Cv = (1.0 - wC) * win.C[ow.b] + wC * win.C[ow.b + 1];
+    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double S0 = (hc * sigma + s0.h * Cv) / (hc + s0.h);
+    double z = s0.zeta + (s0.zeta + s0.h) * S0;
+
+    // ---- AB3 update (ab3_w_advection.pyx:168-170) in FP64 from FP32 derivatives;
+    // the vertical uses this step's FP64 w
+    const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
+    const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
+    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) / 12.0;
+    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) / 12.0;
+    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) / 12.0;
+    ay = encode_axis(yn, NY);
+    ax = encode_axis(xn, NX);
+
+    // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
+    Axis azn;
+    if (ay.ok && ax.ok) {
+        Surface s1 = surface(hdd, win.zeta + rs.s0p * win.zeta_stride, win.zeta + rs.s1p * win.zeta_stride, rs.gt1,
+                             rs.ft1, NY, NX, ay, ax, false);
+        const double S = (z - s1.zeta) / (s1.zeta + s1.h);
+        // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
+        // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
+        // with k, so a walk from the previous bracket finds the reference's k.
+        const double inv = 1.0 / (hc + s1.h);
+        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + s1.h * win.C[k]) * inv; };
+        int k = ow.b;
+        double Sk = S_at(k);
+        while (k > 0 && S < Sk) {
+            --k;
+            Sk = S_at(k);
+        }
+        double Sk1 = S_at(k + 1);
+        while (k < NZ - 1 && S >= Sk1) {
+            ++k;
+            Sk = Sk1;
+            Sk1 = S_at(k + 1);
+        }
+        azn = encode_axis(((double)k - 0.5) + (S - Sk) / (Sk1 - Sk), NZ);
+    } else {
+        azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
+    }
+
+    // ---- store: position, history shift (ab3_w_advection.pyx:173-178), status
+    s.cell[i] = (int32_t)pack_cell(azn, ay, ax, g);
+    s.fz[i] = azn.q;
+    s.fy[i] = ay.q;
+    s.fx[i] = ax.q;
+    s.hz2[i] = dz1;
+    s.hy2[i] = dy1;
+    s.hx2[i] = dx1;
+    s.hz1[i] = dz0;
+    s.hy1[i] = dy0;
+    s.hx1[i] = dx0;
+    if (st_new != st)
+        s.status[i] = st_new;
+    if (s.z)
+        s.z[i] = (float)z;
+}
+
+// ---------------------------------------------------------------- conversions
+
+__global__ void k_split_from_f64(opx_split_state s, opx_grid g, const double *zidx, const double *yidx,
+                                 const double *xidx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n)
+        return;
+    Axis z = encode_axis(zidx[i], g.NZ), y = encode_axis(yidx[i], g.NY), x = encode_axis(xidx[i], g.NX);
+    s.cell[i] = (int32_t)pack_cell(z, y, x, g);
+    s.fz[i] = z.q;
+    s.fy[i] = y.q;
+    s.fx[i] = x.q;
+}
+
+__global__ void k_split_to_f64(opx_split_state s, opx_grid g, double *zidx, double *yidx, double *xidx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n)
+        return;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    Axis z = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+    Axis y = decode_axis(cell, g.xbits, g.ybits);
+    Axis x = decode_axis(cell, 0, g.xbits);
+    z.q = s.fz[i];
+    y.q = s.fy[i];
+    x.q = s.fx[i];
+    const double nan = __builtin_nan("");
+    int64_t o = s.pid ? (int64_t)s.pid[i] : i;
+    zidx[o] = z.ok ? z.pos() : nan;
+    yidx[o] = y.ok ? y.pos() : nan;
+    xidx[o] = x.ok ? x.pos() : nan;
+}
+
+// ---------------------------------------------------------------- re-binning
+
+__device__ __forceinline__ uint32_t spatial_key(uint32_t x, uint32_t y, uint32_t z, int xbits, int ybits,
+                                                int zbits, int xlow)
+{
+    const uint32_t xl = x & ((1u << xlow) - 1u);
+    const uint32_t xh = x >> xlow;
+    const int xhb = xbits - xlow;
+    const int levels = max(xhb, max(ybits, zbits));
+    uint32_t key = 0;
+    int pos = 0;
+    for (int b = 0; b < levels; ++b) {
+        if (b < xhb)
+            key |= ((xh >> b) & 1u) << pos++;
+        if (b < ybits)
+            key |= ((y >> b) & 1u) << pos++;
+        if (b < zbits)
+            key |= ((z >> b) & 1u) << pos++;
+    }
+    return (key << xlow) | xl;
+}
+
+__global__ void k_keys(opx_split_state s, opx_grid g, int xlow, uint32_t inactive_key, uint32_t *keys,
+                       uint32_t *vals, int32_t *n_active)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (i < s.n) {
+        const uint32_t cell = (uint32_t)s.cell[i];
+        act = !(s.status[i] & OPX_INACTIVE);
+        uint32_t key = inactive_key;
+        if (act) {
+            uint32_t x = cell & ((1u << g.xbits) - 1);
+            uint32_t y = (cell >> g.xbits) & ((1u << g.ybits) - 1);
+            uint32_t z = cell >> (g.xbits + g.ybits);
+            key = spatial_key(x, y, z, g.xbits, g.ybits, g.zbits, xlow);
+        }
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+    // one atomic per wavefront
+    unsigned long long m = __ballot(act);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && m)
+        atomicAdd(n_active, (int32_t)__popcll(m));
+}
+
+__global__ void k_permute(opx_split_state src, opx_split_state dst, const uint32_t *__restrict__ perm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= src.n)
+        return;
+    const uint32_t j = perm[i];
+    dst.status[i] = src.status[j];
+    dst.cell[i] = src.cell[j];
+    dst.fz[i] = src.fz[j];
+    dst.fy[i] = src.fy[j];
+    dst.fx[i] = src.fx[j];
+    dst.hz1[i] = src.hz1[j];
+    dst.hy1[i] = src.hy1[j];
+    dst.hx1[i] = src.hx1[j];
+    dst.hz2[i] = src.hz2[j];
+    dst.hy2[i] = src.hy2[j];
+    dst.hx2[i] = src.hx2[j];
+    if (src.pid)
+        dst.pid[i] = src.pid[j];
+    if (src.rel)
+        dst.rel[i] = src.rel[j];
+    if (src.z)
+        dst.z[i] = src.z[j];
+}
+
+__global__ void k_zero_int(int32_t *p) { *p = 0; }
+
+// ---------------------------------------------------------------- host checks
+
+int check_state(const opx_split_state *s, const char *what)
+{
+    OPX_REQUIRE(s != nullptr, "%s: null state", what);
+    OPX_REQUIRE(s->n >= 0 && s->n <= INT32_MAX, "%s: particle count %lld out of range", what, (long long)s->n);
+    if (s->n == 0)
+        return OPX_OK;
+    OPX_REQUIRE(s->status && s->cell && s->fz && s->fy && s->fx, "%s: null status/cell/offset arrays", what);
+    OPX_REQUIRE(s->hz1 && s->hy1 && s->hx1 && s->hz2 && s->hy2 && s->hx2, "%s: null history arrays", what);
+    return OPX_OK;
+}
+
+int check_grid(const opx_grid *g)
+{
+    OPX_REQUIRE(g != nullptr, "null grid");
+    OPX_REQUIRE(g->NZ >= 2 && g->NY >= 3 && g->NX >= 3, "grid %dx%dx%d too small", g->NZ, g->NY, g->NX);
+    OPX_REQUIRE(g->xbits > 0 && g->ybits > 0 && g->zbits > 0 && g->xbits + g->ybits + g->zbits <= 31 &&
+                    (1 << g->xbits) >= g->NX + 5 && (1 << g->ybits) >= g->NY + 5 && (1 << g->zbits) >= g->NZ + 5,
+                "cell bit widths (%d,%d,%d) must hold N+5 values per axis in 31 bits", g->zbits, g->ybits,
+                g->xbits);
+    // largest 3-D field (w: (NZ+1) NY NX) must be indexable with 32 bits
+    OPX_REQUIRE((int64_t)(g->NZ + 1) * g->NY * g->NX < ((int64_t)1 << 32), "grid too large for 32-bit offsets");
+    return OPX_OK;
+}
+
+int check_window(const opx_window *w, const opx_grid *g)
+{
+    OPX_REQUIRE(w != nullptr, "null window");
+    OPX_REQUIRE(w->u && w->v && w->w && w->zeta && w->hdd && w->C, "window has a null field pointer");
+    OPX_REQUIRE(w->dtype3 == OPX_F32 || w->dtype3 == OPX_BF16, "fused step supports F32/BF16 u,v,w (got %d)",
+                w->dtype3);
+    OPX_REQUIRE(w->nslots >= 2, "window needs at least 2 slots (got %d)", w->nslots);
+    OPX_REQUIRE(w->u_stride >= (int64_t)g->NZ * g->NY * (g->NX - 1) &&
+                    w->v_stride >= (int64_t)g->NZ * (g->NY - 1) * g->NX &&
+                    w->w_stride >= (int64_t)(g->NZ + 1) * g->NY * g->NX && w->zeta_stride >= (int64_t)g->NY * g->NX,
+                "window slot strides smaller than the field sizes");
+    return OPX_OK;
+}
+
+inline unsigned grid_xcd(int64_t n)
+{
+    int64_t G = (n + kBlock - 1) / kBlock;
+    G = (G + kXcds - 1) / kXcds * kXcds;
+    return (unsigned)G;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t sort_temp_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 32);
+    return bytes;
+}
+
+} // namespace
+} // namespace opx
+
+using namespace opx;
+
+extern "C" {
+
+int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const double *time_array,
+                      int32_t ntime, int32_t nslots, int32_t global_step, double dt, int32_t *err_flag,
+                      void *stream)
+{
+    OPX_REQUIRE(nrel >= 0, "negative release count");
+    if (nrel == 0)
+        return OPX_OK;
+    OPX_REQUIRE(clocks && table && time_array, "null clock/table/time array");
+    OPX_REQUIRE(ntime >= 2, "time array needs at least 2 entries");
+    OPX_REQUIRE(nslots >= 2, "nslots must be >= 2");
+    hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, as_stream(stream), clocks, table, nrel,
+                       time_array, ntime, nslots, global_step, dt, err_flag);
+    return check_launch("opx_clock_advance");
+}
+
+int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, const int32_t *n_active, void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_ab3_fused"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    if ((rc = check_window(win, g))) return rc;
+    OPX_REQUIRE(table != nullptr, "null release table");
+    if (s->n == 0)
+        return OPX_OK;
+    const dim3 grid(grid_xcd(s->n)), block(kBlock);
+    if (win->dtype3 == OPX_F32)
+        hipLaunchKernelGGL(k_ab3_fused<float>, grid, block, 0, as_stream(stream), *s, *g, *win, table, dt,
+                           n_active);
+    else
+        hipLaunchKernelGGL(k_ab3_fused<uint16_t>, grid, block, 0, as_stream(stream), *s, *g, *win, table, dt,
+                           n_active);
+    return check_launch("opx_ab3_fused");
+}
+
+int opx_split_from_f64(const opx_split_state *s, const opx_grid *g, const double *zidx, const double *yidx,
+                       const double *xidx, void *stream)
+{
+    OPX_REQUIRE(s != nullptr, "null state");
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    if (s->n == 0)
+        return OPX_OK;
+    OPX_REQUIRE(s->cell && s->fz && s->fy && s->fx && zidx && yidx && xidx, "null array");
+    hipLaunchKernelGGL(k_split_from_f64, dim3(grid_for(s->n)), dim3(kBlock), 0, as_stream(stream), *s, *g, zidx,
+                       yidx, xidx);
+    return check_launch("opx_split_from_f64");
+}
+
+int opx_split_to_f64(const opx_split_state *s, const opx_grid *g, double *zidx, double *yidx, double *xidx,
+                     void *stream)
+{
+    OPX_REQUIRE(s != nullptr, "null state");
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    if (s->n == 0)
+        return OPX_OK;
+    OPX_REQUIRE(s->cell && s->fz && s->fy && s->fx && zidx && yidx && xidx, "null array");
+    hipLaunchKernelGGL(k_split_to_f64, dim3(grid_for(s->n)), dim3(kBlock), 0, as_stream(stream), *s, *g, zidx,
+                       yidx, xidx);
+    return check_launch("opx_split_to_f64");
+}
+
+int opx_rebin_workspace_bytes(int64_t n, uint64_t *bytes)
+{
+    OPX_REQUIRE(bytes != nullptr, "null output");
+    OPX_REQUIRE(n >= 0 && n <= INT32_MAX, "particle count out of range");
+    size_t arr = align256((size_t)n * sizeof(uint32_t));
+    *bytes = (uint64_t)(4 * arr + align256(sort_temp_bytes(n)));
+    return OPX_OK;
+}
+
+int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow, void *work,
+              uint64_t work_bytes, int32_t *n_active, void *stream)
+{
+    int rc;
+    if ((rc = check_state(src, "opx_rebin src"))) return rc;
+    if ((rc = check_state(dst, "opx_rebin dst"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(dst->n == src->n, "src/dst sizes differ");
+    OPX_REQUIRE(n_active != nullptr, "null n_active");
+    OPX_REQUIRE(xlow >= 0 && xlow <= g->xbits, "xlow %d out of [0, %d]", xlow, g->xbits);
+    OPX_REQUIRE((src->pid == nullptr) == (dst->pid == nullptr) && (src->rel == nullptr) == (dst->rel == nullptr) &&
+                    (src->z == nullptr) == (dst->z == nullptr),
+                "src/dst optional arrays differ");
+    const int64_t n = src->n;
+    uint64_t need = 0;
+    opx_rebin_workspace_bytes(n, &need);
+    OPX_REQUIRE(work != nullptr && work_bytes >= need, "rebin workspace too small (%llu < %llu bytes)",
+                (unsigned long long)work_bytes, (unsigned long long)need);
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
+    if (n == 0)
+        return check_launch("opx_rebin");
+    const size_t arr = align256((size_t)n * sizeof(uint32_t));
+    char *base = static_cast<char *>(work);
+    uint32_t *k_in = reinterpret_cast<uint32_t *>(base);
+    uint32_t *k_out = reinterpret_cast<uint32_t *>(base + arr);
+    uint32_t *v_in = reinterpret_cast<uint32_t *>(base + 2 * arr);
+    uint32_t *v_out = reinterpret_cast<uint32_t *>(base + 3 * arr);
+    void *temp = base + 4 * arr;
+    size_t temp_bytes = (size_t)(work_bytes - 4 * arr);
+    const int key_bits = g->xbits + g->ybits + g->zbits;
+    const uint32_t inactive_key = 1u << key_bits;
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, inactive_key, k_in, v_in,
+                       n_active);
+    if ((rc = check_launch("opx_rebin keys"))) return rc;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
+                                                      key_bits + 1, st);
+    if (e != hipSuccess) {
+        set_error("opx_rebin: radix sort failed: %s", hipGetErrorString(e));
+        return OPX_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *dst, v_out);
+    return check_launch("opx_rebin permute");
+}
+
+} // extern "C"

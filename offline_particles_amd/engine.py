@@ -1,0 +1,335 @@
+"""Fused AB3 advection engine: the fast form of the hot path.
+
+``AB3Engine`` advances particles with one fused HIP kernel per AB3 step
+(``opx_ab3_fused``: validation -> ab3_w_advection -> ab3_post_w_advection,
+the reference's timesteppers.py:296-305 sequence) over
+
+- split positions (packed int32 cell + 32-bit fixed-point offsets, exact in
+  FP64) and FP32 AB3 history;
+- u, v, w snapshot rings in FP32 or BF16 (zeta FP32; h, dx, dy packed as one
+  FP32 float4 per rho point; C in FP64) taken from a ``Fieldset`` built for the
+  drop-in API;
+- per-release FP64 clocks on the device (``opx_clock_advance`` restates
+  Timestepper.get_time_index / advance_time, timesteppers.py:64-85, 157-161);
+- Morton re-binning every ``rebin_every`` steps (``opx_rebin``), which also
+  moves exited particles behind the active ones so they cost nothing.
+
+Releases: particles are added in groups (``add_release``), each with its
+own start time, start step and number of steps; one launch steps every active
+release.  Results are read back in the order particles were added.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fields import StaticField, TimeDependentField
+from .kernels.status import ParticleStatus
+from .spatial_arrays import Stagger
+
+_ROMS_STAGGERS = {
+    "u": (Stagger.CENTER, Stagger.CENTER, Stagger.INNER),
+    "v": (Stagger.CENTER, Stagger.INNER, Stagger.CENTER),
+    "w": (Stagger.OUTER, Stagger.CENTER, Stagger.CENTER),
+    "zeta": (Stagger.INVARIANT, Stagger.CENTER, Stagger.CENTER),
+    "h": (Stagger.INVARIANT, Stagger.CENTER, Stagger.CENTER),
+    "dx": (Stagger.INVARIANT, Stagger.CENTER, Stagger.CENTER),
+    "dy": (Stagger.INVARIANT, Stagger.CENTER, Stagger.CENTER),
+    "C": (Stagger.OUTER, Stagger.INVARIANT, Stagger.INVARIANT),
+}
+
+_STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
+
+
+def _bits(n: int) -> int:
+    return max(1, int(math.ceil(math.log2(max(n, 2)))))
+
+
+def _ptr(t):
+    return t.data_ptr() if t is not None else None
+
+
+class SplitState:
+    """Device SoA of the split-position state (opx_split_state)."""
+
+    def __init__(self, capacity: int, device, *, with_z: bool = False):
+        n = int(capacity)
+        self.capacity = n
+        self.status = torch.full((n,), ParticleStatus.INACTIVE, dtype=torch.uint8, device=device)
+        self.cell = torch.zeros(n, dtype=torch.int32, device=device)
+        for name in _STATE_F32:
+            # fz, fy, fx hold 32-bit fixed-point offsets (uint32 bits in int32 tensors)
+            dt = torch.int32 if name in ("fz", "fy", "fx") else torch.float32
+            setattr(self, name, torch.zeros(n, dtype=dt, device=device))
+        self.pid = torch.zeros(n, dtype=torch.int32, device=device)
+        self.rel = torch.zeros(n, dtype=torch.int16, device=device)  # read as uint16
+        self.z = torch.zeros(n, dtype=torch.float32, device=device) if with_z else None
+
+    def struct(self, n: int, start: int = 0) -> _lib.opx_split_state:
+        s = _lib.opx_split_state()
+        s.n = int(n)
+
+        def p(t, size):
+            return t.data_ptr() + start * size if t is not None else None
+
+        s.status = p(self.status, 1)
+        s.cell = p(self.cell, 4)
+        for name in _STATE_F32:
+            setattr(s, name, p(getattr(self, name), 4))
+        s.pid = p(self.pid, 4)
+        s.rel = p(self.rel, 2)
+        s.z = p(self.z, 4)
+        return s
+
+
+class AB3Engine:
+    """Fused AB3 stepping of one or many particle releases on one device."""
+
+    def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
+                 capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False):
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if storage_dtype not in (torch.float32, torch.bfloat16):
+            raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
+        self.storage_dtype = storage_dtype
+        self.dt = float(np.float64(dt))
+        self.time_array = np.asarray(time_array, dtype=np.float64)
+        if np.any(self.time_array[1:] <= self.time_array[:-1]):
+            raise ValueError("time_array must be strictly increasing.")
+        self._ta = torch.from_numpy(self.time_array.copy()).to(self.device)
+        self.rebin_every = int(rebin_every)
+        self.xlow = int(xlow)
+        self._build_window(fieldset)
+        self._with_z = with_z
+        self._state = SplitState(capacity, self.device, with_z=with_z)
+        self._spare = None
+        self.n = 0
+        self._n_launch = 0
+        self._n_active_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._err = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._clocks_host: list = []
+        self._clocks = None
+        self._table = None
+        self._release_ranges: list = []
+        self.global_step = 0
+        self._since_rebin = 0
+        self._work = None
+        self._structs = None
+        # optional list collecting (start, end) HIP events around each fused launch
+        self.timer = None
+
+    # ---------------------------------------------------------------- setup
+
+    def _build_window(self, fs):
+        for name, st in _ROMS_STAGGERS.items():
+            f = fs[name]
+            if tuple(f.stagger) != st:
+                raise ValueError(f"field '{name}' has staggering {tuple(map(str, f.stagger))}; the fused step "
+                                 f"needs the ROMS C-grid {tuple(map(str, st))}")
+        for name in ("u", "v", "w", "zeta"):
+            if not isinstance(fs[name], TimeDependentField):
+                raise TypeError(f"field '{name}' must be time dependent")
+        for name in ("h", "dx", "dy", "C"):
+            if not isinstance(fs[name], StaticField):
+                raise TypeError(f"field '{name}' must be static")
+        NZ, NY, NX = fs.z_size, fs.y_size, fs.x_size
+        self.NZ, self.NY, self.NX = NZ, NY, NX
+        snaps = {n: fs[n].data for n in ("u", "v", "w", "zeta")}
+        K = {int(t.shape[0]) for t in snaps.values()}
+        if len(K) != 1:
+            raise ValueError(f"u, v, w, zeta hold different snapshot counts {K}")
+        self.nslots = K.pop()
+        cyc = {fs[n].cyclic for n in ("u", "v", "w", "zeta")}
+        if cyc == {False} and self.nslots != len(self.time_array):
+            raise ValueError("non-cyclic fields need one snapshot per time_array entry")
+        self._u = snaps["u"].to(self.device, self.storage_dtype).contiguous()
+        self._v = snaps["v"].to(self.device, self.storage_dtype).contiguous()
+        self._w = snaps["w"].to(self.device, self.storage_dtype).contiguous()
+        self._zeta = snaps["zeta"].to(self.device, torch.float32).contiguous()
+        h = fs["h"].data.data.to(self.device, torch.float32)
+        dx = fs["dx"].data.data.to(self.device, torch.float32)
+        dy = fs["dy"].data.data.to(self.device, torch.float32)
+        self._hdd = torch.stack([h, dx, dy, torch.zeros_like(h)], dim=-1).contiguous()
+        self._C = fs["C"].data.data.to(self.device, torch.float64).contiguous()
+        hc = fs.constants["hc"]
+        g = _lib.opx_grid()
+        g.NZ, g.NY, g.NX = NZ, NY, NX
+        g.xbits, g.ybits, g.zbits = _bits(NX + 5), _bits(NY + 5), _bits(NZ + 5)
+        g.hc = float(hc)
+        g.zidx_min, g.zidx_max = (float(b) for b in fs.zidx_bounds)
+        g.yidx_min, g.yidx_max = (float(b) for b in fs.yidx_bounds)
+        g.xidx_min, g.xidx_max = (float(b) for b in fs.xidx_bounds)
+        self.grid = g
+        w = _lib.opx_window()
+        w.u, w.v, w.w, w.zeta = (_ptr(t) for t in (self._u, self._v, self._w, self._zeta))
+        w.u_stride = self._u[0].numel()
+        w.v_stride = self._v[0].numel()
+        w.w_stride = self._w[0].numel()
+        w.zeta_stride = self._zeta[0].numel()
+        w.dtype3 = _lib.OPX_F32 if self.storage_dtype == torch.float32 else _lib.OPX_BF16
+        w.nslots = self.nslots
+        w.hdd = _ptr(self._hdd)
+        w.C = _ptr(self._C)
+        self.window = w
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _ensure_capacity(self, n):
+        if n <= self._state.capacity:
+            return
+        cap = max(n, 2 * self._state.capacity)
+        new = SplitState(cap, self.device, with_z=self._with_z)
+        old = self._state
+        for name in ("status", "cell", "pid", "rel") + _STATE_F32 + (("z",) if self._with_z else ()):
+            getattr(new, name)[: self.n].copy_(getattr(old, name)[: self.n])
+        self._state = new
+        self._spare = None
+        self._work = None
+
+    def add_release(self, zidx, yidx, xidx, *, start_time=None, nsteps: int, start_step: int = 0,
+                    status: int = ParticleStatus.MULTISTEP_1) -> int:
+        """Add a group of particles (a release); returns its id.
+
+        The release's clock starts at ``start_time`` (default time_array[0])
+        and it advances at global steps ``start_step`` .. while its
+        ``nsteps`` steps last.  Particles start with ``status``
+        (MULTISTEP_1: Euler -> AB2 -> AB3 start-up).
+        """
+        r = len(self._clocks_host)
+        if r >= 1 << 15:
+            raise ValueError("too many releases")
+        z = torch.as_tensor(np.asarray(zidx, np.float64) if not isinstance(zidx, torch.Tensor) else zidx,
+                            dtype=torch.float64, device=self.device)
+        y = torch.as_tensor(np.asarray(yidx, np.float64) if not isinstance(yidx, torch.Tensor) else yidx,
+                            dtype=torch.float64, device=self.device)
+        x = torch.as_tensor(np.asarray(xidx, np.float64) if not isinstance(xidx, torch.Tensor) else xidx,
+                            dtype=torch.float64, device=self.device)
+        m = int(z.numel())
+        if not (y.numel() == m and x.numel() == m):
+            raise ValueError("zidx, yidx, xidx must have the same length")
+        start = self.n
+        self._ensure_capacity(start + m)
+        S = self._state
+        if m:
+            _lib.call("opx_split_from_f64", ctypes.byref(S.struct(m, start)), ctypes.byref(self.grid),
+                      z.data_ptr(), y.data_ptr(), x.data_ptr(), self._stream())
+            S.status[start:start + m] = int(status)
+            S.pid[start:start + m] = torch.arange(start, start + m, dtype=torch.int32, device=self.device)
+            S.rel[start:start + m] = r
+            for name in _STATE_F32[3:]:
+                getattr(S, name)[start:start + m] = 0.0
+        self.n = start + m
+        self._n_launch = self.n
+        t0 = float(self.time_array[0]) if start_time is None else float(start_time)
+        self._clocks_host.append((t0, int(start_step), int(nsteps)))
+        self._release_ranges.append((start, m))
+        self._upload_clocks()
+        self._structs = None
+        return r
+
+    def _upload_clocks(self):
+        nrel = len(self._clocks_host)
+        arr = (_lib.opx_release_clock * nrel)()
+        if self._clocks is not None:
+            # keep the device clocks of existing releases
+            prev = torch.empty(self._clocks.numel(), dtype=torch.uint8)
+            prev.copy_(self._clocks.cpu())
+            ctypes.memmove(arr, prev.numpy().ctypes.data, prev.numel())
+        for r in range(0 if self._clocks is None else self._clocks.numel() // 16, nrel):
+            t, s0, ns = self._clocks_host[r]
+            arr[r] = _lib.opx_release_clock(t, s0, ns)
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+        self._clocks = torch.from_numpy(raw.copy()).to(self.device)
+        self._table = torch.zeros(nrel * ctypes.sizeof(_lib.opx_release_step), dtype=torch.uint8, device=self.device)
+
+    # ---------------------------------------------------------------- stepping
+
+    def _launch_structs(self):
+        if self._structs is None:
+            self._structs = self._state.struct(self._n_launch)
+        return self._structs
+
+    def rebin(self):
+        """Sort particles into Morton order of their cells; exited particles go last."""
+        if self.n == 0:
+            return
+        if self._spare is None or self._spare.capacity != self._state.capacity:
+            self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
+        if self._work is None:
+            need = ctypes.c_uint64()
+            _lib.call("opx_rebin_workspace_bytes", self.n, ctypes.byref(need))
+            self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
+            self._work_n = self.n
+        if self._work_n < self.n:
+            need = ctypes.c_uint64()
+            _lib.call("opx_rebin_workspace_bytes", self.n, ctypes.byref(need))
+            self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
+            self._work_n = self.n
+        src = self._state.struct(self.n)
+        dst = self._spare.struct(self.n)
+        _lib.call("opx_rebin", ctypes.byref(src), ctypes.byref(dst), ctypes.byref(self.grid), self.xlow,
+                  self._work.data_ptr(), self._work.numel(), self._n_active_dev.data_ptr(), self._stream())
+        self._state, self._spare = self._spare, self._state
+        self._n_launch = int(self._n_active_dev.item())
+        self._structs = None
+        self._since_rebin = 0
+
+    def step(self, nsteps: int = 1) -> None:
+        """Advance every active release by ``nsteps`` AB3 steps."""
+        nrel = len(self._clocks_host)
+        stream = self._stream()
+        for _ in range(int(nsteps)):
+            if self.rebin_every > 0 and self._since_rebin >= self.rebin_every:
+                self.rebin()
+            _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), nrel, self._ta.data_ptr(),
+                      int(self._ta.numel()), self.nslots, self.global_step, self.dt, self._err.data_ptr(), stream)
+            s = self._launch_structs()
+            if self.timer is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            _lib.call("opx_ab3_fused", ctypes.byref(s), ctypes.byref(self.grid), ctypes.byref(self.window),
+                      self._table.data_ptr(), self.dt, None, stream)
+            if self.timer is not None:
+                e1 = torch.cuda.Event(enable_timing=True)
+                e1.record()
+                self.timer.append((e0, e1))
+            self.global_step += 1
+            self._since_rebin += 1
+
+    def check_clock(self) -> None:
+        """Raise ValueError if a release's time left the time array (as the reference does)."""
+        if int(self._err.item()):
+            raise ValueError("Time is out of bounds of the time array.")
+
+    # ---------------------------------------------------------------- results
+
+    def positions(self):
+        """(zidx, yidx, xidx) FP64 device tensors in the order particles were added."""
+        out = [torch.empty(self.n, dtype=torch.float64, device=self.device) for _ in range(3)]
+        if self.n:
+            _lib.call("opx_split_to_f64", ctypes.byref(self._state.struct(self.n)), ctypes.byref(self.grid),
+                      out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), self._stream())
+        return tuple(out)
+
+    def status(self):
+        """Status codes (uint8 device tensor) in the order particles were added."""
+        out = torch.empty(self.n, dtype=torch.uint8, device=self.device)
+        pid = self._state.pid[: self.n].long()
+        out[pid] = self._state.status[: self.n]
+        return out
+
+    def release_slice(self, r: int) -> slice:
+        start, m = self._release_ranges[r]
+        return slice(start, start + m)
+
+    def active_count(self) -> int:
+        return int((self._state.status[: self.n] < 128).sum().item())
+
+    @property
+    def state(self) -> SplitState:
+        return self._state

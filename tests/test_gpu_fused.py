@@ -1,0 +1,120 @@
+"""Fused AB3 engine (split positions, FP32/BF16 fields) against the oracle.
+
+Gate for the FP32 form: max position deviation <= 1e-5 grid-index units after
+100 steps with status codes bit-exact.  BF16 storage is a precision form with
+no reference counterpart; its deviation is reported and bounded loosely here
+(the at-scale 1000-step figure is measured separately).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from offline_particles_amd.engine import AB3Engine
+from offline_particles_amd.workloads import build_fieldset, eddy_workload, turbulent_workload
+
+from ._helpers import POS, max_dev, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(wl, device, *, storage=torch.float32, rebin_every=25, xlow=2, nsteps=100, positions=None,
+            capacity=None):
+    fs = build_fieldset(wl, device=device)
+    pos = wl.positions if positions is None else positions
+    eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
+                    capacity=capacity or max(1, len(pos[0])), rebin_every=rebin_every, xlow=xlow)
+    eng.add_release(*pos, nsteps=nsteps)
+    return eng
+
+
+def _state(eng):
+    z, y, x = (t.cpu().numpy() for t in eng.positions())
+    return {"zidx": z, "yidx": y, "xidx": x, "status": eng.status().cpu().numpy()}
+
+
+def _turb():
+    return turbulent_workload(NX=96, NY=80, NZ=12, nparticles=20_000, steps=100, nsnap_unique=4)
+
+
+def test_fused_fp32_vs_oracle_100_steps(device):
+    wl = _turb()
+    sim_o = run_oracle(wl, "ab3", 100)
+    eng = _engine(wl, device)
+    eng.step(100)
+    eng.check_clock()
+    g = _state(eng)
+    assert np.array_equal(g["status"], sim_o.p["status"]), "status differs"
+    devs = {k: max_dev(g[k], sim_o.p[k]) for k in POS}
+    print("fused fp32 max deviation vs oracle after 100 steps:", devs, flush=True)
+    assert max(devs.values()) <= 1e-5
+
+
+def test_fused_fp32_eddy_vs_oracle(device):
+    wl = eddy_workload(steps=100, dtype=np.float32)
+    sim_o = run_oracle(wl, "ab3", 100)
+    eng = _engine(wl, device)
+    eng.step(100)
+    g = _state(eng)
+    assert np.array_equal(g["status"], sim_o.p["status"])
+    devs = {k: max_dev(g[k], sim_o.p[k]) for k in POS}
+    print("fused fp32 eddy max deviation:", devs, flush=True)
+    assert max(devs.values()) <= 1e-5
+
+
+def test_rebin_does_not_change_results(device):
+    wl = _turb()
+    a = _engine(wl, device, rebin_every=0)
+    b = _engine(wl, device, rebin_every=7, xlow=3)
+    a.step(40)
+    b.step(40)
+    sa, sb = _state(a), _state(b)
+    for k in ("status",) + POS:
+        assert np.array_equal(sa[k], sb[k], equal_nan=True), k
+
+
+def test_bf16_storage_close(device):
+    wl = _turb()
+    sim_o = run_oracle(wl, "ab3", 100)
+    eng = _engine(wl, device, storage=torch.bfloat16)
+    eng.step(100)
+    g = _state(eng)
+    both = (g["status"] < 128) & (sim_o.p["status"] < 128)
+    devs = {k: float(np.max(np.abs(g[k][both] - sim_o.p[k][both]))) for k in POS}
+    print("bf16 max deviation after 100 steps:", devs, flush=True)
+    assert max(devs.values()) < 0.05
+
+
+def test_releases_alone_equal_mix(device):
+    """Each release gives the same result alone and stepped together with others."""
+    wl = _turb()
+    rng = np.random.default_rng(5)
+    n = [3000, 500, 7000]
+    starts = [0.0, 1234.5, 400.0]  # off the dt grid
+    steps = [60, 30, 45]
+    first = [0, 10, 3]
+    pos = [tuple(rng.uniform(1, m - 2, k) for m in (wl.NZ, wl.NY, wl.NX)) for k in n]
+    fs = build_fieldset(wl, device=device)
+    mix = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=sum(n), rebin_every=9)
+    for p, t0, ns, s0 in zip(pos, starts, steps, first):
+        mix.add_release(*p, start_time=t0, nsteps=ns, start_step=s0)
+    mix.step(70)
+    mix.check_clock()
+    got = _state(mix)
+    off = 0
+    for p, t0, ns, s0, k in zip(pos, starts, steps, first, n):
+        alone = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=k, rebin_every=0)
+        alone.add_release(*p, start_time=t0, nsteps=ns, start_step=0)
+        alone.step(ns)
+        ref = _state(alone)
+        for key in ("status",) + POS:
+            assert np.array_equal(got[key][off:off + k], ref[key], equal_nan=True), key
+        off += k
+
+
+def test_clock_leaves_time_array(device):
+    wl = eddy_workload(steps=4, nparticles=100, dtype=np.float32)
+    eng = _engine(wl, device, nsteps=1000)
+    eng.step(len(wl.time_array) * 40)
+    with pytest.raises(ValueError):
+        eng.check_clock()

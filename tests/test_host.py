@@ -1,0 +1,180 @@
+"""Host-side checks that need no GPU.
+
+- libopx.so loads and exports every function include/opx.h declares;
+- the ctypes structs have the C layout of include/opx.h (compiled with gcc);
+- the reference-API mirror: kernel declarations, merge rules, time index and
+  blend-weight semantics, Stagger, Fieldset validation, argument errors.
+"""
+
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from offline_particles_amd import _build, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        _build.build()
+    return _lib.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 16
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    for s in declared:
+        assert hasattr(lib, s)
+    assert lib.opx_abi_version() == 1
+
+
+_STRUCTS = {
+    "opx_field": _lib.opx_field,
+    "opx_roms_fields": _lib.opx_roms_fields,
+    "opx_scalars": _lib.opx_scalars,
+    "opx_particles": _lib.opx_particles,
+    "opx_split_state": _lib.opx_split_state,
+    "opx_grid": _lib.opx_grid,
+    "opx_window": _lib.opx_window,
+    "opx_release_clock": _lib.opx_release_clock,
+    "opx_release_step": _lib.opx_release_step,
+}
+
+
+def test_ctypes_layout_matches_header():
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "opx.h"', "int main(void) {"]
+    for name, cls in _STRUCTS.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "abi.c")
+        exe = os.path.join(d, "abi")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    c = dict(line.rsplit(" ", 1) for line in out.splitlines())
+    for name, cls in _STRUCTS.items():
+        assert int(c[name]) == ctypes.sizeof(cls), name
+        for fname, _ in cls._fields_:
+            assert int(c[f"{name}.{fname}"]) == getattr(cls, fname).offset, f"{name}.{fname}"
+
+
+def test_kernel_declarations_match_reference():
+    from offline_particles_amd.kernels.roms import (
+        ab3_post_w_advection_kernel,
+        ab3_w_advection_kernel,
+        rk2_w_advection_step_1_kernel,
+        rk2_w_advection_step_2_kernel,
+        rk2_w_advection_update_kernel,
+    )
+    from offline_particles_amd.kernels.validation import validation_kernel
+
+    f8, u1, i4 = np.dtype(np.float64), np.dtype(np.uint8), np.dtype(np.int32)
+    hist = ["_dz0", "_dz1", "_dz2", "_dyidx0", "_dyidx1", "_dyidx2", "_dxidx0", "_dxidx1", "_dxidx2"]
+    assert dict(ab3_w_advection_kernel.particle_fields) == {
+        "status": u1, "zidx": f8, "yidx": f8, "xidx": f8, "z": f8, **{k: f8 for k in hist}}
+    assert dict(ab3_w_advection_kernel.scalars) == {"_dt": f8, "hc": f8, "NZ": i4}
+    assert ab3_w_advection_kernel.simulation_fields == frozenset("u v w dx dy h C zeta".split())
+    assert ab3_post_w_advection_kernel.simulation_fields == frozenset({"h", "C", "zeta"})
+    assert dict(rk2_w_advection_step_2_kernel.scalars) == {"_dt": f8, "_RK2_alpha": f8, "hc": f8, "NZ": i4}
+    assert set(rk2_w_advection_step_1_kernel.particle_fields) == {
+        "status", "zidx", "yidx", "xidx", "z", "_dxidx1", "_dyidx1", "_dz1"}
+    assert rk2_w_advection_update_kernel.simulation_fields == frozenset({"h", "C", "zeta"})
+    assert set(validation_kernel.scalars) == {"zidx_min", "zidx_max", "yidx_min", "yidx_max", "xidx_min",
+                                              "xidx_max"}
+    assert len(validation_kernel.functions) == 2
+
+
+def test_merge_rules():
+    from offline_particles_amd.kernels import ParticleKernel, merge_particle_fields, merge_scalars
+
+    k1 = ParticleKernel(lambda *a: None, {"a": np.float64}, {"s": np.float64}, ["u"])
+    k2 = ParticleKernel(lambda *a: None, {"a": np.float32}, {"s": np.int32}, ["v"])
+    with pytest.raises(TypeError):
+        merge_particle_fields([k1, k2])
+    with pytest.raises(TypeError):
+        merge_scalars([k1, k2])
+    merged = merge_particle_fields([k1])
+    assert list(merged)[:4] == ["status", "zidx", "yidx", "xidx"]
+    chained = ParticleKernel.chain(k1, ParticleKernel(lambda *a: None, {}, {}, ["w"]))
+    assert chained.simulation_fields == frozenset({"u", "w"})
+    with pytest.raises(TypeError):
+        ParticleKernel([1], {}, {}, [])
+
+
+def test_time_index_and_blend_weights():
+    from offline_particles_amd import RK2Timestepper
+    from offline_particles_amd.fields import blend_weights
+    from offline_particles_amd.kernels import ParticleKernel
+
+    k = ParticleKernel(lambda *a: None, {}, {}, [])
+    ta = np.array([0.0, 3600.0, 7200.0, 10800.0])
+    ts = RK2Timestepper(ta, np.float64(300.0), k, k, k)
+    assert ts.tidx == 0.0
+    for _ in range(13):
+        ts.advance_time()
+    assert ts.time == 3900.0 and ts.iteration == 13
+    assert ts.tidx == 1 + (3900.0 - 3600.0) / 3600.0
+    with pytest.raises(ValueError):
+        ts.get_time_index(10800.5)
+    with pytest.raises(IndexError):
+        ts.get_time_index(10800.0)
+    with pytest.raises(ValueError):
+        RK2Timestepper(ta, 300.0, k, k, k)  # a Python float dt needs time_unit
+    with pytest.raises(ValueError):
+        RK2Timestepper(ta[::-1], np.float64(1.0), k, k, k)
+    import torch
+
+    It, ft, gt = blend_weights(np.float64(2.3), torch.float32)
+    assert It == 2 and ft == float(np.float32(np.float64(2.3) - 2))
+    assert gt == float(np.float32(1) - np.float32(np.float64(2.3) - 2))
+    It, ft, gt = blend_weights(np.float64(0.75), torch.float64)
+    assert (It, ft, gt) == (0, 0.75, 0.25)
+
+
+def test_stagger_and_fieldset_validation():
+    import torch
+
+    from offline_particles_amd import Fieldset, StaticField, Stagger, TimeDependentField
+
+    assert Stagger("inner").offset == -0.5 and Stagger("outer").expected_size(10) == 11
+    assert Stagger("invariant").offset is None and Stagger.LEFT.offset == 0.5
+    h = StaticField.from_numpy(np.ones((4, 5)), "invariant", "center", "center", device="cpu")
+    fs = Fieldset(3, 2, 4, 5, constants={"hc": 10.0}, h=h)
+    assert fs.constants["zidx_max"] == 1.0 and fs.constants["hc"] == 10.0
+    with pytest.raises(ValueError):
+        fs.add_field("bad", StaticField.from_numpy(np.ones((4, 6)), "invariant", "center", "center", device="cpu"))
+    with pytest.raises(KeyError):
+        fs.add_field("h", h)
+    u = TimeDependentField(np.zeros((3, 2, 4, 4)), "center", "center", "inner", device="cpu")
+    fs.add_field("u", u)
+    fd = u.get_field_data(1.25, None)
+    assert fd.offsets == (0.0, 0.0, -0.5)
+    assert fd.array.ft == 0.25 and fd.array.gt == 0.75 and fd.array.It == 1
+    with pytest.raises(IndexError):
+        u.get_field_data(2.5, None)  # window [2, 3] needs snapshot 3 of 3
+    cyc = TimeDependentField(np.zeros((2, 2, 4, 4)), "center", "center", "inner", cyclic=True, device="cpu")
+    assert cyc.get_field_data(7.5, None).array.It == 7
+    assert torch.equal(cyc.snapshot(7), cyc.snapshot(1))
+
+
+def test_kernel_rejects_host_tensors():
+    from offline_particles_amd.kernels.validation import finite_indices
+    from offline_particles_amd.particles import Particles
+
+    p = Particles(4, device="cpu", status=np.uint8, zidx=np.float64, yidx=np.float64, xidx=np.float64)
+    with pytest.raises(ValueError):
+        finite_indices(p, {}, {})

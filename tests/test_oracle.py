@@ -1,0 +1,272 @@
+"""Pin the CPU oracle before trusting it.
+
+- L0 numerics against golden vectors written by the compiled reference .pxd
+  (tests/golden/l0_vectors.npz) and, where oracle/_ref is built, against the
+  compiled reference directly on fresh random inputs: bit for bit.
+- Closed forms that a plausible mistake would break: affine exactness and
+  extrapolation of the interpolants, the C = sigma vertical coordinate, the RK2
+  step of u = a + b t (distinguishes the code's stage weights from the
+  tableau), the RK2 amplification factor for dx/dt = lambda x, the AB3
+  start-up sequence for u = a + b t, uniform w over a flat bed.
+- The time blend against numpy's dtype semantics, and validation edge cases.
+"""
+
+import glob
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import driver as D
+from offline_particles_amd.workloads import stretching_cs_w
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "l0_vectors.npz")
+REF_DIR = os.path.join(os.path.dirname(HERE), "oracle", "_ref")
+
+
+def _ref():
+    if not glob.glob(os.path.join(REF_DIR, "ref_numerics*.so")):
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    if REF_DIR not in sys.path:
+        sys.path.insert(0, REF_DIR)
+    import ref_numerics
+
+    return ref_numerics
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+# ----------------------------------------------------------------- L0 golden
+
+def test_l0_against_golden_vectors():
+    g = np.load(GOLD)
+    L = D.load()
+    # truncate_index through linear interpolation of the identity array: I0 is observable
+    assert _same(D.linear(g["a1"], g["q1"]), g["lin"])
+    assert _same(D.bilinear(g["a2"], g["q2a"], g["q2b"]), g["bil"])
+    assert _same(D.trilinear(g["a3"], g["q3a"], g["q3b"], g["q3c"]), g["tril"])
+    NZ, hc = int(g["NZ"]), float(g["hc"])
+    assert _same(D.compute_z(g["zidx"], NZ, hc, g["h"], g["Cz"], g["zeta"]), g["compute_z"])
+    assert _same(D.compute_zidx(g["z"], g["h"], g["zeta"], hc, NZ, g["C"], 0.5), g["compute_zidx"])
+    # truncate_index: linear over a = arange gives I0 + f where f = idx - I0 -> recover I0
+    ident = np.arange(7.0)
+    q = g["q1"]
+    fin = np.isfinite(q) & (np.abs(q) < 1e15)
+    lin = D.linear(ident, q[fin])
+    assert np.allclose(lin, q[fin], rtol=0, atol=1e-9)  # identity is reproduced, incl. extrapolation
+    assert L is not None
+
+
+def test_l0_against_compiled_reference_random():
+    R = _ref()
+    rng = np.random.default_rng(99)
+    a3 = rng.standard_normal((6, 7, 9))
+    q = [rng.uniform(-3, 11, 2000) for _ in range(3)]
+    assert _same(D.trilinear(a3, *q), R.ref_trilinear(a3, *q))
+    a2 = rng.standard_normal((9, 4))
+    assert _same(D.bilinear(a2, q[0], q[1]), R.ref_bilinear(a2, q[0], q[1]))
+    a1 = rng.standard_normal(13)
+    assert _same(D.linear(a1, q[2]), R.ref_linear(a1, q[2]))
+    for NZ in (3, 20, 50, 61):
+        C = stretching_cs_w(NZ)
+        h = rng.uniform(20, 4000, 1500)
+        zeta = rng.uniform(-2, 2, 1500)
+        z = -h * rng.uniform(-0.3, 1.3, 1500)
+        assert _same(D.compute_zidx(z, h, zeta, 10.0, NZ, C, 0.5), R.ref_compute_zidx(z, h, zeta, 10.0, NZ, C, 0.5))
+        zi = rng.uniform(-1, NZ, 1500)
+        Cz = rng.uniform(-1, 0, 1500)
+        assert _same(D.compute_z(zi, NZ, 10.0, h, Cz, zeta), R.ref_compute_z(zi, NZ, 10.0, h, Cz, zeta))
+
+
+def test_truncate_index_edges_against_reference():
+    R = _ref()
+    q = np.array([-0.75, -1.0, -2.5, 0.0, 3.999, 4.0, 9.0, 1e19, np.nan, np.inf, -np.inf])
+    I0 = R.ref_truncate_index(q, 4)
+    # (-1, 0) truncates to 0; below -1 clamps to 0; beyond max clamps; NaN/inf/1e19 give 0 on x86-64
+    assert list(I0) == [0, 0, 0, 0, 3, 4, 4, 0, 0, 0, 0]
+    lin = D.linear(np.arange(6.0), q[:7])
+    assert np.allclose(lin, q[:7])
+
+
+# ----------------------------------------------------------------- closed forms
+
+def test_interpolation_reproduces_affine_fields_and_extrapolates():
+    n0, n1, n2 = 5, 6, 7
+    k, j, i = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+    a = 1.5 + 0.25 * k - 0.75 * j + 2.0 * i
+    rng = np.random.default_rng(3)
+    z, y, x = rng.uniform(-2, 9, (3, 500))
+    got = D.trilinear(a, z, y, x)
+    assert np.allclose(got, 1.5 + 0.25 * z - 0.75 * y + 2.0 * x, rtol=0, atol=1e-11)
+    # a corner value is hit exactly at integer indices inside the array
+    assert D.trilinear(a, [2.0], [3.0], [4.0])[0] == a[2, 3, 4]
+
+
+def test_vertical_coordinate_linear_stretching():
+    NZ, hc = 20, 10.0
+    sigma_w = (np.arange(NZ + 1) - NZ) / NZ
+    C = sigma_w.copy()  # C = sigma: S = sigma, z = h sigma with zeta = 0
+    h = np.array([50.0, 200.0, 3000.0])
+    z = np.array([-10.0, -100.0, -2999.0])
+    zidx = D.compute_zidx(z, h, 0.0, hc, NZ, C, 0.5)
+    assert np.allclose(zidx, NZ * (z / h + 1) - 0.5, rtol=0, atol=1e-12)
+    # round trip compute_zidx(compute_z(k)) = k on the stretched grid
+    Cs = stretching_cs_w(NZ)
+    k = np.linspace(-0.5, NZ - 0.5, 57)
+    Cv = np.array([D.linear(Cs, [kk + 0.5])[0] for kk in k])
+    for hh, ze in ((30.0, 0.3), (4000.0, -0.5)):
+        zz = D.compute_z(k, NZ, hc, hh, Cv, ze)
+        back = D.compute_zidx(zz, hh, ze, hc, NZ, Cs, 0.5)
+        assert np.allclose(back, k, rtol=0, atol=1e-9)
+
+
+def _uniform_fields(NZ, NY, NX, u_of_t, nsnap, h=100.0, w=0.0, C=None, dxm=1000.0):
+    """Spatially uniform u(t) snapshots, v = 0, uniform w, flat h, zeta = 0."""
+    if C is None:
+        C = (np.arange(NZ + 1) - NZ) / NZ
+    u = np.stack([np.full((NZ, NY, NX - 1), u_of_t(k)) for k in range(nsnap)])
+    v = np.zeros((nsnap, NZ, NY - 1, NX))
+    ww = np.full((nsnap, NZ + 1, NY, NX), w)
+    from offline_particles_amd.workloads import OFFSETS
+
+    arr = {"u": u, "v": v, "w": ww, "zeta": np.zeros((nsnap, NY, NX)), "h": np.full((NY, NX), h),
+           "dx": np.full((NY, NX), dxm), "dy": np.full((NY, NX), dxm), "C": C}
+    return {k: D.OracleField(a, OFFSETS[k], k in ("u", "v", "w", "zeta"), cyclic=False) for k, a in arr.items()}
+
+
+def _sim(fields, ta, dt, scheme, pos, NZ, NY, NX, status=None):
+    sim = D.OracleSimulation(fields, ta, dt, scheme=scheme, nparticles=len(pos[0]), hc=10.0, NZ=NZ,
+                             bounds=(0, NZ - 1, 0, NY - 1, 0, NX - 1))
+    for key, a in zip(("zidx", "yidx", "xidx"), pos):
+        sim.p[key][:] = a
+    if status is not None:
+        sim.p["status"][:] = status
+    return sim
+
+
+def test_rk2_step_uses_code_weights():
+    """u = a + b t: one RK2 step moves a dt + b t dt + (alpha - 1/2) b dt^2 (in metres)."""
+    a, b, dxm, dt, snap = 0.2, 1e-5, 1000.0, 300.0, 3600.0
+    NZ, NY, NX = 4, 6, 400
+    ta = np.arange(6) * snap
+    fields = _uniform_fields(NZ, NY, NX, lambda k: a + b * ta[k], 6, dxm=dxm)
+    x0 = np.array([10.0, 20.5, 33.3])
+    sim = _sim(fields, ta, dt, "rk2", (np.full(3, 1.5), np.full(3, 2.5), x0), NZ, NY, NX)
+    t = 0.0
+    for _ in range(5):
+        sim.step()
+        t += dt
+    alpha = 2 / 3
+    per_step = lambda tn: (a * dt + b * tn * dt + (alpha - 0.5) * b * dt**2) / dxm
+    expect = x0 + sum(per_step(n * dt) for n in range(5))
+    assert np.allclose(sim.p["xidx"], expect, rtol=0, atol=1e-10)
+    # the Butcher-tableau weights would give b dt^2 / 2 instead: clearly different
+    tableau = x0 + sum((a * dt + b * n * dt * dt + 0.5 * b * dt**2) / dxm for n in range(5))
+    assert np.max(np.abs(tableau - expect)) > 1e-4
+    assert np.allclose(sim.p["zidx"], 1.5, atol=1e-12)  # no vertical motion
+
+
+def test_rk2_amplification_linear_field():
+    """dx/dt = lambda (x - x0): factor 1 + lambda dt + (alpha - 1/2) lambda^2 dt^2 per step."""
+    lam, dt, x0, dxm = 2e-5, 600.0, 50.0, 1000.0
+    NZ, NY, NX = 3, 5, 120
+    ta = np.arange(4) * 7200.0
+    i = np.arange(NX - 1) + 0.5  # u-point positions in rho index units
+    u1 = lam * (i - x0) * dxm
+    from offline_particles_amd.workloads import OFFSETS
+
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4)
+    fields["u"] = D.OracleField(np.broadcast_to(u1, (4, NZ, NY, NX - 1)).copy(), OFFSETS["u"], True, cyclic=False)
+    x = np.array([20.0, 49.0, 70.25, 90.0])
+    sim = _sim(fields, ta, dt, "rk2", (np.full(4, 0.7), np.full(4, 2.0), x), NZ, NY, NX)
+    for _ in range(10):
+        sim.step()
+    alpha = 2 / 3
+    g = 1 + lam * dt + (alpha - 0.5) * (lam * dt) ** 2
+    assert np.allclose(sim.p["xidx"], x0 + (x - x0) * g**10, rtol=0, atol=1e-10)
+
+
+def test_ab3_startup_closed_form():
+    """u = a + b t from MULTISTEP_1: Euler, AB2, then AB3 (exact for linear u)."""
+    a, b, dxm, dt, snap = -0.1, 2e-5, 1000.0, 120.0, 3600.0
+    NZ, NY, NX = 4, 6, 300
+    ta = np.arange(4) * snap
+    fields = _uniform_fields(NZ, NY, NX, lambda k: a + b * ta[k], 4, dxm=dxm)
+    x0 = np.array([100.0, 150.25])
+    sim = _sim(fields, ta, dt, "ab3", (np.full(2, 2.0), np.full(2, 3.0), x0), NZ, NY, NX, status=10)
+    N = 50
+    for n in range(N):
+        sim.step()
+        if n == 0:
+            assert (sim.p["status"] == 11).all()
+        if n == 1:
+            assert (sim.p["status"] == 0).all()
+    tN = N * dt
+    exact = (a * tN + b * tN**2 / 2 - b * dt**2 / 2) / dxm
+    assert np.allclose(sim.p["xidx"] - x0, exact, rtol=0, atol=1e-10)
+    # without the start-up (history left at zero) the answer differs
+    sim2 = _sim(fields, ta, dt, "ab3", (np.full(2, 2.0), np.full(2, 3.0), x0), NZ, NY, NX, status=0)
+    for _ in range(N):
+        sim2.step()
+    assert np.max(np.abs(sim2.p["xidx"] - sim.p["xidx"])) > 1e-3
+
+
+@pytest.mark.parametrize("scheme", ["rk2", "ab3"])
+def test_uniform_w_flat_bed(scheme):
+    """Flat h, zeta = 0, C = sigma, uniform w: dzidx = NZ w dt / h per step."""
+    NZ, NY, NX, h, w, dt = 10, 5, 6, 100.0, 1e-3, 300.0
+    ta = np.arange(4) * 3600.0
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4, h=h, w=w)
+    z0 = np.array([1.0, 4.5, 7.25])
+    sim = _sim(fields, ta, dt, scheme, (z0, np.full(3, 2.0), np.full(3, 3.0)), NZ, NY, NX,
+               status=10 if scheme == "ab3" else 0)
+    for _ in range(6):
+        sim.step()
+    assert np.allclose(sim.p["zidx"], z0 + 6 * NZ * w * dt / h, rtol=0, atol=1e-10)
+
+
+# ----------------------------------------------------------------- blend + time
+
+def test_blend_matches_numpy_dtype_semantics():
+    L = D.load()
+    rng = np.random.default_rng(1)
+    for dtype in (np.float32, np.float64):
+        prev = rng.standard_normal((3, 40, 50)).astype(dtype)
+        nxt = rng.standard_normal((3, 40, 50)).astype(dtype)
+        fld = D.OracleField(np.stack([prev, nxt]), (0.0, 0.0, 0.0), True, cyclic=False)
+        for tidx in (0.0, 0.123456789, 0.5, 0.9999999):
+            got = fld.field_data(tidx, L).copy()
+            It, ft = divmod(tidx, 1)
+            ft = dtype(ft)
+            gt = dtype(1 - ft)
+            exp = np.empty(prev.shape, np.float64)
+            np.add(np.multiply(prev, gt), np.multiply(nxt, ft), out=exp, casting="unsafe")
+            assert _same(got, exp), (dtype, tidx)
+
+
+def test_time_index_errors():
+    ta = np.array([0.0, 10.0, 20.0])
+    fields = _uniform_fields(2, 3, 4, lambda k: 0.0, 3)
+    sim = _sim(fields, ta, 5.0, "ab3", (np.zeros(1), np.zeros(1), np.zeros(1)), 2, 3, 4)
+    assert sim.get_time_index(15.0) == 1.5
+    with pytest.raises(ValueError):
+        sim.get_time_index(25.0)
+    with pytest.raises(IndexError):
+        sim.get_time_index(20.0)  # t == t_last reads past the array, as in the reference
+
+
+def test_validation_edge_cases():
+    NZ, NY, NX = 4, 5, 6
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 3)
+    z = np.array([0.0, 3.0, -1e-300, 3.0 + 4e-16, np.nan, 1.0, 1.0, 1.0, 1.0])
+    y = np.array([0.0, 4.0, 1.0, 1.0, 1.0, np.inf, 1.0, 1.0, 4.000000001])
+    x = np.array([0.0, 5.0, 1.0, 1.0, 1.0, 1.0, -np.inf, 1.0, 1.0])
+    st = np.array([0, 0, 0, 0, 0, 0, 0, 128, 0], np.uint8)
+    sim = _sim(fields, np.arange(3) * 100.0, 10.0, "ab3", (z, y, x), NZ, NY, NX, status=st)
+    sim._validate()
+    assert list(sim.p["status"]) == [0, 0, 130, 130, 129, 129, 129, 128, 130]
