@@ -11,9 +11,10 @@ Forms measured in the same run:
   validation, ab3_w_advection, ab3_post_w_advection HIP kernels), FP32 fields,
   FP64 positions and FP64 interpolation.
 - fused_fp32 / fused_bf16: AB3Engine, one fused HIP kernel per step over
-  split positions (int32 cell + FP32 offsets) with u, v, w stored in FP32 or
-  BF16, Morton re-binning every 25 steps (the re-binning is inside the timed
-  region).
+  split positions (packed int32 cell + 32-bit fixed-point offsets) with u, v,
+  w stored in FP32 or BF16; bin shape, workgroup size and re-sort cadence are
+  picked by the engine's autotuner during warm-up, and the periodic re-binning
+  is inside the timed region.
 The headline ``value`` is the fastest form; every form is listed in ``forms``.
 
 Prints ONE JSON line.  ``value`` is particle-steps/s of the headline form with
@@ -201,7 +202,7 @@ def field_bytes_touched_fused(wl, pos, storage_bytes_3d):
 FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
 
 
-def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow):
+def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False, layout=None, block=128):
     import torch
 
     from offline_particles_amd.engine import AB3Engine
@@ -211,9 +212,15 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow):
     eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                     capacity=wl.nparticles, rebin_every=rebin_every, xlow=xlow)
     del fs
-    eng.add_release(*wl.positions, nsteps=steps + warmup + 2)
+    tune_steps = 3 * 3 * 2 * 4 + 40 + 10 if tune else 0
+    eng.add_release(*wl.positions, nsteps=steps + warmup + tune_steps + 2)
+    if layout is not None:
+        eng.set_layout({"ring": 0, "pair": 1, "quad": 2}[layout])
+    eng.block = block
     eng.rebin()
     eng.step(warmup)
+    if tune:
+        log("autotune:", eng.autotune())
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -245,8 +252,11 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow):
         "algo_bytes": algo,
         "field_bytes_touched": fbytes,
         "achieved_GBs": algo / (kms * 1e-3) / 1e9,
-        "rebin_every": rebin_every,
-        "xlow": xlow,
+        "rebin_every": eng.rebin_every,
+        "xlow": eng.xlow,
+        "block": eng.block,
+        "layout": {0: "ring", 1: "pair", 2: "quad"}[eng.layout],
+        "tuning": eng.tuning,
     }
     del eng
     torch.cuda.empty_cache()
@@ -355,9 +365,15 @@ def main():
     ap.add_argument("--forms", default="plain_fp32,fused_fp32,fused_bf16")
     ap.add_argument("--rebin-every", type=int, default=25)
     ap.add_argument("--xlow", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --xlow/--rebin-every)")
+    ap.add_argument("--layout", choices=("ring", "pair", "quad"), default=None,
+                    help="stencil layout when not tuning (default: pair)")
+    ap.add_argument("--block", type=int, default=128, help="workgroup size when not tuning")
     ap.add_argument("--accuracy", type=int, default=0, metavar="N",
                     help="also report deviation from the FP64 oracle on the first N particles")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -374,9 +390,18 @@ def main():
     device = torch.device("cuda", 0)
     NX, NY, NZ = (int(s) for s in args.grid.split("x"))
     t = time.perf_counter()
-    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles, steps=args.steps + args.warmup + 5,
+    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles, steps=args.steps + args.warmup + 400,
                             dtype=np.float32)
     log(f"workload generated in {time.perf_counter() - t:.1f}s: grid {NX}x{NY}x{NZ}, {args.particles} particles")
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 copies of 1 GiB (torch copy kernel)")
+        del src, dst
 
     acc = None
     if args.accuracy:
@@ -394,7 +419,8 @@ def main():
         log("plain_fp32", forms["plain_fp32"])
     for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
         if name in wanted:
-            forms[name] = run_fused(wl, device, args.steps, args.warmup, dt, args.rebin_every, args.xlow)
+            forms[name] = run_fused(wl, device, args.steps, args.warmup, dt, args.rebin_every, args.xlow,
+                                    tune=not args.no_tune, layout=args.layout, block=args.block)
             log(name, forms[name])
 
     head = max(forms.values(), key=lambda f: f["particle_steps_per_s"])

@@ -185,9 +185,19 @@ typedef struct opx_grid {
     double zidx_min, zidx_max, yidx_min, yidx_max, xidx_min, xidx_max;
 } opx_grid;
 
+/* Layouts of u, v, w stencil data. */
+enum opx_layout {
+    OPX_LAYOUT_RING = 0, /* read snapshots s0 and s1 of the rings: 16 loads per field */
+    OPX_LAYOUT_PAIR = 1, /* packed window, node i = (s0[i], s1[i]): 8 loads per field */
+    OPX_LAYOUT_QUAD = 2, /* packed window, node i = (s0[i], s0[i+1], s1[i], s1[i+1]): 4 loads */
+};
+
 /* A ring of `nslots` snapshots per time-dependent field, contiguous with the
  * given element strides between slots; `hdd` packs the static (h, dx, dy, 0)
- * per rho point as float4. */
+ * per rho point as float4.  Optionally up to two packed stencil windows
+ * (built by opx_pack_window) for the slot pairs pslot[j] = {s0, s1}; a release
+ * whose slots match packed window j reads it in `layout`, others read the
+ * rings. */
 typedef struct opx_window {
     const void *u, *v, *w;  /* dtype3 storage */
     const float *zeta;
@@ -196,7 +206,22 @@ typedef struct opx_window {
     int32_t nslots;
     const float *hdd; /* (NY, NX, 4) */
     const double *C;  /* (NZ + 1) */
+    int32_t layout;   /* opx_layout of the packed windows */
+    int32_t npacked;  /* 0, 1 or 2 */
+    int32_t pslot[2][2];
+    const void *pu[2], *pv[2], *pw[2];
+    const void *p2[2]; /* (NY, NX) float4 {h, zeta[s0], zeta[s1], dx} per packed window */
+    const float *dy;   /* (NY, NX) static dy, read with packed windows */
 } opx_window;
+
+/* Bytes of one packed window of u, v and w for a layout (PAIR or QUAD) and of
+ * its 2-D part. */
+int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t *bytes_u, uint64_t *bytes_v,
+                     uint64_t *bytes_w, uint64_t *bytes_2d);
+/* Build a packed window of ring slots (s0, s1) into caller-owned pu, pv, pw
+ * and p2. */
+int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
+                    void *pv, void *pw, void *p2, void *stream);
 
 /* Per-release FP64 clock kept on the device.  A release advances at global
  * steps >= start_step while steps_left > 0. */
@@ -228,9 +253,11 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
                       double dt, int32_t *err_flag, void *stream);
 
 /* One fused AB3 step (validation -> ab3_w_advection -> ab3_post_w_advection)
- * for particles [0, n) (or [0, *n_active) when n_active is not NULL). */
+ * for particles [0, n) (or [0, *n_active) when n_active is not NULL), one
+ * thread per particle in workgroups of `block` threads (128 or 256; 0 = 128).
+ * NZ must be <= 127. */
 int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
-                  const opx_release_step *table, double dt, const int32_t *n_active, void *stream);
+                  const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream);
 
 /* Split <-> FP64 index conversion.  opx_split_from_f64 writes the state of
  * particle i from idx arrays (and status/rel untouched); opx_split_to_f64

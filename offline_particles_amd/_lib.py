@@ -104,11 +104,16 @@ class opx_grid(ctypes.Structure):
                 ("xidx_min", ctypes.c_double), ("xidx_max", ctypes.c_double)]
 
 
+OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD = 0, 1, 2
+
+
 class opx_window(ctypes.Structure):
     _fields_ = [("u", _vp), ("v", _vp), ("w", _vp), ("zeta", _vp),
                 ("u_stride", ctypes.c_int64), ("v_stride", ctypes.c_int64), ("w_stride", ctypes.c_int64),
                 ("zeta_stride", ctypes.c_int64), ("dtype3", ctypes.c_int32), ("nslots", ctypes.c_int32),
-                ("hdd", _vp), ("C", _vp)]
+                ("hdd", _vp), ("C", _vp), ("layout", ctypes.c_int32), ("npacked", ctypes.c_int32),
+                ("pslot", (ctypes.c_int32 * 2) * 2), ("pu", _vp * 2), ("pv", _vp * 2), ("pw", _vp * 2),
+                ("p2", _vp * 2), ("dy", _vp)]
 
 
 class opx_release_clock(ctypes.Structure):
@@ -125,10 +130,15 @@ _P = ctypes.POINTER
 _SIGNATURES.update({
     "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                           ctypes.c_double, _vp, _vp],
-    "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, _vp],
+    "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,
+                      _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],
+    "opx_packed_bytes": [_P(opx_grid), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_uint64), _P(ctypes.c_uint64),
+                         _P(ctypes.c_uint64), _P(ctypes.c_uint64)],
+    "opx_pack_window": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp,
+                        _vp, _vp],
     "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_uint64,
                   _vp, _vp],
 })

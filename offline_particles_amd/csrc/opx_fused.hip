@@ -30,6 +30,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kXcds = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kMaxC = 128; // largest NZ + 1 whose C is staged in LDS
 
 // ---------------------------------------------------------------- helpers
 
@@ -137,48 +138,99 @@ __device__ __forceinline__ Ax ax_outer(const Axis &a, int N)
 // FP64 weight of the upper node for base b and stagger offset o.
 __device__ __forceinline__ double w64(const Axis &a, int b, double o) { return ((double)(a.c - b) + o) + (double)a.q * (1.0 / kQ); }
 
+// The time blend of one stencil value, p*gt + q*ft with each operation rounded
+// in FP32 (no contraction): the same arithmetic as the reference's FP32
+// full-grid blend (fields.py:426-428), and identical in every stencil layout.
+__device__ __forceinline__ float blend_rn(float p, float q, float gt, float ft)
+{
+#pragma clang fp contract(off)
+    return p * gt + q * ft;
+}
+
 template <typename T>
 __device__ __forceinline__ float blend(const T *__restrict__ p, const T *__restrict__ q, uint32_t i, float gt,
                                        float ft)
 {
-    return ld3(p, i) * gt + ld3(q, i) * ft;
+    return blend_rn(ld3(p, i), ld3(q, i), gt, ft);
 }
 
-// Trilinear interpolation of the blended field (FP32).
+// ---------------------------------------------------------------- stencil corners
+
+// Packed window elements: PAIR node i = (s0[i], s1[i]); QUAD node i =
+// (s0[i], s0[i+1], s1[i], s1[i+1]) (x-neighbour pair of both snapshots).
 template <typename T>
-__device__ __forceinline__ float trilinear(const T *__restrict__ p, const T *__restrict__ q, float gt, float ft,
-                                           int n1, int n2, Ax a0, Ax a1, Ax a2)
+struct Packed;
+template <>
+struct Packed<float> {
+    using pair = float2;
+    using quad = float4;
+};
+template <>
+struct Packed<uint16_t> {
+    using pair = uint32_t; // lo: s0, hi: s1
+    using quad = uint2;    // x: s0[i] | s0[i+1] << 16, y: s1[i] | s1[i+1] << 16
+};
+
+__device__ __forceinline__ float bf_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
+__device__ __forceinline__ float2 unpair(float2 v) { return v; }
+__device__ __forceinline__ float2 unpair(uint32_t v) { return make_float2(bf_lo(v), bf_hi(v)); }
+__device__ __forceinline__ float4 unquad(float4 v) { return v; }
+__device__ __forceinline__ float4 unquad(uint2 v)
 {
-    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
-    const uint32_t b = ((uint32_t)a0.b * (uint32_t)n1 + (uint32_t)a1.b) * (uint32_t)n2 + (uint32_t)a2.b;
-    float v000 = blend(p, q, b, gt, ft), v001 = blend(p, q, b + 1, gt, ft);
-    float v010 = blend(p, q, b + s1, gt, ft), v011 = blend(p, q, b + s1 + 1, gt, ft);
-    float v100 = blend(p, q, b + s0, gt, ft), v101 = blend(p, q, b + s0 + 1, gt, ft);
-    float v110 = blend(p, q, b + s0 + s1, gt, ft), v111 = blend(p, q, b + s0 + s1 + 1, gt, ft);
-    const float f0 = a0.w, f1 = a1.w, f2 = a2.w;
-    const float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
-    float c00 = v000 * g2 + v001 * f2, c01 = v010 * g2 + v011 * f2;
-    float c10 = v100 * g2 + v101 * f2, c11 = v110 * g2 + v111 * f2;
-    return (c00 * g1 + c01 * f1) * g0 + (c10 * g1 + c11 * f1) * f0;
+    return make_float4(bf_lo(v.x), bf_hi(v.x), bf_lo(v.y), bf_hi(v.y));
 }
 
-// Trilinear interpolation in FP64 of the blended field (corner values blended
-// in FP32 like the reference's FP32 blend, then widened).  Used for w: the
-// vertical index moves through layers that can be a fraction of a metre thick.
-template <typename T>
-__device__ __forceinline__ double trilinear64(const T *__restrict__ p, const T *__restrict__ q, float gt, float ft,
-                                              int n1, int n2, int b0, int b1, int b2, double f0, double f1,
-                                              double f2)
+// The 8 blended stencil values (order 000, 001, 010, 011, 100, 101, 110, 111
+// over (axis0, axis1, axis2)) of a field with dims (., n1, n2) and base
+// (b0, b1, b2).  RING reads slot pointers p and q; PAIR and QUAD read packed
+// window p.  The blend is p*gt + q*ft in FP32 in every layout.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void corners(const void *__restrict__ pp, const void *__restrict__ qq, float gt, float ft,
+                                        int n1, int n2, int b0, int b1, int b2, float v[8])
 {
     const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
     const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
-    double v000 = blend(p, q, b, gt, ft), v001 = blend(p, q, b + 1, gt, ft);
-    double v010 = blend(p, q, b + s1, gt, ft), v011 = blend(p, q, b + s1 + 1, gt, ft);
-    double v100 = blend(p, q, b + s0, gt, ft), v101 = blend(p, q, b + s0 + 1, gt, ft);
-    double v110 = blend(p, q, b + s0 + s1, gt, ft), v111 = blend(p, q, b + s0 + s1 + 1, gt, ft);
+    const uint32_t rows[4] = {b, b + s1, b + s0, b + s0 + s1};
+    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            v[2 * r] = blend(p, q, rows[r], gt, ft);
+            v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
+        }
+    } else if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
+        const typename Packed<T>::pair *P = static_cast<const typename Packed<T>::pair *>(pp);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float2 e0 = unpair(P[rows[r]]), e1 = unpair(P[rows[r] + 1]);
+            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
+            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
+        }
+    } else {
+        const typename Packed<T>::quad *P = static_cast<const typename Packed<T>::quad *>(pp);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float4 e = unquad(P[rows[r]]);
+            v[2 * r] = blend_rn(e.x, e.z, gt, ft);
+            v[2 * r + 1] = blend_rn(e.y, e.w, gt, ft);
+        }
+    }
+}
+
+__device__ __forceinline__ float tri32(const float v[8], float f0, float f1, float f2)
+{
+    const float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
+    float c00 = v[0] * g2 + v[1] * f2, c01 = v[2] * g2 + v[3] * f2;
+    float c10 = v[4] * g2 + v[5] * f2, c11 = v[6] * g2 + v[7] * f2;
+    return (c00 * g1 + c01 * f1) * g0 + (c10 * g1 + c11 * f1) * f0;
+}
+
+__device__ __forceinline__ double tri64(const float v[8], double f0, double f1, double f2)
+{
     const double g0 = 1.0 - f0, g1 = 1.0 - f1, g2 = 1.0 - f2;
-    double c00 = v000 * g2 + v001 * f2, c01 = v010 * g2 + v011 * f2;
-    double c10 = v100 * g2 + v101 * f2, c11 = v110 * g2 + v111 * f2;
+    double c00 = (double)v[0] * g2 + (double)v[1] * f2, c01 = (double)v[2] * g2 + (double)v[3] * f2;
+    double c10 = (double)v[4] * g2 + (double)v[5] * f2, c11 = (double)v[6] * g2 + (double)v[7] * f2;
     return (c00 * g1 + c01 * f1) * g0 + (c10 * g1 + c11 * f1) * f0;
 }
 
@@ -189,35 +241,66 @@ __device__ __forceinline__ double bil64(double v00, double v01, double v10, doub
     return gy * gx * v00 + gy * fx * v01 + fy * gx * v10 + fy * fx * v11;
 }
 
-struct Surface {
-    double h, zeta;
-    float dx, dy;
+// The four rho-point corners of the 2-D fields around a particle: h, zeta of
+// the two snapshots, dx, dy, and the base they were read at.
+struct Corners2 {
+    float h[4], z0[4], z1[4], dx[4], dy[4];
+    int by, bx;
 };
 
-// h (and dx, dy) from the packed static metrics and zeta blended in FP32 then
-// widened, bilinear at rho position (y, x) with FP64 weights.
-__device__ __forceinline__ Surface surface(const float4 *__restrict__ hdd, const float *__restrict__ zp,
-                                           const float *__restrict__ zq, float gt, float ft, int NY, int NX,
-                                           const Axis &y, const Axis &x, bool want_dxdy)
+__device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const float *__restrict__ dy, int NX,
+                                             int by, int bx, bool want_dxdy, Corners2 &c)
 {
-    const int by = max(0, min(y.c, NY - 2)), bx = max(0, min(x.c, NX - 2));
-    const double wy = w64(y, by, 0.0), wx = w64(x, bx, 0.0);
     const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
-    float4 m00 = hdd[b], m01 = hdd[b + 1], m10 = hdd[b + NX], m11 = hdd[b + NX + 1];
-    Surface s;
-    s.h = bil64(m00.x, m01.x, m10.x, m11.x, wy, wx);
-    float z00 = zp[b] * gt + zq[b] * ft;
-    float z01 = zp[b + 1] * gt + zq[b + 1] * ft;
-    float z10 = zp[b + NX] * gt + zq[b + NX] * ft;
-    float z11 = zp[b + NX + 1] * gt + zq[b + NX + 1] * ft;
-    s.zeta = bil64(z00, z01, z10, z11, wy, wx);
-    if (want_dxdy) {
-        const float fy = (float)wy, fx = (float)wx;
-        float gy = 1.0f - fy, gx = 1.0f - fx;
-        s.dx = (m00.y * gx + m01.y * fx) * gy + (m10.y * gx + m11.y * fx) * fy;
-        s.dy = (m00.z * gx + m01.z * fx) * gy + (m10.z * gx + m11.z * fx) * fy;
+    const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float4 m = P[o[k]];
+        c.h[k] = m.x;
+        c.z0[k] = m.y;
+        c.z1[k] = m.z;
+        c.dx[k] = m.w;
+        if (want_dxdy)
+            c.dy[k] = dy[o[k]];
     }
-    return s;
+    c.by = by;
+    c.bx = bx;
+}
+
+__device__ __forceinline__ void load2_ring(const float4 *__restrict__ hdd, const float *__restrict__ zp,
+                                           const float *__restrict__ zq, int NX, int by, int bx, bool want_dxdy,
+                                           Corners2 &c)
+{
+    const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
+    const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float4 m = hdd[o[k]];
+        c.h[k] = m.x;
+        c.dx[k] = m.y;
+        c.dy[k] = m.z;
+        c.z0[k] = zp[o[k]];
+        c.z1[k] = zq[o[k]];
+    }
+    c.by = by;
+    c.bx = bx;
+    (void)want_dxdy;
+}
+
+// h and zeta (blended in FP32 like the reference, then widened) bilinear
+// with FP64 weights.
+__device__ __forceinline__ void eval2(const Corners2 &c, float gt, float ft, double wy, double wx, double &h,
+                                      double &zeta)
+{
+    h = bil64(c.h[0], c.h[1], c.h[2], c.h[3], wy, wx);
+    zeta = bil64(blend_rn(c.z0[0], c.z1[0], gt, ft), blend_rn(c.z0[1], c.z1[1], gt, ft),
+                 blend_rn(c.z0[2], c.z1[2], gt, ft), blend_rn(c.z0[3], c.z1[3], gt, ft), wy, wx);
+}
+
+__device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
+{
+    const float gy = 1.0f - fy, gx = 1.0f - fx;
+    return (v[0] * gx + v[1] * fx) * gy + (v[2] * gx + v[3] * fx) * fy;
 }
 
 // XCD-aware block order: XCD k (blocks k, k+8, ...) works on the k-th
@@ -227,7 +310,7 @@ __device__ __forceinline__ int64_t xcd_particle_index()
 {
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const uint32_t lb = (b % kXcds) * (G / kXcds) + b / kXcds;
-    return (int64_t)lb * kBlock + threadIdx.x;
+    return (int64_t)lb * blockDim.x + threadIdx.x;
 }
 
 // ---------------------------------------------------------------- clock
@@ -297,11 +380,17 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
 
 // ---------------------------------------------------------------- fused AB3
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
+template <typename T, int LAYOUT, int BLOCK, int MINW>
+__global__ void __launch_bounds__(BLOCK, MINW)
     k_ab3_fused(opx_split_state s, opx_grid g, opx_window win, const opx_release_step *__restrict__ table,
                 double dt, const int32_t *__restrict__ n_active)
 {
+    // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
+    __shared__ double Cw[kMaxC];
+    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
+        Cw[k] = win.C[k];
+    __syncthreads();
+
     const int64_t i = xcd_particle_index();
     int64_t n = s.n;
     if (n_active)
@@ -338,20 +427,52 @@ __global__ void __launch_bounds__(kBlock)
     }
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
-    const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v), *W = static_cast<const T *>(win.w);
     const float gt = rs.gt, ft = rs.ft;
     const Ax cz = ax_center(az, NZ), cy = ax_center(ay, NY), cx = ax_center(ax, NX);
-    float u = trilinear(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz, cy,
-                        ax_inner(ax, NX));
-    float v = trilinear(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz,
-                        ax_inner(ay, NY), cx);
-    const Ax ow = ax_outer(az, NZ);
-    const double w = trilinear64(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
-                                 cx.b, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    const Ax ixu = ax_inner(ax, NX), iyv = ax_inner(ay, NY), ow = ax_outer(az, NZ);
+    float vu[8], vv[8], vw[8];
+    int j = -1;
+    if constexpr (LAYOUT != OPX_LAYOUT_RING) {
+        if (win.npacked > 0 && win.pslot[0][0] == rs.s0 && win.pslot[0][1] == rs.s1)
+            j = 0;
+        else if (win.npacked > 1 && win.pslot[1][0] == rs.s0 && win.pslot[1][1] == rs.s1)
+            j = 1;
+    }
+    if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
+        const void *pu = j ? win.pu[1] : win.pu[0];
+        const void *pv = j ? win.pv[1] : win.pv[0];
+        const void *pw = j ? win.pw[1] : win.pw[0];
+        corners<T, LAYOUT>(pu, nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ixu.b, vu);
+        corners<T, LAYOUT>(pv, nullptr, gt, ft, NY - 1, NX, cz.b, iyv.b, cx.b, vv);
+        corners<T, LAYOUT>(pw, nullptr, gt, ft, NY, NX, ow.b, cy.b, cx.b, vw);
+    } else {
+        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
+        const T *W = static_cast<const T *>(win.w);
+        corners<T, OPX_LAYOUT_RING>(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b,
+                                    cy.b, ixu.b, vu);
+        corners<T, OPX_LAYOUT_RING>(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b,
+                                    iyv.b, cx.b, vv);
+        corners<T, OPX_LAYOUT_RING>(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
+                                    cx.b, vw);
+    }
+    const float u = tri32(vu, cz.w, cy.w, ixu.w);
+    const float v = tri32(vv, cz.w, iyv.w, cx.w);
+    // w with FP64 weights: the vertical index moves through layers that can be
+    // a fraction of a metre thick
+    const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
     const float4 *hdd = reinterpret_cast<const float4 *>(win.hdd);
-    Surface s0 = surface(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, gt, ft, NY,
-                         NX, ay, ax, true);
-    const float dx0 = u / s0.dx, dy0 = v / s0.dy, dz0 = (float)w;
+    Corners2 c2;
+    const bool packed0 = LAYOUT != OPX_LAYOUT_RING && j >= 0;
+    if (packed0)
+        load2_packed(static_cast<const float4 *>(j ? win.p2[1] : win.p2[0]), win.dy, NX, cy.b, cx.b, true, c2);
+    else
+        load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
+                   c2);
+    double h0, zeta0;
+    const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
+    eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+    const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
+    const float dx0 = u / dxv, dy0 = v / dyv, dz0 = (float)w;
 
     // ---- Euler -> AB2 -> AB3 start-up (ab3_w_advection.pyx:121-138)
     float dx1, dy1, dz1, dx2, dy2, dz2;
@@ -380,10 +501,10 @@ __global__ void __launch_bounds__(kBlock)
     // ---- z at x_n, t_n (ab3_w_advection.pyx:144-165), FP64
     const double hc = g.hc;
     const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
-    const double Cv = (1.0 - wC) * win.C[ow.b] + wC * win.C[ow.b + 1];
+    const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
     const double sigma = (zi + 0.5) / NZ - 1.0;
-    const double S0 = (hc * sigma + s0.h * Cv) / (hc + s0.h);
-    double z = s0.zeta + (s0.zeta + s0.h) * S0;
+    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
+    double z = zeta0 + (zeta0 + h0) * S0;
 
     // ---- AB3 update (ab3_w_advection.pyx:168-170) in FP64 from FP32 derivatives;
     // the vertical uses this step's FP64 w
@@ -398,14 +519,34 @@ __global__ void __launch_bounds__(kBlock)
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
     Axis azn;
     if (ay.ok && ax.ok) {
-        Surface s1 = surface(hdd, win.zeta + rs.s0p * win.zeta_stride, win.zeta + rs.s1p * win.zeta_stride, rs.gt1,
-                             rs.ft1, NY, NX, ay, ax, false);
-        const double S = (z - s1.zeta) / (s1.zeta + s1.h);
+        // h, zeta at t_{n+1} at the new position; most particles stay in their
+        // cell and the zeta snapshot pair does not change, so the corners read
+        // above are reused
+        const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
+        int j1 = -1;
+        if constexpr (LAYOUT != OPX_LAYOUT_RING) {
+            if (win.npacked > 0 && win.pslot[0][0] == rs.s0p && win.pslot[0][1] == rs.s1p)
+                j1 = 0;
+            else if (win.npacked > 1 && win.pslot[1][0] == rs.s0p && win.pslot[1][1] == rs.s1p)
+                j1 = 1;
+        }
+        const bool same_pair = packed0 ? (j1 == j) : (j1 < 0 && rs.s0p == rs.s0 && rs.s1p == rs.s1);
+        if (!(same_pair && by1 == c2.by && bx1 == c2.bx)) {
+            if (LAYOUT != OPX_LAYOUT_RING && j1 >= 0)
+                load2_packed(static_cast<const float4 *>(j1 ? win.p2[1] : win.p2[0]), win.dy, NX, by1, bx1, false,
+                             c2);
+            else
+                load2_ring(hdd, win.zeta + rs.s0p * win.zeta_stride, win.zeta + rs.s1p * win.zeta_stride, NX, by1,
+                           bx1, false, c2);
+        }
+        double h1, zeta1;
+        eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
+        const double S = (z - zeta1) / (zeta1 + h1);
         // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
         // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
         // with k, so a walk from the previous bracket finds the reference's k.
-        const double inv = 1.0 / (hc + s1.h);
-        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + s1.h * win.C[k]) * inv; };
+        const double inv = 1.0 / (hc + h1);
+        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h1 * Cw[k]) * inv; };
         int k = ow.b;
         double Sk = S_at(k);
         while (k > 0 && S < Sk) {
@@ -438,6 +579,46 @@ __global__ void __launch_bounds__(kBlock)
         s.status[i] = st_new;
     if (s.z)
         s.z[i] = (float)z;
+}
+
+// ---------------------------------------------------------------- packed windows
+
+template <typename T, int LAYOUT>
+__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, int64_t n, int n2)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
+        if constexpr (sizeof(T) == 4) {
+            static_cast<float2 *>(out)[i] = make_float2(((const float *)p)[i], ((const float *)q)[i]);
+        } else {
+            static_cast<uint32_t *>(out)[i] = (uint32_t)((const uint16_t *)p)[i] |
+                                              ((uint32_t)((const uint16_t *)q)[i] << 16);
+        }
+    } else {
+        // the x+1 neighbour of the last node of a row is never a stencil node
+        const int64_t i1 = (i % n2) + 1 < n2 ? i + 1 : i;
+        if constexpr (sizeof(T) == 4) {
+            const float *P = (const float *)p, *Q = (const float *)q;
+            static_cast<float4 *>(out)[i] = make_float4(P[i], P[i1], Q[i], Q[i1]);
+        } else {
+            const uint16_t *P = (const uint16_t *)p, *Q = (const uint16_t *)q;
+            static_cast<uint2 *>(out)[i] = make_uint2((uint32_t)P[i] | ((uint32_t)P[i1] << 16),
+                                                      (uint32_t)Q[i] | ((uint32_t)Q[i1] << 16));
+        }
+    }
+}
+
+// p2[i] = {h, zeta[s0], zeta[s1], dx} per rho point
+__global__ void k_pack2(const float4 *__restrict__ hdd, const float *__restrict__ zp, const float *__restrict__ zq,
+                        float4 *__restrict__ out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 m = hdd[i];
+    out[i] = make_float4(m.x, zp[i], zq[i], m.y);
 }
 
 // ---------------------------------------------------------------- conversions
@@ -497,27 +678,36 @@ __device__ __forceinline__ uint32_t spatial_key(uint32_t x, uint32_t y, uint32_t
 }
 
 __global__ void k_keys(opx_split_state s, opx_grid g, int xlow, uint32_t inactive_key, uint32_t *keys,
-                       uint32_t *vals, int32_t *n_active)
+                       uint32_t *vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = false;
-    if (i < s.n) {
-        const uint32_t cell = (uint32_t)s.cell[i];
-        act = !(s.status[i] & OPX_INACTIVE);
-        uint32_t key = inactive_key;
-        if (act) {
-            uint32_t x = cell & ((1u << g.xbits) - 1);
-            uint32_t y = (cell >> g.xbits) & ((1u << g.ybits) - 1);
-            uint32_t z = cell >> (g.xbits + g.ybits);
-            key = spatial_key(x, y, z, g.xbits, g.ybits, g.zbits, xlow);
-        }
-        keys[i] = key;
-        vals[i] = (uint32_t)i;
+    if (i >= s.n)
+        return;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    uint32_t key = inactive_key;
+    if (!(s.status[i] & OPX_INACTIVE)) {
+        uint32_t x = cell & ((1u << g.xbits) - 1);
+        uint32_t y = (cell >> g.xbits) & ((1u << g.ybits) - 1);
+        uint32_t z = cell >> (g.xbits + g.ybits);
+        key = spatial_key(x, y, z, g.xbits, g.ybits, g.zbits, xlow);
     }
-    // one atomic per wavefront
-    unsigned long long m = __ballot(act);
-    if ((threadIdx.x & (warpSize - 1)) == 0 && m)
-        atomicAdd(n_active, (int32_t)__popcll(m));
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+// Number of active particles = index of the first inactive key in the sorted
+// keys (one thread, binary search).
+__global__ void k_count_active(const uint32_t *keys, int64_t n, uint32_t inactive_key, int32_t *n_active)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < inactive_key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    *n_active = (int32_t)lo;
 }
 
 __global__ void k_permute(opx_split_state src, opx_split_state dst, const uint32_t *__restrict__ perm)
@@ -564,6 +754,7 @@ int check_grid(const opx_grid *g)
 {
     OPX_REQUIRE(g != nullptr, "null grid");
     OPX_REQUIRE(g->NZ >= 2 && g->NY >= 3 && g->NX >= 3, "grid %dx%dx%d too small", g->NZ, g->NY, g->NX);
+    OPX_REQUIRE(g->NZ + 1 <= kMaxC, "NZ = %d above the %d levels the fused step keeps in LDS", g->NZ, kMaxC - 1);
     OPX_REQUIRE(g->xbits > 0 && g->ybits > 0 && g->zbits > 0 && g->xbits + g->ybits + g->zbits <= 31 &&
                     (1 << g->xbits) >= g->NX + 5 && (1 << g->ybits) >= g->NY + 5 && (1 << g->zbits) >= g->NZ + 5,
                 "cell bit widths (%d,%d,%d) must hold N+5 values per axis in 31 bits", g->zbits, g->ybits,
@@ -580,6 +771,12 @@ int check_window(const opx_window *w, const opx_grid *g)
     OPX_REQUIRE(w->dtype3 == OPX_F32 || w->dtype3 == OPX_BF16, "fused step supports F32/BF16 u,v,w (got %d)",
                 w->dtype3);
     OPX_REQUIRE(w->nslots >= 2, "window needs at least 2 slots (got %d)", w->nslots);
+    OPX_REQUIRE(w->npacked >= 0 && w->npacked <= 2, "npacked %d not in [0, 2]", w->npacked);
+    OPX_REQUIRE(w->npacked == 0 || w->layout == OPX_LAYOUT_PAIR || w->layout == OPX_LAYOUT_QUAD,
+                "packed windows need layout PAIR or QUAD (got %d)", w->layout);
+    for (int j = 0; j < w->npacked; ++j)
+        OPX_REQUIRE(w->pu[j] && w->pv[j] && w->pw[j] && w->p2[j], "packed window %d has a null pointer", j);
+    OPX_REQUIRE(w->npacked == 0 || w->dy != nullptr, "packed windows need the static dy array");
     OPX_REQUIRE(w->u_stride >= (int64_t)g->NZ * g->NY * (g->NX - 1) &&
                     w->v_stride >= (int64_t)g->NZ * (g->NY - 1) * g->NX &&
                     w->w_stride >= (int64_t)(g->NZ + 1) * g->NY * g->NX && w->zeta_stride >= (int64_t)g->NY * g->NX,
@@ -587,9 +784,9 @@ int check_window(const opx_window *w, const opx_grid *g)
     return OPX_OK;
 }
 
-inline unsigned grid_xcd(int64_t n)
+inline unsigned grid_xcd(int64_t n, int block)
 {
-    int64_t G = (n + kBlock - 1) / kBlock;
+    int64_t G = (n + block - 1) / block;
     G = (G + kXcds - 1) / kXcds * kXcds;
     return (unsigned)G;
 }
@@ -629,7 +826,7 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
 }
 
 int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
-                  const opx_release_step *table, double dt, const int32_t *n_active, void *stream)
+                  const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_ab3_fused"))) return rc;
@@ -638,14 +835,104 @@ int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
     OPX_REQUIRE(table != nullptr, "null release table");
     if (s->n == 0)
         return OPX_OK;
-    const dim3 grid(grid_xcd(s->n)), block(kBlock);
-    if (win->dtype3 == OPX_F32)
-        hipLaunchKernelGGL(k_ab3_fused<float>, grid, block, 0, as_stream(stream), *s, *g, *win, table, dt,
-                           n_active);
-    else
-        hipLaunchKernelGGL(k_ab3_fused<uint16_t>, grid, block, 0, as_stream(stream), *s, *g, *win, table, dt,
-                           n_active);
+    if (block == 0)
+        block = 128;
+    const int minw = 1;
+    OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
+    const dim3 grid(grid_xcd(s->n, block)), blk(block);
+    hipStream_t st = as_stream(stream);
+    const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
+#define OPX_FUSED_LAUNCH(T, L, B, W)                                                                       \
+    if (block == B && layout == L && minw == W)                                                           \
+        hipLaunchKernelGGL((k_ab3_fused<T, L, B, W>), grid, blk, 0, st, *s, *g, *win, table, dt, n_active);
+#define OPX_FUSED_LAUNCH_B(T, L, B) OPX_FUSED_LAUNCH(T, L, B, 1)
+#define OPX_FUSED_LAUNCH_T(T, L) OPX_FUSED_LAUNCH_B(T, L, 128) OPX_FUSED_LAUNCH_B(T, L, 256)
+    if (win->dtype3 == OPX_F32) {
+        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_RING)
+        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_PAIR)
+        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_QUAD)
+    } else {
+        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_RING)
+        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_PAIR)
+        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_QUAD)
+    }
+#undef OPX_FUSED_LAUNCH_T
+#undef OPX_FUSED_LAUNCH_B
+#undef OPX_FUSED_LAUNCH
     return check_launch("opx_ab3_fused");
+}
+
+static int elem_bytes(int dtype3, int layout)
+{
+    const int b = dtype3 == OPX_F32 ? 4 : 2;
+    return layout == OPX_LAYOUT_PAIR ? 2 * b : (layout == OPX_LAYOUT_QUAD ? 4 * b : 0);
+}
+
+int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t *bytes_u, uint64_t *bytes_v,
+                     uint64_t *bytes_w, uint64_t *bytes_2d)
+{
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR or QUAD");
+    OPX_REQUIRE(bytes_u && bytes_v && bytes_w && bytes_2d, "null output");
+    *bytes_2d = (uint64_t)16 * g->NY * g->NX;
+    const uint64_t e = (uint64_t)elem_bytes(dtype3, layout);
+    *bytes_u = e * (uint64_t)g->NZ * g->NY * (g->NX - 1);
+    *bytes_v = e * (uint64_t)g->NZ * (g->NY - 1) * g->NX;
+    *bytes_w = e * (uint64_t)(g->NZ + 1) * g->NY * g->NX;
+    return OPX_OK;
+}
+
+int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
+                    void *pv, void *pw, void *p2, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(win && win->u && win->v && win->w, "null window");
+    OPX_REQUIRE(win->dtype3 == OPX_F32 || win->dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR or QUAD");
+    OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
+                s1, win->nslots);
+    OPX_REQUIRE(pu && pv && pw && p2, "null output window");
+    OPX_REQUIRE(win->hdd && win->zeta, "window without hdd/zeta");
+    hipStream_t st = as_stream(stream);
+    const int64_t n2d = (int64_t)g->NY * g->NX;
+    hipLaunchKernelGGL(k_pack2, dim3(grid_for(n2d)), dim3(kBlock), 0, st, reinterpret_cast<const float4 *>(win->hdd),
+                       win->zeta + s0 * win->zeta_stride, win->zeta + s1 * win->zeta_stride,
+                       static_cast<float4 *>(p2), n2d);
+    if ((rc = check_launch("opx_pack_window 2d"))) return rc;
+    const int64_t nu = (int64_t)g->NZ * g->NY * (g->NX - 1), nv = (int64_t)g->NZ * (g->NY - 1) * g->NX;
+    const int64_t nw = (int64_t)(g->NZ + 1) * g->NY * g->NX;
+    struct F {
+        const void *base;
+        int64_t stride, n;
+        int n2;
+        void *out;
+    } f[3] = {{win->u, win->u_stride, nu, g->NX - 1, pu}, {win->v, win->v_stride, nv, g->NX, pv},
+              {win->w, win->w_stride, nw, g->NX, pw}};
+    for (auto &x : f) {
+        const unsigned grid = grid_for(x.n);
+        if (win->dtype3 == OPX_F32) {
+            const float *b = static_cast<const float *>(x.base);
+            if (layout == OPX_LAYOUT_PAIR)
+                hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
+                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+            else
+                hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_QUAD>), dim3(grid), dim3(kBlock), 0, st,
+                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+        } else {
+            const uint16_t *b = static_cast<const uint16_t *>(x.base);
+            if (layout == OPX_LAYOUT_PAIR)
+                hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
+                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+            else
+                hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_QUAD>), dim3(grid), dim3(kBlock), 0, st,
+                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+        }
+        if ((rc = check_launch("opx_pack_window"))) return rc;
+    }
+    return OPX_OK;
 }
 
 int opx_split_from_f64(const opx_split_state *s, const opx_grid *g, const double *zidx, const double *yidx,
@@ -704,9 +991,10 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     OPX_REQUIRE(work != nullptr && work_bytes >= need, "rebin workspace too small (%llu < %llu bytes)",
                 (unsigned long long)work_bytes, (unsigned long long)need);
     hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
-    if (n == 0)
+    if (n == 0) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
         return check_launch("opx_rebin");
+    }
     const size_t arr = align256((size_t)n * sizeof(uint32_t));
     char *base = static_cast<char *>(work);
     uint32_t *k_in = reinterpret_cast<uint32_t *>(base);
@@ -717,8 +1005,7 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     size_t temp_bytes = (size_t)(work_bytes - 4 * arr);
     const int key_bits = g->xbits + g->ybits + g->zbits;
     const uint32_t inactive_key = 1u << key_bits;
-    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, inactive_key, k_in, v_in,
-                       n_active);
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, inactive_key, k_in, v_in);
     if ((rc = check_launch("opx_rebin keys"))) return rc;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
                                                       key_bits + 1, st);
@@ -726,6 +1013,7 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
         set_error("opx_rebin: radix sort failed: %s", hipGetErrorString(e));
         return OPX_ERR_HIP;
     }
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(1), 0, st, k_out, n, inactive_key, n_active);
     hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *dst, v_out);
     return check_launch("opx_rebin permute");
 }

@@ -121,6 +121,17 @@ class AB3Engine:
         self._structs = None
         # optional list collecting (start, end) HIP events around each fused launch
         self.timer = None
+        self.block = 128  # workgroup size of the fused kernel
+        self.tuning = None
+        # packed stencil windows (PAIR for FP32, QUAD for BF16 by default)
+        self.layout = _lib.OPX_LAYOUT_PAIR if storage_dtype == torch.float32 else _lib.OPX_LAYOUT_QUAD
+        self._packed = []  # [(s0, s1, (pu, pv, pw))], at most 2
+        self._packed_bytes = {}
+        # host mirror of the device clocks (same FP64 arithmetic), used to
+        # know which snapshot pairs the next step reads
+        self._t = np.zeros(0)
+        self._start = np.zeros(0, np.int64)
+        self._left = np.zeros(0, np.int64)
 
     # ---------------------------------------------------------------- setup
 
@@ -154,6 +165,7 @@ class AB3Engine:
         dx = fs["dx"].data.data.to(self.device, torch.float32)
         dy = fs["dy"].data.data.to(self.device, torch.float32)
         self._hdd = torch.stack([h, dx, dy, torch.zeros_like(h)], dim=-1).contiguous()
+        self._dy = dy.contiguous()
         self._C = fs["C"].data.data.to(self.device, torch.float64).contiguous()
         hc = fs.constants["hc"]
         g = _lib.opx_grid()
@@ -174,6 +186,7 @@ class AB3Engine:
         w.nslots = self.nslots
         w.hdd = _ptr(self._hdd)
         w.C = _ptr(self._C)
+        w.dy = _ptr(self._dy)
         self.window = w
 
     def _stream(self):
@@ -227,6 +240,9 @@ class AB3Engine:
         self._n_launch = self.n
         t0 = float(self.time_array[0]) if start_time is None else float(start_time)
         self._clocks_host.append((t0, int(start_step), int(nsteps)))
+        self._t = np.append(self._t, t0)
+        self._start = np.append(self._start, int(start_step))
+        self._left = np.append(self._left, int(nsteps))
         self._release_ranges.append((start, m))
         self._upload_clocks()
         self._structs = None
@@ -279,6 +295,71 @@ class AB3Engine:
         self._structs = None
         self._since_rebin = 0
 
+    # ---------------------------------------------------------------- windows
+
+    def _time_slots(self, t):
+        """Slots (s0, s1) of times t and whether t is inside the time array (timesteppers.py:64-85)."""
+        ta = self.time_array
+        ok = (t >= ta[0]) & (t <= ta[-1])
+        idx = np.searchsorted(ta, t, side="right") - 1
+        ok &= idx + 1 < len(ta)
+        idx = np.clip(idx, 0, len(ta) - 2)
+        tidx = idx + (t - ta[idx]) / (ta[idx + 1] - ta[idx])
+        It = np.floor(tidx).astype(np.int64)
+        return It % self.nslots, (It + 1) % self.nslots, ok
+
+    def _advance_host_clocks(self):
+        """Mirror opx_clock_advance; return the (s0, s1) pairs the step reads."""
+        act = (self._left > 0) & (self.global_step >= self._start)
+        if not act.any():
+            return set()
+        t0 = self._t[act]
+        t1 = t0 + self.dt
+        s0, s1, ok0 = self._time_slots(t0)
+        _, _, ok1 = self._time_slots(t1)
+        ok = ok0 & ok1
+        idx = np.nonzero(act)[0]
+        self._t[idx[ok]] = t1[ok]
+        self._left[idx[ok]] -= 1
+        self._left[idx[~ok]] = 0
+        return set(zip(s0[ok].tolist(), s1[ok].tolist()))
+
+    def _window_buffers(self):
+        key = self.layout
+        if key not in self._packed_bytes:
+            bu, bv, bw, b2 = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+            _lib.call("opx_packed_bytes", ctypes.byref(self.grid), self.window.dtype3, self.layout,
+                      ctypes.byref(bu), ctypes.byref(bv), ctypes.byref(bw), ctypes.byref(b2))
+            self._packed_bytes[key] = (bu.value, bv.value, bw.value, b2.value)
+        return tuple(torch.empty(b, dtype=torch.uint8, device=self.device) for b in self._packed_bytes[key])
+
+    def _ensure_windows(self, pairs):
+        """Pack the stencil windows of the slot pairs this step reads (at most two)."""
+        w = self.window
+        if self.layout == _lib.OPX_LAYOUT_RING:
+            w.npacked = 0
+            return
+        have = [p for p in self._packed if (p[0], p[1]) in pairs]
+        missing = [pr for pr in sorted(pairs) if not any((p[0], p[1]) == pr for p in self._packed)]
+        spare = [p for p in self._packed if (p[0], p[1]) not in pairs]
+        for s0, s1 in missing[: 2 - len(have)]:
+            bufs = spare.pop()[2] if spare else self._window_buffers()
+            _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, int(s0), int(s1),
+                      bufs[0].data_ptr(), bufs[1].data_ptr(), bufs[2].data_ptr(), bufs[3].data_ptr(), self._stream())
+            have.append((s0, s1, bufs))
+        self._packed = have + spare[: 2 - len(have)]
+        w.layout = self.layout
+        w.npacked = len(have)
+        for j, (s0, s1, bufs) in enumerate(have):
+            w.pslot[j][0], w.pslot[j][1] = int(s0), int(s1)
+            w.pu[j], w.pv[j], w.pw[j], w.p2[j] = (b.data_ptr() for b in bufs)
+
+    def set_layout(self, layout: int) -> None:
+        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR or _QUAD)."""
+        if layout != self.layout:
+            self.layout = int(layout)
+            self._packed = []
+
     def step(self, nsteps: int = 1) -> None:
         """Advance every active release by ``nsteps`` AB3 steps."""
         nrel = len(self._clocks_host)
@@ -286,6 +367,7 @@ class AB3Engine:
         for _ in range(int(nsteps)):
             if self.rebin_every > 0 and self._since_rebin >= self.rebin_every:
                 self.rebin()
+            self._ensure_windows(self._advance_host_clocks())
             _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), nrel, self._ta.data_ptr(),
                       int(self._ta.numel()), self.nslots, self.global_step, self.dt, self._err.data_ptr(), stream)
             s = self._launch_structs()
@@ -293,13 +375,75 @@ class AB3Engine:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e0.record()
             _lib.call("opx_ab3_fused", ctypes.byref(s), ctypes.byref(self.grid), ctypes.byref(self.window),
-                      self._table.data_ptr(), self.dt, None, stream)
+                      self._table.data_ptr(), self.dt, None, self.block, stream)
             if self.timer is not None:
                 e1 = torch.cuda.Event(enable_timing=True)
                 e1.record()
                 self.timer.append((e0, e1))
             self.global_step += 1
             self._since_rebin += 1
+
+    # ---------------------------------------------------------------- tuning
+
+    def _time_steps(self, k: int) -> float:
+        """Mean fused-kernel time (ms) over k real steps."""
+        saved, self.timer = self.timer, []
+        self.step(k)
+        torch.cuda.synchronize(self.device)
+        ms = float(np.mean([a.elapsed_time(b) for a, b in self.timer])) if self.timer else float("nan")
+        self.timer = saved
+        return ms
+
+    def autotune(self, *, xlows=(3, 4, 5), blocks=(128, 256), layouts=None, trial_steps: int = 3,
+                 cadence_probe: int = 40, max_cadence: int = 400) -> dict:
+        """Pick the bin shape, workgroup size and re-sort cadence by timing real steps.
+
+        - bin shape: ``xlow`` low bits of the x cell kept contiguous in the sort
+          key (Morton order above them), i.e. bins 2^xlow cells long in x;
+        - stencil layout (ring / pair / quad) and workgroup size of the fused
+          kernel;
+        - re-sort cadence: after a rebin the step time grows as particles drift
+          out of order; with rebin cost R and growth a per step, the average
+          cost per step is minimal at N = sqrt(2 R / a).
+        Tuning advances the simulation (results do not depend on these
+        parameters); it needs ``len(xlows) * len(blocks) * trial_steps +
+        cadence_probe`` steps of the active releases.
+        """
+        import time as _time
+
+        if layouts is None:
+            layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD)
+        saved_every, self.rebin_every = self.rebin_every, 0
+        trials = {}
+        for xlow in xlows:
+            self.xlow = int(xlow)
+            self.rebin()
+            for layout in layouts:
+                self.set_layout(layout)
+                for block in blocks:
+                    self.block = int(block)
+                    self._time_steps(1)  # first launch of a variant (and window packing)
+                    trials[(self.xlow, self.layout, self.block)] = self._time_steps(trial_steps)
+        (self.xlow, layout, self.block), best = min(trials.items(), key=lambda kv: kv[1])
+        self.set_layout(layout)
+        torch.cuda.synchronize(self.device)
+        t0 = _time.perf_counter()
+        self.rebin()
+        torch.cuda.synchronize(self.device)
+        rebin_ms = 1e3 * (_time.perf_counter() - t0)
+        ts = np.array([self._time_steps(1) for _ in range(cadence_probe)])
+        k = np.arange(cadence_probe, dtype=np.float64)
+        slope = float(np.polyfit(k, ts, 1)[0]) if cadence_probe >= 3 else 0.0
+        cadence = int(np.clip(np.sqrt(2.0 * rebin_ms / slope), 5, max_cadence)) if slope > 0 else max_cadence
+        self.rebin_every = cadence if saved_every != 0 else 0
+        self._since_rebin = cadence_probe
+        names = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad"}
+        self.tuning = {
+            "xlow": self.xlow, "layout": names[self.layout], "block": self.block, "rebin_every": self.rebin_every,
+            "kernel_ms": best, "rebin_ms": rebin_ms, "drift_ms_per_step": slope,
+            "trials": {f"xlow={x},{names[lay]},block={b}": round(v, 4) for (x, lay, b), v in trials.items()},
+        }
+        return self.tuning
 
     def check_clock(self) -> None:
         """Raise ValueError if a release's time left the time array (as the reference does)."""

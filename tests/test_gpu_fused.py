@@ -118,3 +118,20 @@ def test_clock_leaves_time_array(device):
     eng.step(len(wl.time_array) * 40)
     with pytest.raises(ValueError):
         eng.check_clock()
+
+
+@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
+def test_stencil_layouts_identical(device, storage):
+    """RING, PAIR and QUAD stencil layouts blend the same values in the same order: identical bits."""
+    from offline_particles_amd import _lib
+
+    wl = _turb()
+    res = []
+    for layout in (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD):
+        eng = _engine(wl, device, storage=storage, rebin_every=11)
+        eng.set_layout(layout)
+        eng.step(100)  # crosses snapshot boundaries (hourly snapshots, dt = 120 s)
+        res.append(_state(eng))
+    for other in res[1:]:
+        for k in ("status",) + POS:
+            assert np.array_equal(res[0][k], other[k], equal_nan=True), k
