@@ -264,6 +264,32 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
 
 # ---------------------------------------------------------------------------
+# HBM traffic measured by rocprofv3 (FETCH_SIZE x2 + WRITE_SIZE x1, see profiles/)
+# ---------------------------------------------------------------------------
+
+_PROFILE_FOR_FORM = {
+    "plain_fp32": "k_ab3_plain_turbulent16M.md",
+    "fused_fp32": "k_ab3_fused_f32_turbulent16M.md",
+    "fused_bf16": "k_ab3_fused_bf16_turbulent16M.md",
+}
+
+
+def profiled_traffic(form, particles, grid):
+    """HBM bytes per launch of the form's dominant kernel from its committed rocprofv3
+    summary, when this run is that profile's workload (16M particles, 1024x768x50)."""
+    if particles != 16_000_000 or tuple(grid) != (1024, 768, 50) or form not in _PROFILE_FOR_FORM:
+        return None
+    path = os.path.join(ROOT, "profiles", _PROFILE_FOR_FORM[form])
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+# ---------------------------------------------------------------------------
 # CPU baseline: reference-structured FP64 oracle on the host cores
 # ---------------------------------------------------------------------------
 
@@ -457,7 +483,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ)),
+            "algorithmic_bytes": head["algo_bytes"],
         },
         "cpu_baseline": cpu,
         "accuracy": acc,
