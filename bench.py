@@ -263,6 +263,84 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     return res
 
 
+def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 50)):
+    """Batching form: uneven releases (own start time, duration, 10 km boxes) in one launch per step.
+
+    The bin shape, layout and workgroup size come from the autotuner run on a
+    probe engine holding a sample of the releases; the main run then steps
+    every release to completion, adding each at its start step, with runs of
+    steps replayed from captured HIP graphs.  Particle-steps are the scheduled
+    ones (size x steps of each release); the fraction of particles that left
+    the domain (and stopped early) is reported beside it.
+    """
+    import torch
+
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.workloads import batched_releases, build_fieldset, turbulent_workload
+
+    NX, NY, NZ = grid
+    t = time.perf_counter()
+    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1, steps=int(96 * 3600 / 120) + 40, dtype=np.float32)
+    rels = batched_releases(wl, nrel=nrel, size_range=(1_000, max_size))
+    total = sum(r.size for r in rels)
+    log(f"batched: {nrel} releases, {total} particles, workload in {time.perf_counter() - t:.1f}s")
+    fs = build_fieldset(wl, device=device)
+    # autotune on a probe: the 40 largest releases started together
+    probe = sorted(rels, key=lambda r: -r.size)[:40]
+    eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
+                    capacity=sum(r.size for r in probe), max_releases=64)
+    for r in probe:
+        eng.add_release(*r.positions, nsteps=400)
+    eng.rebin()
+    tuning = eng.autotune(xlows=(3, 4, 5), blocks=(128,), cadence_probe=30)
+    log("batched autotune:", {k: v for k, v in tuning.items() if k != "trials"})
+    del eng
+    torch.cuda.empty_cache()
+
+    dt = wl.dt
+    start_step = [int(np.floor(r.start_time / dt)) for r in rels]
+    order = np.argsort(start_step, kind="stable")
+    last = max(s + r.nsteps for s, r in zip(start_step, rels))
+    eng = AB3Engine(fs, wl.time_array, np.float64(dt), storage_dtype=storage, device=device, capacity=total,
+                    max_releases=nrel, rebin_every=tuning["rebin_every"], xlow=tuning["xlow"])
+    eng.set_layout({"ring": 0, "pair": 1, "quad": 2}[tuning["layout"]])
+    eng.block = tuning["block"]
+    eng.graph_steps = graph_steps
+    scheduled = sum(r.size * r.nsteps for r in rels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nxt = 0
+    g = 0
+    while g < last:
+        while nxt < nrel and start_step[order[nxt]] <= g:
+            r = rels[order[nxt]]
+            eng.add_release(*r.positions, start_time=r.start_time, nsteps=r.nsteps, start_step=start_step[order[nxt]])
+            nxt += 1
+        upcoming = start_step[order[nxt]] if nxt < nrel else last
+        k = max(1, min(upcoming, last) - g)
+        eng.step(k)
+        g += k
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    eng.check_clock()
+    exited = float((eng.status() >= 128).float().mean().item())
+    res = {
+        "form": "batched_" + ("bf16" if storage == torch.bfloat16 else "fp32"),
+        "releases": nrel,
+        "particles": total,
+        "global_steps": last,
+        "particle_steps_scheduled": scheduled,
+        "seconds": elapsed,
+        "particle_steps_per_s": scheduled / elapsed,
+        "exited_fraction": exited,
+        "graph_steps": graph_steps,
+        "tuning": {k: v for k, v in tuning.items() if k != "trials"},
+    }
+    del eng, fs
+    torch.cuda.empty_cache()
+    return res
+
+
 # ---------------------------------------------------------------------------
 # HBM traffic measured by rocprofv3 (FETCH_SIZE x2 + WRITE_SIZE x1, see profiles/)
 # ---------------------------------------------------------------------------
@@ -398,6 +476,10 @@ def main():
     ap.add_argument("--accuracy", type=int, default=0, metavar="N",
                     help="also report deviation from the FP64 oracle on the first N particles")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--batched", type=int, default=0, metavar="NREL",
+                    help="also run the batching form with NREL releases (256 for the full case)")
+    ap.add_argument("--batched-max-size", type=int, default=1_000_000)
+    ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -450,6 +532,11 @@ def main():
             log(name, forms[name])
 
     head = max(forms.values(), key=lambda f: f["particle_steps_per_s"])
+
+    batching = None
+    if args.batched:
+        batching = run_batched(device, args.batched, args.batched_max_size, torch.bfloat16, args.graph_steps)
+        log("batched", batching)
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -488,6 +575,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "batching": batching,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
     }

@@ -247,10 +247,12 @@ typedef struct opx_release_step {
  * idx + (t - t0)/(t1 - t0), and divmod(tidx, 1) (fields.py:409).  Snapshot It
  * lives in slot It mod nslots.  A release whose time leaves the time array is
  * stopped and *err_flag is set to 1 (the reference raises ValueError /
- * IndexError). */
-int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel,
+ * IndexError).  Graph-capturable form: with nrel_dev != NULL only the first
+ * *nrel_dev (<= nrel) releases are advanced; with step_dev != NULL the global
+ * step is read from *step_dev and incremented on the device after the clocks. */
+int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
                       const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
-                      double dt, int32_t *err_flag, void *stream);
+                      int32_t *step_dev, double dt, int32_t *err_flag, void *stream);
 
 /* One fused AB3 step (validation -> ab3_w_advection -> ab3_post_w_advection)
  * for particles [0, n) (or [0, *n_active) when n_active is not NULL), one
@@ -258,6 +260,13 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
  * NZ must be <= 127. */
 int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                   const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream);
+
+/* opx_ab3_fused with the window descriptor in device memory (`win_dev`, of
+ * storage type dtype3 and stencil layout `layout`), so that a captured graph
+ * stays valid while the host repacks windows between replays. */
+int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
+                      void *stream);
 
 /* Split <-> FP64 index conversion.  opx_split_from_f64 writes the state of
  * particle i from idx arrays (and status/rel untouched); opx_split_to_f64

@@ -128,8 +128,10 @@ class opx_release_step(ctypes.Structure):
 
 _P = ctypes.POINTER
 _SIGNATURES.update({
-    "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+    "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, _vp, _vp],
+    "opx_ab3_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
+                          ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,
                       _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],

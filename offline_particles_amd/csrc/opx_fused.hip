@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -303,16 +304,6 @@ __device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
     return (v[0] * gx + v[1] * fx) * gy + (v[2] * gx + v[3] * fx) * fy;
 }
 
-// XCD-aware block order: XCD k (blocks k, k+8, ...) works on the k-th
-// contiguous eighth of the particle range, so neighbouring (sorted) particles
-// share one L2.  gridDim.x is a multiple of 8.
-__device__ __forceinline__ int64_t xcd_particle_index()
-{
-    const uint32_t G = gridDim.x, b = blockIdx.x;
-    const uint32_t lb = (b % kXcds) * (G / kXcds) + b / kXcds;
-    return (int64_t)lb * blockDim.x + threadIdx.x;
-}
-
 // ---------------------------------------------------------------- clock
 
 // Timestepper.get_time_index (timesteppers.py:64-85): false where the
@@ -350,10 +341,15 @@ __device__ __forceinline__ void weights(double tidx, int nslots, float &ft, floa
     s1 = (int)((I + 1) % nslots);
 }
 
-__global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int nrel, const double *ta, int ntime,
-                        int nslots, int step, double dt, int32_t *err)
+__global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int nrel, const int32_t *nrel_dev,
+                        const double *ta, int ntime, int nslots, int step, const int32_t *step_dev, double dt,
+                        int32_t *err)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nrel_dev)
+        nrel = min(nrel, *nrel_dev);
+    if (step_dev)
+        step = *step_dev;
     if (r >= nrel)
         return;
     opx_release_clock c = clocks[r];
@@ -378,23 +374,16 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
     table[r] = e;
 }
 
+__global__ void k_step_incr(int32_t *step) { *step += 1; }
+
 // ---------------------------------------------------------------- fused AB3
 
-template <typename T, int LAYOUT, int BLOCK, int MINW>
-__global__ void __launch_bounds__(BLOCK, MINW)
-    k_ab3_fused(opx_split_state s, opx_grid g, opx_window win, const opx_release_step *__restrict__ table,
-                double dt, const int32_t *__restrict__ n_active)
+// One particle's fused AB3 step.  Cw is the stretching curve staged in LDS.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                        const opx_window &win, const opx_release_step *__restrict__ table, double dt,
+                                        const double *Cw)
 {
-    // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
-    __shared__ double Cw[kMaxC];
-    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
-        Cw[k] = win.C[k];
-    __syncthreads();
-
-    const int64_t i = xcd_particle_index();
-    int64_t n = s.n;
-    if (n_active)
-        n = min(n, (int64_t)*n_active);
     if (i >= n)
         return;
     const uint8_t st = s.status[i];
@@ -579,6 +568,41 @@ __global__ void __launch_bounds__(BLOCK, MINW)
         s.status[i] = st_new;
     if (s.z)
         s.z[i] = (float)z;
+}
+
+// Fused AB3 step, one particle per thread.  The grid covers s.n particles;
+// the particles actually stepped are the first min(s.n, *n_active), so one
+// captured graph serves any active count up to s.n (surplus blocks exit at
+// once).  XCD k (blocks k, k+8, ...) takes the k-th contiguous eighth of the
+// (sorted) particles in chunks of BLOCK, so neighbouring particles share one
+// L2 and the resident blocks of an XCD sweep one compact region at a time.
+// A loop over chunks would let a smaller grid cover s.n, but it lifts the
+// kernel past 128 VGPRs (3 waves/SIMD instead of 4), which costs more than
+// the surplus blocks.
+template <typename T, int LAYOUT, int BLOCK, bool IND>
+__global__ void __launch_bounds__(BLOCK)
+    k_ab3_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
+                const opx_release_step *__restrict__ table, double dt, const int32_t *__restrict__ n_active)
+{
+    // IND: the window descriptor lives in device memory, so a captured graph
+    // stays valid when the host repacks windows between replays
+    const opx_window win = IND ? *win_dev : win_arg;
+    // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
+    __shared__ double Cw[kMaxC];
+    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
+        Cw[k] = win.C[k];
+    __syncthreads();
+
+    int64_t n = s.n;
+    if (n_active)
+        n = min(n, (int64_t)*n_active);
+    const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+    // the host launches roundup8(ceil(s.n / BLOCK)) blocks, so gridDim.x / 8
+    // >= ceil(chunks / 8) >= c1 - c0: every chunk has a block
+    const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+    const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
+    if (c < c1)
+        ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
 }
 
 // ---------------------------------------------------------------- packed windows
@@ -803,6 +827,33 @@ size_t sort_temp_bytes(int64_t n)
     return bytes;
 }
 
+int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
+                 int layout, int dtype3, const opx_release_step *table, double dt, const int32_t *n_active, int block,
+                 void *stream)
+{
+    const dim3 grid(grid_xcd(s->n, block)), blk(block);
+    hipStream_t st = as_stream(stream);
+    const bool ind = win_dev != nullptr;
+#define OPX_FUSED_LAUNCH(T, L, B, I)                                                                       \
+    if (block == B && layout == L && ind == I)                                                            \
+        hipLaunchKernelGGL((k_ab3_fused<T, L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt, n_active);
+#define OPX_FUSED_LAUNCH_L(T, L)                                                                           \
+    OPX_FUSED_LAUNCH(T, L, 128, false) OPX_FUSED_LAUNCH(T, L, 256, false) OPX_FUSED_LAUNCH(T, L, 128, true) \
+        OPX_FUSED_LAUNCH(T, L, 256, true)
+    if (dtype3 == OPX_F32) {
+        OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_RING)
+        OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_PAIR)
+        OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_QUAD)
+    } else {
+        OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
+        OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
+        OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
+    }
+#undef OPX_FUSED_LAUNCH_L
+#undef OPX_FUSED_LAUNCH
+    return check_launch("opx_ab3_fused");
+}
+
 } // namespace
 } // namespace opx
 
@@ -810,18 +861,23 @@ using namespace opx;
 
 extern "C" {
 
-int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const double *time_array,
-                      int32_t ntime, int32_t nslots, int32_t global_step, double dt, int32_t *err_flag,
-                      void *stream)
+int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
+                      const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
+                      int32_t *step_dev, double dt, int32_t *err_flag, void *stream)
 {
     OPX_REQUIRE(nrel >= 0, "negative release count");
-    if (nrel == 0)
+    if (nrel == 0 && step_dev == nullptr)
         return OPX_OK;
-    OPX_REQUIRE(clocks && table && time_array, "null clock/table/time array");
+    OPX_REQUIRE(nrel == 0 || (clocks && table), "null clock/table");
+    OPX_REQUIRE(time_array != nullptr, "null time array");
     OPX_REQUIRE(ntime >= 2, "time array needs at least 2 entries");
     OPX_REQUIRE(nslots >= 2, "nslots must be >= 2");
-    hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, as_stream(stream), clocks, table, nrel,
-                       time_array, ntime, nslots, global_step, dt, err_flag);
+    hipStream_t st = as_stream(stream);
+    if (nrel > 0)
+        hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, st, clocks, table, nrel, nrel_dev,
+                           time_array, ntime, nslots, global_step, step_dev, dt, err_flag);
+    if (step_dev)
+        hipLaunchKernelGGL(k_step_incr, dim3(1), dim3(1), 0, st, step_dev);
     return check_launch("opx_clock_advance");
 }
 
@@ -837,29 +893,28 @@ int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
         return OPX_OK;
     if (block == 0)
         block = 128;
-    const int minw = 1;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
-    const dim3 grid(grid_xcd(s->n, block)), blk(block);
-    hipStream_t st = as_stream(stream);
     const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
-#define OPX_FUSED_LAUNCH(T, L, B, W)                                                                       \
-    if (block == B && layout == L && minw == W)                                                           \
-        hipLaunchKernelGGL((k_ab3_fused<T, L, B, W>), grid, blk, 0, st, *s, *g, *win, table, dt, n_active);
-#define OPX_FUSED_LAUNCH_B(T, L, B) OPX_FUSED_LAUNCH(T, L, B, 1)
-#define OPX_FUSED_LAUNCH_T(T, L) OPX_FUSED_LAUNCH_B(T, L, 128) OPX_FUSED_LAUNCH_B(T, L, 256)
-    if (win->dtype3 == OPX_F32) {
-        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_RING)
-        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_PAIR)
-        OPX_FUSED_LAUNCH_T(float, OPX_LAYOUT_QUAD)
-    } else {
-        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_RING)
-        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_PAIR)
-        OPX_FUSED_LAUNCH_T(uint16_t, OPX_LAYOUT_QUAD)
-    }
-#undef OPX_FUSED_LAUNCH_T
-#undef OPX_FUSED_LAUNCH_B
-#undef OPX_FUSED_LAUNCH
-    return check_launch("opx_ab3_fused");
+    return launch_fused(s, g, *win, nullptr, layout, win->dtype3, table, dt, n_active, block, stream);
+}
+
+int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
+                      void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_ab3_fused_dev"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
+    OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_QUAD, "unknown layout %d", layout);
+    if (block == 0)
+        block = 128;
+    OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
+    if (s->n == 0)
+        return OPX_OK;
+    opx_window unused{};
+    return launch_fused(s, g, unused, win_dev, layout, dtype3, table, dt, n_active, block, stream);
 }
 
 static int elem_bytes(int dtype3, int layout)

@@ -54,6 +54,19 @@ def _ptr(t):
     return t.data_ptr() if t is not None else None
 
 
+def _h2d(a, dtype, device):
+    """Stream-ordered host->device copy through pinned memory (no host synchronisation)."""
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cpu":
+            return a.to(device=device, dtype=dtype)
+        a = a.numpy()
+    host = torch.from_numpy(np.array(a, dtype=_NP[dtype])).pin_memory()
+    return host.to(device, non_blocking=True)
+
+
+_NP = {torch.float64: np.float64, torch.uint8: np.uint8}
+
+
 class SplitState:
     """Device SoA of the split-position state (opx_split_state)."""
 
@@ -91,7 +104,8 @@ class AB3Engine:
     """Fused AB3 stepping of one or many particle releases on one device."""
 
     def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
-                 capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False):
+                 capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False,
+                 max_releases: int = 256):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if storage_dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
@@ -108,17 +122,26 @@ class AB3Engine:
         self._state = SplitState(capacity, self.device, with_z=with_z)
         self._spare = None
         self.n = 0
-        self._n_launch = 0
+        # device-resident values the step reads, so captured graphs stay valid:
+        # particles to launch, release count, global step, window descriptor
         self._n_active_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._nrel_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._step_dev = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._win_dev = torch.zeros(ctypes.sizeof(_lib.opx_window), dtype=torch.uint8, device=self.device)
+        self._win_dirty = True
         self._err = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._clocks_host: list = []
+        self._rel_cap = 0
         self._clocks = None
         self._table = None
+        self._grow_releases(max(1, int(max_releases)))
         self._release_ranges: list = []
         self.global_step = 0
         self._since_rebin = 0
         self._work = None
         self._structs = None
+        self._graphs = {}
+        self.graph_steps = 0  # > 0: replay captured graphs of this many steps
         # optional list collecting (start, end) HIP events around each fused launch
         self.timer = None
         self.block = 128  # workgroup size of the fused kernel
@@ -203,6 +226,7 @@ class AB3Engine:
         self._state = new
         self._spare = None
         self._work = None
+        self._graphs = {}
 
     def add_release(self, zidx, yidx, xidx, *, start_time=None, nsteps: int, start_step: int = 0,
                     status: int = ParticleStatus.MULTISTEP_1) -> int:
@@ -216,12 +240,7 @@ class AB3Engine:
         r = len(self._clocks_host)
         if r >= 1 << 15:
             raise ValueError("too many releases")
-        z = torch.as_tensor(np.asarray(zidx, np.float64) if not isinstance(zidx, torch.Tensor) else zidx,
-                            dtype=torch.float64, device=self.device)
-        y = torch.as_tensor(np.asarray(yidx, np.float64) if not isinstance(yidx, torch.Tensor) else yidx,
-                            dtype=torch.float64, device=self.device)
-        x = torch.as_tensor(np.asarray(xidx, np.float64) if not isinstance(xidx, torch.Tensor) else xidx,
-                            dtype=torch.float64, device=self.device)
+        z, y, x = (_h2d(a, torch.float64, self.device) for a in (zidx, yidx, xidx))
         m = int(z.numel())
         if not (y.numel() == m and x.numel() == m):
             raise ValueError("zidx, yidx, xidx must have the same length")
@@ -237,37 +256,47 @@ class AB3Engine:
             for name in _STATE_F32[3:]:
                 getattr(S, name)[start:start + m] = 0.0
         self.n = start + m
-        self._n_launch = self.n
+        self._n_active_dev.fill_(self.n)
         t0 = float(self.time_array[0]) if start_time is None else float(start_time)
+        if r >= self._rel_cap:
+            self._grow_releases(2 * self._rel_cap)
         self._clocks_host.append((t0, int(start_step), int(nsteps)))
         self._t = np.append(self._t, t0)
         self._start = np.append(self._start, int(start_step))
         self._left = np.append(self._left, int(nsteps))
         self._release_ranges.append((start, m))
-        self._upload_clocks()
+        clk = np.frombuffer(bytes(_lib.opx_release_clock(t0, int(start_step), int(nsteps))), dtype=np.uint8)
+        cs = ctypes.sizeof(_lib.opx_release_clock)
+        self._clocks[r * cs:(r + 1) * cs].copy_(_h2d(clk, torch.uint8, self.device))
+        self._nrel_dev.fill_(r + 1)
         self._structs = None
         return r
 
-    def _upload_clocks(self):
-        nrel = len(self._clocks_host)
-        arr = (_lib.opx_release_clock * nrel)()
+    def _grow_releases(self, cap):
+        """(Re)allocate the device clock and step tables for `cap` releases."""
+        cs, ts = ctypes.sizeof(_lib.opx_release_clock), ctypes.sizeof(_lib.opx_release_step)
+        clocks = torch.zeros(cap * cs, dtype=torch.uint8, device=self.device)
         if self._clocks is not None:
-            # keep the device clocks of existing releases
-            prev = torch.empty(self._clocks.numel(), dtype=torch.uint8)
-            prev.copy_(self._clocks.cpu())
-            ctypes.memmove(arr, prev.numpy().ctypes.data, prev.numel())
-        for r in range(0 if self._clocks is None else self._clocks.numel() // 16, nrel):
-            t, s0, ns = self._clocks_host[r]
-            arr[r] = _lib.opx_release_clock(t, s0, ns)
-        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-        self._clocks = torch.from_numpy(raw.copy()).to(self.device)
-        self._table = torch.zeros(nrel * ctypes.sizeof(_lib.opx_release_step), dtype=torch.uint8, device=self.device)
+            clocks[: self._clocks.numel()].copy_(self._clocks)
+        self._clocks = clocks
+        self._table = torch.zeros(cap * ts, dtype=torch.uint8, device=self.device)
+        self._rel_cap = cap
+        self._graphs = {}
 
     # ---------------------------------------------------------------- stepping
 
+    def _launch_span(self):
+        """Particles the fused launch covers: a power of two >= n, capped at the capacity.
+
+        The kernel steps min(span, *n_active) particles, so a graph captured for
+        one span serves every particle count up to it; at most half the blocks
+        of a launch find no particle."""
+        return min(self._state.capacity, 1 << max(0, int(self.n) - 1).bit_length())
+
     def _launch_structs(self):
-        if self._structs is None:
-            self._structs = self._state.struct(self._n_launch)
+        span = self._launch_span()
+        if self._structs is None or self._structs.n != span:
+            self._structs = self._state.struct(span)
         return self._structs
 
     def rebin(self):
@@ -276,6 +305,7 @@ class AB3Engine:
             return
         if self._spare is None or self._spare.capacity != self._state.capacity:
             self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
+            self._graphs = {}
         if self._work is None:
             need = ctypes.c_uint64()
             _lib.call("opx_rebin_workspace_bytes", self.n, ctypes.byref(need))
@@ -291,7 +321,6 @@ class AB3Engine:
         _lib.call("opx_rebin", ctypes.byref(src), ctypes.byref(dst), ctypes.byref(self.grid), self.xlow,
                   self._work.data_ptr(), self._work.numel(), self._n_active_dev.data_ptr(), self._stream())
         self._state, self._spare = self._spare, self._state
-        self._n_launch = int(self._n_active_dev.item())
         self._structs = None
         self._since_rebin = 0
 
@@ -337,7 +366,9 @@ class AB3Engine:
         """Pack the stencil windows of the slot pairs this step reads (at most two)."""
         w = self.window
         if self.layout == _lib.OPX_LAYOUT_RING:
-            w.npacked = 0
+            if w.npacked != 0:
+                w.npacked = 0
+                self._win_dirty = True
             return
         have = [p for p in self._packed if (p[0], p[1]) in pairs]
         missing = [pr for pr in sorted(pairs) if not any((p[0], p[1]) == pr for p in self._packed)]
@@ -348,6 +379,10 @@ class AB3Engine:
                       bufs[0].data_ptr(), bufs[1].data_ptr(), bufs[2].data_ptr(), bufs[3].data_ptr(), self._stream())
             have.append((s0, s1, bufs))
         self._packed = have + spare[: 2 - len(have)]
+        new = (self.layout, tuple((int(a), int(b), tuple(x.data_ptr() for x in bufs)) for a, b, bufs in have))
+        if new != getattr(self, "_win_key", None):
+            self._win_key = new
+            self._win_dirty = True
         w.layout = self.layout
         w.npacked = len(have)
         for j, (s0, s1, bufs) in enumerate(have):
@@ -359,29 +394,75 @@ class AB3Engine:
         if layout != self.layout:
             self.layout = int(layout)
             self._packed = []
+            self._win_dirty = True
+
+    def _upload_window(self):
+        if self._win_dirty:
+            raw = np.frombuffer(bytes(self.window), dtype=np.uint8)
+            self._win_dev.copy_(_h2d(raw, torch.uint8, self.device), non_blocking=True)
+            self._win_dirty = False
+
+    def _launch_step(self, stream):
+        """One step on `stream`: device clocks, then the fused kernel (both graph-capturable)."""
+        _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap,
+                  self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots, 0,
+                  self._step_dev.data_ptr(), self.dt, self._err.data_ptr(), stream)
+        s = self._launch_structs()
+        _lib.call("opx_ab3_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
+                  self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self._n_active_dev.data_ptr(),
+                  self.block, stream)
+
+    def _graph(self, k):
+        """A captured graph of k steps for the current buffers and launch shape."""
+        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block)
+        g = self._graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                stream = torch.cuda.current_stream(self.device).cuda_stream
+                for _ in range(k):
+                    self._launch_step(stream)
+            self._graphs[key] = g
+        return g
 
     def step(self, nsteps: int = 1) -> None:
-        """Advance every active release by ``nsteps`` AB3 steps."""
-        nrel = len(self._clocks_host)
+        """Advance every active release by ``nsteps`` AB3 steps.
+
+        With ``graph_steps`` > 0, runs of up to that many steps between
+        re-bins are replayed from captured HIP graphs; the window descriptor,
+        release count, active count and global step live in device memory and
+        the fused kernel walks the active count with a fixed grid, so a graph
+        stays valid while windows are repacked and releases are added.
+        """
         stream = self._stream()
-        for _ in range(int(nsteps)):
+        left = int(nsteps)
+        while left > 0:
             if self.rebin_every > 0 and self._since_rebin >= self.rebin_every:
                 self.rebin()
-            self._ensure_windows(self._advance_host_clocks())
-            _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), nrel, self._ta.data_ptr(),
-                      int(self._ta.numel()), self.nslots, self.global_step, self.dt, self._err.data_ptr(), stream)
-            s = self._launch_structs()
-            if self.timer is not None:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-            _lib.call("opx_ab3_fused", ctypes.byref(s), ctypes.byref(self.grid), ctypes.byref(self.window),
-                      self._table.data_ptr(), self.dt, None, self.block, stream)
-            if self.timer is not None:
-                e1 = torch.cuda.Event(enable_timing=True)
-                e1.record()
-                self.timer.append((e0, e1))
-            self.global_step += 1
-            self._since_rebin += 1
+            k = 1
+            if self.graph_steps > 0 and self.timer is None:
+                k = min(self.graph_steps, left)
+                if self.rebin_every > 0:
+                    k = max(1, min(k, self.rebin_every - self._since_rebin))
+            pairs = set()
+            for _ in range(k):
+                pairs |= self._advance_host_clocks()
+                self.global_step += 1
+            self._ensure_windows(pairs)
+            self._upload_window()
+            if k > 1:
+                self._graph(k).replay()
+            else:
+                if self.timer is not None:
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                self._launch_step(stream)
+                if self.timer is not None:
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e1.record()
+                    self.timer.append((e0, e1))
+            self._since_rebin += k
+            left -= k
 
     # ---------------------------------------------------------------- tuning
 

@@ -370,3 +370,37 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
     elif scheme == "ab3":
         sim.set_particle_field("status", 10)
     return sim
+
+
+@dataclass
+class Release:
+    """One particle release: seed positions, start time (s) and number of steps."""
+
+    positions: tuple
+    start_time: float
+    nsteps: int
+
+    @property
+    def size(self) -> int:
+        return int(self.positions[0].shape[0])
+
+
+def batched_releases(wl: Workload, *, nrel=256, size_range=(1_000, 1_000_000), start_span=24 * 3600.0,
+                     duration_range=(6 * 3600.0, 72 * 3600.0), box_cells=10, seed=3) -> list:
+    """Releases of log-uniform size, uniform start time and duration, seeded in random boxes.
+
+    Each release fills a random ``box_cells`` x ``box_cells`` horizontal box
+    (10 km at 1 km spacing) over the full depth, uniformly in index space.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(size_range[0]), np.log(size_range[1])
+    for _ in range(nrel):
+        m = int(np.exp(rng.uniform(lo, hi)))
+        y0 = rng.uniform(0, wl.NY - 1 - box_cells)
+        x0 = rng.uniform(0, wl.NX - 1 - box_cells)
+        pos = (rng.uniform(0, wl.NZ - 1, m), y0 + rng.uniform(0, box_cells, m), x0 + rng.uniform(0, box_cells, m))
+        t0 = float(rng.uniform(0, start_span))
+        nsteps = int(round(rng.uniform(*duration_range) / wl.dt))
+        out.append(Release(pos, t0, nsteps))
+    return out
