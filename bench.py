@@ -263,6 +263,58 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     return res
 
 
+def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_size=2_000_000, graph_steps=8,
+                tune=True):
+    """Serving form: Poisson arrivals (``rate``/s for ``seconds`` s), sizes log-uniform on
+    [1k, max_size], 1-10 simulated days (720-7200 steps of 120 s), BF16 fields streamed
+    through a 3-slot window, admission by free HBM; every request runs to completion."""
+    import torch
+
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.serving import ReleaseServer, poisson_requests
+    from offline_particles_amd.workloads import build_fieldset, turbulent_workload
+
+    NX, NY, NZ = grid
+    t = time.perf_counter()
+    days = 10
+    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1, steps=int(days * 86400 / 120) * 2 + 40,
+                            nsnap_unique=nsnap, dtype=np.float32)
+    reqs = poisson_requests(wl, rate=rate, duration=seconds, size_range=(1_000, max_size), steps_range=(720, 7200))
+    log(f"serving: {len(reqs)} requests, {sum(r.size for r in reqs)} particles, "
+        f"{sum(r.size * r.nsteps for r in reqs):.3e} particle-steps; workload in {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    fs = build_fieldset(wl, device="cpu", storage_dtype=torch.bfloat16)
+    del wl.fields["u"], wl.fields["v"], wl.fields["w"]
+    tuning = None
+    layout, xlow, block, rebin_every = None, 4, 128, 40
+    if tune:
+        # autotune on a probe: the 40 largest requests started together
+        probe = sorted(reqs, key=lambda r: -r.size)[:40]
+        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
+                        capacity=sum(r.size for r in probe), max_releases=64, stream_slots=3)
+        for r in probe:
+            eng.add_release(r.zidx, r.yidx, r.xidx, nsteps=400)
+        eng.rebin()
+        tuning = eng.autotune(xlows=(4, 5), blocks=(128,), cadence_probe=30)
+        tuning = {k: v for k, v in tuning.items() if k != "trials"}
+        log("serving autotune:", tuning)
+        layout = {"ring": 0, "pair": 1, "quad": 2}[tuning["layout"]]
+        xlow, block, rebin_every = tuning["xlow"], tuning["block"], tuning["rebin_every"]
+        del eng
+        torch.cuda.empty_cache()
+    srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
+                        stream_slots=3, max_releases=4096, rebin_every=rebin_every, xlow=xlow, layout=layout,
+                        block=block, graph_steps=graph_steps)
+    log(f"serving: server ready in {time.perf_counter() - t:.1f}s, capacity {srv.capacity} particles")
+    srv.run(reqs)
+    res = dict(srv.stats)
+    res.update({"form": "served_bf16", "grid": [NX, NY, NZ], "arrival_rate_per_s": rate, "arrival_seconds": seconds,
+                "tuning": tuning})
+    del srv, fs
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 50)):
     """Batching form: uneven releases (own start time, duration, 10 km boxes) in one launch per step.
 
@@ -479,6 +531,9 @@ def main():
     ap.add_argument("--batched", type=int, default=0, metavar="NREL",
                     help="also run the batching form with NREL releases (256 for the full case)")
     ap.add_argument("--batched-max-size", type=int, default=1_000_000)
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
+    ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
@@ -537,6 +592,11 @@ def main():
     if args.batched:
         batching = run_batched(device, args.batched, args.batched_max_size, torch.bfloat16, args.graph_steps)
         log("batched", batching)
+    serving = None
+    if args.serve > 0:
+        sg = tuple(int(v) for v in args.serve_grid.lower().split("x"))
+        serving = run_serving(device, args.serve, grid=sg, graph_steps=args.graph_steps, tune=not args.no_tune)
+        log("serving", serving)
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -576,6 +636,7 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
         "batching": batching,
+        "serving": serving,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
     }

@@ -276,14 +276,32 @@ int opx_split_from_f64(const opx_split_state *s, const opx_grid *g, const double
 int opx_split_to_f64(const opx_split_state *s, const opx_grid *g, double *zidx, double *yidx, double *xidx,
                      void *stream);
 
-/* Re-bin: sort particles by a spatial key (Morton order of the cell with the
- * low `xlow` x bits kept contiguous; inactive particles last) and permute the
- * whole state from `src` into `dst` (same n).  Writes the number of active
- * particles to *n_active (device int).  `work` is caller-owned device memory
- * of at least opx_rebin_workspace_bytes(n) bytes. */
+/* Re-bin: sort particles by a spatial key and permute the whole state from
+ * `src` into `dst` (same n).  Three classes, in this order:
+ *   active  status without the INACTIVE bit and a release that still has steps
+ *           (or not yet started): Morton order of the cell with the low `xlow`
+ *           x bits kept contiguous;
+ *   parked  INACTIVE status, or a release whose steps are done (steps_left 0);
+ *   free    particles of retired releases (steps_left < 0).
+ * The radix sort is stable, so each class keeps its input order outside the
+ * spatial key.  Writes the active count to *n_active and, if n_live is not
+ * NULL, the active + parked count to *n_live (device ints).  `clocks` (nrel
+ * entries, indexed by rel) may be NULL: then only the status decides.  `work`
+ * is caller-owned device memory of at least opx_rebin_workspace_bytes(n). */
 int opx_rebin_workspace_bytes(int64_t n, uint64_t *bytes);
 int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
-              void *work, uint64_t work_bytes, int32_t *n_active, void *stream);
+              const opx_release_clock *clocks, int32_t nrel, void *work, uint64_t work_bytes, int32_t *n_active,
+              int32_t *n_live, void *stream);
+
+/* Results of one release: every particle i < s->n with rel[i] == r writes its
+ * FP64 indices (NaN for a non-finite index) and status at position
+ * pid[i] - pid_base (uint32 arithmetic), i.e. the particles of a release
+ * admitted with pids pid_base .. pid_base + m - 1 come out in admission order.
+ * Particles whose offset is not below m are skipped.  No counterpart in the
+ * reference (its particle arrays are the results); the caller of a served
+ * release reads them from here (`status` may be NULL). */
+int opx_extract_release(const opx_split_state *s, const opx_grid *g, int32_t r, uint32_t pid_base, int64_t m,
+                        double *zidx, double *yidx, double *xidx, uint8_t *status, void *stream);
 
 #ifdef __cplusplus
 }

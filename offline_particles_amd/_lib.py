@@ -141,8 +141,10 @@ _SIGNATURES.update({
                          _P(ctypes.c_uint64), _P(ctypes.c_uint64)],
     "opx_pack_window": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp,
                         _vp, _vp],
-    "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_uint64,
-                  _vp, _vp],
+    "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, _vp,
+                  ctypes.c_uint64, _vp, _vp, _vp],
+    "opx_extract_release": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, _vp,
+                            _vp, _vp, _vp, _vp],
 })
 
 # extra signatures registered by other modules

@@ -701,15 +701,28 @@ __device__ __forceinline__ uint32_t spatial_key(uint32_t x, uint32_t y, uint32_t
     return (key << xlow) | xl;
 }
 
-__global__ void k_keys(opx_split_state s, opx_grid g, int xlow, uint32_t inactive_key, uint32_t *keys,
-                       uint32_t *vals)
+// Sort key per particle: spatial key (active), inactive_key (parked) or
+// inactive_key + 1 (free: retired release).
+__global__ void k_keys(opx_split_state s, opx_grid g, int xlow, const opx_release_clock *__restrict__ clocks,
+                       int nrel, uint32_t inactive_key, uint32_t *keys, uint32_t *vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n)
         return;
     const uint32_t cell = (uint32_t)s.cell[i];
     uint32_t key = inactive_key;
-    if (!(s.status[i] & OPX_INACTIVE)) {
+    bool live = true, parked = false;
+    if (clocks) {
+        const int r = s.rel ? (int)s.rel[i] : 0;
+        if (r < nrel) {
+            const int32_t left = clocks[r].steps_left;
+            live = left >= 0;
+            parked = left == 0;
+        }
+    }
+    if (!live)
+        key = inactive_key + 1;
+    else if (!parked && !(s.status[i] & OPX_INACTIVE)) {
         uint32_t x = cell & ((1u << g.xbits) - 1);
         uint32_t y = (cell >> g.xbits) & ((1u << g.ybits) - 1);
         uint32_t z = cell >> (g.xbits + g.ybits);
@@ -719,19 +732,51 @@ __global__ void k_keys(opx_split_state s, opx_grid g, int xlow, uint32_t inactiv
     vals[i] = (uint32_t)i;
 }
 
-// Number of active particles = index of the first inactive key in the sorted
-// keys (one thread, binary search).
-__global__ void k_count_active(const uint32_t *keys, int64_t n, uint32_t inactive_key, int32_t *n_active)
+// First index of the sorted keys holding a key >= k (binary search).
+__device__ int64_t lower_bound_key(const uint32_t *keys, int64_t n, uint32_t k)
 {
     int64_t lo = 0, hi = n;
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < inactive_key)
+        if (keys[mid] < k)
             lo = mid + 1;
         else
             hi = mid;
     }
-    *n_active = (int32_t)lo;
+    return lo;
+}
+
+// Active count = first parked key; live count = first free key (one thread).
+__global__ void k_count_active(const uint32_t *keys, int64_t n, uint32_t inactive_key, int32_t *n_active,
+                               int32_t *n_live)
+{
+    *n_active = (int32_t)lower_bound_key(keys, n, inactive_key);
+    if (n_live)
+        *n_live = (int32_t)lower_bound_key(keys, n, inactive_key + 1);
+}
+
+__global__ void k_extract_release(opx_split_state s, opx_grid g, uint32_t r, uint32_t pid_base, int64_t m,
+                                  double *zidx, double *yidx, double *xidx, uint8_t *status)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n || (uint32_t)s.rel[i] != r)
+        return;
+    const uint32_t j = (uint32_t)s.pid[i] - pid_base;
+    if ((int64_t)j >= m)
+        return;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    Axis z = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+    Axis y = decode_axis(cell, g.xbits, g.ybits);
+    Axis x = decode_axis(cell, 0, g.xbits);
+    z.q = s.fz[i];
+    y.q = s.fy[i];
+    x.q = s.fx[i];
+    const double nan = __builtin_nan("");
+    zidx[j] = z.ok ? z.pos() : nan;
+    yidx[j] = y.ok ? y.pos() : nan;
+    xidx[j] = x.ok ? x.pos() : nan;
+    if (status)
+        status[j] = s.status[i];
 }
 
 __global__ void k_permute(opx_split_state src, opx_split_state dst, const uint32_t *__restrict__ perm)
@@ -1018,6 +1063,23 @@ int opx_split_to_f64(const opx_split_state *s, const opx_grid *g, double *zidx, 
     return check_launch("opx_split_to_f64");
 }
 
+int opx_extract_release(const opx_split_state *s, const opx_grid *g, int32_t r, uint32_t pid_base, int64_t m,
+                        double *zidx, double *yidx, double *xidx, uint8_t *status, void *stream)
+{
+    OPX_REQUIRE(s != nullptr, "null state");
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(r >= 0 && r <= 0xffff, "release id %d out of [0, 65535]", r);
+    OPX_REQUIRE(m >= 0, "negative release size");
+    if (s->n == 0 || m == 0)
+        return OPX_OK;
+    OPX_REQUIRE(s->cell && s->fz && s->fy && s->fx && s->status && s->pid && s->rel, "null state array");
+    OPX_REQUIRE(zidx && yidx && xidx, "null output");
+    hipLaunchKernelGGL(k_extract_release, dim3(grid_for(s->n)), dim3(kBlock), 0, as_stream(stream), *s, *g,
+                       (uint32_t)r, pid_base, m, zidx, yidx, xidx, status);
+    return check_launch("opx_extract_release");
+}
+
 int opx_rebin_workspace_bytes(int64_t n, uint64_t *bytes)
 {
     OPX_REQUIRE(bytes != nullptr, "null output");
@@ -1027,8 +1089,9 @@ int opx_rebin_workspace_bytes(int64_t n, uint64_t *bytes)
     return OPX_OK;
 }
 
-int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow, void *work,
-              uint64_t work_bytes, int32_t *n_active, void *stream)
+int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+              const opx_release_clock *clocks, int32_t nrel, void *work, uint64_t work_bytes, int32_t *n_active,
+              int32_t *n_live, void *stream)
 {
     int rc;
     if ((rc = check_state(src, "opx_rebin src"))) return rc;
@@ -1037,6 +1100,7 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     OPX_REQUIRE(dst->n == src->n, "src/dst sizes differ");
     OPX_REQUIRE(n_active != nullptr, "null n_active");
     OPX_REQUIRE(xlow >= 0 && xlow <= g->xbits, "xlow %d out of [0, %d]", xlow, g->xbits);
+    OPX_REQUIRE(clocks == nullptr || nrel >= 0, "negative release count");
     OPX_REQUIRE((src->pid == nullptr) == (dst->pid == nullptr) && (src->rel == nullptr) == (dst->rel == nullptr) &&
                     (src->z == nullptr) == (dst->z == nullptr),
                 "src/dst optional arrays differ");
@@ -1048,6 +1112,8 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     hipStream_t st = as_stream(stream);
     if (n == 0) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
+        if (n_live)
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_live);
         return check_launch("opx_rebin");
     }
     const size_t arr = align256((size_t)n * sizeof(uint32_t));
@@ -1060,15 +1126,17 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     size_t temp_bytes = (size_t)(work_bytes - 4 * arr);
     const int key_bits = g->xbits + g->ybits + g->zbits;
     const uint32_t inactive_key = 1u << key_bits;
-    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, inactive_key, k_in, v_in);
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, clocks, (int)nrel,
+                       inactive_key, k_in, v_in);
     if ((rc = check_launch("opx_rebin keys"))) return rc;
+    // keys span [0, inactive_key + 1] = key_bits + 1 bits
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
                                                       key_bits + 1, st);
     if (e != hipSuccess) {
         set_error("opx_rebin: radix sort failed: %s", hipGetErrorString(e));
         return OPX_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(1), 0, st, k_out, n, inactive_key, n_active);
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(1), 0, st, k_out, n, inactive_key, n_active, n_live);
     hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *dst, v_out);
     return check_launch("opx_rebin permute");
 }

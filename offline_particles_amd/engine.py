@@ -16,7 +16,15 @@ the reference's timesteppers.py:296-305 sequence) over
 
 Releases: particles are added in groups (``add_release``), each with its
 own start time, start step and number of steps; one launch steps every active
-release.  Results are read back in the order particles were added.
+release.  Results are read back in the order particles were added, or per
+release (``extract``).  A finished release can be retired: the next re-bin
+drops its particles and recycles its id, so a long-running service reuses
+the same buffers (``offline_particles_amd.serving``).
+
+Snapshot windows: by default every snapshot of u, v, w, zeta is resident.
+With ``stream_slots`` the snapshots stay in pinned host memory and a ring of
+that many device slots is filled on demand (a side stream copies the next
+snapshot while the current ones are in use).
 """
 
 from __future__ import annotations
@@ -70,6 +78,9 @@ _NP = {torch.float64: np.float64, torch.uint8: np.uint8}
 class SplitState:
     """Device SoA of the split-position state (opx_split_state)."""
 
+    # status 1 + cell 4 + offsets 3 x 4 + FP32 history 6 x 4 + pid 4 + rel 2
+    BYTES_PER_PARTICLE = 47
+
     def __init__(self, capacity: int, device, *, with_z: bool = False):
         n = int(capacity)
         self.capacity = n
@@ -105,7 +116,7 @@ class AB3Engine:
 
     def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
                  capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False,
-                 max_releases: int = 256):
+                 max_releases: int = 256, stream_slots: int | None = None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if storage_dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
@@ -117,6 +128,9 @@ class AB3Engine:
         self._ta = torch.from_numpy(self.time_array.copy()).to(self.device)
         self.rebin_every = int(rebin_every)
         self.xlow = int(xlow)
+        self._stream_slots = None if stream_slots is None else int(stream_slots)
+        if self._stream_slots is not None and self._stream_slots < 3:
+            raise ValueError("a streamed window needs at least 3 slots")
         self._build_window(fieldset)
         self._with_z = with_z
         self._state = SplitState(capacity, self.device, with_z=with_z)
@@ -136,6 +150,14 @@ class AB3Engine:
         self._table = None
         self._grow_releases(max(1, int(max_releases)))
         self._release_ranges: list = []
+        # release ids: pid range (base, size) per id, ids free for reuse, and
+        # retired releases whose particles the next re-bin drops
+        self._release_pids: dict = {}
+        self._free_ids: list = []
+        self._retired: list = []
+        self._retired_particles = 0
+        self._next_pid = 0
+        self._compacted = False
         self.global_step = 0
         self._since_rebin = 0
         self._work = None
@@ -148,7 +170,8 @@ class AB3Engine:
         self.tuning = None
         # packed stencil windows (PAIR for FP32, QUAD for BF16 by default)
         self.layout = _lib.OPX_LAYOUT_PAIR if storage_dtype == torch.float32 else _lib.OPX_LAYOUT_QUAD
-        self._packed = []  # [(s0, s1, (pu, pv, pw))], at most 2
+        self._packed = []  # [(s0, s1, (pu, pv, pw, p2))], at most 2
+        self._spare_bufs = []  # packed-window buffers not in use
         self._packed_bytes = {}
         # host mirror of the device clocks (same FP64 arithmetic), used to
         # know which snapshot pairs the next step reads
@@ -176,14 +199,28 @@ class AB3Engine:
         K = {int(t.shape[0]) for t in snaps.values()}
         if len(K) != 1:
             raise ValueError(f"u, v, w, zeta hold different snapshot counts {K}")
-        self.nslots = K.pop()
+        self.nsnap = K.pop()
         cyc = {fs[n].cyclic for n in ("u", "v", "w", "zeta")}
-        if cyc == {False} and self.nslots != len(self.time_array):
+        if cyc == {False} and self.nsnap != len(self.time_array):
             raise ValueError("non-cyclic fields need one snapshot per time_array entry")
-        self._u = snaps["u"].to(self.device, self.storage_dtype).contiguous()
-        self._v = snaps["v"].to(self.device, self.storage_dtype).contiguous()
-        self._w = snaps["w"].to(self.device, self.storage_dtype).contiguous()
-        self._zeta = snaps["zeta"].to(self.device, torch.float32).contiguous()
+        self.nslots = self.nsnap
+        dts = {"u": self.storage_dtype, "v": self.storage_dtype, "w": self.storage_dtype, "zeta": torch.float32}
+        if self._stream_slots is None:
+            self._u, self._v, self._w, self._zeta = (snaps[n].to(self.device, dts[n]).contiguous()
+                                                     for n in ("u", "v", "w", "zeta"))
+            self._src = None
+        else:
+            # host source (pinned, storage dtype) and a ring of device slots;
+            # snapshot It lives in slot It % nslots while it is needed
+            self.nslots = self._stream_slots
+            self._src = {n: snaps[n].to("cpu", dts[n]).contiguous().pin_memory() for n in ("u", "v", "w", "zeta")}
+            self._u, self._v, self._w, self._zeta = (
+                torch.empty((self.nslots,) + tuple(snaps[n].shape[1:]), dtype=dts[n], device=self.device)
+                for n in ("u", "v", "w", "zeta"))
+            self._ring_It = [None] * self.nslots
+            self._ring_ready = [None] * self.nslots
+            self._copy_stream = torch.cuda.Stream(self.device)
+            self.snapshot_loads = 0
         h = fs["h"].data.data.to(self.device, torch.float32)
         dx = fs["dx"].data.data.to(self.device, torch.float32)
         dy = fs["dy"].data.data.to(self.device, torch.float32)
@@ -215,10 +252,34 @@ class AB3Engine:
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
-    def _ensure_capacity(self, n):
+    def reserve(self, capacity: int) -> None:
+        """Allocate the particle buffers, the re-bin target and the re-bin workspace
+        for `capacity` particles up front (no allocation while stepping after this)."""
+        self._ensure_capacity(int(capacity), exact=True)
+        self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
+        self._alloc_work(self._state.capacity)
+        self._graphs = {}
+
+    def _alloc_work(self, n):
+        need = ctypes.c_uint64()
+        _lib.call("opx_rebin_workspace_bytes", int(n), ctypes.byref(need))
+        self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
+        self._work_n = int(n)
+
+    def packed_window_bytes(self):
+        """Bytes of one packed stencil window (u, v, w, 2-D) in the current layout."""
+        key = self.layout
+        if key not in self._packed_bytes:
+            bu, bv, bw, b2 = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+            _lib.call("opx_packed_bytes", ctypes.byref(self.grid), self.window.dtype3, self.layout,
+                      ctypes.byref(bu), ctypes.byref(bv), ctypes.byref(bw), ctypes.byref(b2))
+            self._packed_bytes[key] = (bu.value, bv.value, bw.value, b2.value)
+        return self._packed_bytes[key]
+
+    def _ensure_capacity(self, n, exact=False):
         if n <= self._state.capacity:
             return
-        cap = max(n, 2 * self._state.capacity)
+        cap = n if exact else max(n, 2 * self._state.capacity)
         new = SplitState(cap, self.device, with_z=self._with_z)
         old = self._state
         for name in ("status", "cell", "pid", "rel") + _STATE_F32 + (("z",) if self._with_z else ()):
@@ -237,7 +298,7 @@ class AB3Engine:
         ``nsteps`` steps last.  Particles start with ``status``
         (MULTISTEP_1: Euler -> AB2 -> AB3 start-up).
         """
-        r = len(self._clocks_host)
+        r = self._free_ids.pop() if self._free_ids else len(self._clocks_host)
         if r >= 1 << 15:
             raise ValueError("too many releases")
         z, y, x = (_h2d(a, torch.float64, self.device) for a in (zidx, yidx, xidx))
@@ -251,26 +312,79 @@ class AB3Engine:
             _lib.call("opx_split_from_f64", ctypes.byref(S.struct(m, start)), ctypes.byref(self.grid),
                       z.data_ptr(), y.data_ptr(), x.data_ptr(), self._stream())
             S.status[start:start + m] = int(status)
-            S.pid[start:start + m] = torch.arange(start, start + m, dtype=torch.int32, device=self.device)
+            # pids count admissions (uint32, wrapping), stored as int32 bits
+            base = self._next_pid
+            pids = torch.arange(base, base + m, dtype=torch.int64, device=self.device)
+            S.pid[start:start + m] = torch.where(pids >= 1 << 31, pids - (1 << 32), pids).to(torch.int32)
             S.rel[start:start + m] = r
             for name in _STATE_F32[3:]:
                 getattr(S, name)[start:start + m] = 0.0
         self.n = start + m
         self._n_active_dev.fill_(self.n)
+        self._release_pids[r] = (self._next_pid, m)
+        self._next_pid = (self._next_pid + m) & 0xFFFFFFFF
         t0 = float(self.time_array[0]) if start_time is None else float(start_time)
         if r >= self._rel_cap:
             self._grow_releases(2 * self._rel_cap)
-        self._clocks_host.append((t0, int(start_step), int(nsteps)))
-        self._t = np.append(self._t, t0)
-        self._start = np.append(self._start, int(start_step))
-        self._left = np.append(self._left, int(nsteps))
-        self._release_ranges.append((start, m))
+        if r == len(self._clocks_host):
+            self._clocks_host.append((t0, int(start_step), int(nsteps)))
+            self._t = np.append(self._t, t0)
+            self._start = np.append(self._start, int(start_step))
+            self._left = np.append(self._left, int(nsteps))
+            self._release_ranges.append((start, m))
+        else:
+            self._clocks_host[r] = (t0, int(start_step), int(nsteps))
+            self._t[r], self._start[r], self._left[r] = t0, int(start_step), int(nsteps)
+            self._release_ranges[r] = (start, m)
         clk = np.frombuffer(bytes(_lib.opx_release_clock(t0, int(start_step), int(nsteps))), dtype=np.uint8)
         cs = ctypes.sizeof(_lib.opx_release_clock)
         self._clocks[r * cs:(r + 1) * cs].copy_(_h2d(clk, torch.uint8, self.device))
-        self._nrel_dev.fill_(r + 1)
+        self._nrel_dev.fill_(len(self._clocks_host))
         self._structs = None
         return r
+
+    def steps_left(self, r: int) -> int:
+        """Steps release r still has to run (host mirror of its device clock; -1 once retired)."""
+        return int(self._left[r])
+
+    def retire(self, r: int) -> None:
+        """Give release r up: it stops stepping and the next re-bin drops its
+        particles and frees its id.  Read its results (``extract``) first."""
+        if r in self._retired or self._left[r] < 0:
+            raise ValueError(f"release {r} is already retired")
+        self._left[r] = -1
+        cs = ctypes.sizeof(_lib.opx_release_clock)
+        off = r * cs + _lib.opx_release_clock.steps_left.offset
+        val = np.array([-1], dtype=np.int32).view(np.uint8)
+        self._clocks[off:off + 4].copy_(_h2d(val, torch.uint8, self.device), non_blocking=True)
+        self._retired.append(r)
+        self._retired_particles += self._release_pids[r][1]
+
+    @property
+    def releases_in_use(self) -> int:
+        """Release ids held: running, finished or retired but not yet dropped by a re-bin."""
+        return len(self._clocks_host) - len(self._free_ids)
+
+    @property
+    def retired_releases(self) -> int:
+        """Retired releases whose particles and ids the next re-bin frees."""
+        return len(self._retired)
+
+    @property
+    def retired_particles(self) -> int:
+        """Particles of retired releases still held (dropped by the next re-bin)."""
+        return self._retired_particles
+
+    def extract(self, r: int):
+        """(zidx, yidx, xidx, status) of release r as device tensors, in admission order."""
+        base, m = self._release_pids[r]
+        out = [torch.empty(m, dtype=torch.float64, device=self.device) for _ in range(3)]
+        st = torch.empty(m, dtype=torch.uint8, device=self.device)
+        if m:
+            _lib.call("opx_extract_release", ctypes.byref(self._state.struct(self.n)), ctypes.byref(self.grid), int(r),
+                      int(base), int(m), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), st.data_ptr(),
+                      self._stream())
+        return out[0], out[1], out[2], st
 
     def _grow_releases(self, cap):
         """(Re)allocate the device clock and step tables for `cap` releases."""
@@ -306,64 +420,109 @@ class AB3Engine:
         if self._spare is None or self._spare.capacity != self._state.capacity:
             self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
             self._graphs = {}
-        if self._work is None:
-            need = ctypes.c_uint64()
-            _lib.call("opx_rebin_workspace_bytes", self.n, ctypes.byref(need))
-            self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
-            self._work_n = self.n
-        if self._work_n < self.n:
-            need = ctypes.c_uint64()
-            _lib.call("opx_rebin_workspace_bytes", self.n, ctypes.byref(need))
-            self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
-            self._work_n = self.n
+        if self._work is None or self._work_n < self.n:
+            self._alloc_work(self.n)
         src = self._state.struct(self.n)
         dst = self._spare.struct(self.n)
         _lib.call("opx_rebin", ctypes.byref(src), ctypes.byref(dst), ctypes.byref(self.grid), self.xlow,
-                  self._work.data_ptr(), self._work.numel(), self._n_active_dev.data_ptr(), self._stream())
+                  self._clocks.data_ptr(), len(self._clocks_host), self._work.data_ptr(), self._work.numel(),
+                  self._n_active_dev.data_ptr(), None, self._stream())
         self._state, self._spare = self._spare, self._state
         self._structs = None
         self._since_rebin = 0
+        if self._retired:
+            # retired releases' particles now sit behind every live one
+            self.n -= self._retired_particles
+            self._compacted = True
+            for r in self._retired:
+                del self._release_pids[r]
+            self._free_ids.extend(sorted(self._retired, reverse=True))
+            self._retired = []
+            self._retired_particles = 0
 
     # ---------------------------------------------------------------- windows
 
-    def _time_slots(self, t):
-        """Slots (s0, s1) of times t and whether t is inside the time array (timesteppers.py:64-85)."""
+    def _time_index(self, t):
+        """Snapshot index It = floor(tidx) of times t and whether t is inside the time array
+        (timesteppers.py:64-85)."""
         ta = self.time_array
         ok = (t >= ta[0]) & (t <= ta[-1])
         idx = np.searchsorted(ta, t, side="right") - 1
         ok &= idx + 1 < len(ta)
         idx = np.clip(idx, 0, len(ta) - 2)
         tidx = idx + (t - ta[idx]) / (ta[idx + 1] - ta[idx])
-        It = np.floor(tidx).astype(np.int64)
-        return It % self.nslots, (It + 1) % self.nslots, ok
+        return np.floor(tidx).astype(np.int64), ok
 
     def _advance_host_clocks(self):
-        """Mirror opx_clock_advance; return the (s0, s1) pairs the step reads."""
+        """Mirror opx_clock_advance; return the snapshot pairs (It, It + 1) the step reads
+        (at t for the step, at t + dt for the post step)."""
         act = (self._left > 0) & (self.global_step >= self._start)
         if not act.any():
             return set()
         t0 = self._t[act]
         t1 = t0 + self.dt
-        s0, s1, ok0 = self._time_slots(t0)
-        _, _, ok1 = self._time_slots(t1)
+        i0, ok0 = self._time_index(t0)
+        i1, ok1 = self._time_index(t1)
         ok = ok0 & ok1
         idx = np.nonzero(act)[0]
         self._t[idx[ok]] = t1[ok]
         self._left[idx[ok]] -= 1
         self._left[idx[~ok]] = 0
-        return set(zip(s0[ok].tolist(), s1[ok].tolist()))
+        its = np.unique(np.concatenate([i0[ok], i1[ok]]))
+        return {(int(i), int(i) + 1) for i in its}
+
+    # ---------------------------------------------------------------- streamed snapshots
+
+    def _load_slot(self, It: int) -> None:
+        """Copy snapshot It into ring slot It % nslots on the side stream, after the
+        work already queued on the compute stream (which may still read the slot)."""
+        slot = It % self.nslots
+        src = It % self.nsnap
+        compute = torch.cuda.current_stream(self.device)
+        self._copy_stream.wait_stream(compute)
+        with torch.cuda.stream(self._copy_stream):
+            for dst, name in ((self._u, "u"), (self._v, "v"), (self._w, "w"), (self._zeta, "zeta")):
+                dst[slot].copy_(self._src[name][src], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self._copy_stream)
+        self._ring_It[slot] = It
+        self._ring_ready[slot] = ev
+        self.snapshot_loads += 1
+        # packed windows built from the slot's old contents are stale
+        stale = [p for p in self._packed if slot in (p[0], p[1])]
+        if stale:
+            self._packed = [p for p in self._packed if slot not in (p[0], p[1])]
+            self._spare_bufs.extend(p[2] for p in stale)
+
+    def _ensure_snapshots(self, pairs) -> None:
+        """Make the snapshots of `pairs` resident; prefetch the one after them."""
+        if self._src is None or not pairs:
+            return
+        need = sorted({i for p in pairs for i in p})
+        slots = [i % self.nslots for i in need]
+        if len(set(slots)) != len(slots):
+            raise ValueError(f"active releases read snapshots {need[0]}..{need[-1]}, more than the "
+                             f"{self.nslots}-slot streamed window holds")
+        compute = torch.cuda.current_stream(self.device)
+        for It, slot in zip(need, slots):
+            if self._ring_It[slot] != It:
+                self._load_slot(It)
+            ev = self._ring_ready[slot]
+            if ev is not None:
+                compute.wait_event(ev)
+                self._ring_ready[slot] = None
+        nxt = need[-1] + 1
+        slot = nxt % self.nslots
+        cur = self._ring_It[slot]
+        if cur != nxt and (cur is None or cur < need[0]):
+            self._load_slot(nxt)
 
     def _window_buffers(self):
-        key = self.layout
-        if key not in self._packed_bytes:
-            bu, bv, bw, b2 = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-            _lib.call("opx_packed_bytes", ctypes.byref(self.grid), self.window.dtype3, self.layout,
-                      ctypes.byref(bu), ctypes.byref(bv), ctypes.byref(bw), ctypes.byref(b2))
-            self._packed_bytes[key] = (bu.value, bv.value, bw.value, b2.value)
-        return tuple(torch.empty(b, dtype=torch.uint8, device=self.device) for b in self._packed_bytes[key])
+        return tuple(torch.empty(b, dtype=torch.uint8, device=self.device) for b in self.packed_window_bytes())
 
     def _ensure_windows(self, pairs):
-        """Pack the stencil windows of the slot pairs this step reads (at most two)."""
+        """Pack the stencil windows of the snapshot pairs this step reads (at most two)."""
+        pairs = {(a % self.nslots, b % self.nslots) for a, b in pairs}
         w = self.window
         if self.layout == _lib.OPX_LAYOUT_RING:
             if w.npacked != 0:
@@ -374,7 +533,12 @@ class AB3Engine:
         missing = [pr for pr in sorted(pairs) if not any((p[0], p[1]) == pr for p in self._packed)]
         spare = [p for p in self._packed if (p[0], p[1]) not in pairs]
         for s0, s1 in missing[: 2 - len(have)]:
-            bufs = spare.pop()[2] if spare else self._window_buffers()
+            if spare:
+                bufs = spare.pop()[2]
+            elif self._spare_bufs:
+                bufs = self._spare_bufs.pop()
+            else:
+                bufs = self._window_buffers()
             _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, int(s0), int(s1),
                       bufs[0].data_ptr(), bufs[1].data_ptr(), bufs[2].data_ptr(), bufs[3].data_ptr(), self._stream())
             have.append((s0, s1, bufs))
@@ -394,6 +558,7 @@ class AB3Engine:
         if layout != self.layout:
             self.layout = int(layout)
             self._packed = []
+            self._spare_bufs = []
             self._win_dirty = True
 
     def _upload_window(self):
@@ -431,8 +596,9 @@ class AB3Engine:
         With ``graph_steps`` > 0, runs of up to that many steps between
         re-bins are replayed from captured HIP graphs; the window descriptor,
         release count, active count and global step live in device memory and
-        the fused kernel walks the active count with a fixed grid, so a graph
-        stays valid while windows are repacked and releases are added.
+        the fused kernel's grid covers a power-of-two span of particles, so a
+        graph stays valid while windows are repacked, snapshots are streamed
+        and releases are added or retired.
         """
         stream = self._stream()
         left = int(nsteps)
@@ -448,6 +614,7 @@ class AB3Engine:
             for _ in range(k):
                 pairs |= self._advance_host_clocks()
                 self.global_step += 1
+            self._ensure_snapshots(pairs)
             self._ensure_windows(pairs)
             self._upload_window()
             if k > 1:
@@ -535,6 +702,8 @@ class AB3Engine:
 
     def positions(self):
         """(zidx, yidx, xidx) FP64 device tensors in the order particles were added."""
+        if self._compacted:
+            raise RuntimeError("releases were retired: read results per release with extract()")
         out = [torch.empty(self.n, dtype=torch.float64, device=self.device) for _ in range(3)]
         if self.n:
             _lib.call("opx_split_to_f64", ctypes.byref(self._state.struct(self.n)), ctypes.byref(self.grid),
@@ -543,12 +712,17 @@ class AB3Engine:
 
     def status(self):
         """Status codes (uint8 device tensor) in the order particles were added."""
+        if self._compacted:
+            raise RuntimeError("releases were retired: read results per release with extract()")
         out = torch.empty(self.n, dtype=torch.uint8, device=self.device)
         pid = self._state.pid[: self.n].long()
         out[pid] = self._state.status[: self.n]
         return out
 
     def release_slice(self, r: int) -> slice:
+        """Positions of release r in positions()/status() (valid while no release was retired)."""
+        if self._compacted:
+            raise RuntimeError("releases were retired: read results per release with extract()")
         start, m = self._release_ranges[r]
         return slice(start, start + m)
 
