@@ -235,14 +235,17 @@ typedef struct opx_release_clock {
  * blend at t_n in slots (s0, s1) with (ft, gt); zeta at t_{n+1} in slots
  * (s0p, s1p) with (ft1, gt1); FP32-rounded like fields.py:423-424. */
 typedef struct opx_release_step {
-    float ft, gt, ft1, gt1;
-    int32_t s0, s1, s0p, s1p;
+    float ft, gt;   /* at t (AB3 step, RK2 stage 1) */
+    float ft1, gt1; /* at t + dt (AB3 post step, RK2 update) */
+    float ftm, gtm; /* at t + alpha dt (RK2 stage 2; 0 when alpha is 0) */
+    int32_t s0, s1, s0p, s1p, s0m, s1m;
     int32_t active;
     int32_t pad_;
 } opx_release_step;
 
 /* Advance `nrel` release clocks by one step of `dt` and write their step
- * weights.  The time index restates Timestepper.get_time_index
+ * weights (with alpha > 0 also those of the RK2 intermediate time t + alpha dt,
+ * timesteppers.py:250-251).  The time index restates Timestepper.get_time_index
  * (timesteppers.py:64-85) on the device: searchsorted over time_array, then
  * idx + (t - t0)/(t1 - t0), and divmod(tidx, 1) (fields.py:409).  Snapshot It
  * lives in slot It mod nslots.  A release whose time leaves the time array is
@@ -252,7 +255,7 @@ typedef struct opx_release_step {
  * step is read from *step_dev and incremented on the device after the clocks. */
 int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
                       const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
-                      int32_t *step_dev, double dt, int32_t *err_flag, void *stream);
+                      int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream);
 
 /* One fused AB3 step (validation -> ab3_w_advection -> ab3_post_w_advection)
  * for particles [0, n) (or [0, *n_active) when n_active is not NULL), one
@@ -267,6 +270,21 @@ int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
 int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                       int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
                       void *stream);
+
+/* One fused RK2 step (validation -> _rk2_step_1 at t -> _rk2_step_2 at
+ * t + alpha dt -> _rk2_update at t + dt, rk2_w_advection.pyx:31-354 in the
+ * order of timesteppers.py:241-257) per particle, with the weights b1 =
+ * 1/(2 alpha) on stage 1 and b2 = 1 - b1 on stage 2 as the reference code has
+ * them.  Positions, z and the stage derivatives are FP64 inside the step;
+ * u/dx and v/dy are FP32 (fields stored in FP32 or BF16), w FP64.  The AB3
+ * history arrays of `s` are not used (may be NULL).  Same launch rules as
+ * opx_ab3_fused; `table` must come from opx_clock_advance with this alpha. */
+int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
+                  void *stream);
+int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, double alpha,
+                      const int32_t *n_active, int32_t block, void *stream);
 
 /* Split <-> FP64 index conversion.  opx_split_from_f64 writes the state of
  * particle i from idx arrays (and status/rel untouched); opx_split_to_f64

@@ -122,14 +122,19 @@ class opx_release_clock(ctypes.Structure):
 
 class opx_release_step(ctypes.Structure):
     _fields_ = [("ft", ctypes.c_float), ("gt", ctypes.c_float), ("ft1", ctypes.c_float), ("gt1", ctypes.c_float),
+                ("ftm", ctypes.c_float), ("gtm", ctypes.c_float),
                 ("s0", ctypes.c_int32), ("s1", ctypes.c_int32), ("s0p", ctypes.c_int32), ("s1p", ctypes.c_int32),
-                ("active", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("s0m", ctypes.c_int32), ("s1m", ctypes.c_int32), ("active", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 _P = ctypes.POINTER
 _SIGNATURES.update({
     "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
-                          ctypes.c_double, _vp, _vp],
+                          ctypes.c_double, ctypes.c_double, _vp, _vp],
+    "opx_rk2_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
+                          ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32, _vp],
+    "opx_rk2_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, ctypes.c_double, _vp,
+                      ctypes.c_int32, _vp],
     "opx_ab3_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,

@@ -306,6 +306,15 @@ __device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
 
 // ---------------------------------------------------------------- clock
 
+// a + b * c with both operations rounded (no FMA): the reference's
+// intermediate time `time + alpha * dt` (timesteppers.py:250).
+__device__ __forceinline__ double add_mul_rn(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+
+
 // Timestepper.get_time_index (timesteppers.py:64-85): false where the
 // reference raises (time outside the array, or t == t_last).
 __device__ bool time_index(const double *__restrict__ ta, int n, double t, double &tidx)
@@ -343,7 +352,7 @@ __device__ __forceinline__ void weights(double tidx, int nslots, float &ft, floa
 
 __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int nrel, const int32_t *nrel_dev,
                         const double *ta, int ntime, int nslots, int step, const int32_t *step_dev, double dt,
-                        int32_t *err)
+                        double alpha, int32_t *err)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (nrel_dev)
@@ -356,11 +365,16 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
     opx_release_step e{};
     e.active = 0;
     if (c.steps_left > 0 && step >= c.start_step) {
-        double tidx0, tidx1;
+        double tidx0, tidx1, tidxm = 0.0;
         double t1 = c.t + dt; // advance_time: time += dt (timesteppers.py:159)
-        if (time_index(ta, ntime, c.t, tidx0) && time_index(ta, ntime, t1, tidx1)) {
+        bool ok = time_index(ta, ntime, c.t, tidx0) && time_index(ta, ntime, t1, tidx1);
+        if (alpha > 0.0) // RK2 intermediate time (timesteppers.py:250-251)
+            ok = ok && time_index(ta, ntime, add_mul_rn(c.t, alpha, dt), tidxm);
+        if (ok) {
             weights(tidx0, nslots, e.ft, e.gt, e.s0, e.s1);
             weights(tidx1, nslots, e.ft1, e.gt1, e.s0p, e.s1p);
+            if (alpha > 0.0)
+                weights(tidxm, nslots, e.ftm, e.gtm, e.s0m, e.s1m);
             e.active = 1;
             c.t = t1;
             c.steps_left -= 1;
@@ -679,6 +693,235 @@ __global__ void k_split_to_f64(opx_split_state s, opx_grid g, double *zidx, doub
     xidx[o] = x.ok ? x.pos() : nan;
 }
 
+// ---------------------------------------------------------------- fused RK2
+
+// `(Py_ssize_t)p` clamped to [0, hi] (linear.pxd:3-9); NaN and values outside
+// the int64 range convert to INT64_MIN on x86-64, i.e. clamp to 0.
+__device__ __forceinline__ int trunc_clamp(double p, int hi)
+{
+    if (!(p > -9223372036854775808.0 && p < 9223372036854775808.0))
+        return 0;
+    const long long i = (long long)p;
+    return i < 0 ? 0 : (i > hi ? hi : (int)i);
+}
+
+// Interpolation axis of an FP64 index: base truncate(idx + off) clamped to
+// [0, size-2] and the unclamped FP64 weight of the upper node.
+struct AxD {
+    int b;
+    double w;
+};
+
+__device__ __forceinline__ AxD axd(double idx, double off, int size)
+{
+    const double p = idx + off;
+    const int b = trunc_clamp(p, size - 2);
+    return {b, p - (double)b};
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ int packed_index(const opx_window &win, int s0, int s1)
+{
+    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        return -1;
+    } else {
+        if (win.npacked > 0 && win.pslot[0][0] == s0 && win.pslot[0][1] == s1)
+            return 0;
+        if (win.npacked > 1 && win.pslot[1][0] == s0 && win.pslot[1][1] == s1)
+            return 1;
+        return -1;
+    }
+}
+
+// u, v (FP32 trilinear) and w (FP64 weights) blended at slots (s0, s1) at an
+// FP64 position (rk2_w_advection.pyx:124-150).
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void sample_uvw(const opx_window &win, const opx_grid &g, int s0, int s1, float gt,
+                                           float ft, double zi, double yi, double xi, float &u, float &v, double &w)
+{
+    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const AxD cz = axd(zi, 0.0, NZ), cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
+    const AxD ix = axd(xi, -0.5, NX - 1), iy = axd(yi, -0.5, NY - 1), oz = axd(zi, 0.5, NZ + 1);
+    float vu[8], vv[8], vw[8];
+    const int j = packed_index<LAYOUT>(win, s0, s1);
+    if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
+        corners<T, LAYOUT>(j ? win.pu[1] : win.pu[0], nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ix.b, vu);
+        corners<T, LAYOUT>(j ? win.pv[1] : win.pv[0], nullptr, gt, ft, NY - 1, NX, cz.b, iy.b, cx.b, vv);
+        corners<T, LAYOUT>(j ? win.pw[1] : win.pw[0], nullptr, gt, ft, NY, NX, oz.b, cy.b, cx.b, vw);
+    } else {
+        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
+        const T *W = static_cast<const T *>(win.w);
+        corners<T, OPX_LAYOUT_RING>(U + s0 * win.u_stride, U + s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b, cy.b,
+                                    ix.b, vu);
+        corners<T, OPX_LAYOUT_RING>(V + s0 * win.v_stride, V + s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b, iy.b,
+                                    cx.b, vv);
+        corners<T, OPX_LAYOUT_RING>(W + s0 * win.w_stride, W + s1 * win.w_stride, gt, ft, NY, NX, oz.b, cy.b, cx.b,
+                                    vw);
+    }
+    u = tri32(vu, (float)cz.w, (float)cy.w, (float)ix.w);
+    v = tri32(vv, (float)cz.w, (float)iy.w, (float)cx.w);
+    w = tri64(vw, oz.w, cy.w, cx.w);
+}
+
+// h and zeta (FP64 weights; zeta blended in FP32) and, if asked, dx and dy
+// (FP32) at an FP64 horizontal position.
+template <int LAYOUT>
+__device__ __forceinline__ void sample_2d(const opx_window &win, int s0, int s1, float gt, float ft, int NY, int NX,
+                                          double yi, double xi, bool want_dxdy, double &h, double &zeta, float &dxv,
+                                          float &dyv)
+{
+    const AxD cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
+    Corners2 c2;
+    const int j = packed_index<LAYOUT>(win, s0, s1);
+    if (LAYOUT != OPX_LAYOUT_RING && j >= 0)
+        load2_packed(static_cast<const float4 *>(j ? win.p2[1] : win.p2[0]), win.dy, NX, cy.b, cx.b, want_dxdy, c2);
+    else
+        load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
+                   win.zeta + s1 * win.zeta_stride, NX, cy.b, cx.b, want_dxdy, c2);
+    eval2(c2, gt, ft, cy.w, cx.w, h, zeta);
+    if (want_dxdy) {
+        dxv = bil32(c2.dx, (float)cy.w, (float)cx.w);
+        dyv = bil32(c2.dy, (float)cy.w, (float)cx.w);
+    }
+}
+
+// zidx of depth z (compute_zidx, _vertical_coordinate.pxd:41-103): bracket
+// S_k <= S < S_{k+1} on the w-levels clamped to [0, NZ-1], found by a walk
+// from level k0 (S_k increases with k), then the linear inverse.
+__device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, double hc, int NZ, const double *Cw,
+                                              int k0)
+{
+    const double S = (z - zeta) / (zeta + h);
+    const double inv = 1.0 / (hc + h);
+    auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h * Cw[k]) * inv; };
+    int k = max(0, min(k0, NZ - 1));
+    double Sk = S_at(k);
+    while (k > 0 && S < Sk) {
+        --k;
+        Sk = S_at(k);
+    }
+    double Sk1 = S_at(k + 1);
+    while (k < NZ - 1 && S >= Sk1) {
+        ++k;
+        Sk = Sk1;
+        Sk1 = S_at(k + 1);
+    }
+    return ((double)k - 0.5) + (S - Sk) / (Sk1 - Sk);
+}
+
+// One particle's fused RK2 step: validation at t, stage 1 at (x_n, t), stage 2
+// at (x*, t + alpha dt), update and zidx at t + dt.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                        const opx_window &win, const opx_release_step *__restrict__ table, double dt,
+                                        double alpha, const double *Cw)
+{
+    if (i >= n)
+        return;
+    const uint8_t st = s.status[i];
+    if (st & OPX_INACTIVE)
+        return;
+    const int r = s.rel ? (int)s.rel[i] : 0;
+    const opx_release_step rs = table[r];
+    if (!rs.active)
+        return;
+
+    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    Axis az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+    Axis ay = decode_axis(cell, g.xbits, g.ybits);
+    Axis ax = decode_axis(cell, 0, g.xbits);
+
+    // ---- validation (validation.pyx:21-77), the RK2 pre-step kernel
+    if (!(az.ok && ay.ok && ax.ok)) {
+        s.status[i] = OPX_NONFINITE;
+        return;
+    }
+    az.q = s.fz[i];
+    ay.q = s.fy[i];
+    ax.q = s.fx[i];
+    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+    if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
+          xi <= g.xidx_max)) {
+        s.status[i] = OPX_OUT_OF_DOMAIN;
+        return;
+    }
+    const double hc = g.hc;
+
+    // ---- stage 1 (rk2_w_advection.pyx:31-153): z at x_n and k1 = (u/dx, v/dy, w)
+    double h0, zeta0;
+    float dxv, dyv;
+    sample_2d<LAYOUT>(win, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, true, h0, zeta0, dxv, dyv);
+    const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
+    const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
+    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
+    const double z = zeta0 + (zeta0 + h0) * S0;
+    float u1, v1;
+    double w1;
+    sample_uvw<T, LAYOUT>(win, g, rs.s0, rs.s1, rs.gt, rs.ft, zi, yi, xi, u1, v1, w1);
+    const double kx1 = (double)(u1 / dxv), ky1 = (double)(v1 / dyv);
+
+    // ---- stage 2 (:155-275) at x* = x_n + alpha dt k1, time t + alpha dt
+    const double adt = alpha * dt;
+    const double zs = z + adt * w1, ys = yi + adt * ky1, xs = xi + adt * kx1;
+    double hs, zetas;
+    float dxs, dys;
+    sample_2d<LAYOUT>(win, rs.s0m, rs.s1m, rs.gtm, rs.ftm, NY, NX, ys, xs, true, hs, zetas, dxs, dys);
+    const double zis = zidx_from_z(zs, hs, zetas, hc, NZ, Cw, oz.b);
+    float u2, v2;
+    double w2;
+    sample_uvw<T, LAYOUT>(win, g, rs.s0m, rs.s1m, rs.gtm, rs.ftm, zis, ys, xs, u2, v2, w2);
+    const double kx2 = (double)(u2 / dxs), ky2 = (double)(v2 / dys);
+
+    // ---- update (:278-354): b1 = 1/(2 alpha) on stage 1, b2 = 1 - b1 on stage 2
+    const double b1 = 1.0 / (2.0 * alpha), b2 = 1.0 - b1;
+    const double zn = z + b1 * dt * w1 + b2 * dt * w2;
+    const double yn = yi + b1 * dt * ky1 + b2 * dt * ky2;
+    const double xn = xi + b1 * dt * kx1 + b2 * dt * kx2;
+    ay = encode_axis(yn, NY);
+    ax = encode_axis(xn, NX);
+    Axis azn;
+    if (ay.ok && ax.ok) {
+        double h1, zeta1;
+        float unused0 = 0.f, unused1 = 0.f;
+        sample_2d<LAYOUT>(win, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, ay.pos(), ax.pos(), false, h1, zeta1, unused0,
+                          unused1);
+        int k0 = (int)floor(zis + 0.5);
+        azn = encode_axis(zidx_from_z(zn, h1, zeta1, hc, NZ, Cw, k0), NZ);
+    } else {
+        azn = encode_axis(__builtin_nan(""), NZ);
+    }
+    s.cell[i] = (int32_t)pack_cell(azn, ay, ax, g);
+    s.fz[i] = azn.q;
+    s.fy[i] = ay.q;
+    s.fx[i] = ax.q;
+    if (s.z)
+        s.z[i] = (float)zn;
+}
+
+template <typename T, int LAYOUT, int BLOCK, bool IND>
+__global__ void __launch_bounds__(BLOCK)
+    k_rk2_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
+                const opx_release_step *__restrict__ table, double dt, double alpha,
+                const int32_t *__restrict__ n_active)
+{
+    const opx_window win = IND ? *win_dev : win_arg;
+    __shared__ double Cw[kMaxC];
+    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
+        Cw[k] = win.C[k];
+    __syncthreads();
+
+    int64_t n = s.n;
+    if (n_active)
+        n = min(n, (int64_t)*n_active);
+    const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+    const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+    const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
+    if (c < c1)
+        rk2_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, alpha, Cw);
+}
+
 // ---------------------------------------------------------------- re-binning
 
 __device__ __forceinline__ uint32_t spatial_key(uint32_t x, uint32_t y, uint32_t z, int xbits, int ybits,
@@ -808,14 +1051,15 @@ __global__ void k_zero_int(int32_t *p) { *p = 0; }
 
 // ---------------------------------------------------------------- host checks
 
-int check_state(const opx_split_state *s, const char *what)
+int check_state(const opx_split_state *s, const char *what, bool need_hist = true)
 {
     OPX_REQUIRE(s != nullptr, "%s: null state", what);
     OPX_REQUIRE(s->n >= 0 && s->n <= INT32_MAX, "%s: particle count %lld out of range", what, (long long)s->n);
     if (s->n == 0)
         return OPX_OK;
     OPX_REQUIRE(s->status && s->cell && s->fz && s->fy && s->fx, "%s: null status/cell/offset arrays", what);
-    OPX_REQUIRE(s->hz1 && s->hy1 && s->hx1 && s->hz2 && s->hy2 && s->hx2, "%s: null history arrays", what);
+    OPX_REQUIRE(!need_hist || (s->hz1 && s->hy1 && s->hx1 && s->hz2 && s->hy2 && s->hx2), "%s: null history arrays",
+                what);
     return OPX_OK;
 }
 
@@ -899,6 +1143,34 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
     return check_launch("opx_ab3_fused");
 }
 
+int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
+               int layout, int dtype3, const opx_release_step *table, double dt, double alpha,
+               const int32_t *n_active, int block, void *stream)
+{
+    const dim3 grid(grid_xcd(s->n, block)), blk(block);
+    hipStream_t st = as_stream(stream);
+    const bool ind = win_dev != nullptr;
+#define OPX_RK2_LAUNCH(T, L, B, I)                                                                         \
+    if (block == B && layout == L && ind == I)                                                            \
+        hipLaunchKernelGGL((k_rk2_fused<T, L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt, alpha, \
+                           n_active);
+#define OPX_RK2_LAUNCH_L(T, L)                                                                             \
+    OPX_RK2_LAUNCH(T, L, 128, false) OPX_RK2_LAUNCH(T, L, 256, false) OPX_RK2_LAUNCH(T, L, 128, true)     \
+        OPX_RK2_LAUNCH(T, L, 256, true)
+    if (dtype3 == OPX_F32) {
+        OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_RING)
+        OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_PAIR)
+        OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_QUAD)
+    } else {
+        OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
+        OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
+        OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
+    }
+#undef OPX_RK2_LAUNCH_L
+#undef OPX_RK2_LAUNCH
+    return check_launch("opx_rk2_fused");
+}
+
 } // namespace
 } // namespace opx
 
@@ -908,9 +1180,10 @@ extern "C" {
 
 int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
                       const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
-                      int32_t *step_dev, double dt, int32_t *err_flag, void *stream)
+                      int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream)
 {
     OPX_REQUIRE(nrel >= 0, "negative release count");
+    OPX_REQUIRE(alpha >= 0.0, "alpha must be >= 0");
     if (nrel == 0 && step_dev == nullptr)
         return OPX_OK;
     OPX_REQUIRE(nrel == 0 || (clocks && table), "null clock/table");
@@ -920,7 +1193,7 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
     hipStream_t st = as_stream(stream);
     if (nrel > 0)
         hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, st, clocks, table, nrel, nrel_dev,
-                           time_array, ntime, nslots, global_step, step_dev, dt, err_flag);
+                           time_array, ntime, nslots, global_step, step_dev, dt, alpha, err_flag);
     if (step_dev)
         hipLaunchKernelGGL(k_step_incr, dim3(1), dim3(1), 0, st, step_dev);
     return check_launch("opx_clock_advance");
@@ -960,6 +1233,45 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
         return OPX_OK;
     opx_window unused{};
     return launch_fused(s, g, unused, win_dev, layout, dtype3, table, dt, n_active, block, stream);
+}
+
+int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
+                  void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_rk2_fused", false))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    if ((rc = check_window(win, g))) return rc;
+    OPX_REQUIRE(table != nullptr, "null release table");
+    OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
+    if (s->n == 0)
+        return OPX_OK;
+    if (block == 0)
+        block = 128;
+    OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
+    const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
+    return launch_rk2(s, g, *win, nullptr, layout, win->dtype3, table, dt, alpha, n_active, block, stream);
+}
+
+int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, double alpha,
+                      const int32_t *n_active, int32_t block, void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_rk2_fused_dev", false))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
+    OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_QUAD, "unknown layout %d", layout);
+    OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
+    if (block == 0)
+        block = 128;
+    OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
+    if (s->n == 0)
+        return OPX_OK;
+    opx_window unused{};
+    return launch_rk2(s, g, unused, win_dev, layout, dtype3, table, dt, alpha, n_active, block, stream);
 }
 
 static int elem_bytes(int dtype3, int layout)

@@ -1,8 +1,10 @@
-"""Fused AB3 advection engine: the fast form of the hot path.
+"""Fused advection engine: the fast form of the hot path.
 
-``AB3Engine`` advances particles with one fused HIP kernel per AB3 step
+``AB3Engine`` advances particles with one fused HIP kernel per step: AB3
 (``opx_ab3_fused``: validation -> ab3_w_advection -> ab3_post_w_advection,
-the reference's timesteppers.py:296-305 sequence) over
+the reference's timesteppers.py:296-305 sequence) or, with ``scheme="rk2"``,
+RK2 (``opx_rk2_fused``: validation -> _rk2_step_1 -> _rk2_step_2 ->
+_rk2_update, timesteppers.py:241-257), over
 
 - split positions (packed int32 cell + 32-bit fixed-point offsets, exact in
   FP64) and FP32 AB3 history;
@@ -116,11 +118,18 @@ class AB3Engine:
 
     def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
                  capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False,
-                 max_releases: int = 256, stream_slots: int | None = None):
+                 max_releases: int = 256, stream_slots: int | None = None, scheme: str = "ab3",
+                 alpha: float = 2 / 3):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if storage_dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
         self.storage_dtype = storage_dtype
+        if scheme not in ("ab3", "rk2"):
+            raise ValueError(f"scheme must be 'ab3' or 'rk2', not {scheme!r}")
+        self.scheme = scheme
+        self.alpha = float(alpha)
+        if scheme == "rk2" and not self.alpha > 0:
+            raise ValueError("RK2 needs alpha > 0")
         self.dt = float(np.float64(dt))
         self.time_array = np.asarray(time_array, dtype=np.float64)
         if np.any(self.time_array[1:] <= self.time_array[:-1]):
@@ -290,14 +299,16 @@ class AB3Engine:
         self._graphs = {}
 
     def add_release(self, zidx, yidx, xidx, *, start_time=None, nsteps: int, start_step: int = 0,
-                    status: int = ParticleStatus.MULTISTEP_1) -> int:
+                    status: int | None = None) -> int:
         """Add a group of particles (a release); returns its id.
 
         The release's clock starts at ``start_time`` (default time_array[0])
         and it advances at global steps ``start_step`` .. while its
-        ``nsteps`` steps last.  Particles start with ``status``
-        (MULTISTEP_1: Euler -> AB2 -> AB3 start-up).
+        ``nsteps`` steps last.  Particles start with ``status`` (default
+        MULTISTEP_1 for AB3: Euler -> AB2 -> AB3 start-up; NORMAL for RK2).
         """
+        if status is None:
+            status = ParticleStatus.MULTISTEP_1 if self.scheme == "ab3" else ParticleStatus.NORMAL
         r = self._free_ids.pop() if self._free_ids else len(self._clocks_host)
         if r >= 1 << 15:
             raise ValueError("too many releases")
@@ -464,11 +475,17 @@ class AB3Engine:
         i0, ok0 = self._time_index(t0)
         i1, ok1 = self._time_index(t1)
         ok = ok0 & ok1
+        parts = [i0, i1]
+        if self.scheme == "rk2":
+            # intermediate time of stage 2: time + alpha * dt (timesteppers.py:250)
+            im, okm = self._time_index(t0 + self.alpha * self.dt)
+            ok &= okm
+            parts.append(im)
         idx = np.nonzero(act)[0]
         self._t[idx[ok]] = t1[ok]
         self._left[idx[ok]] -= 1
         self._left[idx[~ok]] = 0
-        its = np.unique(np.concatenate([i0[ok], i1[ok]]))
+        its = np.unique(np.concatenate([p[ok] for p in parts]))
         return {(int(i), int(i) + 1) for i in its}
 
     # ---------------------------------------------------------------- streamed snapshots
@@ -569,13 +586,19 @@ class AB3Engine:
 
     def _launch_step(self, stream):
         """One step on `stream`: device clocks, then the fused kernel (both graph-capturable)."""
+        rk2 = self.scheme == "rk2"
         _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap,
                   self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots, 0,
-                  self._step_dev.data_ptr(), self.dt, self._err.data_ptr(), stream)
+                  self._step_dev.data_ptr(), self.dt, self.alpha if rk2 else 0.0, self._err.data_ptr(), stream)
         s = self._launch_structs()
-        _lib.call("opx_ab3_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
-                  self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self._n_active_dev.data_ptr(),
-                  self.block, stream)
+        if rk2:
+            _lib.call("opx_rk2_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
+                      self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self.alpha,
+                      self._n_active_dev.data_ptr(), self.block, stream)
+        else:
+            _lib.call("opx_ab3_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
+                      self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self._n_active_dev.data_ptr(),
+                      self.block, stream)
 
     def _graph(self, k):
         """A captured graph of k steps for the current buffers and launch shape."""

@@ -19,11 +19,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _engine(wl, device, *, storage=torch.float32, rebin_every=25, xlow=2, nsteps=100, positions=None,
-            capacity=None):
+            capacity=None, scheme="ab3"):
     fs = build_fieldset(wl, device=device)
     pos = wl.positions if positions is None else positions
     eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
-                    capacity=capacity or max(1, len(pos[0])), rebin_every=rebin_every, xlow=xlow)
+                    capacity=capacity or max(1, len(pos[0])), rebin_every=rebin_every, xlow=xlow, scheme=scheme)
     eng.add_release(*pos, nsteps=nsteps)
     return eng
 
@@ -164,3 +164,51 @@ def test_graph_replays_match_eager(device):
         for k in ("status",) + POS:
             assert np.array_equal(a[k], b[k], equal_nan=True), (chunk, k)
     assert len(engines[1]._graphs) > 0
+
+
+# ---------------------------------------------------------------- fused RK2
+
+def test_fused_rk2_fp32_vs_oracle_100_steps(device):
+    wl = _turb()
+    sim_o = run_oracle(wl, "rk2", 100)
+    eng = _engine(wl, device, scheme="rk2")
+    eng.step(100)
+    eng.check_clock()
+    g = _state(eng)
+    assert np.array_equal(g["status"], sim_o.p["status"]), "status differs"
+    devs = {k: max_dev(g[k], sim_o.p[k]) for k in POS}
+    print("fused rk2 fp32 max deviation vs oracle after 100 steps:", devs, flush=True)
+    assert max(devs.values()) <= 1e-5
+
+
+def test_fused_rk2_eddy_vs_oracle(device):
+    """RK2 on the eddy case (dt = 300 s, snapshots 3600 s apart: stage 2 and
+    the update cross snapshot boundaries)."""
+    wl = eddy_workload(steps=100, dtype=np.float32)
+    sim_o = run_oracle(wl, "rk2", 100)
+    eng = _engine(wl, device, scheme="rk2")
+    eng.step(100)
+    g = _state(eng)
+    assert np.array_equal(g["status"], sim_o.p["status"])
+    devs = {k: max_dev(g[k], sim_o.p[k]) for k in POS}
+    print("fused rk2 eddy max deviation:", devs, flush=True)
+    assert max(devs.values()) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
+def test_fused_rk2_layouts_and_graphs_identical(device, storage):
+    """RK2: RING/PAIR/QUAD layouts, re-binning and graph replays give identical bits."""
+    from offline_particles_amd import _lib
+
+    wl = _turb()
+    res = []
+    for layout, graph_steps, rebin in ((_lib.OPX_LAYOUT_RING, 0, 0), (_lib.OPX_LAYOUT_PAIR, 5, 11),
+                                       (_lib.OPX_LAYOUT_QUAD, 8, 7)):
+        eng = _engine(wl, device, storage=storage, rebin_every=rebin, scheme="rk2")
+        eng.set_layout(layout)
+        eng.graph_steps = graph_steps
+        eng.step(70)
+        res.append(_state(eng))
+    for other in res[1:]:
+        for k in ("status",) + POS:
+            assert np.array_equal(res[0][k], other[k], equal_nan=True), k
