@@ -140,6 +140,27 @@ int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const
                              void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Diagnostics at particle positions (FP64, blended field data as in the */
+/* entry points above; particles with the INACTIVE bit are skipped).   */
+/* ------------------------------------------------------------------ */
+
+/* compute_z_kernel (kernels/roms/_roms_core.pyx:17-112): out[i] = depth z of
+ * (zidx, yidx, xidx)[i] from h and zeta (2-D, y/x) and C (1-D, z). */
+int opx_compute_z(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,
+                  const opx_field *h, const opx_field *zeta, const opx_field *C, double hc, int32_t NZ, double *out,
+                  void *stream);
+
+/* Output interpolation kernels (kernels/output/linear_interpolation.pyx:34-145):
+ * linear over one particle index, bilinear over two (the field's active
+ * dims in order), trilinear over (zidx, yidx, xidx). */
+int opx_interp_linear(int64_t n, const uint8_t *status, const double *idx, const opx_field *f, double *out,
+                      void *stream);
+int opx_interp_bilinear(int64_t n, const uint8_t *status, const double *idx0, const double *idx1,
+                        const opx_field *f, double *out, void *stream);
+int opx_interp_trilinear(int64_t n, const uint8_t *status, const double *zidx, const double *yidx,
+                         const double *xidx, const opx_field *f, double *out, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Fused form: one kernel per AB3 step over split positions.           */
 /*                                                                     */
 /* The reference's AB3 step is three launches, each with a full-grid   */

@@ -212,6 +212,53 @@ __global__ void __launch_bounds__(kBlock) k_ab3_post(opx_particles p, opx_scalar
     p.zidx[i] = compute_zidx(p.z[i], h_value, zeta_value, s.hc, s.NZ, f.C);
 }
 
+// ---------------- diagnostics: compute_z and output interpolation ----------------
+
+// compute_z_kernel (kernels/roms/_roms_core.pyx:17-75)
+__global__ void __launch_bounds__(kBlock)
+    k_compute_z(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,
+                opx_field h, opx_field zeta, opx_field C, double hc, int NZ, double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (status[i] & OPX_INACTIVE))
+        return;
+    double h_value = bilinear_f64(h, yidx[i], xidx[i]);
+    double zeta_value = bilinear_f64(zeta, yidx[i], xidx[i]);
+    double C_value = linear_f64(C, zidx[i]);
+    out[i] = compute_z(zidx[i], NZ, hc, h_value, C_value, zeta_value);
+}
+
+// _linear/_bilinear/_trilinear_interpolation_kernel_function
+// (kernels/output/linear_interpolation.pyx:34-145)
+__global__ void __launch_bounds__(kBlock)
+    k_interp_linear(int64_t n, const uint8_t *status, const double *idx, opx_field f, double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (status[i] & OPX_INACTIVE))
+        return;
+    out[i] = linear_f64(f, idx[i]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_interp_bilinear(int64_t n, const uint8_t *status, const double *idx0, const double *idx1, opx_field f,
+                      double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (status[i] & OPX_INACTIVE))
+        return;
+    out[i] = bilinear_f64(f, idx0[i], idx1[i]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_interp_trilinear(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,
+                       opx_field f, double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (status[i] & OPX_INACTIVE))
+        return;
+    out[i] = trilinear_f64(f, zidx[i], yidx[i], xidx[i]);
+}
+
 // ---------------- argument checks ----------------
 
 int check_particles(const opx_particles *p, bool positions_only, bool ab3, bool rk2_stage2)
@@ -329,6 +376,66 @@ int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const
     if (p->n == 0) return OPX_OK;
     hipLaunchKernelGGL(k_ab3_post, dim3(grid_for(p->n)), dim3(kBlock), 0, as_stream(stream), *p, *s, *f);
     return check_launch("opx_ab3_post_w_advection");
+}
+
+int opx_compute_z(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,
+                  const opx_field *h, const opx_field *zeta, const opx_field *C, double hc, int32_t NZ, double *out,
+                  void *stream)
+{
+    OPX_REQUIRE(n >= 0, "negative particle count %lld", (long long)n);
+    OPX_REQUIRE(h && zeta && C, "null field");
+    int rc;
+    if ((rc = check_field(*h, 2, "h", true))) return rc;
+    if ((rc = check_field(*zeta, 2, "zeta", true))) return rc;
+    if ((rc = check_field(*C, 1, "C", true))) return rc;
+    OPX_REQUIRE(NZ > 0, "NZ must be positive");
+    if (n == 0) return OPX_OK;
+    OPX_REQUIRE(status && zidx && yidx && xidx && out, "null particle array");
+    hipLaunchKernelGGL(k_compute_z, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, status, zidx, yidx,
+                       xidx, *h, *zeta, *C, hc, (int)NZ, out);
+    return check_launch("opx_compute_z");
+}
+
+int opx_interp_linear(int64_t n, const uint8_t *status, const double *idx, const opx_field *f, double *out,
+                      void *stream)
+{
+    OPX_REQUIRE(n >= 0, "negative particle count %lld", (long long)n);
+    OPX_REQUIRE(f != nullptr, "null field");
+    int rc;
+    if ((rc = check_field(*f, 1, "field", true))) return rc;
+    if (n == 0) return OPX_OK;
+    OPX_REQUIRE(status && idx && out, "null particle array");
+    hipLaunchKernelGGL(k_interp_linear, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, status, idx, *f,
+                       out);
+    return check_launch("opx_interp_linear");
+}
+
+int opx_interp_bilinear(int64_t n, const uint8_t *status, const double *idx0, const double *idx1,
+                        const opx_field *f, double *out, void *stream)
+{
+    OPX_REQUIRE(n >= 0, "negative particle count %lld", (long long)n);
+    OPX_REQUIRE(f != nullptr, "null field");
+    int rc;
+    if ((rc = check_field(*f, 2, "field", true))) return rc;
+    if (n == 0) return OPX_OK;
+    OPX_REQUIRE(status && idx0 && idx1 && out, "null particle array");
+    hipLaunchKernelGGL(k_interp_bilinear, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, status, idx0,
+                       idx1, *f, out);
+    return check_launch("opx_interp_bilinear");
+}
+
+int opx_interp_trilinear(int64_t n, const uint8_t *status, const double *zidx, const double *yidx,
+                         const double *xidx, const opx_field *f, double *out, void *stream)
+{
+    OPX_REQUIRE(n >= 0, "negative particle count %lld", (long long)n);
+    OPX_REQUIRE(f != nullptr, "null field");
+    int rc;
+    if ((rc = check_field(*f, 3, "field", true))) return rc;
+    if (n == 0) return OPX_OK;
+    OPX_REQUIRE(status && zidx && yidx && xidx && out, "null particle array");
+    hipLaunchKernelGGL(k_interp_trilinear, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), n, status, zidx,
+                       yidx, xidx, *f, out);
+    return check_launch("opx_interp_trilinear");
 }
 
 } // extern "C"

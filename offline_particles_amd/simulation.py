@@ -4,13 +4,15 @@
 ``set_indices`` / ``set_particle_field`` write the device particle arrays.
 Events and schedulers are the reference's minimal host-side plumbing
 (events/_events.py, events/_schedulers.py) so that event kernels launch at the
-same iterations/times; output writers (zarr) are not part of this package.
+same iterations/times.  Output writers (``add_output_writer``) turn their
+outputs into events as the reference's builder does (simulation.py:316-367).
 """
 
 from __future__ import annotations
 
 import dataclasses
 import itertools
+import types
 from typing import Callable
 
 import numpy as np
@@ -124,6 +126,7 @@ class Simulation:
         self._fieldset = fieldset
         self._iteration_scheduler = iteration_scheduler or IterationScheduler()
         self._time_scheduler = time_scheduler or TimeScheduler()
+        self._output_writers = types.MappingProxyType(dict(output_writers or {}))
         self._launcher = Launcher(fieldset)
         self._launcher.maybe_increase_index_padding(timestepper.index_padding)
         self._launcher.register_scalar_data_sources_from_object(self._timestepper)
@@ -147,6 +150,7 @@ class Simulation:
     particles = property(lambda self: self._particles_view)
     iteration_scheduler = property(lambda self: self._iteration_scheduler)
     time_scheduler = property(lambda self: self._time_scheduler)
+    output_writers = property(lambda self: self._output_writers)
 
     @property
     def state(self) -> SimulationState:
@@ -189,13 +193,14 @@ class Simulation:
 
 
 class SimulationBuilder:
-    """Builds a Simulation with events (simulation.py:232-367; output writers omitted)."""
+    """Builds a Simulation with events and output writers (simulation.py:232-367)."""
 
     def __init__(self, timestepper, fieldset) -> None:
         self._timestepper = timestepper
         self._fieldset = fieldset
         self._iteration_scheduler = IterationScheduler()
         self._time_scheduler = TimeScheduler()
+        self._output_writers: dict = {}
 
     def every_n(self, n: int, event: Event, *, first: int | None = None) -> None:
         if n <= 0:
@@ -222,6 +227,19 @@ class SimulationBuilder:
         else:
             raise ValueError("Either n or dt must be specified.")
 
+    def add_output_writer(self, builder, *, n=None, dt=None, first=None) -> None:
+        """Add an output writer; its events run every n iterations or every dt."""
+        name = builder.name
+        if name in self._output_writers:
+            raise ValueError(f"Output writer '{name}' already exists.")
+        self._output_writers[name] = (builder, {"n": n, "dt": dt, "first": first})
+
     def build_simulation(self, nparticles: int, *, device=None) -> Simulation:
+        writers = {}
+        time_type = self._timestepper.time_array.dtype
+        for name, (builder, kwargs) in self._output_writers.items():
+            writers[name] = builder.build(nparticles, time_type)
+            for event in writers[name].create_events():
+                self.add_event(event, **kwargs)
         return Simulation(nparticles, self._timestepper, self._fieldset, self._iteration_scheduler,
-                          self._time_scheduler, {}, device=device)
+                          self._time_scheduler, writers, device=device)

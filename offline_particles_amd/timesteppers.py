@@ -43,6 +43,10 @@ class Timestepper(abc.ABC):
         self._index_padding = 0
         self.set_index_padding(index_padding)
 
+    @property
+    def time_array(self) -> np.ndarray:
+        return self._time_array
+
     def get_time_index(self, time) -> np.float64:
         """Fractional snapshot index of ``time`` (timesteppers.py:64-85)."""
         time_array = self._time_array
