@@ -139,6 +139,17 @@ int opx_ab3_w_advection(const opx_particles *p, const opx_scalars *s, const opx_
 int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const opx_roms_fields *f,
                              void *stream);
 
+/* kernels/roms/ab3_isopycnal_following.pyx:26-173 (_ab3_isopycnal_following):
+ * the AB3 step of opx_ab3_w_advection with the vertical rate
+ * dz0 = w_max * drho / (rho_scale + |drho|), drho = rho(x) - rho_t[i], where
+ * `rho` is a 3-D (blended) density field and rho_t the per-particle target
+ * density (the particle field "rho_t"); f->w is not read.  Its post step
+ * (_ab3_post_isopycnal_following, :175-229) is the same computation as
+ * opx_ab3_post_w_advection. */
+int opx_ab3_isopycnal_following(const opx_particles *p, const double *rho_t, const opx_scalars *s,
+                                const opx_roms_fields *f, const opx_field *rho, double w_max, double rho_scale,
+                                void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Diagnostics at particle positions (FP64, blended field data as in the */
 /* entry points above; particles with the INACTIVE bit are skipped).   */

@@ -133,6 +133,8 @@ _SIGNATURES.update({
                           ctypes.c_double, ctypes.c_double, _vp, _vp],
     "opx_rk2_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32, _vp],
+    "opx_ab3_isopycnal_following": [_P(opx_particles), _vp, _P(opx_scalars), _P(opx_roms_fields), _P(opx_field),
+                                    ctypes.c_double, ctypes.c_double, _vp],
     "opx_compute_z": [ctypes.c_int64, _vp, _vp, _vp, _vp, _P(opx_field), _P(opx_field), _P(opx_field),
                       ctypes.c_double, ctypes.c_int32, _vp, _vp],
     "opx_interp_linear": [ctypes.c_int64, _vp, _vp, _P(opx_field), _vp, _vp],

@@ -142,9 +142,18 @@ __global__ void __launch_bounds__(kBlock) k_rk2_update(opx_particles p, opx_scal
 
 // ---------------- AB3 (ab3_w_advection.pyx) ----------------
 
-__global__ void __launch_bounds__(kBlock) k_ab3(opx_particles p, opx_scalars s, opx_roms_fields f)
+// ISO: the vertical rate is dz0 = w_max * drho / (rho_scale + |drho|) with
+// drho = rho(x) - rho_t (ab3_isopycnal_following.pyx:126-134) instead of w.
+struct IsoArgs {
+    opx_field rho;
+    const double *rho_t;
+    double w_max, rho_scale;
+};
+
+template <bool ISO>
+__device__ __forceinline__ void ab3_particle(int64_t i, const opx_particles &p, const opx_scalars &s,
+                                             const opx_roms_fields &f, const IsoArgs &iso)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n)
         return;
     uint8_t st = p.status[i];
@@ -159,7 +168,14 @@ __global__ void __launch_bounds__(kBlock) k_ab3(opx_particles p, opx_scalars s, 
     double v_value = trilinear_f64(f.v, zi, yi, xi);
     double dy_value = bilinear_f64(f.dy, yi, xi);
     double dy0 = v_value / dy_value;
-    double dz0 = trilinear_f64(f.w, zi, yi, xi);
+    double dz0;
+    if constexpr (ISO) {
+        double rho_value = trilinear_f64(iso.rho, zi, yi, xi);
+        double delta_rho = rho_value - iso.rho_t[i];
+        dz0 = iso.w_max * delta_rho / (iso.rho_scale + fabs(delta_rho));
+    } else {
+        dz0 = trilinear_f64(f.w, zi, yi, xi);
+    }
 
     double dx1, dy1, dz1, dx2, dy2, dz2;
     if (st == OPX_MULTISTEP_1) { // :122-131
@@ -199,6 +215,16 @@ __global__ void __launch_bounds__(kBlock) k_ab3(opx_particles p, opx_scalars s, 
     p.dz1[i] = dz0;
     p.dxidx1[i] = dx0;
     p.dyidx1[i] = dy0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ab3(opx_particles p, opx_scalars s, opx_roms_fields f)
+{
+    ab3_particle<false>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, p, s, f, IsoArgs{});
+}
+
+__global__ void __launch_bounds__(kBlock) k_ab3_iso(opx_particles p, opx_scalars s, opx_roms_fields f, IsoArgs iso)
+{
+    ab3_particle<true>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, p, s, f, iso);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ab3_post(opx_particles p, opx_scalars s, opx_roms_fields f)
@@ -280,14 +306,14 @@ int check_particles(const opx_particles *p, bool positions_only, bool ab3, bool 
     return OPX_OK;
 }
 
-int check_fields(const opx_roms_fields *f, bool full)
+int check_fields(const opx_roms_fields *f, bool full, bool need_w = true)
 {
     OPX_REQUIRE(f != nullptr, "null fields");
     int rc;
     if (full) {
         if ((rc = check_field(f->u, 3, "u", true))) return rc;
         if ((rc = check_field(f->v, 3, "v", true))) return rc;
-        if ((rc = check_field(f->w, 3, "w", true))) return rc;
+        if (need_w && (rc = check_field(f->w, 3, "w", true))) return rc;
         if ((rc = check_field(f->dx, 2, "dx", true))) return rc;
         if ((rc = check_field(f->dy, 2, "dy", true))) return rc;
     }
@@ -376,6 +402,23 @@ int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const
     if (p->n == 0) return OPX_OK;
     hipLaunchKernelGGL(k_ab3_post, dim3(grid_for(p->n)), dim3(kBlock), 0, as_stream(stream), *p, *s, *f);
     return check_launch("opx_ab3_post_w_advection");
+}
+
+int opx_ab3_isopycnal_following(const opx_particles *p, const double *rho_t, const opx_scalars *s,
+                                const opx_roms_fields *f, const opx_field *rho, double w_max, double rho_scale,
+                                void *stream)
+{
+    int rc = check_particles(p, false, true, false);
+    if (rc) return rc;
+    if ((rc = check_scalars(s, true))) return rc;
+    if ((rc = check_fields(f, true, false))) return rc;
+    OPX_REQUIRE(rho != nullptr, "null rho field");
+    if ((rc = check_field(*rho, 3, "rho", true))) return rc;
+    if (p->n == 0) return OPX_OK;
+    OPX_REQUIRE(rho_t != nullptr, "null rho_t");
+    IsoArgs iso{*rho, rho_t, w_max, rho_scale};
+    hipLaunchKernelGGL(k_ab3_iso, dim3(grid_for(p->n)), dim3(kBlock), 0, as_stream(stream), *p, *s, *f, iso);
+    return check_launch("opx_ab3_isopycnal_following");
 }
 
 int opx_compute_z(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,

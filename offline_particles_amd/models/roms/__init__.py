@@ -1,6 +1,7 @@
 """ROMS timestepper factories (mirrors the reference's models/roms/__init__.py:37-106).
 
-``rk2_w_advection_timestepper`` and ``ab3_w_advection_timestepper`` compose
+``rk2_w_advection_timestepper``, ``ab3_w_advection_timestepper`` and
+``ab3_isopycnal_following_timestepper`` (:109-140) compose
 the same kernels in the same order as the reference: the validation
 pre-step, the RK2 stages (alpha = 2/3 by default) or the AB3 kernel and its
 post-step, and an index padding of 5.
@@ -9,6 +10,8 @@ post-step, and an index padding of 5.
 from __future__ import annotations
 
 from ...kernels.roms import (
+    ab3_isopycnal_following_kernel,
+    ab3_post_isopycnal_following_kernel,
     ab3_post_w_advection_kernel,
     ab3_w_advection_kernel,
     rk2_w_advection_step_1_kernel,
@@ -22,6 +25,7 @@ __all__ = [
     "ab3_w_advection_kernel", "ab3_post_w_advection_kernel", "ab3_w_advection_timestepper",
     "rk2_w_advection_step_1_kernel", "rk2_w_advection_step_2_kernel", "rk2_w_advection_update_kernel",
     "rk2_w_advection_timestepper",
+    "ab3_isopycnal_following_kernel", "ab3_post_isopycnal_following_kernel", "ab3_isopycnal_following_timestepper",
 ]
 
 
@@ -41,3 +45,11 @@ def ab3_w_advection_timestepper(time_array, dt, *, time=None, time_unit=None, it
     return ABTimestepper(time_array, dt, ab_kernel=ab3_w_advection_kernel, time=time, time_unit=time_unit,
                          iteration=iteration, index_padding=index_padding, pre_step_kernel=validation_kernel,
                          post_step_kernel=ab3_post_w_advection_kernel)
+
+
+def ab3_isopycnal_following_timestepper(time_array, dt, *, time=None, time_unit=None, iteration: int = 0,
+                                        index_padding: int = 5) -> ABTimestepper:
+    """AB3 timestepper with the ROMS isopycnal following kernels."""
+    return ABTimestepper(time_array, dt, ab_kernel=ab3_isopycnal_following_kernel, time=time, time_unit=time_unit,
+                         iteration=iteration, index_padding=index_padding, pre_step_kernel=validation_kernel,
+                         post_step_kernel=ab3_post_isopycnal_following_kernel)

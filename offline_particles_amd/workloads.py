@@ -46,6 +46,7 @@ OFFSETS = {
     "h": (0.0, 0.0),
     "zeta": (0.0, 0.0),
     "C": (0.5,),
+    "rho": (0.0, 0.0, 0.0),
 }
 STAGGERS = {
     "u": ("center", "center", "inner"),
@@ -56,8 +57,9 @@ STAGGERS = {
     "h": ("invariant", "center", "center"),
     "zeta": ("invariant", "center", "center"),
     "C": ("outer", "invariant", "invariant"),
+    "rho": ("center", "center", "center"),
 }
-TIME_DEPENDENT = ("u", "v", "w", "zeta")
+TIME_DEPENDENT = ("u", "v", "w", "zeta", "rho")
 
 
 def stretching_cs_w(NZ: int, theta_s: float = THETA_S, theta_b: float = THETA_B) -> np.ndarray:
@@ -319,7 +321,26 @@ def turbulent_workload(*, NX=1024, NY=768, NZ=50, nparticles=4_000_000, steps=10
     return wl
 
 
-def build_fieldset(wl: Workload, *, device=None, storage_dtype=None, cyclic: bool = True):
+def add_density(wl: Workload, *, rho0: float = 1026.0, drho: float = 2.0, amp: float = 0.05, seed: int = 11):
+    """Add a time-dependent density field "rho" at rho points (NZ, NY, NX):
+    stably stratified (denser with depth, ``drho`` over the column) plus a
+    smooth horizontal perturbation that drifts between snapshots."""
+    rng = np.random.default_rng(seed)
+    K = wl.fields["u"].shape[0]
+    NZ, NY, NX = wl.NZ, wl.NY, wl.NX
+    zr = (np.arange(NZ) + 0.5) / NZ  # 0 at the bottom, 1 at the top
+    y = np.arange(NY)[:, None]
+    x = np.arange(NX)[None, :]
+    ph = rng.uniform(0, 2 * np.pi, 2)
+    snaps = []
+    for n in range(K):
+        pert = amp * np.sin(2 * np.pi * x / 37.0 + ph[0] + 2 * np.pi * n / K) * np.cos(2 * np.pi * y / 29.0 + ph[1])
+        snaps.append(rho0 + drho * (1.0 - zr)[:, None, None] + pert[None] * np.sin(np.pi * zr)[:, None, None])
+    wl.fields["rho"] = np.stack(snaps).astype(wl.fields["u"].dtype)
+    return wl
+
+
+def build_fieldset(wl: Workload, *, device=None, storage_dtype=None, cyclic: bool = True, constants=None):
     """A Fieldset on the device holding the workload's fields.
 
     ``storage_dtype`` (e.g. torch.bfloat16) overrides the storage type of
@@ -336,7 +357,9 @@ def build_fieldset(wl: Workload, *, device=None, storage_dtype=None, cyclic: boo
             fields[name] = TimeDependentField(arr, zs, ys, xs, cyclic=cyclic, device=device, storage_dtype=sd)
         else:
             fields[name] = StaticField.from_numpy(arr, zs, ys, xs, device=device)
-    return Fieldset(len(wl.time_array), wl.NZ, wl.NY, wl.NX, constants={"hc": wl.hc, "NZ": wl.NZ}, **fields)
+    consts = {"hc": wl.hc, "NZ": wl.NZ}
+    consts.update(constants or {})
+    return Fieldset(len(wl.time_array), wl.NZ, wl.NY, wl.NX, constants=consts, **fields)
 
 
 def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=None, positions=None,

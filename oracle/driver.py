@@ -121,6 +121,8 @@ def load():
     }.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = None
+    L.orc_ab3_iso.argtypes = [P, S, F, ctypes.POINTER(_Field), _dp, ctypes.c_double, ctypes.c_double]
+    L.orc_ab3_iso.restype = None
     L.orc_blend_f64.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.orc_blend_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
                                 ctypes.c_float, _dp]
@@ -250,6 +252,7 @@ class OracleField:
 
 _KERNEL_FIELDS = {
     "full": ("u", "v", "w", "dx", "dy", "h", "C", "zeta"),
+    "iso": ("u", "v", "dx", "dy", "h", "C", "zeta"),
     "vertical": ("h", "C", "zeta"),
 }
 
@@ -260,16 +263,19 @@ _PARTICLE_FIELDS = ("zidx", "yidx", "xidx", "z", "dz0", "dz1", "dz2", "dyidx0", 
 class OracleSimulation:
     """Reference-structured simulation on the CPU oracle.
 
-    ``scheme`` is ``"rk2"`` (rk2_w_advection_timestepper) or ``"ab3"``
-    (ab3_w_advection_timestepper); both use the validation pre-step.
+    ``scheme`` is ``"rk2"`` (rk2_w_advection_timestepper), ``"ab3"``
+    (ab3_w_advection_timestepper) or ``"ab3_iso"``
+    (ab3_isopycnal_following_timestepper: ``fields`` has "rho", and
+    ``rho_t``, ``w_max``, ``rho_scale`` are given); all use the validation
+    pre-step.
     Particle state lives in ``self.p`` (numpy arrays named as the reference's
     particle fields, with RK2's ``_d*1/_d*2`` sharing the AB3 names).
     """
 
     def __init__(self, fields: dict, time_array, dt, *, scheme: str, nparticles: int, hc: float,
                  NZ: int, bounds, alpha: float = 2 / 3, time=None, threads: int | None = None,
-                 bbox: bool = True):
-        if scheme not in ("rk2", "ab3"):
+                 bbox: bool = True, rho_t=None, w_max: float = 0.0, rho_scale: float = 1.0):
+        if scheme not in ("rk2", "ab3", "ab3_iso"):
             raise ValueError(scheme)
         self.L = load()
         if threads:
@@ -295,6 +301,10 @@ class OracleSimulation:
         self._scalars = _Scalars(float(self.dt), float(alpha), float(hc), int(NZ), 0,
                                  zmin, zmax, ymin, ymax, xmin, xmax)
         self._bbox = np.zeros(6)
+        if scheme == "ab3_iso":
+            self.p["rho_t"] = np.ascontiguousarray(np.broadcast_to(np.asarray(rho_t, np.float64), (n,)))
+            self.w_max = float(w_max)
+            self.rho_scale = float(rho_scale)
 
     # timesteppers.py:64-85
     def get_time_index(self, time):
@@ -351,6 +361,21 @@ class OracleSimulation:
             self._launch(L.orc_rk2_step_2, intermediate_tidx, "full")
             self.advance_time()
             self._launch(L.orc_rk2_update, self.tidx, "vertical")
+        elif self.scheme == "ab3_iso":
+            # ABTimestepper (timesteppers.py:296-305) with the isopycnal kernels
+            # (models/roms/__init__.py:109-140)
+            self._validate()
+            if self.do_bbox:
+                self.L.orc_bbox(ctypes.byref(self._cp), _ptr(self._bbox))
+            F = self._fields(self.tidx, "iso")
+            rf = self.fields["rho"]
+            ra = rf.field_data(self.tidx, self.L)
+            self._keep_rho = ra
+            R = _Field(_ptr(ra), *ra.shape, *rf.offsets)
+            L.orc_ab3_iso(ctypes.byref(self._cp), ctypes.byref(self._scalars), ctypes.byref(F), ctypes.byref(R),
+                          _ptr(self.p["rho_t"]), self.w_max, self.rho_scale)
+            self.advance_time()
+            self._launch(L.orc_ab3_post, self.tidx, "vertical")
         else:
             # timesteppers.py:296-305
             self._validate()

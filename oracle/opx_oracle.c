@@ -461,6 +461,67 @@ void orc_ab3_post(orc_particles *p, const orc_scalars *s, const orc_fields *F)
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* L1: AB3 isopycnal following (kernels/roms/ab3_isopycnal_following.pyx) */
+/* ------------------------------------------------------------------ */
+
+/* _ab3_isopycnal_following (ab3_isopycnal_following.pyx:26-173): the AB3
+ * step of orc_ab3 with the vertical rate dz0 = w_max * drho / (rho_scale +
+ * |drho|), drho = rho(x) - rho_t (:126-134), in place of w.  The post step is
+ * orc_ab3_post (:175-229 is the same computation). */
+void orc_ab3_iso(orc_particles *p, const orc_scalars *s, const orc_fields *F, const orc_field *rho,
+                 const double *rho_t, double w_max, double rho_scale)
+{
+    int64_t i, n = p->n;
+    const double dt = s->dt, hc = s->hc;
+    const int NZ = s->NZ;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; ++i) {
+        if (p->status[i] & ORC_INACTIVE)
+            continue;
+        double u_value = tril(&F->u, p->zidx[i], p->yidx[i], p->xidx[i]);
+        double dx_value = bil(&F->dx, p->yidx[i], p->xidx[i]);
+        p->dxidx0[i] = u_value / dx_value;
+        double v_value = tril(&F->v, p->zidx[i], p->yidx[i], p->xidx[i]);
+        double dy_value = bil(&F->dy, p->yidx[i], p->xidx[i]);
+        p->dyidx0[i] = v_value / dy_value;
+        double rho_value = tril(rho, p->zidx[i], p->yidx[i], p->xidx[i]);
+        double delta_rho = rho_value - rho_t[i];
+        p->dz0[i] = w_max * delta_rho / (rho_scale + fabs(delta_rho));
+
+        if (p->status[i] == ORC_MULTISTEP_1) {
+            p->dxidx1[i] = p->dxidx0[i];
+            p->dyidx1[i] = p->dyidx0[i];
+            p->dz1[i] = p->dz0[i];
+            p->dxidx2[i] = p->dxidx0[i];
+            p->dyidx2[i] = p->dyidx0[i];
+            p->dz2[i] = p->dz0[i];
+            p->status[i] = ORC_MULTISTEP_2;
+        } else if (p->status[i] == ORC_MULTISTEP_2) {
+            p->dxidx2[i] = -p->dxidx0[i] + 2.0 * p->dxidx1[i];
+            p->dyidx2[i] = -p->dyidx0[i] + 2.0 * p->dyidx1[i];
+            p->dz2[i] = -p->dz0[i] + 2.0 * p->dz1[i];
+            p->status[i] = ORC_NORMAL;
+        }
+
+        double h_value = bil(&F->h, p->yidx[i], p->xidx[i]);
+        double zeta_value = bil(&F->zeta, p->yidx[i], p->xidx[i]);
+        double C_value = orc_linear(F->C.a, F->C.n0, p->zidx[i] + F->C.o0);
+        p->z[i] = orc_compute_z(p->zidx[i], NZ, hc, h_value, C_value, zeta_value);
+
+        p->z[i] += dt * (23.0 * p->dz0[i] - 16.0 * p->dz1[i] + 5.0 * p->dz2[i]) / 12.0;
+        p->yidx[i] += dt * (23.0 * p->dyidx0[i] - 16.0 * p->dyidx1[i] + 5.0 * p->dyidx2[i]) / 12.0;
+        p->xidx[i] += dt * (23.0 * p->dxidx0[i] - 16.0 * p->dxidx1[i] + 5.0 * p->dxidx2[i]) / 12.0;
+
+        p->dz2[i] = p->dz1[i];
+        p->dxidx2[i] = p->dxidx1[i];
+        p->dyidx2[i] = p->dyidx1[i];
+        p->dz1[i] = p->dz0[i];
+        p->dxidx1[i] = p->dxidx0[i];
+        p->dyidx1[i] = p->dyidx0[i];
+    }
+}
+
 /* bounding box reductions (launcher.py:122-148); min/max of each index. */
 void orc_bbox(const orc_particles *p, double *out6)
 {

@@ -15,17 +15,17 @@ def oracle_fields(wl):
     return {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in wl.fields.items()}
 
 
-def run_oracle(wl, scheme, steps, *, positions=None, status=None, threads=None):
+def run_oracle(wl, scheme, steps, *, positions=None, status=None, threads=None, **kw):
     from oracle import OracleSimulation
 
     pos = wl.positions if positions is None else positions
     sim = OracleSimulation(oracle_fields(wl), wl.time_array, wl.dt, scheme=scheme, nparticles=len(pos[0]),
-                           hc=wl.hc, NZ=wl.NZ, bounds=wl.bounds, threads=threads)
+                           hc=wl.hc, NZ=wl.NZ, bounds=wl.bounds, threads=threads, **kw)
     for k, a in zip(POS, pos):
         sim.p[k][:] = a
     if status is not None:
         sim.p["status"][:] = status
-    elif scheme == "ab3":
+    elif scheme in ("ab3", "ab3_iso"):
         sim.p["status"][:] = 10
     sim.run(steps)
     return sim

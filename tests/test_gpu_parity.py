@@ -70,3 +70,26 @@ def test_zero_particles(device):
     for _ in range(3):
         sim_g.step()
     assert len(sim_g.particles) == 0
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 1e-5)])
+def test_isopycnal_following_100_steps(device, dtype, tol):
+    """AB3 isopycnal following (models/roms/__init__.py:109-140) against the oracle."""
+    from offline_particles_amd import SimulationBuilder
+    from offline_particles_amd.models import roms
+    from offline_particles_amd.workloads import add_density, build_fieldset
+
+    wl = add_density(eddy_workload(steps=100, nparticles=5000, dtype=dtype))
+    n = wl.nparticles
+    rho_t = np.random.default_rng(2).uniform(1026.2, 1027.8, n)
+    consts = {"w_max": 5e-4, "rho_scale": 0.2}
+    sim_o = run_oracle(wl, "ab3_iso", 100, rho_t=rho_t, **consts)
+    fs = build_fieldset(wl, device=device, constants=consts)
+    ts = roms.ab3_isopycnal_following_timestepper(wl.time_array, np.float64(wl.dt))
+    sim_g = SimulationBuilder(ts, fs).build_simulation(n, device=device)
+    sim_g.set_indices(*wl.positions)
+    sim_g.set_particle_field("status", 10)
+    sim_g.set_particle_field("rho_t", rho_t)
+    for _ in range(100):
+        sim_g.step()
+    _compare(sim_o, sim_g, tol)

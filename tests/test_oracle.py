@@ -230,6 +230,60 @@ def test_uniform_w_flat_bed(scheme):
     assert np.allclose(sim.p["zidx"], z0 + 6 * NZ * w * dt / h, rtol=0, atol=1e-10)
 
 
+def _iso_sim(fields, rho, ta, dt, pos, NZ, NY, NX, rho_t, w_max, rho_scale):
+    from offline_particles_amd.workloads import OFFSETS
+
+    fields = dict(fields)
+    fields["rho"] = D.OracleField(rho, OFFSETS["rho"], True, cyclic=False)
+    sim = D.OracleSimulation(fields, ta, dt, scheme="ab3_iso", nparticles=len(pos[0]), hc=10.0, NZ=NZ,
+                             bounds=(0, NZ - 1, 0, NY - 1, 0, NX - 1), rho_t=rho_t, w_max=w_max,
+                             rho_scale=rho_scale)
+    for key, a in zip(("zidx", "yidx", "xidx"), pos):
+        sim.p[key][:] = a
+    sim.p["status"][:] = 10
+    return sim
+
+
+def test_ab3_iso_uniform_density_closed_form():
+    """Uniform rho: w = w_max drho / (rho_scale + |drho|) is constant, so with a
+    flat bed, zeta = 0 and C = sigma, zidx moves NZ w dt / h per step (up when the
+    particle is denser than its target, down when lighter)."""
+    NZ, NY, NX, h, dt = 10, 5, 6, 100.0, 300.0
+    ta = np.arange(4) * 3600.0
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4, h=h, w=0.0)
+    rho0, w_max, rho_scale = 1026.0, 2e-4, 0.3
+    rho = np.full((4, NZ, NY, NX), rho0)
+    rho_t = np.array([1025.5, 1026.0, 1027.0])
+    z0 = np.array([5.0, 4.5, 3.25])
+    sim = _iso_sim(fields, rho, ta, dt, (z0, np.full(3, 2.0), np.full(3, 3.0)), NZ, NY, NX, rho_t, w_max,
+                   rho_scale)
+    for _ in range(6):
+        sim.step()
+    d = rho0 - rho_t
+    w = w_max * d / (rho_scale + np.abs(d))
+    assert np.allclose(sim.p["zidx"], z0 + 6 * NZ * w * dt / h, rtol=0, atol=1e-10)
+    assert sim.p["zidx"][0] > z0[0] and sim.p["zidx"][1] == z0[1] and sim.p["zidx"][2] < z0[2]
+
+
+def test_ab3_iso_converges_to_target_isopycnal():
+    """Stable stratification rho = r0 + g (NZ - 1 - zidx): particles settle on
+    the level whose density is their target (the drive has the right sign and
+    the density is read at rho points)."""
+    NZ, NY, NX, h, dt = 12, 5, 6, 100.0, 300.0
+    ta = np.arange(40) * 3600.0
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 40, h=h, w=0.0)
+    r0, g = 1025.0, 0.25
+    k = np.arange(NZ, dtype=np.float64)
+    rho = np.broadcast_to((r0 + g * (NZ - 1 - k))[None, :, None, None], (40, NZ, NY, NX)).copy()
+    target_level = np.array([2.5, 6.0, 9.75])
+    rho_t = r0 + g * (NZ - 1 - target_level)
+    z0 = np.array([8.0, 1.0, 4.0])
+    sim = _iso_sim(fields, rho, ta, dt, (z0, np.full(3, 2.0), np.full(3, 3.0)), NZ, NY, NX, rho_t, 1e-3, 0.05)
+    for _ in range(400):
+        sim.step()
+    assert np.allclose(sim.p["zidx"], target_level, rtol=0, atol=1e-6)
+
+
 # ----------------------------------------------------------------- blend + time
 
 def test_blend_matches_numpy_dtype_semantics():
