@@ -222,6 +222,8 @@ enum opx_layout {
     OPX_LAYOUT_RING = 0, /* read snapshots s0 and s1 of the rings: 16 loads per field */
     OPX_LAYOUT_PAIR = 1, /* packed window, node i = (s0[i], s1[i]): 8 loads per field */
     OPX_LAYOUT_QUAD = 2, /* packed window, node i = (s0[i], s0[i+1], s1[i], s1[i+1]): 4 loads */
+    OPX_LAYOUT_PAIR_LDS = 3, /* PAIR windows; the AB3 step stages each wave's stencil box in LDS
+                                (64-thread workgroups; `block` is ignored) */
 };
 
 /* A ring of `nslots` snapshots per time-dependent field, contiguous with the

@@ -31,7 +31,7 @@ COMMON = [
 # source -> extra flags
 SOURCES = {
     "opx_plain.hip": ["-ffp-contract=off"],
-    "opx_fused.hip": [],
+    "opx_fused.hip": ["-ffp-contract=off"],
 }
 
 

@@ -304,6 +304,31 @@ __device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
     return (v[0] * gx + v[1] * fx) * gy + (v[2] * gx + v[3] * fx) * fy;
 }
 
+// Pointer a[1] if j else a[0], as a bit select: the compiler then keeps both
+// pointers in registers instead of forming a dynamically indexed (scratch)
+// copy of the window's pointer pair.
+__device__ __forceinline__ const void *pick(int j, const void *const a[2])
+{
+    const uintptr_t p0 = reinterpret_cast<uintptr_t>(a[0]), p1 = reinterpret_cast<uintptr_t>(a[1]);
+    const uintptr_t m = (uintptr_t)0 - (uintptr_t)(j != 0);
+    return reinterpret_cast<const void *>(p0 ^ ((p0 ^ p1) & m));
+}
+
+// Which of the (at most two) packed windows holds slot pair (s0, s1); -1: none.
+template <int LAYOUT>
+__device__ __forceinline__ int packed_index(const opx_window &win, int s0, int s1)
+{
+    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        return -1;
+    } else {
+        if (win.npacked > 0 && win.pslot[0][0] == s0 && win.pslot[0][1] == s1)
+            return 0;
+        if (win.npacked > 1 && win.pslot[1][0] == s0 && win.pslot[1][1] == s1)
+            return 1;
+        return -1;
+    }
+}
+
 // ---------------------------------------------------------------- clock
 
 // a + b * c with both operations rounded (no FMA): the reference's
@@ -392,42 +417,133 @@ __global__ void k_step_incr(int32_t *step) { *step += 1; }
 
 // ---------------------------------------------------------------- fused AB3
 
-// One particle's fused AB3 step.  Cw is the stretching curve staged in LDS.
-template <typename T, int LAYOUT>
-__device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
-                                        const opx_window &win, const opx_release_step *__restrict__ table, double dt,
-                                        const double *Cw)
+// A particle that passed the step's validation: decoded position, status and
+// its release's weights.
+struct Pre {
+    Axis az, ay, ax;
+    opx_release_step rs;
+    uint8_t st;
+    bool sample; // stepped (and sampled) this step
+};
+
+// Validation prologue of the fused step (validation.pyx:21-77, inclusive
+// bounds in FP64); flags NONFINITE / OUT_OF_DOMAIN particles.
+__device__ __forceinline__ Pre ab3_pre(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                       const opx_release_step *__restrict__ table)
 {
+    Pre p;
+    p.sample = false;
     if (i >= n)
-        return;
-    const uint8_t st = s.status[i];
-    if (st & OPX_INACTIVE)
-        return;
+        return p;
+    p.st = s.status[i];
+    if (p.st & OPX_INACTIVE)
+        return p;
     const int r = s.rel ? (int)s.rel[i] : 0;
-    const opx_release_step rs = table[r];
-    if (!rs.active)
-        return;
-
-    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    p.rs = table[r];
+    if (!p.rs.active)
+        return p;
     const uint32_t cell = (uint32_t)s.cell[i];
-    Axis az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
-    Axis ay = decode_axis(cell, g.xbits, g.ybits);
-    Axis ax = decode_axis(cell, 0, g.xbits);
-
-    // ---- validation (validation.pyx:21-77), inclusive bounds in FP64
-    if (!(az.ok && ay.ok && ax.ok)) {
+    p.az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+    p.ay = decode_axis(cell, g.xbits, g.ybits);
+    p.ax = decode_axis(cell, 0, g.xbits);
+    if (!(p.az.ok && p.ay.ok && p.ax.ok)) {
         s.status[i] = OPX_NONFINITE;
-        return;
+        return p;
     }
-    az.q = s.fz[i];
-    ay.q = s.fy[i];
-    ax.q = s.fx[i];
-    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+    p.az.q = s.fz[i];
+    p.ay.q = s.fy[i];
+    p.ax.q = s.fx[i];
+    const double zi = p.az.pos(), yi = p.ay.pos(), xi = p.ax.pos();
     if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
           xi <= g.xidx_max)) {
         s.status[i] = OPX_OUT_OF_DOMAIN;
-        return;
+        return p;
     }
+    p.sample = true;
+    return p;
+}
+
+// ---------------------------------------------------------------- wave tiles in LDS
+
+// Nodes [lo0, lo0 + d0) x [lo1, lo1 + d1) x [lo2, lo2 + d2) of a field, row-major
+// (x fastest) at dword `off` of the LDS pool.
+struct Box {
+    int lo0, lo1, lo2, d1, d2, off;
+};
+
+// The stencil boxes of one wave's particles: u, v, w of packed window j and
+// the 2-D (h, zeta, dx | dy) box t2 (lo1/lo2: y/x origin, d1/d2: extents;
+// its dy values follow the float4 values at dword off + 4 * d1 * d2).
+struct Tile {
+    Box u, v, w, t2;
+    int j;
+};
+
+template <typename T>
+__device__ __forceinline__ void corners_tile(const uint32_t *pool, const Box &b, int b0, int b1, int b2, float gt,
+                                             float ft, float v[8])
+{
+    const int base = ((b0 - b.lo0) * b.d1 + (b1 - b.lo1)) * b.d2 + (b2 - b.lo2);
+    const int plane = b.d1 * b.d2;
+    const int rows[4] = {base, base + b.d2, base + plane, base + plane + b.d2};
+    if constexpr (sizeof(T) == 4) {
+        const float2 *P = reinterpret_cast<const float2 *>(pool + b.off);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float2 e0 = P[rows[r]], e1 = P[rows[r] + 1];
+            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
+            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
+        }
+    } else {
+        const uint32_t *P = pool + b.off;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float2 e0 = unpair(P[rows[r]]), e1 = unpair(P[rows[r] + 1]);
+            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
+            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
+        }
+    }
+}
+
+__device__ __forceinline__ void load2_tile(const uint32_t *pool, const Box &b, int by, int bx, bool want_dxdy,
+                                           Corners2 &c)
+{
+    const int base = (by - b.lo1) * b.d2 + (bx - b.lo2);
+    const int o[4] = {base, base + 1, base + b.d2, base + b.d2 + 1};
+    const float4 *P = reinterpret_cast<const float4 *>(pool + b.off);
+    const float *D = reinterpret_cast<const float *>(pool + b.off + 4 * b.d1 * b.d2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 m = P[o[k]];
+        c.h[k] = m.x;
+        c.z0[k] = m.y;
+        c.z1[k] = m.z;
+        c.dx[k] = m.w;
+        if (want_dxdy)
+            c.dy[k] = D[o[k]];
+    }
+    c.by = by;
+    c.bx = bx;
+}
+
+__device__ __forceinline__ bool in_box2(const Box &b, int by, int bx)
+{
+    return by >= b.lo1 && by + 1 < b.lo1 + b.d1 && bx >= b.lo2 && bx + 1 < b.lo2 + b.d2;
+}
+
+// One particle's fused AB3 step after ab3_pre.  Cw is the stretching curve
+// staged in LDS; with TILE the packed window tile.j and the 2-D fields at the
+// particle are read from the wave's LDS tile.
+template <typename T, int LAYOUT, bool TILE>
+__device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_split_state &s, const opx_grid &g,
+                                         const opx_window &win, double dt, const double *Cw, const uint32_t *pool,
+                                         const Tile &tile)
+{
+    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const opx_release_step &rs = p.rs;
+    const uint8_t st = p.st;
+    Axis az = p.az, ay = p.ay, ax = p.ax;
+    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
     const float gt = rs.gt, ft = rs.ft;
@@ -441,10 +557,15 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
         else if (win.npacked > 1 && win.pslot[1][0] == rs.s0 && win.pslot[1][1] == rs.s1)
             j = 1;
     }
-    if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
-        const void *pu = j ? win.pu[1] : win.pu[0];
-        const void *pv = j ? win.pv[1] : win.pv[0];
-        const void *pw = j ? win.pw[1] : win.pw[0];
+    const bool tiled = TILE && j == tile.j;
+    if (tiled) {
+        corners_tile<T>(pool, tile.u, cz.b, cy.b, ixu.b, gt, ft, vu);
+        corners_tile<T>(pool, tile.v, cz.b, iyv.b, cx.b, gt, ft, vv);
+        corners_tile<T>(pool, tile.w, ow.b, cy.b, cx.b, gt, ft, vw);
+    } else if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
+        const void *pu = pick(j, win.pu);
+        const void *pv = pick(j, win.pv);
+        const void *pw = pick(j, win.pw);
         corners<T, LAYOUT>(pu, nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ixu.b, vu);
         corners<T, LAYOUT>(pv, nullptr, gt, ft, NY - 1, NX, cz.b, iyv.b, cx.b, vv);
         corners<T, LAYOUT>(pw, nullptr, gt, ft, NY, NX, ow.b, cy.b, cx.b, vw);
@@ -466,8 +587,10 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
     const float4 *hdd = reinterpret_cast<const float4 *>(win.hdd);
     Corners2 c2;
     const bool packed0 = LAYOUT != OPX_LAYOUT_RING && j >= 0;
-    if (packed0)
-        load2_packed(static_cast<const float4 *>(j ? win.p2[1] : win.p2[0]), win.dy, NX, cy.b, cx.b, true, c2);
+    if (tiled)
+        load2_tile(pool, tile.t2, cy.b, cx.b, true, c2);
+    else if (packed0)
+        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, cy.b, cx.b, true, c2);
     else
         load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
                    c2);
@@ -535,8 +658,10 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
         }
         const bool same_pair = packed0 ? (j1 == j) : (j1 < 0 && rs.s0p == rs.s0 && rs.s1p == rs.s1);
         if (!(same_pair && by1 == c2.by && bx1 == c2.bx)) {
-            if (LAYOUT != OPX_LAYOUT_RING && j1 >= 0)
-                load2_packed(static_cast<const float4 *>(j1 ? win.p2[1] : win.p2[0]), win.dy, NX, by1, bx1, false,
+            if (tiled && j1 == j && in_box2(tile.t2, by1, bx1))
+                load2_tile(pool, tile.t2, by1, bx1, false, c2);
+            else if (LAYOUT != OPX_LAYOUT_RING && j1 >= 0)
+                load2_packed(static_cast<const float4 *>(pick(j1, win.p2)), win.dy, NX, by1, bx1, false,
                              c2);
             else
                 load2_ring(hdd, win.zeta + rs.s0p * win.zeta_stride, win.zeta + rs.s1p * win.zeta_stride, NX, by1,
@@ -584,6 +709,16 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
         s.z[i] = (float)z;
 }
 
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                        const opx_window &win, const opx_release_step *__restrict__ table, double dt,
+                                        const double *Cw)
+{
+    const Pre p = ab3_pre(i, n, s, g, table);
+    if (p.sample)
+        ab3_body<T, LAYOUT, false>(i, p, s, g, win, dt, Cw, nullptr, Tile{});
+}
+
 // Fused AB3 step, one particle per thread.  The grid covers s.n particles;
 // the particles actually stepped are the first min(s.n, *n_active), so one
 // captured graph serves any active count up to s.n (surplus blocks exit at
@@ -617,6 +752,164 @@ __global__ void __launch_bounds__(BLOCK)
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
     if (c < c1)
         ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
+}
+
+// ---------------------------------------------------------------- LDS-tiled fused AB3 (one wave per workgroup)
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void global_void;
+
+constexpr int kTile3 = 4096; // LDS dwords for the u, v, w boxes of a wave (16 KB)
+constexpr int kTile2 = 256;  // nodes of the 2-D box (5 dwords each: float4 + dy)
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Node range [lo, lo + d) holding both stencil nodes of every base the cells
+// [cmin, cmax] produce on one axis; the base functions are those of ax_center
+// (clamp(c, 0, N-2)), ax_inner (clamp(c - 1 + upper, 0, N-3)) and ax_outer
+// (clamp(c + upper, 0, N-1)), all non-decreasing in c.
+__device__ __forceinline__ void range_center(int cmin, int cmax, int N, int &lo, int &d)
+{
+    lo = max(0, min(cmin, N - 2));
+    d = max(0, min(cmax, N - 2)) + 2 - lo;
+}
+__device__ __forceinline__ void range_inner(int cmin, int cmax, int N, int &lo, int &d)
+{
+    lo = max(0, min(cmin - 1, N - 3));
+    d = max(0, min(cmax, N - 3)) + 2 - lo;
+}
+__device__ __forceinline__ void range_outer(int cmin, int cmax, int N, int &lo, int &d)
+{
+    lo = max(0, min(cmin, N - 1));
+    d = max(0, min(cmax + 1, N - 1)) + 2 - lo;
+}
+
+// ceil(2^32 / d): floor(e * m / 2^32) = floor(e / d) for e * (d - 1) < 2^32.
+__device__ __forceinline__ uint64_t div_magic(uint32_t d) { return ((1ull << 32) + d - 1) / d; }
+
+// Copy box b (d0 planes, EPN dwords per node) of a field with dims (., n1, n2)
+// into the LDS pool with direct-to-LDS loads: lane l of the instruction that
+// covers pool dwords [base, base + 64) fetches dword base + l of the box.
+template <int EPN>
+__device__ __forceinline__ void stage_box(uint32_t *pool, const Box &b, int d0, const void *src, int n1, int n2,
+                                          int lane)
+{
+    const uint32_t nd = (uint32_t)(d0 * b.d1 * b.d2 * EPN);
+    const uint64_t m2 = div_magic((uint32_t)b.d2), m1 = div_magic((uint32_t)b.d1);
+    const uint32_t *S = static_cast<const uint32_t *>(src);
+    for (uint32_t base = 0; base < nd; base += 64) {
+        const uint32_t dw = base + (uint32_t)lane;
+        if (dw < nd) {
+            const uint32_t e = dw / EPN;
+            const uint32_t row = (uint32_t)(((uint64_t)e * m2) >> 32);
+            const uint32_t x = e - row * (uint32_t)b.d2;
+            const uint32_t zz = (uint32_t)(((uint64_t)row * m1) >> 32);
+            const uint32_t yy = row - zz * (uint32_t)b.d1;
+            const uint64_t node = ((uint64_t)(b.lo0 + (int)zz) * (uint64_t)n1 + (uint64_t)(b.lo1 + (int)yy)) *
+                                      (uint64_t)n2 +
+                                  (uint64_t)(b.lo2 + (int)x);
+            const uint32_t *a = S + node * EPN + dw % EPN;
+            __builtin_amdgcn_global_load_lds((global_void *)a, (lds_void *)(pool + b.off + base), 4, 0, 0);
+        }
+    }
+}
+
+// Fused AB3 step with the stencils of each wave staged in LDS (PAIR windows):
+// the wave's particles are neighbours in the Morton order, so the boxes around
+// their cells are small; the boxes are copied with coalesced direct-to-LDS
+// loads and every stencil read then hits LDS.  A wave whose boxes exceed the
+// pool, that holds particles of different packed windows or none reads from
+// global memory as k_ab3_fused does (identical arithmetic either way).
+template <typename T, bool IND>
+__global__ void __launch_bounds__(64)
+    k_ab3_tile(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
+               const opx_release_step *__restrict__ table, double dt, const int32_t *__restrict__ n_active)
+{
+    constexpr int EPN = sizeof(typename Packed<T>::pair) / 4;
+    const opx_window win = IND ? *win_dev : win_arg;
+    __shared__ double Cw[kMaxC];
+    __shared__ __attribute__((aligned(16))) uint32_t pool[kTile3 + 5 * kTile2];
+    const int lane = (int)threadIdx.x;
+    for (int k = lane; k <= g.NZ; k += 64)
+        Cw[k] = win.C[k];
+
+    int64_t n = s.n;
+    if (n_active)
+        n = min(n, (int64_t)*n_active);
+    const int64_t chunks = (n + 63) / 64;
+    const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+    const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
+    const int64_t i = c < c1 ? c * 64 + lane : n;
+    const Pre p = ab3_pre(i, n, s, g, table);
+
+    // ---- the wave's cell range and packed window
+    constexpr int BIG = 1 << 30;
+    const int j = p.sample ? packed_index<OPX_LAYOUT_PAIR>(win, p.rs.s0, p.rs.s1) : -1;
+    const int zmin = uniform(wave_min(p.sample ? p.az.c : BIG)), zmax = uniform(wave_max(p.sample ? p.az.c : -BIG));
+    const int ymin = uniform(wave_min(p.sample ? p.ay.c : BIG)), ymax = uniform(wave_max(p.sample ? p.ay.c : -BIG));
+    const int xmin = uniform(wave_min(p.sample ? p.ax.c : BIG)), xmax = uniform(wave_max(p.sample ? p.ax.c : -BIG));
+    const int jmin = uniform(wave_min(p.sample ? j : BIG)), jmax = uniform(wave_max(p.sample ? j : -BIG));
+    Tile t;
+    bool use = zmin <= zmax && jmin == jmax && jmin >= 0;
+    if (use) {
+        const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+        int du0, dv0, dw0;
+        range_center(zmin, zmax, NZ, t.u.lo0, du0);
+        range_center(ymin, ymax, NY, t.u.lo1, t.u.d1);
+        range_inner(xmin, xmax, NX, t.u.lo2, t.u.d2);
+        range_center(zmin, zmax, NZ, t.v.lo0, dv0);
+        range_inner(ymin, ymax, NY, t.v.lo1, t.v.d1);
+        range_center(xmin, xmax, NX, t.v.lo2, t.v.d2);
+        range_outer(zmin, zmax, NZ, t.w.lo0, dw0);
+        range_center(ymin, ymax, NY, t.w.lo1, t.w.d1);
+        range_center(xmin, xmax, NX, t.w.lo2, t.w.d2);
+        t.t2.lo0 = 0;
+        range_center(ymin, ymax, NY, t.t2.lo1, t.t2.d1);
+        range_center(xmin, xmax, NX, t.t2.lo2, t.t2.d2);
+        // 64-bit volumes: a wave of scattered particles spans the whole grid
+        const int64_t vu = (int64_t)du0 * t.u.d1 * t.u.d2, vv = (int64_t)dv0 * t.v.d1 * t.v.d2;
+        const int64_t vw = (int64_t)dw0 * t.w.d1 * t.w.d2, v2 = (int64_t)t.t2.d1 * t.t2.d2;
+        use = (vu + vv + vw) * EPN <= kTile3 && v2 <= kTile2;
+        t.u.off = 0;
+        t.v.off = (int)(vu * EPN);
+        t.w.off = (int)((vu + vv) * EPN);
+        t.t2.off = kTile3;
+        t.j = jmin;
+        if (use) {
+            const int jj = t.j;
+            stage_box<EPN>(pool, t.u, du0, pick(jj, win.pu), NY, NX - 1, lane);
+            stage_box<EPN>(pool, t.v, dv0, pick(jj, win.pv), NY - 1, NX, lane);
+            stage_box<EPN>(pool, t.w, dw0, pick(jj, win.pw), NY, NX, lane);
+            stage_box<4>(pool, t.t2, 1, pick(jj, win.p2), 1, NX, lane);
+            Box tdy = t.t2;
+            tdy.off = kTile3 + 4 * (int)v2;
+            stage_box<1>(pool, tdy, 1, win.dy, 1, NX, lane);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!p.sample)
+        return;
+    if (use)
+        ab3_body<T, OPX_LAYOUT_PAIR, true>(i, p, s, g, win, dt, Cw, pool, t);
+    else
+        ab3_body<T, OPX_LAYOUT_PAIR, false>(i, p, s, g, win, dt, Cw, nullptr, t);
 }
 
 // ---------------------------------------------------------------- packed windows
@@ -719,19 +1012,6 @@ __device__ __forceinline__ AxD axd(double idx, double off, int size)
     return {b, p - (double)b};
 }
 
-template <int LAYOUT>
-__device__ __forceinline__ int packed_index(const opx_window &win, int s0, int s1)
-{
-    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
-        return -1;
-    } else {
-        if (win.npacked > 0 && win.pslot[0][0] == s0 && win.pslot[0][1] == s1)
-            return 0;
-        if (win.npacked > 1 && win.pslot[1][0] == s0 && win.pslot[1][1] == s1)
-            return 1;
-        return -1;
-    }
-}
 
 // u, v (FP32 trilinear) and w (FP64 weights) blended at slots (s0, s1) at an
 // FP64 position (rk2_w_advection.pyx:124-150).
@@ -745,9 +1025,9 @@ __device__ __forceinline__ void sample_uvw(const opx_window &win, const opx_grid
     float vu[8], vv[8], vw[8];
     const int j = packed_index<LAYOUT>(win, s0, s1);
     if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
-        corners<T, LAYOUT>(j ? win.pu[1] : win.pu[0], nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ix.b, vu);
-        corners<T, LAYOUT>(j ? win.pv[1] : win.pv[0], nullptr, gt, ft, NY - 1, NX, cz.b, iy.b, cx.b, vv);
-        corners<T, LAYOUT>(j ? win.pw[1] : win.pw[0], nullptr, gt, ft, NY, NX, oz.b, cy.b, cx.b, vw);
+        corners<T, LAYOUT>(pick(j, win.pu), nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ix.b, vu);
+        corners<T, LAYOUT>(pick(j, win.pv), nullptr, gt, ft, NY - 1, NX, cz.b, iy.b, cx.b, vv);
+        corners<T, LAYOUT>(pick(j, win.pw), nullptr, gt, ft, NY, NX, oz.b, cy.b, cx.b, vw);
     } else {
         const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
         const T *W = static_cast<const T *>(win.w);
@@ -774,7 +1054,7 @@ __device__ __forceinline__ void sample_2d(const opx_window &win, int s0, int s1,
     Corners2 c2;
     const int j = packed_index<LAYOUT>(win, s0, s1);
     if (LAYOUT != OPX_LAYOUT_RING && j >= 0)
-        load2_packed(static_cast<const float4 *>(j ? win.p2[1] : win.p2[0]), win.dy, NX, cy.b, cx.b, want_dxdy, c2);
+        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, cy.b, cx.b, want_dxdy, c2);
     else
         load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
                    win.zeta + s1 * win.zeta_stride, NX, cy.b, cx.b, want_dxdy, c2);
@@ -1085,8 +1365,9 @@ int check_window(const opx_window *w, const opx_grid *g)
                 w->dtype3);
     OPX_REQUIRE(w->nslots >= 2, "window needs at least 2 slots (got %d)", w->nslots);
     OPX_REQUIRE(w->npacked >= 0 && w->npacked <= 2, "npacked %d not in [0, 2]", w->npacked);
-    OPX_REQUIRE(w->npacked == 0 || w->layout == OPX_LAYOUT_PAIR || w->layout == OPX_LAYOUT_QUAD,
-                "packed windows need layout PAIR or QUAD (got %d)", w->layout);
+    OPX_REQUIRE(w->npacked == 0 || w->layout == OPX_LAYOUT_PAIR || w->layout == OPX_LAYOUT_QUAD ||
+                    w->layout == OPX_LAYOUT_PAIR_LDS,
+                "packed windows need layout PAIR, QUAD or PAIR_LDS (got %d)", w->layout);
     for (int j = 0; j < w->npacked; ++j)
         OPX_REQUIRE(w->pu[j] && w->pv[j] && w->pw[j] && w->p2[j], "packed window %d has a null pointer", j);
     OPX_REQUIRE(w->npacked == 0 || w->dy != nullptr, "packed windows need the static dy array");
@@ -1120,9 +1401,29 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
                  int layout, int dtype3, const opx_release_step *table, double dt, const int32_t *n_active, int block,
                  void *stream)
 {
-    const dim3 grid(grid_xcd(s->n, block)), blk(block);
     hipStream_t st = as_stream(stream);
     const bool ind = win_dev != nullptr;
+    if (layout == OPX_LAYOUT_PAIR_LDS) {
+        // one wave per workgroup, stencils staged in LDS (packed PAIR windows)
+        const dim3 grid(grid_xcd(s->n, 64)), blk(64);
+        if (dtype3 == OPX_F32) {
+            if (ind)
+                hipLaunchKernelGGL((k_ab3_tile<float, true>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt,
+                                   n_active);
+            else
+                hipLaunchKernelGGL((k_ab3_tile<float, false>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt,
+                                   n_active);
+        } else {
+            if (ind)
+                hipLaunchKernelGGL((k_ab3_tile<uint16_t, true>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table,
+                                   dt, n_active);
+            else
+                hipLaunchKernelGGL((k_ab3_tile<uint16_t, false>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table,
+                                   dt, n_active);
+        }
+        return check_launch("opx_ab3_fused");
+    }
+    const dim3 grid(grid_xcd(s->n, block)), blk(block);
 #define OPX_FUSED_LAUNCH(T, L, B, I)                                                                       \
     if (block == B && layout == L && ind == I)                                                            \
         hipLaunchKernelGGL((k_ab3_fused<T, L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt, n_active);
@@ -1147,6 +1448,8 @@ int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &wi
                int layout, int dtype3, const opx_release_step *table, double dt, double alpha,
                const int32_t *n_active, int block, void *stream)
 {
+    if (layout == OPX_LAYOUT_PAIR_LDS) // the RK2 step reads PAIR windows directly
+        layout = OPX_LAYOUT_PAIR;
     const dim3 grid(grid_xcd(s->n, block)), blk(block);
     hipStream_t st = as_stream(stream);
     const bool ind = win_dev != nullptr;
@@ -1225,7 +1528,7 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_QUAD, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_PAIR_LDS, "unknown layout %d", layout);
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
@@ -1263,7 +1566,7 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_QUAD, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_PAIR_LDS, "unknown layout %d", layout);
     OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
     if (block == 0)
         block = 128;
@@ -1286,7 +1589,9 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
     int rc;
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR or QUAD");
+    if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
+        layout = OPX_LAYOUT_PAIR;
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR, QUAD or PAIR_LDS");
     OPX_REQUIRE(bytes_u && bytes_v && bytes_w && bytes_2d, "null output");
     *bytes_2d = (uint64_t)16 * g->NY * g->NX;
     const uint64_t e = (uint64_t)elem_bytes(dtype3, layout);
@@ -1303,7 +1608,9 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win && win->u && win->v && win->w, "null window");
     OPX_REQUIRE(win->dtype3 == OPX_F32 || win->dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR or QUAD");
+    if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
+        layout = OPX_LAYOUT_PAIR;
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR, QUAD or PAIR_LDS");
     OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
                 s1, win->nslots);
     OPX_REQUIRE(pu && pv && pw && p2, "null output window");

@@ -55,6 +55,9 @@ _ROMS_STAGGERS = {
 
 _STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
 
+LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad",
+                _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds"}
+
 
 def _bits(n: int) -> int:
     return max(1, int(math.ceil(math.log2(max(n, 2)))))
@@ -571,7 +574,7 @@ class AB3Engine:
             w.pu[j], w.pv[j], w.pw[j], w.p2[j] = (b.data_ptr() for b in bufs)
 
     def set_layout(self, layout: int) -> None:
-        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR or _QUAD)."""
+        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD or _PAIR_LDS)."""
         if layout != self.layout:
             self.layout = int(layout)
             self._packed = []
@@ -683,7 +686,7 @@ class AB3Engine:
         import time as _time
 
         if layouts is None:
-            layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD)
+            layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS)
         saved_every, self.rebin_every = self.rebin_every, 0
         trials = {}
         for xlow in xlows:
@@ -691,7 +694,8 @@ class AB3Engine:
             self.rebin()
             for layout in layouts:
                 self.set_layout(layout)
-                for block in blocks:
+                # the LDS-tiled step runs one wave per workgroup
+                for block in (blocks if layout != _lib.OPX_LAYOUT_PAIR_LDS else blocks[:1]):
                     self.block = int(block)
                     self._time_steps(1)  # first launch of a variant (and window packing)
                     trials[(self.xlow, self.layout, self.block)] = self._time_steps(trial_steps)
@@ -708,7 +712,7 @@ class AB3Engine:
         cadence = int(np.clip(np.sqrt(2.0 * rebin_ms / slope), 5, max_cadence)) if slope > 0 else max_cadence
         self.rebin_every = cadence if saved_every != 0 else 0
         self._since_rebin = cadence_probe
-        names = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad"}
+        names = LAYOUT_NAMES
         self.tuning = {
             "xlow": self.xlow, "layout": names[self.layout], "block": self.block, "rebin_every": self.rebin_every,
             "kernel_ms": best, "rebin_ms": rebin_ms, "drift_ms_per_step": slope,

@@ -122,12 +122,12 @@ def test_clock_leaves_time_array(device):
 
 @pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
 def test_stencil_layouts_identical(device, storage):
-    """RING, PAIR and QUAD stencil layouts blend the same values in the same order: identical bits."""
+    """RING, PAIR, QUAD and LDS-staged PAIR stencils blend the same values in the same order: identical bits."""
     from offline_particles_amd import _lib
 
     wl = _turb()
     res = []
-    for layout in (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD):
+    for layout in (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS):
         eng = _engine(wl, device, storage=storage, rebin_every=11)
         eng.set_layout(layout)
         eng.step(100)  # crosses snapshot boundaries (hourly snapshots, dt = 120 s)
@@ -212,3 +212,24 @@ def test_fused_rk2_layouts_and_graphs_identical(device, storage):
     for other in res[1:]:
         for k in ("status",) + POS:
             assert np.array_equal(res[0][k], other[k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
+def test_lds_tiles_staged_and_fallback(device, storage):
+    """The LDS-tiled step matches the global-read step bit for bit both on waves
+    whose stencil boxes fit the LDS pool (sorted particles) and on waves that
+    fall back to global reads (unsorted particles spread over the grid), with
+    particles leaving the domain and a snapshot boundary in the run."""
+    from offline_particles_amd import _lib
+
+    wl = _turb()
+    for rebin in (9, 0):
+        res = []
+        for layout in (_lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_PAIR_LDS):
+            eng = _engine(wl, device, storage=storage, rebin_every=rebin, xlow=1)
+            eng.set_layout(layout)
+            eng.graph_steps = 4
+            eng.step(45)
+            res.append(_state(eng))
+        for k in ("status",) + POS:
+            assert np.array_equal(res[0][k], res[1][k], equal_nan=True), (rebin, k)
