@@ -224,6 +224,8 @@ enum opx_layout {
     OPX_LAYOUT_QUAD = 2, /* packed window, node i = (s0[i], s0[i+1], s1[i], s1[i+1]): 4 loads */
     OPX_LAYOUT_PAIR_LDS = 3, /* PAIR windows; the AB3 step stages each wave's stencil box in LDS
                                 (64-thread workgroups; `block` is ignored) */
+    OPX_LAYOUT_OCT = 4,      /* packed window, node i = the (y, x) corners i, i+1, i+n2, i+n2+1 of s0
+                                and of s1: one load per z-plane (16 B for BF16, 32 B for FP32) */
 };
 
 /* A ring of `nslots` snapshots per time-dependent field, contiguous with the
@@ -265,16 +267,18 @@ typedef struct opx_release_clock {
     int32_t steps_left;
 } opx_release_clock;
 
-/* Per-release weights of one step, written by opx_clock_advance: u/v/w/zeta
- * blend at t_n in slots (s0, s1) with (ft, gt); zeta at t_{n+1} in slots
- * (s0p, s1p) with (ft1, gt1); FP32-rounded like fields.py:423-424. */
+/* Per-release weights of one step, written by opx_clock_advance (16 bytes,
+ * one vector load per particle): u/v/w/zeta blend at t_n with (ft, 1 - ft),
+ * zeta at t_{n+1} with (ft1, 1 - ft1), FP32-rounded like fields.py:423-424.
+ * The ring holds at most 1023 slots. */
 typedef struct opx_release_step {
-    float ft, gt;   /* at t (AB3 step, RK2 stage 1) */
-    float ft1, gt1; /* at t + dt (AB3 post step, RK2 update) */
-    float ftm, gtm; /* at t + alpha dt (RK2 stage 2; 0 when alpha is 0) */
-    int32_t s0, s1, s0p, s1p, s0m, s1m;
-    int32_t active;
-    int32_t pad_;
+    float ft;  /* at t (AB3 step, RK2 stage 1); gt = 1.0f - ft */
+    float ft1; /* at t + dt (AB3 post step, RK2 update) */
+    float ftm; /* at t + alpha dt (RK2 stage 2; 0 when alpha is 0) */
+    /* ring slots of the snapshot pairs and the active flag: slot s0 of t in
+     * bits 0-9, of t + dt in bits 10-19, of t + alpha dt in bits 20-29, active
+     * in bit 30; the pair's second slot is (s0 + 1) mod nslots */
+    uint32_t slots;
 } opx_release_step;
 
 /* Advance `nrel` release clocks by one step of `dt` and write their step

@@ -104,7 +104,7 @@ class opx_grid(ctypes.Structure):
                 ("xidx_min", ctypes.c_double), ("xidx_max", ctypes.c_double)]
 
 
-OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD, OPX_LAYOUT_PAIR_LDS = 0, 1, 2, 3
+OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD, OPX_LAYOUT_PAIR_LDS, OPX_LAYOUT_OCT = 0, 1, 2, 3, 4
 
 
 class opx_window(ctypes.Structure):
@@ -121,10 +121,7 @@ class opx_release_clock(ctypes.Structure):
 
 
 class opx_release_step(ctypes.Structure):
-    _fields_ = [("ft", ctypes.c_float), ("gt", ctypes.c_float), ("ft1", ctypes.c_float), ("gt1", ctypes.c_float),
-                ("ftm", ctypes.c_float), ("gtm", ctypes.c_float),
-                ("s0", ctypes.c_int32), ("s1", ctypes.c_int32), ("s0p", ctypes.c_int32), ("s1p", ctypes.c_int32),
-                ("s0m", ctypes.c_int32), ("s1m", ctypes.c_int32), ("active", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+    _fields_ = [("ft", ctypes.c_float), ("ft1", ctypes.c_float), ("ftm", ctypes.c_float), ("slots", ctypes.c_uint32)]
 
 
 _P = ctypes.POINTER

@@ -35,10 +35,42 @@ constexpr int kMaxC = 128; // largest NZ + 1 whose C is staged in LDS
 
 // ---------------------------------------------------------------- helpers
 
-__device__ __forceinline__ float ld3(const float *__restrict__ p, uint32_t i) { return p[i]; }
+// Pointers read from the window descriptor are generic; loads through them
+// compile to FLAT instructions, which also count against the LDS counter.
+// Field data always lives in global memory: load through global pointers.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gbl(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// p[i] of a 4, 8, 16 or 32-byte element type through a global pointer.
+template <typename V>
+__device__ __forceinline__ V gload(const V *p, uint32_t i)
+{
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16 || sizeof(V) == 32, "element size");
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(V, gbl(reinterpret_cast<const uint32_t *>(p))[i]);
+    } else if constexpr (sizeof(V) == 8) {
+        return __builtin_bit_cast(V, gbl(reinterpret_cast<const u32x2 *>(p))[i]);
+    } else if constexpr (sizeof(V) == 16) {
+        return __builtin_bit_cast(V, gbl(reinterpret_cast<const u32x4 *>(p))[i]);
+    } else {
+        struct Two {
+            u32x4 a, b;
+        };
+        const auto *q = gbl(reinterpret_cast<const u32x4 *>(p));
+        return __builtin_bit_cast(V, Two{q[2 * i], q[2 * i + 1]});
+    }
+}
+
+__device__ __forceinline__ float ld3(const float *__restrict__ p, uint32_t i) { return gbl(p)[i]; }
 __device__ __forceinline__ float ld3(const uint16_t *__restrict__ p, uint32_t i)
 {
-    return __uint_as_float((uint32_t)p[i] << 16);
+    return __uint_as_float((uint32_t)gbl(p)[i] << 16);
 }
 
 constexpr double kQ = 4294967296.0; // 2^32: one cell in fixed-point offset units
@@ -161,15 +193,21 @@ __device__ __forceinline__ float blend(const T *__restrict__ p, const T *__restr
 // (s0[i], s0[i+1], s1[i], s1[i+1]) (x-neighbour pair of both snapshots).
 template <typename T>
 struct Packed;
+// OCT node i = the 2 x 2 (y, x) corners i, i+1, i+n2, i+n2+1 of both snapshots.
+struct float8 {
+    float4 a, b; // a: s0 corners, b: s1 corners (order i, i+1, i+n2, i+n2+1)
+};
 template <>
 struct Packed<float> {
     using pair = float2;
     using quad = float4;
+    using oct = float8;
 };
 template <>
 struct Packed<uint16_t> {
     using pair = uint32_t; // lo: s0, hi: s1
     using quad = uint2;    // x: s0[i] | s0[i+1] << 16, y: s1[i] | s1[i+1] << 16
+    using oct = uint4;     // x: s0[i] | s0[i+1] << 16, y: s0[i+n2] | s0[i+n2+1] << 16, z, w: the same of s1
 };
 
 __device__ __forceinline__ float bf_lo(uint32_t v) { return __uint_as_float(v << 16); }
@@ -201,20 +239,40 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
             v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
         }
     } else if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
-        const typename Packed<T>::pair *P = static_cast<const typename Packed<T>::pair *>(pp);
+        const auto *P = static_cast<const typename Packed<T>::pair *>(pp);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float2 e0 = unpair(P[rows[r]]), e1 = unpair(P[rows[r] + 1]);
+            float2 e0 = unpair(gload(P, rows[r])), e1 = unpair(gload(P, rows[r] + 1));
             v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
             v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
         }
-    } else {
-        const typename Packed<T>::quad *P = static_cast<const typename Packed<T>::quad *>(pp);
+    } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
+        const auto *P = static_cast<const typename Packed<T>::quad *>(pp);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float4 e = unquad(P[rows[r]]);
+            float4 e = unquad(gload(P, rows[r]));
             v[2 * r] = blend_rn(e.x, e.z, gt, ft);
             v[2 * r + 1] = blend_rn(e.y, e.w, gt, ft);
+        }
+    } else {
+        // OCT: one node per z-plane holds that plane's four (y, x) corners of both snapshots
+        const auto *P = static_cast<const typename Packed<T>::oct *>(pp);
+        const uint32_t planes[2] = {rows[0], rows[2]};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if constexpr (sizeof(T) == 4) {
+                const float8 e = gload(P, planes[k]);
+                v[4 * k + 0] = blend_rn(e.a.x, e.b.x, gt, ft);
+                v[4 * k + 1] = blend_rn(e.a.y, e.b.y, gt, ft);
+                v[4 * k + 2] = blend_rn(e.a.z, e.b.z, gt, ft);
+                v[4 * k + 3] = blend_rn(e.a.w, e.b.w, gt, ft);
+            } else {
+                const uint4 e = gload(P, planes[k]);
+                v[4 * k + 0] = blend_rn(bf_lo(e.x), bf_lo(e.z), gt, ft);
+                v[4 * k + 1] = blend_rn(bf_hi(e.x), bf_hi(e.z), gt, ft);
+                v[4 * k + 2] = blend_rn(bf_lo(e.y), bf_lo(e.w), gt, ft);
+                v[4 * k + 3] = blend_rn(bf_hi(e.y), bf_hi(e.w), gt, ft);
+            }
         }
     }
 }
@@ -249,14 +307,15 @@ struct Corners2 {
     int by, bx;
 };
 
-__device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const float *__restrict__ dy, int NX,
+__device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const float *__restrict__ dy_, int NX,
                                              int by, int bx, bool want_dxdy, Corners2 &c)
 {
+    const auto *dy = gbl(dy_);
     const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
     const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float4 m = P[o[k]];
+        float4 m = gload(P, o[k]);
         c.h[k] = m.x;
         c.z0[k] = m.y;
         c.z1[k] = m.z;
@@ -268,15 +327,16 @@ __device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const
     c.bx = bx;
 }
 
-__device__ __forceinline__ void load2_ring(const float4 *__restrict__ hdd, const float *__restrict__ zp,
-                                           const float *__restrict__ zq, int NX, int by, int bx, bool want_dxdy,
+__device__ __forceinline__ void load2_ring(const float4 *__restrict__ hdd, const float *__restrict__ zp_,
+                                           const float *__restrict__ zq_, int NX, int by, int bx, bool want_dxdy,
                                            Corners2 &c)
 {
+    const auto *zp = gbl(zp_), *zq = gbl(zq_);
     const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
     const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float4 m = hdd[o[k]];
+        float4 m = gload(hdd, o[k]);
         c.h[k] = m.x;
         c.dx[k] = m.y;
         c.dy[k] = m.z;
@@ -312,6 +372,37 @@ __device__ __forceinline__ const void *pick(int j, const void *const a[2])
     const uintptr_t p0 = reinterpret_cast<uintptr_t>(a[0]), p1 = reinterpret_cast<uintptr_t>(a[1]);
     const uintptr_t m = (uintptr_t)0 - (uintptr_t)(j != 0);
     return reinterpret_cast<const void *>(p0 ^ ((p0 ^ p1) & m));
+}
+
+// A release's step weights and ring slots, unpacked from its 16-byte table entry.
+struct Step {
+    float ft, gt, ft1, gt1, ftm, gtm;
+    int s0, s1, s0p, s1p, s0m, s1m;
+    bool active;
+};
+
+constexpr int kMaxSlots = 1023; // slot fields of opx_release_step are 10 bits
+
+__device__ __forceinline__ int next_slot(int s, int nslots) { return s + 1 == nslots ? 0 : s + 1; }
+
+__device__ __forceinline__ Step load_release(const opx_release_step *__restrict__ table, int r, int nslots)
+{
+    const opx_release_step e = gload(table, (uint32_t)r);
+    Step t;
+    t.ft = e.ft;
+    t.gt = 1.0f - e.ft;
+    t.ft1 = e.ft1;
+    t.gt1 = 1.0f - e.ft1;
+    t.ftm = e.ftm;
+    t.gtm = 1.0f - e.ftm;
+    t.s0 = (int)(e.slots & 1023u);
+    t.s0p = (int)((e.slots >> 10) & 1023u);
+    t.s0m = (int)((e.slots >> 20) & 1023u);
+    t.s1 = next_slot(t.s0, nslots);
+    t.s1p = next_slot(t.s0p, nslots);
+    t.s1m = next_slot(t.s0m, nslots);
+    t.active = (e.slots >> 30) & 1u;
+    return t;
 }
 
 // Which of the (at most two) packed windows holds slot pair (s0, s1); -1: none.
@@ -388,7 +479,8 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
         return;
     opx_release_clock c = clocks[r];
     opx_release_step e{};
-    e.active = 0;
+    float gt_unused;
+    int s1_unused, s0 = 0, s0p = 0, s0m = 0, active = 0;
     if (c.steps_left > 0 && step >= c.start_step) {
         double tidx0, tidx1, tidxm = 0.0;
         double t1 = c.t + dt; // advance_time: time += dt (timesteppers.py:159)
@@ -396,11 +488,11 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
         if (alpha > 0.0) // RK2 intermediate time (timesteppers.py:250-251)
             ok = ok && time_index(ta, ntime, add_mul_rn(c.t, alpha, dt), tidxm);
         if (ok) {
-            weights(tidx0, nslots, e.ft, e.gt, e.s0, e.s1);
-            weights(tidx1, nslots, e.ft1, e.gt1, e.s0p, e.s1p);
+            weights(tidx0, nslots, e.ft, gt_unused, s0, s1_unused);
+            weights(tidx1, nslots, e.ft1, gt_unused, s0p, s1_unused);
             if (alpha > 0.0)
-                weights(tidxm, nslots, e.ftm, e.gtm, e.s0m, e.s1m);
-            e.active = 1;
+                weights(tidxm, nslots, e.ftm, gt_unused, s0m, s1_unused);
+            active = 1;
             c.t = t1;
             c.steps_left -= 1;
         } else {
@@ -410,6 +502,7 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
         }
         clocks[r] = c;
     }
+    e.slots = (uint32_t)s0 | ((uint32_t)s0p << 10) | ((uint32_t)s0m << 20) | ((uint32_t)active << 30);
     table[r] = e;
 }
 
@@ -421,7 +514,7 @@ __global__ void k_step_incr(int32_t *step) { *step += 1; }
 // its release's weights.
 struct Pre {
     Axis az, ay, ax;
-    opx_release_step rs;
+    Step rs;
     uint8_t st;
     bool sample; // stepped (and sampled) this step
 };
@@ -429,7 +522,7 @@ struct Pre {
 // Validation prologue of the fused step (validation.pyx:21-77, inclusive
 // bounds in FP64); flags NONFINITE / OUT_OF_DOMAIN particles.
 __device__ __forceinline__ Pre ab3_pre(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
-                                       const opx_release_step *__restrict__ table)
+                                       const opx_release_step *__restrict__ table, int nslots)
 {
     Pre p;
     p.sample = false;
@@ -439,7 +532,7 @@ __device__ __forceinline__ Pre ab3_pre(int64_t i, int64_t n, const opx_split_sta
     if (p.st & OPX_INACTIVE)
         return p;
     const int r = s.rel ? (int)s.rel[i] : 0;
-    p.rs = table[r];
+    p.rs = load_release(table, r, nslots);
     if (!p.rs.active)
         return p;
     const uint32_t cell = (uint32_t)s.cell[i];
@@ -540,7 +633,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
                                          const Tile &tile)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
-    const opx_release_step &rs = p.rs;
+    const Step &rs = p.rs;
     const uint8_t st = p.st;
     Axis az = p.az, ay = p.ay, ax = p.ax;
     const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
@@ -714,7 +807,7 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
                                         const opx_window &win, const opx_release_step *__restrict__ table, double dt,
                                         const double *Cw)
 {
-    const Pre p = ab3_pre(i, n, s, g, table);
+    const Pre p = ab3_pre(i, n, s, g, table, win.nslots);
     if (p.sample)
         ab3_body<T, LAYOUT, false>(i, p, s, g, win, dt, Cw, nullptr, Tile{});
 }
@@ -739,7 +832,7 @@ __global__ void __launch_bounds__(BLOCK)
     // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
     __shared__ double Cw[kMaxC];
     for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
-        Cw[k] = win.C[k];
+        Cw[k] = gbl(win.C)[k];
     __syncthreads();
 
     int64_t n = s.n;
@@ -847,7 +940,7 @@ __global__ void __launch_bounds__(64)
     __shared__ __attribute__((aligned(16))) uint32_t pool[kTile3 + 5 * kTile2];
     const int lane = (int)threadIdx.x;
     for (int k = lane; k <= g.NZ; k += 64)
-        Cw[k] = win.C[k];
+        Cw[k] = gbl(win.C)[k];
 
     int64_t n = s.n;
     if (n_active)
@@ -856,7 +949,7 @@ __global__ void __launch_bounds__(64)
     const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
     const int64_t i = c < c1 ? c * 64 + lane : n;
-    const Pre p = ab3_pre(i, n, s, g, table);
+    const Pre p = ab3_pre(i, n, s, g, table, win.nslots);
 
     // ---- the wave's cell range and packed window
     constexpr int BIG = 1 << 30;
@@ -915,11 +1008,31 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------------------------------- packed windows
 
 template <typename T, int LAYOUT>
-__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, int64_t n, int n2)
+__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, int64_t n, int n1,
+                       int n2)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
+        // neighbours beyond the last row / column are never stencil nodes of a base here
+        const int64_t x1 = (i % n2) + 1 < n2 ? 1 : 0;
+        const int64_t y1 = ((i / n2) % n1) + 1 < n1 ? n2 : 0;
+        const int64_t c[4] = {i, i + x1, i + y1, i + y1 + x1};
+        if constexpr (sizeof(T) == 4) {
+            const float *P = (const float *)p, *Q = (const float *)q;
+            float8 e;
+            e.a = make_float4(P[c[0]], P[c[1]], P[c[2]], P[c[3]]);
+            e.b = make_float4(Q[c[0]], Q[c[1]], Q[c[2]], Q[c[3]]);
+            static_cast<float8 *>(out)[i] = e;
+        } else {
+            const uint16_t *P = (const uint16_t *)p, *Q = (const uint16_t *)q;
+            static_cast<uint4 *>(out)[i] =
+                make_uint4((uint32_t)P[c[0]] | ((uint32_t)P[c[1]] << 16), (uint32_t)P[c[2]] | ((uint32_t)P[c[3]] << 16),
+                           (uint32_t)Q[c[0]] | ((uint32_t)Q[c[1]] << 16), (uint32_t)Q[c[2]] | ((uint32_t)Q[c[3]] << 16));
+        }
+        return;
+    }
     if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
         if constexpr (sizeof(T) == 4) {
             static_cast<float2 *>(out)[i] = make_float2(((const float *)p)[i], ((const float *)q)[i]);
@@ -1102,7 +1215,7 @@ __device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_st
     if (st & OPX_INACTIVE)
         return;
     const int r = s.rel ? (int)s.rel[i] : 0;
-    const opx_release_step rs = table[r];
+    const Step rs = load_release(table, r, win.nslots);
     if (!rs.active)
         return;
 
@@ -1189,7 +1302,7 @@ __global__ void __launch_bounds__(BLOCK)
     const opx_window win = IND ? *win_dev : win_arg;
     __shared__ double Cw[kMaxC];
     for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
-        Cw[k] = win.C[k];
+        Cw[k] = gbl(win.C)[k];
     __syncthreads();
 
     int64_t n = s.n;
@@ -1363,11 +1476,11 @@ int check_window(const opx_window *w, const opx_grid *g)
     OPX_REQUIRE(w->u && w->v && w->w && w->zeta && w->hdd && w->C, "window has a null field pointer");
     OPX_REQUIRE(w->dtype3 == OPX_F32 || w->dtype3 == OPX_BF16, "fused step supports F32/BF16 u,v,w (got %d)",
                 w->dtype3);
-    OPX_REQUIRE(w->nslots >= 2, "window needs at least 2 slots (got %d)", w->nslots);
+    OPX_REQUIRE(w->nslots >= 2 && w->nslots <= kMaxSlots, "window needs 2 to %d slots (got %d)", kMaxSlots,
+                w->nslots);
     OPX_REQUIRE(w->npacked >= 0 && w->npacked <= 2, "npacked %d not in [0, 2]", w->npacked);
-    OPX_REQUIRE(w->npacked == 0 || w->layout == OPX_LAYOUT_PAIR || w->layout == OPX_LAYOUT_QUAD ||
-                    w->layout == OPX_LAYOUT_PAIR_LDS,
-                "packed windows need layout PAIR, QUAD or PAIR_LDS (got %d)", w->layout);
+    OPX_REQUIRE(w->npacked == 0 || (w->layout >= OPX_LAYOUT_PAIR && w->layout <= OPX_LAYOUT_OCT),
+                "packed windows need layout PAIR, QUAD, PAIR_LDS or OCT (got %d)", w->layout);
     for (int j = 0; j < w->npacked; ++j)
         OPX_REQUIRE(w->pu[j] && w->pv[j] && w->pw[j] && w->p2[j], "packed window %d has a null pointer", j);
     OPX_REQUIRE(w->npacked == 0 || w->dy != nullptr, "packed windows need the static dy array");
@@ -1434,10 +1547,12 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_RING)
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_PAIR)
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_QUAD)
+        OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_OCT)
     } else {
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
+        OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_OCT)
     }
 #undef OPX_FUSED_LAUNCH_L
 #undef OPX_FUSED_LAUNCH
@@ -1464,10 +1579,12 @@ int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &wi
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_RING)
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_PAIR)
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_QUAD)
+        OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_OCT)
     } else {
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
+        OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_OCT)
     }
 #undef OPX_RK2_LAUNCH_L
 #undef OPX_RK2_LAUNCH
@@ -1492,7 +1609,7 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
     OPX_REQUIRE(nrel == 0 || (clocks && table), "null clock/table");
     OPX_REQUIRE(time_array != nullptr, "null time array");
     OPX_REQUIRE(ntime >= 2, "time array needs at least 2 entries");
-    OPX_REQUIRE(nslots >= 2, "nslots must be >= 2");
+    OPX_REQUIRE(nslots >= 2 && nslots <= kMaxSlots, "nslots must be in [2, %d]", kMaxSlots);
     hipStream_t st = as_stream(stream);
     if (nrel > 0)
         hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, st, clocks, table, nrel, nrel_dev,
@@ -1528,7 +1645,7 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_PAIR_LDS, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_OCT, "unknown layout %d", layout);
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
@@ -1566,7 +1683,7 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_PAIR_LDS, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_OCT, "unknown layout %d", layout);
     OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
     if (block == 0)
         block = 128;
@@ -1580,7 +1697,7 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
 static int elem_bytes(int dtype3, int layout)
 {
     const int b = dtype3 == OPX_F32 ? 4 : 2;
-    return layout == OPX_LAYOUT_PAIR ? 2 * b : (layout == OPX_LAYOUT_QUAD ? 4 * b : 0);
+    return layout == OPX_LAYOUT_PAIR ? 2 * b : (layout == OPX_LAYOUT_QUAD ? 4 * b : (layout == OPX_LAYOUT_OCT ? 8 * b : 0));
 }
 
 int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t *bytes_u, uint64_t *bytes_v,
@@ -1591,7 +1708,8 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
         layout = OPX_LAYOUT_PAIR;
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR, QUAD or PAIR_LDS");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT,
+                "layout must be PAIR, QUAD, OCT or PAIR_LDS");
     OPX_REQUIRE(bytes_u && bytes_v && bytes_w && bytes_2d, "null output");
     *bytes_2d = (uint64_t)16 * g->NY * g->NX;
     const uint64_t e = (uint64_t)elem_bytes(dtype3, layout);
@@ -1610,7 +1728,8 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
     OPX_REQUIRE(win->dtype3 == OPX_F32 || win->dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
         layout = OPX_LAYOUT_PAIR;
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD, "layout must be PAIR, QUAD or PAIR_LDS");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT,
+                "layout must be PAIR, QUAD, OCT or PAIR_LDS");
     OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
                 s1, win->nslots);
     OPX_REQUIRE(pu && pv && pw && p2, "null output window");
@@ -1626,29 +1745,33 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
     struct F {
         const void *base;
         int64_t stride, n;
-        int n2;
+        int n1, n2;
         void *out;
-    } f[3] = {{win->u, win->u_stride, nu, g->NX - 1, pu}, {win->v, win->v_stride, nv, g->NX, pv},
-              {win->w, win->w_stride, nw, g->NX, pw}};
+    } f[3] = {{win->u, win->u_stride, nu, g->NY, g->NX - 1, pu}, {win->v, win->v_stride, nv, g->NY - 1, g->NX, pv},
+              {win->w, win->w_stride, nw, g->NY, g->NX, pw}};
     for (auto &x : f) {
         const unsigned grid = grid_for(x.n);
+#define OPX_PACK(T, L)                                                                                      \
+    hipLaunchKernelGGL((k_pack<T, L>), dim3(grid), dim3(kBlock), 0, st, b + s0 * x.stride, b + s1 * x.stride, \
+                       x.out, x.n, x.n1, x.n2)
         if (win->dtype3 == OPX_F32) {
             const float *b = static_cast<const float *>(x.base);
             if (layout == OPX_LAYOUT_PAIR)
-                hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
-                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+                OPX_PACK(float, OPX_LAYOUT_PAIR);
+            else if (layout == OPX_LAYOUT_QUAD)
+                OPX_PACK(float, OPX_LAYOUT_QUAD);
             else
-                hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_QUAD>), dim3(grid), dim3(kBlock), 0, st,
-                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+                OPX_PACK(float, OPX_LAYOUT_OCT);
         } else {
             const uint16_t *b = static_cast<const uint16_t *>(x.base);
             if (layout == OPX_LAYOUT_PAIR)
-                hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
-                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+                OPX_PACK(uint16_t, OPX_LAYOUT_PAIR);
+            else if (layout == OPX_LAYOUT_QUAD)
+                OPX_PACK(uint16_t, OPX_LAYOUT_QUAD);
             else
-                hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_QUAD>), dim3(grid), dim3(kBlock), 0, st,
-                                   b + s0 * x.stride, b + s1 * x.stride, x.out, x.n, x.n2);
+                OPX_PACK(uint16_t, OPX_LAYOUT_OCT);
         }
+#undef OPX_PACK
         if ((rc = check_launch("opx_pack_window"))) return rc;
     }
     return OPX_OK;

@@ -56,7 +56,7 @@ _ROMS_STAGGERS = {
 _STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
 
 LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad",
-                _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds"}
+                _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct"}
 
 
 def _bits(n: int) -> int:
@@ -574,7 +574,7 @@ class AB3Engine:
             w.pu[j], w.pv[j], w.pw[j], w.p2[j] = (b.data_ptr() for b in bufs)
 
     def set_layout(self, layout: int) -> None:
-        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD or _PAIR_LDS)."""
+        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD, _PAIR_LDS or _OCT)."""
         if layout != self.layout:
             self.layout = int(layout)
             self._packed = []
@@ -686,7 +686,8 @@ class AB3Engine:
         import time as _time
 
         if layouts is None:
-            layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS)
+            layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
+                       _lib.OPX_LAYOUT_OCT)
         saved_every, self.rebin_every = self.rebin_every, 0
         trials = {}
         for xlow in xlows:
