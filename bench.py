@@ -15,7 +15,16 @@ Forms measured in the same run:
   w stored in FP32 or BF16; bin shape, workgroup size and re-sort cadence are
   picked by the engine's autotuner during warm-up, and the periodic re-binning
   is inside the timed region.
-The headline ``value`` is the fastest form; every form is listed in ``forms``.
+- batching (``--batched NREL``, default 256): uneven releases (sizes log-uniform
+  1k-1M, start over 24 h, 6-72 h long, 10 km boxes) on the same grid in BF16,
+  stepped together from captured HIP graphs with the autotuned bin shape,
+  layout and re-sort cadence; reported under ``batching``.
+- serving (``--serve SECONDS``, default 5): Poisson arrivals at 20/s for that
+  many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 BF16
+  grid streamed through a 3-snapshot window, admission by free HBM; every
+  request runs to completion; p50/p99 latency under ``serving``.
+The headline ``value`` is the fastest 16M-particle form; every form is listed
+in ``forms``.
 
 Prints ONE JSON line.  ``value`` is particle-steps/s of the headline form with
 all inputs resident in HBM; particle-steps count the particles active during
@@ -200,9 +209,14 @@ def field_bytes_touched_fused(wl, pos, storage_bytes_3d):
 # cell (4), 3 offsets (12) and 6 history values (24) and writes cell, offsets
 # and history (40 B).
 FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
+# fused RK2 kernel: reads status, release id, cell, 3 offsets; writes cell and
+# offsets (no history).  Its field term is counted at the step's start
+# position only (a lower bound: stage 2 reads around x*).
+FUSED_RK2_STATE_BYTES = 1 + 2 + 4 + 12 + 16
 
 
-def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False, layout=None, block=128):
+def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False, layout=None, block=128,
+              scheme="ab3"):
     import torch
 
     from offline_particles_amd.engine import AB3Engine
@@ -210,10 +224,12 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
     fs = build_fieldset(wl, device=device)
     eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
-                    capacity=wl.nparticles, rebin_every=rebin_every, xlow=xlow)
+                    capacity=wl.nparticles, rebin_every=rebin_every, xlow=xlow, scheme=scheme)
     del fs
-    tune_steps = 3 * 3 * 2 * 4 + 40 + 10 if tune else 0
-    eng.add_release(*wl.positions, nsteps=steps + warmup + tune_steps + 2)
+    # the release runs as long as the time array allows; the timed steps must
+    # all be real steps of every particle (checked below)
+    nsteps_max = int((wl.time_array[-1] - wl.time_array[0]) // wl.dt) - 2
+    eng.add_release(*wl.positions, nsteps=nsteps_max)
     if layout is not None:
         eng.set_layout({"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4}[layout])
     eng.block = block
@@ -238,16 +254,19 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.timer = None
     a1 = eng.active_count()
     eng.check_clock()
+    if eng.steps_left(0) <= 0:
+        raise RuntimeError("the release ran out of steps inside the timed region; lengthen the time array")
     elapsed = t1 - t0
     actn = 0.5 * (a0 + a1)
-    algo = actn * FUSED_STATE_BYTES + fbytes
-    name = "fused_bf16" if storage == torch.bfloat16 else "fused_fp32"
+    algo = actn * (FUSED_STATE_BYTES if scheme == "ab3" else FUSED_RK2_STATE_BYTES) + fbytes
+    name = ("fused_" if scheme == "ab3" else "fused_rk2_") + ("bf16" if storage == torch.bfloat16 else "fp32")
     res = {
         "form": name,
         "particle_steps_per_s": actn * steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
         "active_particles": actn,
-        "kernel": f"k_ab3_fused<{'bf16' if sb == 2 else 'f32'}> (opx_ab3_fused)",
+        "kernel": (f"k_ab3_fused<{'bf16' if sb == 2 else 'f32'}> (opx_ab3_fused)" if scheme == "ab3"
+                   else f"k_rk2_fused<{'bf16' if sb == 2 else 'f32'}> (opx_rk2_fused)"),
         "kernel_ms": kms,
         "algo_bytes": algo,
         "field_bytes_touched": fbytes,
@@ -264,7 +283,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
 
 def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_size=2_000_000, graph_steps=8,
-                tune=True):
+                tune=True, max_particles=256_000_000):
     """Serving form: Poisson arrivals (``rate``/s for ``seconds`` s), sizes log-uniform on
     [1k, max_size], 1-10 simulated days (720-7200 steps of 120 s), BF16 fields streamed
     through a 3-slot window, admission by free HBM; every request runs to completion."""
@@ -304,7 +323,7 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
         torch.cuda.empty_cache()
     srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
                         stream_slots=3, max_releases=4096, rebin_every=rebin_every, xlow=xlow, layout=layout,
-                        block=block, graph_steps=graph_steps)
+                        block=block, graph_steps=graph_steps, max_particles=max_particles)
     log(f"serving: server ready in {time.perf_counter() - t:.1f}s, capacity {srv.capacity} particles")
     srv.run(reqs)
     res = dict(srv.stats)
@@ -518,7 +537,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-particles", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain_fp32,fused_fp32,fused_bf16")
+    ap.add_argument("--forms", default="plain_fp32,fused_fp32,fused_bf16,fused_rk2_fp32")
     ap.add_argument("--rebin-every", type=int, default=25)
     ap.add_argument("--xlow", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --xlow/--rebin-every)")
@@ -528,10 +547,10 @@ def main():
     ap.add_argument("--accuracy", type=int, default=0, metavar="N",
                     help="also report deviation from the FP64 oracle on the first N particles")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
-    ap.add_argument("--batched", type=int, default=0, metavar="NREL",
+    ap.add_argument("--batched", type=int, default=256, metavar="NREL",
                     help="also run the batching form with NREL releases (256 for the full case)")
     ap.add_argument("--batched-max-size", type=int, default=1_000_000)
-    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+    ap.add_argument("--serve", type=float, default=5.0, metavar="SECONDS",
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
@@ -580,13 +599,15 @@ def main():
     if "plain_fp32" in wanted:
         forms["plain_fp32"] = run_plain(wl, device, args.steps, args.warmup)
         log("plain_fp32", forms["plain_fp32"])
-    for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
+    for name, dt, scheme in (("fused_fp32", torch.float32, "ab3"), ("fused_bf16", torch.bfloat16, "ab3"),
+                             ("fused_rk2_fp32", torch.float32, "rk2"), ("fused_rk2_bf16", torch.bfloat16, "rk2")):
         if name in wanted:
             forms[name] = run_fused(wl, device, args.steps, args.warmup, dt, args.rebin_every, args.xlow,
-                                    tune=not args.no_tune, layout=args.layout, block=args.block)
+                                    tune=not args.no_tune, layout=args.layout, block=args.block, scheme=scheme)
             log(name, forms[name])
 
-    head = max(forms.values(), key=lambda f: f["particle_steps_per_s"])
+    # the headline is the fastest AB3 form at the 16M-particle workload
+    head = max((f for k, f in forms.items() if "rk2" not in k), key=lambda f: f["particle_steps_per_s"])
 
     batching = None
     if args.batched:

@@ -233,6 +233,7 @@ class AB3Engine:
             self._ring_ready = [None] * self.nslots
             self._copy_stream = torch.cuda.Stream(self.device)
             self.snapshot_loads = 0
+            self.snapshot_bytes = 0
         h = fs["h"].data.data.to(self.device, torch.float32)
         dx = fs["dx"].data.data.to(self.device, torch.float32)
         dy = fs["dy"].data.data.to(self.device, torch.float32)
@@ -508,6 +509,7 @@ class AB3Engine:
         self._ring_It[slot] = It
         self._ring_ready[slot] = ev
         self.snapshot_loads += 1
+        self.snapshot_bytes += sum(self._src[n][src].nbytes for n in ("u", "v", "w", "zeta"))
         # packed windows built from the slot's old contents are stale
         stale = [p for p in self._packed if slot in (p[0], p[1])]
         if stale:

@@ -259,6 +259,7 @@ class ReleaseServer:
             "peak_live_particles": int(peak_live),
             "compactions": compactions,
             "snapshot_loads": getattr(eng, "snapshot_loads", 0),
+            "snapshot_h2d_GBs": getattr(eng, "snapshot_bytes", 0) / elapsed / 1e9 if elapsed > 0 else 0.0,
         }
         return done
 
