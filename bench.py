@@ -31,8 +31,10 @@ all inputs resident in HBM; particle-steps count the particles active during
 the timed steps (average of the active counts before and after).
 ``roofline`` is computed for the dominant kernel from algorithmic bytes
 (compulsory particle state + distinct field nodes touched) over its average
-duration measured with HIP events on its stream.  ``cpu_baseline`` times the
-reference-structured FP64 CPU oracle on the host cores over a bounded sample.
+duration measured with HIP events on its stream.  The CPU leg times the
+reference-structured FP64 CPU oracle on the host cores over a bounded sample
+(``cpu_baseline``) and runs it as the checker of the GPU forms' positions
+after 1000 steps on a 100k-particle subsample (``accuracy``).
 """
 
 from __future__ import annotations
@@ -420,6 +422,7 @@ _PROFILE_FOR_FORM = {
     "plain_fp32": "k_ab3_plain_turbulent16M.md",
     "fused_fp32": "k_ab3_fused_f32_turbulent16M.md",
     "fused_bf16": "k_ab3_fused_bf16_turbulent16M.md",
+    "fused_rk2_fp32": "k_rk2_fused_f32_turbulent16M.md",
 }
 
 
@@ -527,6 +530,52 @@ def accuracy(wl, device, nsub, steps):
     return out
 
 
+def build_output(args, forms, grid, cpu, acc, batching, serving) -> dict:
+    """The JSON line: headline = the fastest AB3 form at the 16M-particle workload."""
+    NX, NY, NZ = grid
+    ab3 = [f for k, f in forms.items() if "rk2" not in k] or list(forms.values())
+    head = max(ab3, key=lambda f: f["particle_steps_per_s"])
+
+    out = {
+        "metric": "particle-steps/s (16M, AB3) plain vs batched/BF16; p50/p99 release latency",
+        "value": head["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head["form"].startswith("plain") else "f32",
+        "data": "synthetic (seeded turbulent ROMS fields, uniform random particles)",
+        "config": {
+            "workload": f"{'RK2' if 'rk2' in head['form'] else 'AB3'} dt=120s, {args.particles} particles, "
+                        f"{NX}x{NY}x{NZ} grid, 8 hourly snapshots (FP32 data; the {head['form']} form)",
+            "form": head["form"],
+            "particles": args.particles,
+            "grid": [NX, NY, NZ],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": head["kernel"],
+            "achieved": head["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ)),
+            "algorithmic_bytes": head["algo_bytes"],
+        },
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+        "batching": batching,
+        "serving": serving,
+        "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in forms.items()},
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,7 +593,7 @@ def main():
     ap.add_argument("--layout", choices=("ring", "pair", "quad", "pair_lds", "oct"), default=None,
                     help="stencil layout when not tuning (default: pair)")
     ap.add_argument("--block", type=int, default=128, help="workgroup size when not tuning")
-    ap.add_argument("--accuracy", type=int, default=0, metavar="N",
+    ap.add_argument("--accuracy", type=int, default=100_000, metavar="N",
                     help="also report deviation from the FP64 oracle on the first N particles")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--batched", type=int, default=256, metavar="NREL",
@@ -585,15 +634,6 @@ def main():
         log("copy calibration: 3 copies of 1 GiB (torch copy kernel)")
         del src, dst
 
-    acc = None
-    if args.accuracy:
-        wl_acc = wl
-        if args.accuracy_steps + 5 > args.steps + args.warmup + 5:
-            wl_acc = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.accuracy,
-                                        steps=args.accuracy_steps + 5, dtype=np.float32)
-        acc = accuracy(wl_acc, device, args.accuracy, args.accuracy_steps)
-        del wl_acc
-
     forms = {}
     wanted = args.forms.split(",")
     if "plain_fp32" in wanted:
@@ -606,9 +646,6 @@ def main():
                                     tune=not args.no_tune, layout=args.layout, block=args.block, scheme=scheme)
             log(name, forms[name])
 
-    # the headline is the fastest AB3 form at the 16M-particle workload
-    head = max((f for k, f in forms.items() if "rk2" not in k), key=lambda f: f["particle_steps_per_s"])
-
     batching = None
     if args.batched:
         batching = run_batched(device, args.batched, args.batched_max_size, torch.bfloat16, args.graph_steps)
@@ -618,49 +655,23 @@ def main():
         sg = tuple(int(v) for v in args.serve_grid.lower().split("x"))
         serving = run_serving(device, args.serve, grid=sg, graph_steps=args.graph_steps, tune=not args.no_tune)
         log("serving", serving)
-    cpu = None
+    # CPU leg: the FP64 oracle is timed on a bounded sample of the workload
+    # (the baseline), and run as the checker of every GPU form's positions on
+    # a subsample over 1000 steps (the accuracy report)
+    cpu = acc = None
     if not args.no_cpu:
         t = time.perf_counter()
         cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_particles)
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s:", cpu)
+        if args.accuracy:
+            wl_acc = wl
+            if args.accuracy_steps + 5 > args.steps + args.warmup + 5:
+                wl_acc = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.accuracy,
+                                            steps=args.accuracy_steps + 5, dtype=np.float32)
+            acc = accuracy(wl_acc, device, args.accuracy, args.accuracy_steps)
+            del wl_acc
 
-    out = {
-        "metric": "particle-steps/s (16M, AB3) plain vs batched/BF16; p50/p99 release latency",
-        "value": head["particle_steps_per_s"],
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": {"plain_fp32": "f64", "fused_fp32": "f32", "fused_bf16": "f32"}[head["form"]],
-        "data": "synthetic (seeded turbulent ROMS fields, uniform random particles)",
-        "config": {
-            "workload": f"AB3 dt=120s, {args.particles} particles, {NX}x{NY}x{NZ} grid, 8 hourly snapshots "
-                        f"(FP32 data; the {head['form']} form)",
-            "form": head["form"],
-            "particles": args.particles,
-            "grid": [NX, NY, NZ],
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": head["kernel"],
-            "achieved": head["achieved_GBs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ)),
-            "algorithmic_bytes": head["algo_bytes"],
-        },
-        "cpu_baseline": cpu,
-        "accuracy": acc,
-        "batching": batching,
-        "serving": serving,
-        "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                  for k, v in forms.items()},
-    }
+    out = build_output(args, forms, (NX, NY, NZ), cpu, acc, batching, serving)
     print(json.dumps(out))
 
 

@@ -1,0 +1,45 @@
+"""The bench JSON line's contract, built from synthetic form records (no GPU)."""
+
+import json
+import types
+
+import bench
+
+
+def _form(name, ps, kernel="k"):
+    return {"form": name, "particle_steps_per_s": ps, "ms_per_step": 1.0, "kernel": kernel, "kernel_ms": 0.8,
+            "algo_bytes": 2.0e9, "achieved_GBs": 2500.0, "active_particles": 16e6}
+
+
+def _args(**kw):
+    a = dict(steps=50, warmup=5, particles=16_000_000)
+    a.update(kw)
+    return types.SimpleNamespace(**a)
+
+
+def test_headline_is_fastest_ab3_form_and_keys_present():
+    forms = {"plain_fp32": _form("plain_fp32", 1e9), "fused_fp32": _form("fused_fp32", 1.7e10),
+             "fused_bf16": _form("fused_bf16", 1.9e10), "fused_rk2_fp32": _form("fused_rk2_fp32", 5e10)}
+    out = bench.build_output(_args(), forms, (1024, 768, 50), {"value": 3e7, "unit": "particle-steps/s",
+                                                               "cores": 16, "kind": "port", "sample": "s"},
+                             None, None, None)
+    json.dumps(out)
+    assert out["value"] == 1.9e10 and out["config"]["form"] == "fused_bf16"  # RK2 is not the headline
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+              "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert k in out, k
+    r = out["roofline"]
+    assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["unit"] == "GB/s"
+    assert abs(r["frac"] - 2500.0 / 8000.0) < 1e-12
+    assert r["traffic"] is not None  # read from the committed profile of this form and workload
+    assert out["dtype"] == "f32" and out["n_gpus"] == 1 and out["vs_baseline"] is None
+
+
+def test_rk2_only_run_and_other_workloads():
+    forms = {"fused_rk2_fp32": _form("fused_rk2_fp32", 1e10)}
+    out = bench.build_output(_args(particles=1000), forms, (64, 48, 8), None, None, None, None)
+    assert out["config"]["form"] == "fused_rk2_fp32" and out["config"]["workload"].startswith("RK2")
+    assert out["roofline"]["traffic"] is None  # no committed profile for this workload
+    out = bench.build_output(_args(), {"plain_fp32": _form("plain_fp32", 1e9)}, (1024, 768, 50), None, None,
+                             None, None)
+    assert out["dtype"] == "f64"

@@ -46,7 +46,7 @@ def test_output_events_vs_oracle(device, dtype):
     assert {o.name: o.particle_field for o in outs} == {
         n: f"_output_{wl.fields[n].dtype}" for n in ("u", "w", "zeta", "h", "C")}
     sim = b.build_simulation(len(wl.positions[0]), device=device)
-    # round 0 ran at construction, with every index 0
+    # the first write round ran at construction, with every index 0
     snaps = [(float(sim.tidx), np.zeros(len(wl.positions[0])), np.zeros(len(wl.positions[0])),
               np.zeros(len(wl.positions[0])), np.zeros(len(wl.positions[0]), np.uint8))]
     sim.set_indices(*wl.positions)
