@@ -893,8 +893,10 @@ __device__ __forceinline__ void range_outer(int cmin, int cmax, int N, int &lo, 
     d = max(0, min(cmax + 1, N - 1)) + 2 - lo;
 }
 
-// ceil(2^32 / d): floor(e * m / 2^32) = floor(e / d) for e * (d - 1) < 2^32.
-__device__ __forceinline__ uint64_t div_magic(uint32_t d) { return ((1ull << 32) + d - 1) / d; }
+// m = ceil(2^32 / d) for d >= 2 (from a 32-bit division), so that
+// umulhi(e, m) = floor(e / d) for e * (d - 1) < 2^32; d = 1 is handled by the caller.
+__device__ __forceinline__ uint32_t div_magic(uint32_t d) { return d > 1 ? 0xffffffffu / d + 1u : 0u; }
+__device__ __forceinline__ uint32_t div_by(uint32_t e, uint32_t d, uint32_t m) { return d > 1 ? __umulhi(e, m) : e; }
 
 // Copy box b (d0 planes, EPN dwords per node) of a field with dims (., n1, n2)
 // into the LDS pool with direct-to-LDS loads: lane l of the instruction that
@@ -904,16 +906,17 @@ __device__ __forceinline__ void stage_box(uint32_t *pool, const Box &b, int d0, 
                                           int lane)
 {
     const uint32_t nd = (uint32_t)(d0 * b.d1 * b.d2 * EPN);
-    const uint64_t m2 = div_magic((uint32_t)b.d2), m1 = div_magic((uint32_t)b.d1);
+    const uint32_t d2 = (uint32_t)b.d2, d1 = (uint32_t)b.d1;
+    const uint32_t m2 = div_magic(d2), m1 = div_magic(d1);
     const uint32_t *S = static_cast<const uint32_t *>(src);
     for (uint32_t base = 0; base < nd; base += 64) {
         const uint32_t dw = base + (uint32_t)lane;
         if (dw < nd) {
             const uint32_t e = dw / EPN;
-            const uint32_t row = (uint32_t)(((uint64_t)e * m2) >> 32);
-            const uint32_t x = e - row * (uint32_t)b.d2;
-            const uint32_t zz = (uint32_t)(((uint64_t)row * m1) >> 32);
-            const uint32_t yy = row - zz * (uint32_t)b.d1;
+            const uint32_t row = div_by(e, d2, m2);
+            const uint32_t x = e - row * d2;
+            const uint32_t zz = div_by(row, d1, m1);
+            const uint32_t yy = row - zz * d1;
             const uint64_t node = ((uint64_t)(b.lo0 + (int)zz) * (uint64_t)n1 + (uint64_t)(b.lo1 + (int)yy)) *
                                       (uint64_t)n2 +
                                   (uint64_t)(b.lo2 + (int)x);
