@@ -231,7 +231,7 @@ class AB3Engine:
                 for n in ("u", "v", "w", "zeta"))
             self._ring_It = [None] * self.nslots
             self._ring_ready = [None] * self.nslots
-            self._copy_stream = torch.cuda.Stream(self.device)
+            self._copy_streams = [torch.cuda.Stream(self.device) for _ in range(3)]
             self.snapshot_loads = 0
             self.snapshot_bytes = 0
         h = fs["h"].data.data.to(self.device, torch.float32)
@@ -500,14 +500,20 @@ class AB3Engine:
         slot = It % self.nslots
         src = It % self.nsnap
         compute = torch.cuda.current_stream(self.device)
-        self._copy_stream.wait_stream(compute)
-        with torch.cuda.stream(self._copy_stream):
-            for dst, name in ((self._u, "u"), (self._v, "v"), (self._w, "w"), (self._zeta, "zeta")):
-                dst[slot].copy_(self._src[name][src], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(self._copy_stream)
+        # one side stream per 3-D field: the copies run on separate DMA queues
+        events = []
+        for k, (dst, names) in enumerate(((self._u, ("u",)), (self._v, ("v",)), (self._w, ("w", "zeta")))):
+            cs = self._copy_streams[k]
+            cs.wait_stream(compute)
+            with torch.cuda.stream(cs):
+                for name in names:
+                    ring = self._zeta if name == "zeta" else dst
+                    ring[slot].copy_(self._src[name][src], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(cs)
+            events.append(ev)
         self._ring_It[slot] = It
-        self._ring_ready[slot] = ev
+        self._ring_ready[slot] = events
         self.snapshot_loads += 1
         self.snapshot_bytes += sum(self._src[n][src].nbytes for n in ("u", "v", "w", "zeta"))
         # packed windows built from the slot's old contents are stale
@@ -529,9 +535,10 @@ class AB3Engine:
         for It, slot in zip(need, slots):
             if self._ring_It[slot] != It:
                 self._load_slot(It)
-            ev = self._ring_ready[slot]
-            if ev is not None:
-                compute.wait_event(ev)
+            evs = self._ring_ready[slot]
+            if evs is not None:
+                for ev in evs:
+                    compute.wait_event(ev)
                 self._ring_ready[slot] = None
         nxt = need[-1] + 1
         slot = nxt % self.nslots
