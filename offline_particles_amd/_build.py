@@ -1,9 +1,9 @@
 """Build libopx.so (the HIP kernels + C-ABI) in-tree for gfx950.
 
-Each translation unit gets its own floating-point contraction policy:
-the plain drop-in kernels are compiled with ``-ffp-contract=off`` so their FP64
-results match the reference bit for bit; the fused kernels allow contraction.
-Packed-FP32 instructions are disabled for all device code.
+Both translation units are compiled with ``-ffp-contract=off``: the plain
+drop-in kernels so their FP64 results match the reference bit for bit, the
+fused kernels so every stencil layout, launch form and release mix rounds
+identically. Packed-FP32 instructions are disabled for all device code.
 """
 
 from __future__ import annotations

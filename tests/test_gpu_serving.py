@@ -152,3 +152,24 @@ def test_serving_realtime_latency(device):
     assert 0 < st["latency_p50_s"] <= st["latency_p99_s"] < 60
     for r in done:
         assert r.completed_at >= r.arrival
+
+
+def test_finished_release_is_parked_by_rebin(device):
+    """A release whose steps are done stops moving; the next re-bin moves its
+    particles behind the active ones (not launched) without changing them."""
+    wl = _wl()
+    rng = np.random.default_rng(9)
+    fs = build_fieldset(wl, device=device)
+    eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=3000, rebin_every=0)
+    eng.add_release(*_release(wl, rng, 1000), start_time=0.0, nsteps=10)
+    eng.add_release(*_release(wl, rng, 2000), start_time=0.0, nsteps=40)
+    eng.step(10)
+    before = [t.cpu().numpy().copy() for t in eng.extract(0)]
+    eng.rebin()
+    n_active = int(eng._n_active_dev.item())
+    live = int((eng.status()[1000:] < 128).sum().item())
+    assert n_active == live  # only the running release's active particles are launched
+    eng.step(30)
+    after = [t.cpu().numpy() for t in eng.extract(0)]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b, equal_nan=True)
