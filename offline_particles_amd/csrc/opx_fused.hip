@@ -67,6 +67,37 @@ __device__ __forceinline__ V gload(const V *p, uint32_t i)
     }
 }
 
+// p[i] and p[i + 1] of a 4 or 8-byte element type as ONE 8 or 16-byte load.
+// Multi-dword global loads need only dword alignment (the two low address
+// bits are ignored), so the pair needs no more than the element's alignment.
+// The compiler does not merge the two loads by itself: with a 32-bit index
+// i + 1 may wrap, so it cannot prove the second address is the first + size.
+// The casts are spelled out: passed through a template such as gbl(), the
+// typedefs would lose their reduced alignment.
+typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef const __attribute__((address_space(1))) u32x2_a4 *gptr_u32x2_a4;
+typedef const __attribute__((address_space(1))) u32x4_a8 *gptr_u32x4_a8;
+
+template <typename V>
+__device__ __forceinline__ void gload_pair(const V *p, uint32_t i, V &a, V &b)
+{
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8, "element size");
+    // Elements are copied out before the bit casts: a bit cast of a vector
+    // element expression (t.y) reads element 0.
+    if constexpr (sizeof(V) == 4) {
+        const u32x2_a4 t = *(gptr_u32x2_a4)(reinterpret_cast<const uint32_t *>(p) + i);
+        const uint32_t x = t.x, y = t.y;
+        a = __builtin_bit_cast(V, x);
+        b = __builtin_bit_cast(V, y);
+    } else {
+        const u32x4_a8 t = *(gptr_u32x4_a8)(reinterpret_cast<const u32x2 *>(p) + i);
+        const u32x2 x = {t.x, t.y}, y = {t.z, t.w};
+        a = __builtin_bit_cast(V, x);
+        b = __builtin_bit_cast(V, y);
+    }
+}
+
 __device__ __forceinline__ float ld3(const float *__restrict__ p, uint32_t i) { return gbl(p)[i]; }
 __device__ __forceinline__ float ld3(const uint16_t *__restrict__ p, uint32_t i)
 {
@@ -239,10 +270,13 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
             v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
         }
     } else if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
-        const auto *P = static_cast<const typename Packed<T>::pair *>(pp);
+        using Pair = typename Packed<T>::pair;
+        const auto *P = static_cast<const Pair *>(pp);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float2 e0 = unpair(gload(P, rows[r])), e1 = unpair(gload(P, rows[r] + 1));
+            Pair a, b;
+            gload_pair(P, rows[r], a, b); // the x-neighbours (i, i+1) in one load
+            const float2 e0 = unpair(a), e1 = unpair(b);
             v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
             v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
         }
@@ -310,7 +344,6 @@ struct Corners2 {
 __device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const float *__restrict__ dy_, int NX,
                                              int by, int bx, bool want_dxdy, Corners2 &c)
 {
-    const auto *dy = gbl(dy_);
     const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
     const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
 #pragma unroll
@@ -320,8 +353,10 @@ __device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const
         c.z0[k] = m.y;
         c.z1[k] = m.z;
         c.dx[k] = m.w;
-        if (want_dxdy)
-            c.dy[k] = dy[o[k]];
+    }
+    if (want_dxdy) {
+        gload_pair(dy_, o[0], c.dy[0], c.dy[1]);
+        gload_pair(dy_, o[2], c.dy[2], c.dy[3]);
     }
     c.by = by;
     c.bx = bx;
