@@ -216,6 +216,9 @@ FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
 # position only (a lower bound: stage 2 reads around x*).
 FUSED_RK2_STATE_BYTES = 1 + 2 + 4 + 12 + 16
 
+# stencil layout names -> OPX_LAYOUT_* values of the C-ABI
+LAYOUTS = {"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4, "ypair": 5}
+
 
 def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False, layout=None, block=128,
               scheme="ab3"):
@@ -233,7 +236,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     nsteps_max = int((wl.time_array[-1] - wl.time_array[0]) // wl.dt) - 2
     eng.add_release(*wl.positions, nsteps=nsteps_max)
     if layout is not None:
-        eng.set_layout({"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4}[layout])
+        eng.set_layout(LAYOUTS[layout])
     eng.block = block
     eng.rebin()
     eng.step(warmup)
@@ -276,7 +279,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
         "rebin_every": eng.rebin_every,
         "xlow": eng.xlow,
         "block": eng.block,
-        "layout": {0: "ring", 1: "pair", 2: "quad", 3: "pair_lds", 4: "oct"}[eng.layout],
+        "layout": {v: k for k, v in LAYOUTS.items()}[eng.layout],
         "tuning": eng.tuning,
     }
     del eng
@@ -319,7 +322,7 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
         tuning = eng.autotune(xlows=(4, 5), blocks=(128,), cadence_probe=30)
         tuning = {k: v for k, v in tuning.items() if k != "trials"}
         log("serving autotune:", tuning)
-        layout = {"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4}[tuning["layout"]]
+        layout = LAYOUTS[tuning["layout"]]
         xlow, block, rebin_every = tuning["xlow"], tuning["block"], tuning["rebin_every"]
         del eng
         torch.cuda.empty_cache()
@@ -376,7 +379,7 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     last = max(s + r.nsteps for s, r in zip(start_step, rels))
     eng = AB3Engine(fs, wl.time_array, np.float64(dt), storage_dtype=storage, device=device, capacity=total,
                     max_releases=nrel, rebin_every=tuning["rebin_every"], xlow=tuning["xlow"])
-    eng.set_layout({"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4}[tuning["layout"]])
+    eng.set_layout(LAYOUTS[tuning["layout"]])
     eng.block = tuning["block"]
     eng.graph_steps = graph_steps
     scheduled = sum(r.size * r.nsteps for r in rels)
@@ -590,7 +593,7 @@ def main():
     ap.add_argument("--rebin-every", type=int, default=25)
     ap.add_argument("--xlow", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --xlow/--rebin-every)")
-    ap.add_argument("--layout", choices=("ring", "pair", "quad", "pair_lds", "oct"), default=None,
+    ap.add_argument("--layout", choices=tuple(LAYOUTS), default=None,
                     help="stencil layout when not tuning (default: pair)")
     ap.add_argument("--block", type=int, default=128, help="workgroup size when not tuning")
     ap.add_argument("--accuracy", type=int, default=100_000, metavar="N",

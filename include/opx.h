@@ -226,6 +226,9 @@ enum opx_layout {
                                 (64-thread workgroups; `block` is ignored) */
     OPX_LAYOUT_OCT = 4,      /* packed window, node i = the (y, x) corners i, i+1, i+n2, i+n2+1 of s0
                                 and of s1: one load per z-plane (16 B for BF16, 32 B for FP32) */
+    OPX_LAYOUT_YPAIR = 5,    /* packed window, node i = (s0[i], s1[i], s0[i+n2], s1[i+n2]); nodes i, i+1
+                                hold a z-plane's four (y, x) corners: one 16-byte load per z-plane
+                                for BF16, two for FP32 */
 };
 
 /* A ring of `nslots` snapshots per time-dependent field, contiguous with the

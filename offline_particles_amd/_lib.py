@@ -104,7 +104,7 @@ class opx_grid(ctypes.Structure):
                 ("xidx_min", ctypes.c_double), ("xidx_max", ctypes.c_double)]
 
 
-OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD, OPX_LAYOUT_PAIR_LDS, OPX_LAYOUT_OCT = 0, 1, 2, 3, 4
+OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD, OPX_LAYOUT_PAIR_LDS, OPX_LAYOUT_OCT, OPX_LAYOUT_YPAIR = 0, 1, 2, 3, 4, 5
 
 
 class opx_window(ctypes.Structure):

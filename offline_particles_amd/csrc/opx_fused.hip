@@ -233,12 +233,14 @@ struct Packed<float> {
     using pair = float2;
     using quad = float4;
     using oct = float8;
+    using ypair = float4; // s0[i], s1[i], s0[i+n2], s1[i+n2]
 };
 template <>
 struct Packed<uint16_t> {
     using pair = uint32_t; // lo: s0, hi: s1
     using quad = uint2;    // x: s0[i] | s0[i+1] << 16, y: s1[i] | s1[i+1] << 16
     using oct = uint4;     // x: s0[i] | s0[i+1] << 16, y: s0[i+n2] | s0[i+n2+1] << 16, z, w: the same of s1
+    using ypair = uint2;   // x: s0[i] | s1[i] << 16, y: s0[i+n2] | s1[i+n2] << 16
 };
 
 __device__ __forceinline__ float bf_lo(uint32_t v) { return __uint_as_float(v << 16); }
@@ -287,6 +289,27 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
             float4 e = unquad(gload(P, rows[r]));
             v[2 * r] = blend_rn(e.x, e.z, gt, ft);
             v[2 * r + 1] = blend_rn(e.y, e.w, gt, ft);
+        }
+    } else if constexpr (LAYOUT == OPX_LAYOUT_YPAIR) {
+        // nodes i, i+1 of a z-plane: (y, x), (y+1, x) and (y, x+1), (y+1, x+1) of both snapshots
+        const uint32_t planes[2] = {rows[0], rows[2]};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if constexpr (sizeof(T) == 4) {
+                const auto *P = static_cast<const float4 *>(pp);
+                const float4 a = gload(P, planes[k]), c = gload(P, planes[k] + 1);
+                v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
+                v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
+                v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
+                v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
+            } else {
+                uint2 a, c;
+                gload_pair(static_cast<const uint2 *>(pp), planes[k], a, c); // one 16-byte load
+                v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
+                v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
+                v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
+                v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
+            }
         }
     } else {
         // OCT: one node per z-plane holds that plane's four (y, x) corners of both snapshots
@@ -1071,6 +1094,19 @@ __global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *_
         }
         return;
     }
+    if constexpr (LAYOUT == OPX_LAYOUT_YPAIR) {
+        // the y+1 neighbour of the last row is never a stencil node of a base here
+        const int64_t y1 = ((i / n2) % n1) + 1 < n1 ? i + n2 : i;
+        if constexpr (sizeof(T) == 4) {
+            const float *P = (const float *)p, *Q = (const float *)q;
+            static_cast<float4 *>(out)[i] = make_float4(P[i], Q[i], P[y1], Q[y1]);
+        } else {
+            const uint16_t *P = (const uint16_t *)p, *Q = (const uint16_t *)q;
+            static_cast<uint2 *>(out)[i] = make_uint2((uint32_t)P[i] | ((uint32_t)Q[i] << 16),
+                                                      (uint32_t)P[y1] | ((uint32_t)Q[y1] << 16));
+        }
+        return;
+    }
     if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
         if constexpr (sizeof(T) == 4) {
             static_cast<float2 *>(out)[i] = make_float2(((const float *)p)[i], ((const float *)q)[i]);
@@ -1517,8 +1553,8 @@ int check_window(const opx_window *w, const opx_grid *g)
     OPX_REQUIRE(w->nslots >= 2 && w->nslots <= kMaxSlots, "window needs 2 to %d slots (got %d)", kMaxSlots,
                 w->nslots);
     OPX_REQUIRE(w->npacked >= 0 && w->npacked <= 2, "npacked %d not in [0, 2]", w->npacked);
-    OPX_REQUIRE(w->npacked == 0 || (w->layout >= OPX_LAYOUT_PAIR && w->layout <= OPX_LAYOUT_OCT),
-                "packed windows need layout PAIR, QUAD, PAIR_LDS or OCT (got %d)", w->layout);
+    OPX_REQUIRE(w->npacked == 0 || (w->layout >= OPX_LAYOUT_PAIR && w->layout <= OPX_LAYOUT_YPAIR),
+                "packed windows need layout PAIR, QUAD, PAIR_LDS, OCT or YPAIR (got %d)", w->layout);
     for (int j = 0; j < w->npacked; ++j)
         OPX_REQUIRE(w->pu[j] && w->pv[j] && w->pw[j] && w->p2[j], "packed window %d has a null pointer", j);
     OPX_REQUIRE(w->npacked == 0 || w->dy != nullptr, "packed windows need the static dy array");
@@ -1586,11 +1622,13 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_PAIR)
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_QUAD)
         OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_OCT)
+        OPX_FUSED_LAUNCH_L(float, OPX_LAYOUT_YPAIR)
     } else {
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
         OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_OCT)
+        OPX_FUSED_LAUNCH_L(uint16_t, OPX_LAYOUT_YPAIR)
     }
 #undef OPX_FUSED_LAUNCH_L
 #undef OPX_FUSED_LAUNCH
@@ -1618,11 +1656,13 @@ int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &wi
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_PAIR)
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_QUAD)
         OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_OCT)
+        OPX_RK2_LAUNCH_L(float, OPX_LAYOUT_YPAIR)
     } else {
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_RING)
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_PAIR)
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_QUAD)
         OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_OCT)
+        OPX_RK2_LAUNCH_L(uint16_t, OPX_LAYOUT_YPAIR)
     }
 #undef OPX_RK2_LAUNCH_L
 #undef OPX_RK2_LAUNCH
@@ -1683,7 +1723,7 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_OCT, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
@@ -1721,7 +1761,7 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
-    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_OCT, "unknown layout %d", layout);
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
     OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
     if (block == 0)
         block = 128;
@@ -1735,7 +1775,13 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
 static int elem_bytes(int dtype3, int layout)
 {
     const int b = dtype3 == OPX_F32 ? 4 : 2;
-    return layout == OPX_LAYOUT_PAIR ? 2 * b : (layout == OPX_LAYOUT_QUAD ? 4 * b : (layout == OPX_LAYOUT_OCT ? 8 * b : 0));
+    switch (layout) {
+    case OPX_LAYOUT_PAIR: return 2 * b;
+    case OPX_LAYOUT_QUAD: return 4 * b;
+    case OPX_LAYOUT_OCT: return 8 * b;
+    case OPX_LAYOUT_YPAIR: return 4 * b;
+    default: return 0;
+    }
 }
 
 int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t *bytes_u, uint64_t *bytes_v,
@@ -1746,8 +1792,9 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
         layout = OPX_LAYOUT_PAIR;
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT,
-                "layout must be PAIR, QUAD, OCT or PAIR_LDS");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT ||
+                    layout == OPX_LAYOUT_YPAIR,
+                "layout must be PAIR, QUAD, OCT, YPAIR or PAIR_LDS");
     OPX_REQUIRE(bytes_u && bytes_v && bytes_w && bytes_2d, "null output");
     *bytes_2d = (uint64_t)16 * g->NY * g->NX;
     const uint64_t e = (uint64_t)elem_bytes(dtype3, layout);
@@ -1766,8 +1813,9 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
     OPX_REQUIRE(win->dtype3 == OPX_F32 || win->dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
         layout = OPX_LAYOUT_PAIR;
-    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT,
-                "layout must be PAIR, QUAD, OCT or PAIR_LDS");
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT ||
+                    layout == OPX_LAYOUT_YPAIR,
+                "layout must be PAIR, QUAD, OCT, YPAIR or PAIR_LDS");
     OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
                 s1, win->nslots);
     OPX_REQUIRE(pu && pv && pw && p2, "null output window");
@@ -1798,6 +1846,8 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
                 OPX_PACK(float, OPX_LAYOUT_PAIR);
             else if (layout == OPX_LAYOUT_QUAD)
                 OPX_PACK(float, OPX_LAYOUT_QUAD);
+            else if (layout == OPX_LAYOUT_YPAIR)
+                OPX_PACK(float, OPX_LAYOUT_YPAIR);
             else
                 OPX_PACK(float, OPX_LAYOUT_OCT);
         } else {
@@ -1806,6 +1856,8 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
                 OPX_PACK(uint16_t, OPX_LAYOUT_PAIR);
             else if (layout == OPX_LAYOUT_QUAD)
                 OPX_PACK(uint16_t, OPX_LAYOUT_QUAD);
+            else if (layout == OPX_LAYOUT_YPAIR)
+                OPX_PACK(uint16_t, OPX_LAYOUT_YPAIR);
             else
                 OPX_PACK(uint16_t, OPX_LAYOUT_OCT);
         }

@@ -56,7 +56,7 @@ _ROMS_STAGGERS = {
 _STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
 
 LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad",
-                _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct"}
+                _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct", _lib.OPX_LAYOUT_YPAIR: "ypair"}
 
 
 def _bits(n: int) -> int:
@@ -583,7 +583,7 @@ class AB3Engine:
             w.pu[j], w.pv[j], w.pw[j], w.p2[j] = (b.data_ptr() for b in bufs)
 
     def set_layout(self, layout: int) -> None:
-        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD, _PAIR_LDS or _OCT)."""
+        """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD, _PAIR_LDS, _OCT or _YPAIR)."""
         if layout != self.layout:
             self.layout = int(layout)
             self._packed = []
@@ -683,8 +683,8 @@ class AB3Engine:
 
         - bin shape: ``xlow`` low bits of the x cell kept contiguous in the sort
           key (Morton order above them), i.e. bins 2^xlow cells long in x;
-        - stencil layout (ring / pair / quad) and workgroup size of the fused
-          kernel;
+        - stencil layout (ring / pair / quad / pair_lds / oct / ypair) and
+          workgroup size of the fused kernel;
         - re-sort cadence: after a rebin the step time grows as particles drift
           out of order; with rebin cost R and growth a per step, the average
           cost per step is minimal at N = sqrt(2 R / a).
@@ -696,7 +696,7 @@ class AB3Engine:
 
         if layouts is None:
             layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
-                       _lib.OPX_LAYOUT_OCT)
+                       _lib.OPX_LAYOUT_OCT, _lib.OPX_LAYOUT_YPAIR)
         saved_every, self.rebin_every = self.rebin_every, 0
         trials = {}
         for xlow in xlows:

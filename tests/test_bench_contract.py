@@ -43,3 +43,9 @@ def test_rk2_only_run_and_other_workloads():
     out = bench.build_output(_args(), {"plain_fp32": _form("plain_fp32", 1e9)}, (1024, 768, 50), None, None,
                              None, None)
     assert out["dtype"] == "f64"
+
+
+def test_layout_names_match_engine():
+    from offline_particles_amd.engine import LAYOUT_NAMES
+
+    assert bench.LAYOUTS == {name: value for value, name in LAYOUT_NAMES.items()}

@@ -122,13 +122,14 @@ def test_clock_leaves_time_array(device):
 
 @pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
 def test_stencil_layouts_identical(device, storage):
-    """RING, PAIR, QUAD, OCT and LDS-staged PAIR stencils blend the same values in the same order: identical bits."""
+    """RING, PAIR, QUAD, OCT, YPAIR and LDS-staged PAIR stencils blend the same values in the same order:
+    identical bits."""
     from offline_particles_amd import _lib
 
     wl = _turb()
     res = []
     for layout in (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
-                   _lib.OPX_LAYOUT_OCT):
+                   _lib.OPX_LAYOUT_OCT, _lib.OPX_LAYOUT_YPAIR):
         eng = _engine(wl, device, storage=storage, rebin_every=11)
         eng.set_layout(layout)
         eng.step(100)  # crosses snapshot boundaries (hourly snapshots, dt = 120 s)
@@ -198,13 +199,14 @@ def test_fused_rk2_eddy_vs_oracle(device):
 
 @pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
 def test_fused_rk2_layouts_and_graphs_identical(device, storage):
-    """RK2: RING/PAIR/QUAD layouts, re-binning and graph replays give identical bits."""
+    """RK2: every stencil layout, re-binning and graph replays give identical bits."""
     from offline_particles_amd import _lib
 
     wl = _turb()
     res = []
     for layout, graph_steps, rebin in ((_lib.OPX_LAYOUT_RING, 0, 0), (_lib.OPX_LAYOUT_PAIR, 5, 11),
-                                       (_lib.OPX_LAYOUT_QUAD, 8, 7), (_lib.OPX_LAYOUT_OCT, 3, 5)):
+                                       (_lib.OPX_LAYOUT_QUAD, 8, 7), (_lib.OPX_LAYOUT_OCT, 3, 5),
+                                       (_lib.OPX_LAYOUT_YPAIR, 4, 9)):
         eng = _engine(wl, device, storage=storage, rebin_every=rebin, scheme="rk2")
         eng.set_layout(layout)
         eng.graph_steps = graph_steps
