@@ -221,7 +221,7 @@ LAYOUTS = {"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4, "ypair": 5}
 
 
 def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False, layout=None, block=128,
-              scheme="ab3"):
+              scheme="ab3", launch_steps=1):
     import torch
 
     from offline_particles_amd.engine import AB3Engine
@@ -238,6 +238,8 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     if layout is not None:
         eng.set_layout(LAYOUTS[layout])
     eng.block = block
+    if scheme == "ab3":
+        eng.launch_steps = launch_steps
     eng.rebin()
     eng.step(warmup)
     if tune:
@@ -255,7 +257,9 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.step(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kms = float(np.mean([a.elapsed_time(b) for a, b in eng.timer]))
+    # per launch: the launches that took the full step count (a re-bin can cut one short)
+    m = max(n for _, _, n in eng.timer)
+    kms = float(np.mean([a.elapsed_time(b) for a, b, n in eng.timer if n == m]))
     eng.timer = None
     a1 = eng.active_count()
     eng.check_clock()
@@ -273,6 +277,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
         "kernel": (f"k_ab3_fused<{'bf16' if sb == 2 else 'f32'}> (opx_ab3_fused)" if scheme == "ab3"
                    else f"k_rk2_fused<{'bf16' if sb == 2 else 'f32'}> (opx_rk2_fused)"),
         "kernel_ms": kms,
+        "steps_per_launch": m,
         "algo_bytes": algo,
         "field_bytes_touched": fbytes,
         "achieved_GBs": algo / (kms * 1e-3) / 1e9,
@@ -606,6 +611,7 @@ def main():
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
+    ap.add_argument("--launch-steps", type=int, default=1, help="AB3 steps per fused launch (1..64)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -646,7 +652,8 @@ def main():
                              ("fused_rk2_fp32", torch.float32, "rk2"), ("fused_rk2_bf16", torch.bfloat16, "rk2")):
         if name in wanted:
             forms[name] = run_fused(wl, device, args.steps, args.warmup, dt, args.rebin_every, args.xlow,
-                                    tune=not args.no_tune, layout=args.layout, block=args.block, scheme=scheme)
+                                    tune=not args.no_tune, layout=args.layout, block=args.block, scheme=scheme,
+                                    launch_steps=args.launch_steps)
             log(name, forms[name])
 
     batching = None

@@ -298,6 +298,15 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
                       const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
                       int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream);
 
+/* opx_clock_advance over `nsteps` (1..64) consecutive global steps: step k's
+ * entries go to table[k * table_stride + r] (table_stride >= nrel), and the
+ * device step counter advances by nsteps.  The same per-step arithmetic as
+ * nsteps calls of opx_clock_advance. */
+int opx_clock_advance_steps(opx_release_clock *clocks, opx_release_step *table, int32_t table_stride,
+                            int32_t nsteps, int32_t nrel, const int32_t *nrel_dev, const double *time_array,
+                            int32_t ntime, int32_t nslots, int32_t global_step, int32_t *step_dev, double dt,
+                            double alpha, int32_t *err_flag, void *stream);
+
 /* One fused AB3 step (validation -> ab3_w_advection -> ab3_post_w_advection)
  * for particles [0, n) (or [0, *n_active) when n_active is not NULL), one
  * thread per particle in workgroups of `block` threads (128 or 256; 0 = 128).
@@ -311,6 +320,22 @@ int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
 int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                       int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
                       void *stream);
+
+/* `nsteps` (1..64) fused AB3 steps in one launch, with step k's release
+ * weights at table[k * table_stride + release] (from opx_clock_advance_steps).
+ * Each particle's state stays in registers between the steps, and the u, v, w
+ * and 2-D stencil nodes it read are blended again at the next step's weights
+ * while its cells and snapshot pair are unchanged: the results are those of
+ * nsteps one-step launches (timesteppers.py:296-305 applied nsteps times),
+ * bit for bit.  No counterpart in the reference, which launches per step. */
+int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                        const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
+                        const int32_t *n_active, int32_t block, void *stream);
+
+/* opx_ab3_fused_steps with the window descriptor in device memory. */
+int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                            int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                            double dt, const int32_t *n_active, int32_t block, void *stream);
 
 /* One fused RK2 step (validation -> _rk2_step_1 at t -> _rk2_step_2 at
  * t + alpha dt -> _rk2_update at t + dt, rk2_w_advection.pyx:31-354 in the

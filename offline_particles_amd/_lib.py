@@ -128,6 +128,8 @@ _P = ctypes.POINTER
 _SIGNATURES.update({
     "opx_clock_advance": [_vp, _vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, ctypes.c_double, _vp, _vp],
+    "opx_clock_advance_steps": [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, ctypes.c_int32,
+                                ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp],
     "opx_rk2_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_ab3_isopycnal_following": [_P(opx_particles), _vp, _P(opx_scalars), _P(opx_roms_fields), _P(opx_field),
@@ -143,6 +145,10 @@ _SIGNATURES.update({
                           ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,
                       _vp],
+    "opx_ab3_fused_steps": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_double, _vp, ctypes.c_int32, _vp],
+    "opx_ab3_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],

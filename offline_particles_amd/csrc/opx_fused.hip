@@ -253,84 +253,144 @@ __device__ __forceinline__ float4 unquad(uint2 v)
     return make_float4(bf_lo(v.x), bf_hi(v.x), bf_lo(v.y), bf_hi(v.y));
 }
 
-// The 8 blended stencil values (order 000, 001, 010, 011, 100, 101, 110, 111
-// over (axis0, axis1, axis2)) of a field with dims (., n1, n2) and base
-// (b0, b1, b2).  RING reads slot pointers p and q; PAIR and QUAD read packed
-// window p.  The blend is p*gt + q*ft in FP32 in every layout.
+// Raw stencil nodes of one field at one base: both snapshots, before the time
+// blend.  A particle that stays in its cells keeps them and blends them again
+// at the next step's weights instead of reloading them.
 template <typename T, int LAYOUT>
-__device__ __forceinline__ void corners(const void *__restrict__ pp, const void *__restrict__ qq, float gt, float ft,
-                                        int n1, int n2, int b0, int b1, int b2, float v[8])
+struct Raw3;
+template <typename T>
+struct Raw3<T, OPX_LAYOUT_PAIR> {
+    typename Packed<T>::pair n[8]; // the 8 corners in stencil order
+};
+template <typename T>
+struct Raw3<T, OPX_LAYOUT_QUAD> {
+    typename Packed<T>::quad n[4]; // the 4 rows (axis0, axis1) of x-pairs
+};
+template <typename T>
+struct Raw3<T, OPX_LAYOUT_OCT> {
+    typename Packed<T>::oct n[2]; // the 2 z-planes
+};
+template <typename T>
+struct Raw3<T, OPX_LAYOUT_YPAIR> {
+    typename Packed<T>::ypair n[4]; // nodes i, i+1 of the 2 z-planes
+};
+
+// Loads the raw nodes of a field with dims (., n1, n2) at base (b0, b1, b2)
+// from packed window pp.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void load_raw(const void *__restrict__ pp, int n1, int n2, int b0, int b1, int b2,
+                                         Raw3<T, LAYOUT> &r)
 {
     const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
     const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
     const uint32_t rows[4] = {b, b + s1, b + s0, b + s0 + s1};
-    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
-        const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
+    if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
+        const auto *P = static_cast<const typename Packed<T>::pair *>(pp);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            v[2 * r] = blend(p, q, rows[r], gt, ft);
-            v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
-        }
-    } else if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
-        using Pair = typename Packed<T>::pair;
-        const auto *P = static_cast<const Pair *>(pp);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            Pair a, b;
-            gload_pair(P, rows[r], a, b); // the x-neighbours (i, i+1) in one load
-            const float2 e0 = unpair(a), e1 = unpair(b);
-            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
-            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
-        }
+        for (int k = 0; k < 4; ++k)
+            gload_pair(P, rows[k], r.n[2 * k], r.n[2 * k + 1]); // the x-neighbours (i, i+1) in one load
     } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
         const auto *P = static_cast<const typename Packed<T>::quad *>(pp);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            float4 e = unquad(gload(P, rows[r]));
-            v[2 * r] = blend_rn(e.x, e.z, gt, ft);
-            v[2 * r + 1] = blend_rn(e.y, e.w, gt, ft);
-        }
-    } else if constexpr (LAYOUT == OPX_LAYOUT_YPAIR) {
-        // nodes i, i+1 of a z-plane: (y, x), (y+1, x) and (y, x+1), (y+1, x+1) of both snapshots
-        const uint32_t planes[2] = {rows[0], rows[2]};
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if constexpr (sizeof(T) == 4) {
-                const auto *P = static_cast<const float4 *>(pp);
-                const float4 a = gload(P, planes[k]), c = gload(P, planes[k] + 1);
-                v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
-                v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
-                v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
-                v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
-            } else {
-                uint2 a, c;
-                gload_pair(static_cast<const uint2 *>(pp), planes[k], a, c); // one 16-byte load
-                v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
-                v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
-                v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
-                v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
-            }
-        }
-    } else {
-        // OCT: one node per z-plane holds that plane's four (y, x) corners of both snapshots
+        for (int k = 0; k < 4; ++k)
+            r.n[k] = gload(P, rows[k]);
+    } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
+        // one node per z-plane holds that plane's four (y, x) corners of both snapshots
         const auto *P = static_cast<const typename Packed<T>::oct *>(pp);
-        const uint32_t planes[2] = {rows[0], rows[2]};
+        r.n[0] = gload(P, rows[0]);
+        r.n[1] = gload(P, rows[2]);
+    } else {
+        // YPAIR: nodes i, i+1 of a z-plane: (y, x), (y+1, x) and (y, x+1), (y+1, x+1)
+        const auto *P = static_cast<const typename Packed<T>::ypair *>(pp);
+        if constexpr (sizeof(T) == 4) {
+            r.n[0] = gload(P, rows[0]);
+            r.n[1] = gload(P, rows[0] + 1);
+            r.n[2] = gload(P, rows[2]);
+            r.n[3] = gload(P, rows[2] + 1);
+        } else {
+            gload_pair(P, rows[0], r.n[0], r.n[1]); // one 16-byte load per z-plane
+            gload_pair(P, rows[2], r.n[2], r.n[3]);
+        }
+    }
+}
+
+// The 8 blended stencil values (order 000, 001, 010, 011, 100, 101, 110, 111
+// over (axis0, axis1, axis2)) of raw nodes: p*gt + q*ft in FP32, the same
+// operations in every layout.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void blend_raw(const Raw3<T, LAYOUT> &r, float gt, float ft, float v[8])
+{
+    if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float2 e = unpair(r.n[k]);
+            v[k] = blend_rn(e.x, e.y, gt, ft);
+        }
+    } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 e = unquad(r.n[k]);
+            v[2 * k] = blend_rn(e.x, e.z, gt, ft);
+            v[2 * k + 1] = blend_rn(e.y, e.w, gt, ft);
+        }
+    } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if constexpr (sizeof(T) == 4) {
-                const float8 e = gload(P, planes[k]);
+                const float8 &e = r.n[k];
                 v[4 * k + 0] = blend_rn(e.a.x, e.b.x, gt, ft);
                 v[4 * k + 1] = blend_rn(e.a.y, e.b.y, gt, ft);
                 v[4 * k + 2] = blend_rn(e.a.z, e.b.z, gt, ft);
                 v[4 * k + 3] = blend_rn(e.a.w, e.b.w, gt, ft);
             } else {
-                const uint4 e = gload(P, planes[k]);
+                const uint4 &e = r.n[k];
                 v[4 * k + 0] = blend_rn(bf_lo(e.x), bf_lo(e.z), gt, ft);
                 v[4 * k + 1] = blend_rn(bf_hi(e.x), bf_hi(e.z), gt, ft);
                 v[4 * k + 2] = blend_rn(bf_lo(e.y), bf_lo(e.w), gt, ft);
                 v[4 * k + 3] = blend_rn(bf_hi(e.y), bf_hi(e.w), gt, ft);
             }
         }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if constexpr (sizeof(T) == 4) {
+                const float4 &a = r.n[2 * k], &c = r.n[2 * k + 1];
+                v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
+                v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
+                v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
+                v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
+            } else {
+                const uint2 &a = r.n[2 * k], &c = r.n[2 * k + 1];
+                v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
+                v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
+                v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
+                v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
+            }
+        }
+    }
+}
+
+// The 8 blended stencil values of a field with dims (., n1, n2) and base
+// (b0, b1, b2).  RING reads slot pointers p and q; the packed layouts read
+// window p.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void corners(const void *__restrict__ pp, const void *__restrict__ qq, float gt, float ft,
+                                        int n1, int n2, int b0, int b1, int b2, float v[8])
+{
+    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
+        const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
+        const uint32_t rows[4] = {b, b + s1, b + s0, b + s0 + s1};
+        const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            v[2 * r] = blend(p, q, rows[r], gt, ft);
+            v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
+        }
+    } else {
+        Raw3<T, LAYOUT> r;
+        load_raw<T, LAYOUT>(pp, n1, n2, b0, b1, b2, r);
+        blend_raw<T, LAYOUT>(r, gt, ft, v);
     }
 }
 
@@ -440,6 +500,7 @@ struct Step {
 };
 
 constexpr int kMaxSlots = 1023; // slot fields of opx_release_step are 10 bits
+constexpr int kMaxLaunchSteps = 64; // steps one fused launch may take
 
 __device__ __forceinline__ int next_slot(int s, int nslots) { return s + 1 == nslots ? 0 : s + 1; }
 
@@ -524,18 +585,11 @@ __device__ __forceinline__ void weights(double tidx, int nslots, float &ft, floa
     s1 = (int)((I + 1) % nslots);
 }
 
-__global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int nrel, const int32_t *nrel_dev,
-                        const double *ta, int ntime, int nslots, int step, const int32_t *step_dev, double dt,
-                        double alpha, int32_t *err)
+// One release's clock at global step `step`: its table entry (weights and
+// slots at t, t + dt and the RK2 time t + alpha dt), then the advance.
+__device__ __forceinline__ void clock_step(opx_release_clock &c, opx_release_step *entry, const double *ta,
+                                           int ntime, int nslots, int step, double dt, double alpha, int32_t *err)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nrel_dev)
-        nrel = min(nrel, *nrel_dev);
-    if (step_dev)
-        step = *step_dev;
-    if (r >= nrel)
-        return;
-    opx_release_clock c = clocks[r];
     opx_release_step e{};
     float gt_unused;
     int s1_unused, s0 = 0, s0p = 0, s0m = 0, active = 0;
@@ -558,13 +612,31 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
             if (err)
                 *err = 1;
         }
-        clocks[r] = c;
     }
     e.slots = (uint32_t)s0 | ((uint32_t)s0p << 10) | ((uint32_t)s0m << 20) | ((uint32_t)active << 30);
-    table[r] = e;
+    *entry = e;
 }
 
-__global__ void k_step_incr(int32_t *step) { *step += 1; }
+// Advances each release's clock over `nsteps` global steps from `step`,
+// writing step k's entry to table[k * tstride + r].
+__global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int tstride, int nsteps, int nrel,
+                        const int32_t *nrel_dev, const double *ta, int ntime, int nslots, int step,
+                        const int32_t *step_dev, double dt, double alpha, int32_t *err)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nrel_dev)
+        nrel = min(nrel, *nrel_dev);
+    if (step_dev)
+        step = *step_dev;
+    if (r >= nrel)
+        return;
+    opx_release_clock c = clocks[r];
+    for (int k = 0; k < nsteps; ++k)
+        clock_step(c, table + (int64_t)k * tstride + r, ta, ntime, nslots, step + k, dt, alpha, err);
+    clocks[r] = c;
+}
+
+__global__ void k_step_incr(int32_t *step, int nsteps) { *step += nsteps; }
 
 // ---------------------------------------------------------------- fused AB3
 
@@ -870,6 +942,277 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
         ab3_body<T, LAYOUT, false>(i, p, s, g, win, dt, Cw, nullptr, Tile{});
 }
 
+// A particle's state while one launch steps it in registers.
+struct PState {
+    Axis az, ay, ax;
+    float hz1, hy1, hx1, hz2, hy2, hx2; // derivative history f_{n-1}, f_{n-2}
+    uint8_t st;
+    double z;
+};
+
+template <typename T>
+struct Raw3<T, OPX_LAYOUT_RING> {}; // the ring path caches no u, v, w nodes
+
+// A particle's stencils across the steps of one launch: the raw u, v, w nodes
+// of its last sample, valid while the snapshot pair (first slot s0) and their
+// bases are unchanged, and the 2-D corners c2 at (c2.by, c2.bx) of the pair
+// with first slot s0_2d.  Particles move ~0.04 cells per step, so most steps
+// blend cached nodes at the new time weights instead of gathering them.
+template <typename T, int LAYOUT>
+struct StencilCache {
+    Raw3<T, LAYOUT> u, v, w;
+    int bz, by, bx, bxu, byv, bzw; // center z, y, x, inner x, inner y and outer z bases of the nodes
+    int s0;                        // -1: no u, v, w nodes cached
+    int s0_2d;                     // -1: no 2-D corners cached
+    Corners2 c2;
+};
+
+// The 2-D corners (h, zeta of pair (s0, s1), dx, dy) at base (by, bx), from
+// packed window j or the rings; kept if already cached.
+__device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, int s1, int NX, int by, int bx,
+                                        Corners2 &c2, int &s0_2d)
+{
+    if (s0_2d == s0 && c2.by == by && c2.bx == bx)
+        return;
+    if (j >= 0)
+        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, by, bx, true, c2);
+    else
+        load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
+                   win.zeta + s1 * win.zeta_stride, NX, by, bx, true, c2);
+    s0_2d = s0;
+}
+
+// One AB3 step of a validated particle in registers (ab3_w_advection.pyx:20-229,
+// the same operations as ab3_body).
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
+                                         const opx_window &win, double dt, const double *Cw)
+{
+    const int NZ = g.NZ, NY = g.NY, NX = g.NX;
+    const Axis az = p.az;
+    Axis ay = p.ay, ax = p.ax;
+    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+
+    // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
+    const float gt = rs.gt, ft = rs.ft;
+    const Ax cz = ax_center(az, NZ), cy = ax_center(ay, NY), cx = ax_center(ax, NX);
+    const Ax ixu = ax_inner(ax, NX), iyv = ax_inner(ay, NY), ow = ax_outer(az, NZ);
+    float vu[8], vv[8], vw[8];
+    const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1);
+    bool ring = true;
+    if constexpr (LAYOUT != OPX_LAYOUT_RING) {
+        if (j >= 0) {
+            ring = false;
+            const bool same = c.s0 == rs.s0;
+            if (!(same && c.bz == cz.b && c.by == cy.b && c.bxu == ixu.b))
+                load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, cz.b, cy.b, ixu.b, c.u);
+            if (!(same && c.bz == cz.b && c.byv == iyv.b && c.bx == cx.b))
+                load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, cz.b, iyv.b, cx.b, c.v);
+            if (!(same && c.bzw == ow.b && c.by == cy.b && c.bx == cx.b))
+                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, ow.b, cy.b, cx.b, c.w);
+            c.s0 = rs.s0;
+            c.bz = cz.b;
+            c.by = cy.b;
+            c.bx = cx.b;
+            c.bxu = ixu.b;
+            c.byv = iyv.b;
+            c.bzw = ow.b;
+            blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
+            blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
+            blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
+        }
+    }
+    if (ring) {
+        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
+        const T *W = static_cast<const T *>(win.w);
+        corners<T, OPX_LAYOUT_RING>(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b,
+                                    cy.b, ixu.b, vu);
+        corners<T, OPX_LAYOUT_RING>(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b,
+                                    iyv.b, cx.b, vv);
+        corners<T, OPX_LAYOUT_RING>(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
+                                    cx.b, vw);
+        c.s0 = -1;
+    }
+    const float u = tri32(vu, cz.w, cy.w, ixu.w);
+    const float v = tri32(vv, cz.w, iyv.w, cx.w);
+    // w with FP64 weights: the vertical index moves through layers that can be
+    // a fraction of a metre thick
+    const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    Corners2 &c2 = c.c2;
+    sample2(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
+    double h0, zeta0;
+    const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
+    eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+    const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
+    const float dx0 = u / dxv, dy0 = v / dyv, dz0 = (float)w;
+
+    // ---- Euler -> AB2 -> AB3 start-up (ab3_w_advection.pyx:121-138)
+    const uint8_t st = p.st;
+    float dx1, dy1, dz1, dx2, dy2, dz2;
+    if (st == OPX_MULTISTEP_1) {
+        dx1 = dx2 = dx0;
+        dy1 = dy2 = dy0;
+        dz1 = dz2 = dz0;
+        p.st = OPX_MULTISTEP_2;
+    } else {
+        dx1 = p.hx1;
+        dy1 = p.hy1;
+        dz1 = p.hz1;
+        if (st == OPX_MULTISTEP_2) {
+            dx2 = -dx0 + 2.0f * dx1;
+            dy2 = -dy0 + 2.0f * dy1;
+            dz2 = -dz0 + 2.0f * dz1;
+            p.st = OPX_NORMAL;
+        } else {
+            dx2 = p.hx2;
+            dy2 = p.hy2;
+            dz2 = p.hz2;
+        }
+    }
+
+    // ---- z at x_n, t_n (ab3_w_advection.pyx:144-165), FP64
+    const double hc = g.hc;
+    const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
+    const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
+    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
+    double z = zeta0 + (zeta0 + h0) * S0;
+
+    // ---- AB3 update (ab3_w_advection.pyx:168-170) in FP64 from FP32 derivatives;
+    // the vertical uses this step's FP64 w
+    const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
+    const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
+    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) / 12.0;
+    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) / 12.0;
+    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) / 12.0;
+    ay = encode_axis(yn, NY);
+    ax = encode_axis(xn, NX);
+
+    // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
+    Axis azn;
+    if (ay.ok && ax.ok) {
+        // h, zeta at t_{n+1} at the new position: most particles stay in their
+        // cell and the zeta snapshot pair does not change, so the corners read
+        // above are reused
+        const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
+        sample2(win, packed_index<LAYOUT>(win, rs.s0p, rs.s1p), rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
+        double h1, zeta1;
+        eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
+        const double S = (z - zeta1) / (zeta1 + h1);
+        // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
+        // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
+        // with k, so a walk from the previous bracket finds the reference's k.
+        const double inv = 1.0 / (hc + h1);
+        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h1 * Cw[k]) * inv; };
+        int k = ow.b;
+        double Sk = S_at(k);
+        while (k > 0 && S < Sk) {
+            --k;
+            Sk = S_at(k);
+        }
+        double Sk1 = S_at(k + 1);
+        while (k < NZ - 1 && S >= Sk1) {
+            ++k;
+            Sk = Sk1;
+            Sk1 = S_at(k + 1);
+        }
+        azn = encode_axis(((double)k - 0.5) + (S - Sk) / (Sk1 - Sk), NZ);
+    } else {
+        azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
+    }
+
+    // ---- history shift (ab3_w_advection.pyx:173-178)
+    p.az = azn;
+    p.ay = ay;
+    p.ax = ax;
+    p.hz2 = dz1;
+    p.hy2 = dy1;
+    p.hx2 = dx1;
+    p.hz1 = dz0;
+    p.hy1 = dy0;
+    p.hx1 = dx0;
+    p.z = z;
+}
+
+// `nsteps` fused AB3 steps of particle i; step k reads its release's
+// weights from table[k * tstride + release].  The particle's state
+// is loaded once, stepped in registers and stored once: the same values as
+// nsteps one-step launches.  Each step validates first (validation.pyx:21-77,
+// inclusive bounds in FP64); a flagged particle stops.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                        const opx_window &win, const opx_release_step *__restrict__ table,
+                                        int tstride, int nsteps, double dt, const double *Cw)
+{
+    if (i >= n)
+        return;
+    const uint8_t st0 = s.status[i];
+    if (st0 & OPX_INACTIVE)
+        return;
+    const int r = s.rel ? (int)s.rel[i] : 0;
+    PState p;
+    p.st = st0;
+    StencilCache<T, LAYOUT> c;
+    c.s0 = -1;
+    c.s0_2d = -1;
+    bool loaded = false, moved = false;
+    for (int k = 0; k < nsteps; ++k) {
+        const Step rs = load_release(table + (int64_t)k * tstride, r, win.nslots);
+        if (!rs.active)
+            continue;
+        if (!loaded) {
+            const uint32_t cell = (uint32_t)s.cell[i];
+            p.az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+            p.ay = decode_axis(cell, g.xbits, g.ybits);
+            p.ax = decode_axis(cell, 0, g.xbits);
+            if (p.az.ok && p.ay.ok && p.ax.ok) {
+                p.az.q = s.fz[i];
+                p.ay.q = s.fy[i];
+                p.ax.q = s.fx[i];
+            }
+            if (p.st != OPX_MULTISTEP_1) {
+                p.hz1 = s.hz1[i];
+                p.hy1 = s.hy1[i];
+                p.hx1 = s.hx1[i];
+                if (p.st != OPX_MULTISTEP_2) {
+                    p.hz2 = s.hz2[i];
+                    p.hy2 = s.hy2[i];
+                    p.hx2 = s.hx2[i];
+                }
+            }
+            loaded = true;
+        }
+        if (!(p.az.ok && p.ay.ok && p.ax.ok)) {
+            p.st = OPX_NONFINITE;
+            break;
+        }
+        const double zi = p.az.pos(), yi = p.ay.pos(), xi = p.ax.pos();
+        if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
+              xi <= g.xidx_max)) {
+            p.st = OPX_OUT_OF_DOMAIN;
+            break;
+        }
+        ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw);
+        moved = true;
+    }
+    if (moved) {
+        s.cell[i] = (int32_t)pack_cell(p.az, p.ay, p.ax, g);
+        s.fz[i] = p.az.q;
+        s.fy[i] = p.ay.q;
+        s.fx[i] = p.ax.q;
+        s.hz2[i] = p.hz2;
+        s.hy2[i] = p.hy2;
+        s.hx2[i] = p.hx2;
+        s.hz1[i] = p.hz1;
+        s.hy1[i] = p.hy1;
+        s.hx1[i] = p.hx1;
+        if (s.z)
+            s.z[i] = (float)p.z;
+    }
+    if (p.st != st0)
+        s.status[i] = p.st;
+}
+
 // Fused AB3 step, one particle per thread.  The grid covers s.n particles;
 // the particles actually stepped are the first min(s.n, *n_active), so one
 // captured graph serves any active count up to s.n (surplus blocks exit at
@@ -879,10 +1222,11 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
 // A loop over chunks would let a smaller grid cover s.n, but it lifts the
 // kernel past 128 VGPRs (3 waves/SIMD instead of 4), which costs more than
 // the surplus blocks.
-template <typename T, int LAYOUT, int BLOCK, bool IND>
+template <typename T, int LAYOUT, int BLOCK, bool IND, bool MULTI>
 __global__ void __launch_bounds__(BLOCK)
     k_ab3_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
-                const opx_release_step *__restrict__ table, double dt, const int32_t *__restrict__ n_active)
+                const opx_release_step *__restrict__ table, int tstride, int nsteps, double dt,
+                const int32_t *__restrict__ n_active)
 {
     // IND: the window descriptor lives in device memory, so a captured graph
     // stays valid when the host repacks windows between replays
@@ -901,8 +1245,12 @@ __global__ void __launch_bounds__(BLOCK)
     // >= ceil(chunks / 8) >= c1 - c0: every chunk has a block
     const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
-    if (c < c1)
-        ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
+    if (c < c1) {
+        if constexpr (MULTI)
+            ab3_run<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt, Cw);
+        else
+            ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
+    }
 }
 
 // ---------------------------------------------------------------- LDS-tiled fused AB3 (one wave per workgroup)
@@ -1585,11 +1933,13 @@ size_t sort_temp_bytes(int64_t n)
 }
 
 int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
-                 int layout, int dtype3, const opx_release_step *table, double dt, const int32_t *n_active, int block,
-                 void *stream)
+                 int layout, int dtype3, const opx_release_step *table, int tstride, int nsteps, double dt,
+                 const int32_t *n_active, int block, void *stream)
 {
     hipStream_t st = as_stream(stream);
     const bool ind = win_dev != nullptr;
+    if (layout == OPX_LAYOUT_PAIR_LDS && nsteps > 1) // several steps per launch read PAIR windows directly
+        layout = OPX_LAYOUT_PAIR;
     if (layout == OPX_LAYOUT_PAIR_LDS) {
         // one wave per workgroup, stencils staged in LDS (packed PAIR windows)
         const dim3 grid(grid_xcd(s->n, 64)), blk(64);
@@ -1612,8 +1962,14 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
     }
     const dim3 grid(grid_xcd(s->n, block)), blk(block);
 #define OPX_FUSED_LAUNCH(T, L, B, I)                                                                       \
-    if (block == B && layout == L && ind == I)                                                            \
-        hipLaunchKernelGGL((k_ab3_fused<T, L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt, n_active);
+    if (block == B && layout == L && ind == I) {                                                          \
+        if (nsteps > 1)                                                                                   \
+            hipLaunchKernelGGL((k_ab3_fused<T, L, B, I, true>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
+                               table, tstride, nsteps, dt, n_active);                                     \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_ab3_fused<T, L, B, I, false>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
+                               table, tstride, nsteps, dt, n_active);                                     \
+    }
 #define OPX_FUSED_LAUNCH_L(T, L)                                                                           \
     OPX_FUSED_LAUNCH(T, L, 128, false) OPX_FUSED_LAUNCH(T, L, 256, false) OPX_FUSED_LAUNCH(T, L, 128, true) \
         OPX_FUSED_LAUNCH(T, L, 256, true)
@@ -1676,12 +2032,16 @@ using namespace opx;
 
 extern "C" {
 
-int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
-                      const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
-                      int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream)
+int opx_clock_advance_steps(opx_release_clock *clocks, opx_release_step *table, int32_t table_stride,
+                            int32_t nsteps, int32_t nrel, const int32_t *nrel_dev, const double *time_array,
+                            int32_t ntime, int32_t nslots, int32_t global_step, int32_t *step_dev, double dt,
+                            double alpha, int32_t *err_flag, void *stream)
 {
     OPX_REQUIRE(nrel >= 0, "negative release count");
     OPX_REQUIRE(alpha >= 0.0, "alpha must be >= 0");
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= nrel, "table_stride %d below the release count %d", table_stride,
+                nrel);
     if (nrel == 0 && step_dev == nullptr)
         return OPX_OK;
     OPX_REQUIRE(nrel == 0 || (clocks && table), "null clock/table");
@@ -1690,33 +2050,51 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
     OPX_REQUIRE(nslots >= 2 && nslots <= kMaxSlots, "nslots must be in [2, %d]", kMaxSlots);
     hipStream_t st = as_stream(stream);
     if (nrel > 0)
-        hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, st, clocks, table, nrel, nrel_dev,
-                           time_array, ntime, nslots, global_step, step_dev, dt, alpha, err_flag);
+        hipLaunchKernelGGL(k_clock, dim3((nrel + 127) / 128), dim3(128), 0, st, clocks, table, table_stride, nsteps,
+                           nrel, nrel_dev, time_array, ntime, nslots, global_step, step_dev, dt, alpha, err_flag);
     if (step_dev)
-        hipLaunchKernelGGL(k_step_incr, dim3(1), dim3(1), 0, st, step_dev);
+        hipLaunchKernelGGL(k_step_incr, dim3(1), dim3(1), 0, st, step_dev, nsteps);
     return check_launch("opx_clock_advance");
 }
 
-int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
-                  const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream)
+int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_t nrel, const int32_t *nrel_dev,
+                      const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
+                      int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream)
+{
+    return opx_clock_advance_steps(clocks, table, 0, 1, nrel, nrel_dev, time_array, ntime, nslots, global_step,
+                                   step_dev, dt, alpha, err_flag, stream);
+}
+
+int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                        const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
+                        const int32_t *n_active, int32_t block, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_ab3_fused"))) return rc;
     if ((rc = check_grid(g))) return rc;
     if ((rc = check_window(win, g))) return rc;
     OPX_REQUIRE(table != nullptr, "null release table");
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= 1, "table_stride must be >= 1 for several steps");
     if (s->n == 0)
         return OPX_OK;
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
     const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
-    return launch_fused(s, g, *win, nullptr, layout, win->dtype3, table, dt, n_active, block, stream);
+    return launch_fused(s, g, *win, nullptr, layout, win->dtype3, table, table_stride, nsteps, dt, n_active, block,
+                        stream);
 }
 
-int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
-                      int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
-                      void *stream)
+int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream)
+{
+    return opx_ab3_fused_steps(s, g, win, table, 0, 1, dt, n_active, block, stream);
+}
+
+int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                            int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                            double dt, const int32_t *n_active, int32_t block, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_ab3_fused_dev"))) return rc;
@@ -1724,13 +2102,23 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= 1, "table_stride must be >= 1 for several steps");
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
     if (s->n == 0)
         return OPX_OK;
     opx_window unused{};
-    return launch_fused(s, g, unused, win_dev, layout, dtype3, table, dt, n_active, block, stream);
+    return launch_fused(s, g, unused, win_dev, layout, dtype3, table, table_stride, nsteps, dt, n_active, block,
+                        stream);
+}
+
+int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
+                      void *stream)
+{
+    return opx_ab3_fused_steps_dev(s, g, win_dev, dtype3, layout, table, 0, 1, dt, n_active, block, stream);
 }
 
 int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,

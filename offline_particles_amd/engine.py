@@ -55,6 +55,9 @@ _ROMS_STAGGERS = {
 
 _STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
 
+# steps one fused launch may take (the release table holds one row per step)
+MAX_LAUNCH_STEPS = 64
+
 LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad",
                 _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct", _lib.OPX_LAYOUT_YPAIR: "ypair"}
 
@@ -176,7 +179,10 @@ class AB3Engine:
         self._structs = None
         self._graphs = {}
         self.graph_steps = 0  # > 0: replay captured graphs of this many steps
-        # optional list collecting (start, end) HIP events around each fused launch
+        # AB3 steps per fused launch (1..MAX_LAUNCH_STEPS): the state stays in
+        # registers between them and unchanged stencils are not read again
+        self.launch_steps = 1
+        # optional list collecting (start, end, steps) around each fused launch
         self.timer = None
         self.block = 128  # workgroup size of the fused kernel
         self.tuning = None
@@ -408,7 +414,8 @@ class AB3Engine:
         if self._clocks is not None:
             clocks[: self._clocks.numel()].copy_(self._clocks)
         self._clocks = clocks
-        self._table = torch.zeros(cap * ts, dtype=torch.uint8, device=self.device)
+        # one row of `cap` entries per step of a launch
+        self._table = torch.zeros(MAX_LAUNCH_STEPS * cap * ts, dtype=torch.uint8, device=self.device)
         self._rel_cap = cap
         self._graphs = {}
 
@@ -596,32 +603,39 @@ class AB3Engine:
             self._win_dev.copy_(_h2d(raw, torch.uint8, self.device), non_blocking=True)
             self._win_dirty = False
 
-    def _launch_step(self, stream):
-        """One step on `stream`: device clocks, then the fused kernel (both graph-capturable)."""
+    def _launch_steps(self, stream, m: int = 1):
+        """m steps on `stream`: device clocks, then one fused launch (both graph-capturable)."""
         rk2 = self.scheme == "rk2"
-        _lib.call("opx_clock_advance", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap,
-                  self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots, 0,
-                  self._step_dev.data_ptr(), self.dt, self.alpha if rk2 else 0.0, self._err.data_ptr(), stream)
+        if rk2 and m != 1:
+            raise ValueError("the fused RK2 step runs one step per launch")
+        _lib.call("opx_clock_advance_steps", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap, int(m),
+                  self._rel_cap, self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots,
+                  0, self._step_dev.data_ptr(), self.dt, self.alpha if rk2 else 0.0, self._err.data_ptr(), stream)
         s = self._launch_structs()
         if rk2:
             _lib.call("opx_rk2_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
                       self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self.alpha,
                       self._n_active_dev.data_ptr(), self.block, stream)
         else:
-            _lib.call("opx_ab3_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
-                      self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self._n_active_dev.data_ptr(),
-                      self.block, stream)
+            _lib.call("opx_ab3_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
+                      self.window.dtype3, self.layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,
+                      self._n_active_dev.data_ptr(), self.block, stream)
+
+    def _launch_sizes(self, k):
+        """Steps per launch for a run of k steps."""
+        m = self.launch_steps if self.scheme == "ab3" else 1
+        return [m] * (k // m) + ([k % m] if k % m else [])
 
     def _graph(self, k):
         """A captured graph of k steps for the current buffers and launch shape."""
-        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block)
+        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps)
         g = self._graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 stream = torch.cuda.current_stream(self.device).cuda_stream
-                for _ in range(k):
-                    self._launch_step(stream)
+                for m in self._launch_sizes(k):
+                    self._launch_steps(stream, m)
             self._graphs[key] = g
         return g
 
@@ -640,11 +654,11 @@ class AB3Engine:
         while left > 0:
             if self.rebin_every > 0 and self._since_rebin >= self.rebin_every:
                 self.rebin()
-            k = 1
+            k = min(self.launch_steps if self.scheme == "ab3" else 1, left)
             if self.graph_steps > 0 and self.timer is None:
-                k = min(self.graph_steps, left)
-                if self.rebin_every > 0:
-                    k = max(1, min(k, self.rebin_every - self._since_rebin))
+                k = min(max(self.graph_steps, k), left)
+            if self.rebin_every > 0:
+                k = max(1, min(k, self.rebin_every - self._since_rebin))
             pairs = set()
             for _ in range(k):
                 pairs |= self._advance_host_clocks()
@@ -652,28 +666,30 @@ class AB3Engine:
             self._ensure_snapshots(pairs)
             self._ensure_windows(pairs)
             self._upload_window()
-            if k > 1:
+            if self.graph_steps > 0 and self.timer is None and k > 1:
                 self._graph(k).replay()
             else:
-                if self.timer is not None:
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                self._launch_step(stream)
-                if self.timer is not None:
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e1.record()
-                    self.timer.append((e0, e1))
+                for m in self._launch_sizes(k):
+                    if self.timer is not None:
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e0.record()
+                    self._launch_steps(stream, m)
+                    if self.timer is not None:
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e1.record()
+                        self.timer.append((e0, e1, m))
             self._since_rebin += k
             left -= k
 
     # ---------------------------------------------------------------- tuning
 
     def _time_steps(self, k: int) -> float:
-        """Mean fused-kernel time (ms) over k real steps."""
+        """Fused-kernel time (ms) per step over k real steps."""
         saved, self.timer = self.timer, []
         self.step(k)
         torch.cuda.synchronize(self.device)
-        ms = float(np.mean([a.elapsed_time(b) for a, b in self.timer])) if self.timer else float("nan")
+        ms = (sum(a.elapsed_time(b) for a, b, _ in self.timer) / sum(m for _, _, m in self.timer)
+              if self.timer else float("nan"))
         self.timer = saved
         return ms
 

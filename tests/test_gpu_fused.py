@@ -236,3 +236,48 @@ def test_lds_tiles_staged_and_fallback(device, storage):
             res.append(_state(eng))
         for k in ("status",) + POS:
             assert np.array_equal(res[0][k], res[1][k], equal_nan=True), (rebin, k)
+
+
+@pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("layout", ["ring", "pair", "quad", "oct", "ypair"])
+def test_several_steps_per_launch_identical(device, storage, layout):
+    """k AB3 steps per launch (state in registers, unchanged stencils blended again) give the bits of one
+    step per launch: across snapshot boundaries, re-bins and graph replays, with releases that start, end
+    and cross a time-array edge inside a launch, and particles that leave the domain."""
+    from offline_particles_amd.engine import LAYOUT_NAMES
+
+    code = {v: k for k, v in LAYOUT_NAMES.items()}[layout]
+    wl = _turb()
+    rng = np.random.default_rng(11)
+    n = [6000, 900, 3000]
+    starts = [0.0, 1234.5, 400.0]  # off the dt grid
+    steps = [95, 30, 41]
+    first = [0, 13, 5]
+    pos = [tuple(rng.uniform(0.5, m - 1.5, k) for m in (wl.NZ, wl.NY, wl.NX)) for k in n]
+    # a release at the lateral edges (some drift out mid-launch) with one non-finite position
+    m_edge = 800
+    ze = rng.uniform(0.5, wl.NZ - 1.5, m_edge)
+    ye = np.where(rng.random(m_edge) < 0.5, 0.01, wl.NY - 1.01)
+    xe = np.where(rng.random(m_edge) < 0.5, 0.01, wl.NX - 1.01)
+    ye[::2] = rng.uniform(0.5, wl.NY - 1.5, m_edge // 2)
+    xe[0] = np.nan
+    pos.append((ze, ye, xe))
+    n, starts, steps, first = n + [m_edge], starts + [0.0], steps + [90], first + [2]
+    fs = build_fieldset(wl, device=device)
+    res = []
+    for launch_steps, graph_steps in ((1, 0), (3, 0), (8, 0), (8, 8), (16, 5), (64, 0)):
+        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
+                        capacity=sum(n), rebin_every=23)
+        eng.set_layout(code)
+        for p, t0, ns, s0 in zip(pos, starts, steps, first):
+            eng.add_release(*p, start_time=t0, nsteps=ns, start_step=s0)
+        eng.launch_steps = launch_steps
+        eng.graph_steps = graph_steps
+        eng.step(100)
+        eng.check_clock()
+        res.append(_state(eng))
+    assert (res[0]["status"] == 130).sum() > 10, "too few particles left the domain"
+    assert (res[0]["status"] == 129).any(), "the non-finite particle was not flagged"
+    for other in res[1:]:
+        for k in ("status",) + POS:
+            assert np.array_equal(res[0][k], other[k], equal_nan=True), k
