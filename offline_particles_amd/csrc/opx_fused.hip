@@ -275,112 +275,124 @@ struct Raw3<T, OPX_LAYOUT_YPAIR> {
     typename Packed<T>::ypair n[4]; // nodes i, i+1 of the 2 z-planes
 };
 
-// Loads the raw nodes of a field with dims (., n1, n2) at base (b0, b1, b2)
-// from packed window pp.
+// Raw nodes come in units loaded and blended together: a row of x-neighbours
+// (PAIR, QUAD: 4 units) or a z-plane (OCT, YPAIR: 2 units).
+template <int LAYOUT>
+constexpr int kRawUnits = (LAYOUT == OPX_LAYOUT_OCT || LAYOUT == OPX_LAYOUT_YPAIR) ? 2 : 4;
+
+// Loads unit k of the raw nodes of a field whose stencil rows (axis0, axis1)
+// start at packed nodes rows[0..3].
 template <typename T, int LAYOUT>
-__device__ __forceinline__ void load_raw(const void *__restrict__ pp, int n1, int n2, int b0, int b1, int b2,
-                                         Raw3<T, LAYOUT> &r)
+__device__ __forceinline__ void load_unit(const void *__restrict__ pp, const uint32_t rows[4], int k,
+                                          Raw3<T, LAYOUT> &r)
 {
-    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
-    const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
-    const uint32_t rows[4] = {b, b + s1, b + s0, b + s0 + s1};
     if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
-        const auto *P = static_cast<const typename Packed<T>::pair *>(pp);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            gload_pair(P, rows[k], r.n[2 * k], r.n[2 * k + 1]); // the x-neighbours (i, i+1) in one load
+        // the x-neighbours (i, i+1) in one load
+        gload_pair(static_cast<const typename Packed<T>::pair *>(pp), rows[k], r.n[2 * k], r.n[2 * k + 1]);
     } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
-        const auto *P = static_cast<const typename Packed<T>::quad *>(pp);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            r.n[k] = gload(P, rows[k]);
+        r.n[k] = gload(static_cast<const typename Packed<T>::quad *>(pp), rows[k]);
     } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
         // one node per z-plane holds that plane's four (y, x) corners of both snapshots
-        const auto *P = static_cast<const typename Packed<T>::oct *>(pp);
-        r.n[0] = gload(P, rows[0]);
-        r.n[1] = gload(P, rows[2]);
+        r.n[k] = gload(static_cast<const typename Packed<T>::oct *>(pp), rows[2 * k]);
     } else {
         // YPAIR: nodes i, i+1 of a z-plane: (y, x), (y+1, x) and (y, x+1), (y+1, x+1)
         const auto *P = static_cast<const typename Packed<T>::ypair *>(pp);
         if constexpr (sizeof(T) == 4) {
-            r.n[0] = gload(P, rows[0]);
-            r.n[1] = gload(P, rows[0] + 1);
-            r.n[2] = gload(P, rows[2]);
-            r.n[3] = gload(P, rows[2] + 1);
+            r.n[2 * k] = gload(P, rows[2 * k]);
+            r.n[2 * k + 1] = gload(P, rows[2 * k] + 1);
         } else {
-            gload_pair(P, rows[0], r.n[0], r.n[1]); // one 16-byte load per z-plane
-            gload_pair(P, rows[2], r.n[2], r.n[3]);
+            gload_pair(P, rows[2 * k], r.n[2 * k], r.n[2 * k + 1]); // one 16-byte load per z-plane
         }
     }
 }
 
-// The 8 blended stencil values (order 000, 001, 010, 011, 100, 101, 110, 111
-// over (axis0, axis1, axis2)) of raw nodes: p*gt + q*ft in FP32, the same
-// operations in every layout.
+// Blends unit k of raw nodes into stencil values v (order 000, 001, 010, 011,
+// 100, 101, 110, 111 over (axis0, axis1, axis2)): p*gt + q*ft in FP32, the
+// same operations in every layout.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void blend_unit(const Raw3<T, LAYOUT> &r, int k, float gt, float ft, float v[8])
+{
+    if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
+        const float2 e0 = unpair(r.n[2 * k]), e1 = unpair(r.n[2 * k + 1]);
+        v[2 * k] = blend_rn(e0.x, e0.y, gt, ft);
+        v[2 * k + 1] = blend_rn(e1.x, e1.y, gt, ft);
+    } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
+        const float4 e = unquad(r.n[k]);
+        v[2 * k] = blend_rn(e.x, e.z, gt, ft);
+        v[2 * k + 1] = blend_rn(e.y, e.w, gt, ft);
+    } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
+        if constexpr (sizeof(T) == 4) {
+            const float8 &e = r.n[k];
+            v[4 * k + 0] = blend_rn(e.a.x, e.b.x, gt, ft);
+            v[4 * k + 1] = blend_rn(e.a.y, e.b.y, gt, ft);
+            v[4 * k + 2] = blend_rn(e.a.z, e.b.z, gt, ft);
+            v[4 * k + 3] = blend_rn(e.a.w, e.b.w, gt, ft);
+        } else {
+            const uint4 &e = r.n[k];
+            v[4 * k + 0] = blend_rn(bf_lo(e.x), bf_lo(e.z), gt, ft);
+            v[4 * k + 1] = blend_rn(bf_hi(e.x), bf_hi(e.z), gt, ft);
+            v[4 * k + 2] = blend_rn(bf_lo(e.y), bf_lo(e.w), gt, ft);
+            v[4 * k + 3] = blend_rn(bf_hi(e.y), bf_hi(e.w), gt, ft);
+        }
+    } else {
+        if constexpr (sizeof(T) == 4) {
+            const float4 &a = r.n[2 * k], &c = r.n[2 * k + 1];
+            v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
+            v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
+            v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
+            v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
+        } else {
+            const uint2 &a = r.n[2 * k], &c = r.n[2 * k + 1];
+            v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
+            v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
+            v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
+            v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
+        }
+    }
+}
+
+// First packed node of each stencil row (axis0, axis1) of a field with dims
+// (., n1, n2) at base (b0, b1, b2).
+__device__ __forceinline__ void stencil_rows(int n1, int n2, int b0, int b1, int b2, uint32_t rows[4])
+{
+    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
+    const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
+    rows[0] = b;
+    rows[1] = b + s1;
+    rows[2] = b + s0;
+    rows[3] = b + s0 + s1;
+}
+
+// All raw nodes of a field at base (b0, b1, b2) from packed window pp.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void load_raw(const void *__restrict__ pp, int n1, int n2, int b0, int b1, int b2,
+                                         Raw3<T, LAYOUT> &r)
+{
+    uint32_t rows[4];
+    stencil_rows(n1, n2, b0, b1, b2, rows);
+#pragma unroll
+    for (int k = 0; k < kRawUnits<LAYOUT>; ++k)
+        load_unit<T, LAYOUT>(pp, rows, k, r);
+}
+
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void blend_raw(const Raw3<T, LAYOUT> &r, float gt, float ft, float v[8])
 {
-    if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float2 e = unpair(r.n[k]);
-            v[k] = blend_rn(e.x, e.y, gt, ft);
-        }
-    } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float4 e = unquad(r.n[k]);
-            v[2 * k] = blend_rn(e.x, e.z, gt, ft);
-            v[2 * k + 1] = blend_rn(e.y, e.w, gt, ft);
-        }
-    } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if constexpr (sizeof(T) == 4) {
-                const float8 &e = r.n[k];
-                v[4 * k + 0] = blend_rn(e.a.x, e.b.x, gt, ft);
-                v[4 * k + 1] = blend_rn(e.a.y, e.b.y, gt, ft);
-                v[4 * k + 2] = blend_rn(e.a.z, e.b.z, gt, ft);
-                v[4 * k + 3] = blend_rn(e.a.w, e.b.w, gt, ft);
-            } else {
-                const uint4 &e = r.n[k];
-                v[4 * k + 0] = blend_rn(bf_lo(e.x), bf_lo(e.z), gt, ft);
-                v[4 * k + 1] = blend_rn(bf_hi(e.x), bf_hi(e.z), gt, ft);
-                v[4 * k + 2] = blend_rn(bf_lo(e.y), bf_lo(e.w), gt, ft);
-                v[4 * k + 3] = blend_rn(bf_hi(e.y), bf_hi(e.w), gt, ft);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if constexpr (sizeof(T) == 4) {
-                const float4 &a = r.n[2 * k], &c = r.n[2 * k + 1];
-                v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
-                v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
-                v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
-                v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
-            } else {
-                const uint2 &a = r.n[2 * k], &c = r.n[2 * k + 1];
-                v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
-                v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
-                v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
-                v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
-            }
-        }
-    }
+    for (int k = 0; k < kRawUnits<LAYOUT>; ++k)
+        blend_unit<T, LAYOUT>(r, k, gt, ft, v);
 }
 
 // The 8 blended stencil values of a field with dims (., n1, n2) and base
 // (b0, b1, b2).  RING reads slot pointers p and q; the packed layouts read
-// window p.
+// window p, blending each unit as soon as it is loaded.
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void corners(const void *__restrict__ pp, const void *__restrict__ qq, float gt, float ft,
                                         int n1, int n2, int b0, int b1, int b2, float v[8])
 {
+    uint32_t rows[4];
+    stencil_rows(n1, n2, b0, b1, b2, rows);
     if constexpr (LAYOUT == OPX_LAYOUT_RING) {
-        const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
-        const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
-        const uint32_t rows[4] = {b, b + s1, b + s0, b + s0 + s1};
         const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -389,8 +401,11 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
         }
     } else {
         Raw3<T, LAYOUT> r;
-        load_raw<T, LAYOUT>(pp, n1, n2, b0, b1, b2, r);
-        blend_raw<T, LAYOUT>(r, gt, ft, v);
+#pragma unroll
+        for (int k = 0; k < kRawUnits<LAYOUT>; ++k) {
+            load_unit<T, LAYOUT>(pp, rows, k, r);
+            blend_unit<T, LAYOUT>(r, k, gt, ft, v);
+        }
     }
 }
 
@@ -947,7 +962,6 @@ struct PState {
     Axis az, ay, ax;
     float hz1, hy1, hx1, hz2, hy2, hx2; // derivative history f_{n-1}, f_{n-2}
     uint8_t st;
-    double z;
 };
 
 template <typename T>
@@ -961,11 +975,17 @@ struct Raw3<T, OPX_LAYOUT_RING> {}; // the ring path caches no u, v, w nodes
 template <typename T, int LAYOUT>
 struct StencilCache {
     Raw3<T, LAYOUT> u, v, w;
-    int bz, by, bx, bxu, byv, bzw; // center z, y, x, inner x, inner y and outer z bases of the nodes
-    int s0;                        // -1: no u, v, w nodes cached
-    int s0_2d;                     // -1: no 2-D corners cached
+    uint32_t ku, kv, kw; // bases of the nodes, packed as a cell (base_key)
+    int s0;              // -1: no u, v, w nodes cached
+    int s0_2d;           // -1: no 2-D corners cached
     Corners2 c2;
 };
+
+// A field's stencil base (b0, b1, b2) packed like a cell: one compare per field.
+__device__ __forceinline__ uint32_t base_key(int b0, int b1, int b2, const opx_grid &g)
+{
+    return ((uint32_t)b0 << (g.xbits + g.ybits)) | ((uint32_t)b1 << g.xbits) | (uint32_t)b2;
+}
 
 // The 2-D corners (h, zeta of pair (s0, s1), dx, dy) at base (by, bx), from
 // packed window j or the rings; kept if already cached.
@@ -986,7 +1006,7 @@ __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, in
 // the same operations as ab3_body).
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
-                                         const opx_window &win, double dt, const double *Cw)
+                                         const opx_window &win, double dt, const double *Cw, float *zout)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const Axis az = p.az;
@@ -1004,19 +1024,18 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         if (j >= 0) {
             ring = false;
             const bool same = c.s0 == rs.s0;
-            if (!(same && c.bz == cz.b && c.by == cy.b && c.bxu == ixu.b))
+            const uint32_t ku = base_key(cz.b, cy.b, ixu.b, g), kv = base_key(cz.b, iyv.b, cx.b, g);
+            const uint32_t kw = base_key(ow.b, cy.b, cx.b, g);
+            if (!(same && c.ku == ku))
                 load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, cz.b, cy.b, ixu.b, c.u);
-            if (!(same && c.bz == cz.b && c.byv == iyv.b && c.bx == cx.b))
+            if (!(same && c.kv == kv))
                 load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, cz.b, iyv.b, cx.b, c.v);
-            if (!(same && c.bzw == ow.b && c.by == cy.b && c.bx == cx.b))
+            if (!(same && c.kw == kw))
                 load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, ow.b, cy.b, cx.b, c.w);
             c.s0 = rs.s0;
-            c.bz = cz.b;
-            c.by = cy.b;
-            c.bx = cx.b;
-            c.bxu = ixu.b;
-            c.byv = iyv.b;
-            c.bzw = ow.b;
+            c.ku = ku;
+            c.kv = kv;
+            c.kw = kw;
             blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
             blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
             blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
@@ -1131,7 +1150,8 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     p.hz1 = dz0;
     p.hy1 = dy0;
     p.hx1 = dx0;
-    p.z = z;
+    if (zout)
+        *zout = (float)z;
 }
 
 // `nsteps` fused AB3 steps of particle i; step k reads its release's
@@ -1156,6 +1176,7 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
     c.s0 = -1;
     c.s0_2d = -1;
     bool loaded = false, moved = false;
+#pragma unroll 1
     for (int k = 0; k < nsteps; ++k) {
         const Step rs = load_release(table + (int64_t)k * tstride, r, win.nslots);
         if (!rs.active)
@@ -1192,7 +1213,7 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
             p.st = OPX_OUT_OF_DOMAIN;
             break;
         }
-        ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw);
+        ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, s.z ? s.z + i : nullptr);
         moved = true;
     }
     if (moved) {
@@ -1206,8 +1227,6 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
         s.hz1[i] = p.hz1;
         s.hy1[i] = p.hy1;
         s.hx1[i] = p.hx1;
-        if (s.z)
-            s.z[i] = (float)p.z;
     }
     if (p.st != st0)
         s.status[i] = p.st;
@@ -1223,7 +1242,7 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
 // kernel past 128 VGPRs (3 waves/SIMD instead of 4), which costs more than
 // the surplus blocks.
 template <typename T, int LAYOUT, int BLOCK, bool IND, bool MULTI>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI && sizeof(T) == 2 ? 3 : 1)))
     k_ab3_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
                 const opx_release_step *__restrict__ table, int tstride, int nsteps, double dt,
                 const int32_t *__restrict__ n_active)
