@@ -243,7 +243,8 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.rebin()
     eng.step(warmup)
     if tune:
-        log("autotune:", eng.autotune())
+        # steps per launch from 1, 8 and 16 (16 is the profiled launch size)
+        log("autotune:", eng.autotune(launch_steps=(1, 8, 16) if scheme == "ab3" else (1,)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -267,7 +268,9 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
         raise RuntimeError("the release ran out of steps inside the timed region; lengthen the time array")
     elapsed = t1 - t0
     actn = 0.5 * (a0 + a1)
+    # algorithmic bytes per particle-step times the particle-steps of one launch
     algo = actn * (FUSED_STATE_BYTES if scheme == "ab3" else FUSED_RK2_STATE_BYTES) + fbytes
+    algo_launch = m * algo
     name = ("fused_" if scheme == "ab3" else "fused_rk2_") + ("bf16" if storage == torch.bfloat16 else "fp32")
     res = {
         "form": name,
@@ -278,9 +281,10 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
                    else f"k_rk2_fused<{'bf16' if sb == 2 else 'f32'}> (opx_rk2_fused)"),
         "kernel_ms": kms,
         "steps_per_launch": m,
-        "algo_bytes": algo,
+        "algo_bytes": algo_launch,
+        "algo_bytes_per_step": algo,
         "field_bytes_touched": fbytes,
-        "achieved_GBs": algo / (kms * 1e-3) / 1e9,
+        "achieved_GBs": algo_launch / (kms * 1e-3) / 1e9,
         "rebin_every": eng.rebin_every,
         "xlow": eng.xlow,
         "block": eng.block,
@@ -315,7 +319,7 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
     fs = build_fieldset(wl, device="cpu", storage_dtype=torch.bfloat16)
     del wl.fields["u"], wl.fields["v"], wl.fields["w"]
     tuning = None
-    layout, xlow, block, rebin_every = None, 4, 128, 40
+    layout, xlow, block, rebin_every, launch_steps = None, 4, 128, 40, 1
     if tune:
         # autotune on a probe: the 40 largest requests started together
         probe = sorted(reqs, key=lambda r: -r.size)[:40]
@@ -324,16 +328,20 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
         for r in probe:
             eng.add_release(r.zidx, r.yidx, r.xidx, nsteps=400)
         eng.rebin()
-        tuning = eng.autotune(xlows=(4, 5), blocks=(128,), cadence_probe=30)
+        # launches stay within a server chunk (graph_steps) and the 3-slot window
+        tuning = eng.autotune(xlows=(4, 5), blocks=(128,), cadence_probe=30,
+                              launch_steps=tuple(m for m in (1, 4, 8) if m <= max(1, graph_steps)))
         tuning = {k: v for k, v in tuning.items() if k != "trials"}
         log("serving autotune:", tuning)
         layout = LAYOUTS[tuning["layout"]]
         xlow, block, rebin_every = tuning["xlow"], tuning["block"], tuning["rebin_every"]
+        launch_steps = tuning["launch_steps"]
         del eng
         torch.cuda.empty_cache()
     srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
                         stream_slots=3, max_releases=4096, rebin_every=rebin_every, xlow=xlow, layout=layout,
-                        block=block, graph_steps=graph_steps, max_particles=max_particles)
+                        block=block, graph_steps=graph_steps, max_particles=max_particles,
+                        launch_steps=launch_steps)
     log(f"serving: server ready in {time.perf_counter() - t:.1f}s, capacity {srv.capacity} particles")
     srv.run(reqs)
     res = dict(srv.stats)
@@ -386,6 +394,7 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
                     max_releases=nrel, rebin_every=tuning["rebin_every"], xlow=tuning["xlow"])
     eng.set_layout(LAYOUTS[tuning["layout"]])
     eng.block = tuning["block"]
+    eng.launch_steps = tuning["launch_steps"]
     eng.graph_steps = graph_steps
     scheduled = sum(r.size * r.nsteps for r in rels)
     torch.cuda.synchronize()
@@ -426,20 +435,25 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
 # HBM traffic measured by rocprofv3 (FETCH_SIZE x2 + WRITE_SIZE x1, see profiles/)
 # ---------------------------------------------------------------------------
 
+# (form, steps per launch) -> summary
 _PROFILE_FOR_FORM = {
-    "plain_fp32": "k_ab3_plain_turbulent16M.md",
-    "fused_fp32": "k_ab3_fused_f32_turbulent16M.md",
-    "fused_bf16": "k_ab3_fused_bf16_turbulent16M.md",
-    "fused_rk2_fp32": "k_rk2_fused_f32_turbulent16M.md",
+    ("plain_fp32", 1): "k_ab3_plain_turbulent16M.md",
+    ("fused_fp32", 1): "k_ab3_fused_f32_turbulent16M.md",
+    ("fused_bf16", 1): "k_ab3_fused_bf16_turbulent16M.md",
+    ("fused_fp32", 16): "k_ab3_fused_steps16_f32_turbulent16M.md",
+    ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M.md",
+    ("fused_rk2_fp32", 1): "k_rk2_fused_f32_turbulent16M.md",
 }
 
 
-def profiled_traffic(form, particles, grid):
+def profiled_traffic(form, particles, grid, steps_per_launch=1):
     """HBM bytes per launch of the form's dominant kernel from its committed rocprofv3
-    summary, when this run is that profile's workload (16M particles, 1024x768x50)."""
-    if particles != 16_000_000 or tuple(grid) != (1024, 768, 50) or form not in _PROFILE_FOR_FORM:
+    summary, when this run is that profile's workload (16M particles, 1024x768x50,
+    the same steps per launch)."""
+    key = (form, steps_per_launch)
+    if particles != 16_000_000 or tuple(grid) != (1024, 768, 50) or key not in _PROFILE_FOR_FORM:
         return None
-    path = os.path.join(ROOT, "profiles", _PROFILE_FOR_FORM[form])
+    path = os.path.join(ROOT, "profiles", _PROFILE_FOR_FORM[key])
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -571,8 +585,10 @@ def build_output(args, forms, grid, cpu, acc, batching, serving) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ)),
+            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ),
+                                        head.get("steps_per_launch", 1)),
             "algorithmic_bytes": head["algo_bytes"],
+            "steps_per_launch": head.get("steps_per_launch", 1),
         },
         "cpu_baseline": cpu,
         "accuracy": acc,
@@ -630,7 +646,7 @@ def main():
     device = torch.device("cuda", 0)
     NX, NY, NZ = (int(s) for s in args.grid.split("x"))
     t = time.perf_counter()
-    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles, steps=args.steps + args.warmup + 400,
+    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles, steps=args.steps + args.warmup + 600,
                             dtype=np.float32)
     log(f"workload generated in {time.perf_counter() - t:.1f}s: grid {NX}x{NY}x{NZ}, {args.particles} particles")
 

@@ -270,6 +270,12 @@ typedef struct opx_release_clock {
     int32_t steps_left;
 } opx_release_clock;
 
+/* Values written to the device error flags (err_flag arguments). */
+enum opx_err_flag {
+    OPX_FLAG_TIME_OUT_OF_RANGE = 1, /* a release's time left the time array */
+    OPX_FLAG_UNPACKED_PAIR = 2,     /* a several-step launch met a snapshot pair its window does not pack */
+};
+
 /* Per-release weights of one step, written by opx_clock_advance (16 bytes,
  * one vector load per particle): u/v/w/zeta blend at t_n with (ft, 1 - ft),
  * zeta at t_{n+1} with (ft1, 1 - ft1), FP32-rounded like fields.py:423-424.
@@ -327,15 +333,19 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
  * and 2-D stencil nodes it read are blended again at the next step's weights
  * while its cells and snapshot pair are unchanged: the results are those of
  * nsteps one-step launches (timesteppers.py:296-305 applied nsteps times),
- * bit for bit.  No counterpart in the reference, which launches per step. */
+ * bit for bit.  With nsteps > 1 and a packed layout, every snapshot pair the
+ * steps read (at t and t + dt) must be one of the window's packed pairs: a
+ * particle that meets another pair stops, unchanged from that step on, and
+ * *err_flag (if not NULL) is set to OPX_FLAG_UNPACKED_PAIR; the RING layout
+ * reads any pair.  No counterpart in the reference, which launches per step. */
 int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                         const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
-                        const int32_t *n_active, int32_t block, void *stream);
+                        const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);
 
 /* opx_ab3_fused_steps with the window descriptor in device memory. */
 int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                             int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
-                            double dt, const int32_t *n_active, int32_t block, void *stream);
+                            double dt, const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);
 
 /* One fused RK2 step (validation -> _rk2_step_1 at t -> _rk2_step_2 at
  * t + alpha dt -> _rk2_update at t + dt, rk2_w_advection.pyx:31-354 in the

@@ -105,6 +105,7 @@ class opx_grid(ctypes.Structure):
 
 
 OPX_LAYOUT_RING, OPX_LAYOUT_PAIR, OPX_LAYOUT_QUAD, OPX_LAYOUT_PAIR_LDS, OPX_LAYOUT_OCT, OPX_LAYOUT_YPAIR = 0, 1, 2, 3, 4, 5
+OPX_FLAG_TIME_OUT_OF_RANGE, OPX_FLAG_UNPACKED_PAIR = 1, 2
 
 
 class opx_window(ctypes.Structure):
@@ -146,9 +147,9 @@ _SIGNATURES.update({
     "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,
                       _vp],
     "opx_ab3_fused_steps": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_int32, ctypes.c_int32,
-                            ctypes.c_double, _vp, ctypes.c_int32, _vp],
+                            ctypes.c_double, _vp, ctypes.c_int32, _vp, _vp],
     "opx_ab3_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
-                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, ctypes.c_int32, _vp],
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, ctypes.c_int32, _vp, _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],

@@ -33,6 +33,18 @@ constexpr int kBlock = 256;
 constexpr int kXcds = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxC = 128; // largest NZ + 1 whose C is staged in LDS
 
+// The w-level tables every particle reads, staged in LDS by a workgroup:
+// L[k] = C[k] (Cs_w) and L[kMaxC + k] = k / NZ - 1, the w-level sigma of the
+// S-coordinate (_vertical_coordinate.pxd:19-35), the same FP64 values the
+// formula gives inline.
+__device__ __forceinline__ void stage_levels(double *L, const double *C, int NZ, int t, int nt)
+{
+    for (int k = t; k <= NZ; k += nt) {
+        L[k] = C[k];
+        L[kMaxC + k] = (double)k / NZ - 1.0;
+    }
+}
+
 // ---------------------------------------------------------------- helpers
 
 // Pointers read from the window descriptor are generic; loads through them
@@ -625,7 +637,7 @@ __device__ __forceinline__ void clock_step(opx_release_clock &c, opx_release_ste
         } else {
             c.steps_left = 0;
             if (err)
-                *err = 1;
+                *err = OPX_FLAG_TIME_OUT_OF_RANGE;
         }
     }
     e.slots = (uint32_t)s0 | ((uint32_t)s0p << 10) | ((uint32_t)s0m << 20) | ((uint32_t)active << 30);
@@ -912,7 +924,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
         // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
         // with k, so a walk from the previous bracket finds the reference's k.
         const double inv = 1.0 / (hc + h1);
-        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h1 * Cw[k]) * inv; };
+        auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h1 * Cw[k]) * inv; };
         int k = ow.b;
         double Sk = S_at(k);
         while (k > 0 && S < Sk) {
@@ -989,23 +1001,26 @@ __device__ __forceinline__ uint32_t base_key(int b0, int b1, int b2, const opx_g
 
 // The 2-D corners (h, zeta of pair (s0, s1), dx, dy) at base (by, bx), from
 // packed window j or the rings; kept if already cached.
+template <int LAYOUT>
 __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, int s1, int NX, int by, int bx,
                                         Corners2 &c2, int &s0_2d)
 {
     if (s0_2d == s0 && c2.by == by && c2.bx == bx)
         return;
-    if (j >= 0)
-        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, by, bx, true, c2);
-    else
+    if constexpr (LAYOUT == OPX_LAYOUT_RING)
         load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
                    win.zeta + s1 * win.zeta_stride, NX, by, bx, true, c2);
+    else
+        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, by, bx, true, c2);
     s0_2d = s0;
 }
 
 // One AB3 step of a validated particle in registers (ab3_w_advection.pyx:20-229,
-// the same operations as ab3_body).
+// the same operations as ab3_body).  A packed LAYOUT reads only its packed
+// windows: it returns false, leaving the particle unchanged, if a snapshot pair
+// the step needs is not packed (the caller runs such steps with RING).
 template <typename T, int LAYOUT>
-__device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
+__device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
                                          const opx_window &win, double dt, const double *Cw, float *zout)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
@@ -1019,10 +1034,20 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const Ax ixu = ax_inner(ax, NX), iyv = ax_inner(ay, NY), ow = ax_outer(az, NZ);
     float vu[8], vv[8], vw[8];
     const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1);
-    bool ring = true;
-    if constexpr (LAYOUT != OPX_LAYOUT_RING) {
-        if (j >= 0) {
-            ring = false;
+    const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
+    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
+        const T *W = static_cast<const T *>(win.w);
+        corners<T, OPX_LAYOUT_RING>(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b,
+                                    cy.b, ixu.b, vu);
+        corners<T, OPX_LAYOUT_RING>(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b,
+                                    iyv.b, cx.b, vv);
+        corners<T, OPX_LAYOUT_RING>(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
+                                    cx.b, vw);
+    } else {
+        if (j < 0 || j1 < 0)
+            return false;
+        {
             const bool same = c.s0 == rs.s0;
             const uint32_t ku = base_key(cz.b, cy.b, ixu.b, g), kv = base_key(cz.b, iyv.b, cx.b, g);
             const uint32_t kw = base_key(ow.b, cy.b, cx.b, g);
@@ -1041,24 +1066,13 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
             blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
         }
     }
-    if (ring) {
-        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
-        const T *W = static_cast<const T *>(win.w);
-        corners<T, OPX_LAYOUT_RING>(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b,
-                                    cy.b, ixu.b, vu);
-        corners<T, OPX_LAYOUT_RING>(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b,
-                                    iyv.b, cx.b, vv);
-        corners<T, OPX_LAYOUT_RING>(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
-                                    cx.b, vw);
-        c.s0 = -1;
-    }
     const float u = tri32(vu, cz.w, cy.w, ixu.w);
     const float v = tri32(vv, cz.w, iyv.w, cx.w);
     // w with FP64 weights: the vertical index moves through layers that can be
     // a fraction of a metre thick
     const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
     Corners2 &c2 = c.c2;
-    sample2(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
+    sample2<LAYOUT>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
     double h0, zeta0;
     const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
     eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
@@ -1114,7 +1128,7 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         // cell and the zeta snapshot pair does not change, so the corners read
         // above are reused
         const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
-        sample2(win, packed_index<LAYOUT>(win, rs.s0p, rs.s1p), rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
+        sample2<LAYOUT>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
         eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
         const double S = (z - zeta1) / (zeta1 + h1);
@@ -1122,7 +1136,7 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
         // with k, so a walk from the previous bracket finds the reference's k.
         const double inv = 1.0 / (hc + h1);
-        auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h1 * Cw[k]) * inv; };
+        auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h1 * Cw[k]) * inv; };
         int k = ow.b;
         double Sk = S_at(k);
         while (k > 0 && S < Sk) {
@@ -1152,6 +1166,7 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     p.hx1 = dx0;
     if (zout)
         *zout = (float)z;
+    return true;
 }
 
 // `nsteps` fused AB3 steps of particle i; step k reads its release's
@@ -1162,7 +1177,7 @@ __device__ __forceinline__ void ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
                                         const opx_window &win, const opx_release_step *__restrict__ table,
-                                        int tstride, int nsteps, double dt, const double *Cw)
+                                        int tstride, int nsteps, double dt, const double *Cw, int32_t *err)
 {
     if (i >= n)
         return;
@@ -1213,7 +1228,11 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
             p.st = OPX_OUT_OF_DOMAIN;
             break;
         }
-        ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, s.z ? s.z + i : nullptr);
+        if (!ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, s.z ? s.z + i : nullptr)) {
+            if (err)
+                *err = OPX_FLAG_UNPACKED_PAIR;
+            break;
+        }
         moved = true;
     }
     if (moved) {
@@ -1242,18 +1261,17 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
 // kernel past 128 VGPRs (3 waves/SIMD instead of 4), which costs more than
 // the surplus blocks.
 template <typename T, int LAYOUT, int BLOCK, bool IND, bool MULTI>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI && sizeof(T) == 2 ? 3 : 1)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 1)))
     k_ab3_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
                 const opx_release_step *__restrict__ table, int tstride, int nsteps, double dt,
-                const int32_t *__restrict__ n_active)
+                const int32_t *__restrict__ n_active, int32_t *err)
 {
     // IND: the window descriptor lives in device memory, so a captured graph
     // stays valid when the host repacks windows between replays
     const opx_window win = IND ? *win_dev : win_arg;
     // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
-    __shared__ double Cw[kMaxC];
-    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
-        Cw[k] = gbl(win.C)[k];
+    __shared__ double Cw[2 * kMaxC];
+    stage_levels(Cw, win.C, g.NZ, threadIdx.x, BLOCK);
     __syncthreads();
 
     int64_t n = s.n;
@@ -1266,7 +1284,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULT
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
     if (c < c1) {
         if constexpr (MULTI)
-            ab3_run<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt, Cw);
+            ab3_run<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt, Cw, err);
         else
             ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
     }
@@ -1364,11 +1382,10 @@ __global__ void __launch_bounds__(64)
 {
     constexpr int EPN = sizeof(typename Packed<T>::pair) / 4;
     const opx_window win = IND ? *win_dev : win_arg;
-    __shared__ double Cw[kMaxC];
+    __shared__ double Cw[2 * kMaxC];
     __shared__ __attribute__((aligned(16))) uint32_t pool[kTile3 + 5 * kTile2];
     const int lane = (int)threadIdx.x;
-    for (int k = lane; k <= g.NZ; k += 64)
-        Cw[k] = gbl(win.C)[k];
+    stage_levels(Cw, win.C, g.NZ, lane, 64);
 
     int64_t n = s.n;
     if (n_active)
@@ -1627,7 +1644,7 @@ __device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, d
 {
     const double S = (z - zeta) / (zeta + h);
     const double inv = 1.0 / (hc + h);
-    auto S_at = [&](int k) { return (hc * ((double)k / NZ - 1.0) + h * Cw[k]) * inv; };
+    auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h * Cw[k]) * inv; };
     int k = max(0, min(k0, NZ - 1));
     double Sk = S_at(k);
     while (k > 0 && S < Sk) {
@@ -1741,9 +1758,8 @@ __global__ void __launch_bounds__(BLOCK)
                 const int32_t *__restrict__ n_active)
 {
     const opx_window win = IND ? *win_dev : win_arg;
-    __shared__ double Cw[kMaxC];
-    for (int k = threadIdx.x; k <= g.NZ; k += BLOCK)
-        Cw[k] = gbl(win.C)[k];
+    __shared__ double Cw[2 * kMaxC];
+    stage_levels(Cw, win.C, g.NZ, threadIdx.x, BLOCK);
     __syncthreads();
 
     int64_t n = s.n;
@@ -1953,7 +1969,7 @@ size_t sort_temp_bytes(int64_t n)
 
 int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
                  int layout, int dtype3, const opx_release_step *table, int tstride, int nsteps, double dt,
-                 const int32_t *n_active, int block, void *stream)
+                 const int32_t *n_active, int block, int32_t *err, void *stream)
 {
     hipStream_t st = as_stream(stream);
     const bool ind = win_dev != nullptr;
@@ -1984,10 +2000,10 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
     if (block == B && layout == L && ind == I) {                                                          \
         if (nsteps > 1)                                                                                   \
             hipLaunchKernelGGL((k_ab3_fused<T, L, B, I, true>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
-                               table, tstride, nsteps, dt, n_active);                                     \
+                               table, tstride, nsteps, dt, n_active, err);                                \
         else                                                                                              \
             hipLaunchKernelGGL((k_ab3_fused<T, L, B, I, false>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
-                               table, tstride, nsteps, dt, n_active);                                     \
+                               table, tstride, nsteps, dt, n_active, err);                                \
     }
 #define OPX_FUSED_LAUNCH_L(T, L)                                                                           \
     OPX_FUSED_LAUNCH(T, L, 128, false) OPX_FUSED_LAUNCH(T, L, 256, false) OPX_FUSED_LAUNCH(T, L, 128, true) \
@@ -2086,7 +2102,7 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
 
 int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                         const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
-                        const int32_t *n_active, int32_t block, void *stream)
+                        const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_ab3_fused"))) return rc;
@@ -2102,18 +2118,18 @@ int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_w
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
     const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
     return launch_fused(s, g, *win, nullptr, layout, win->dtype3, table, table_stride, nsteps, dt, n_active, block,
-                        stream);
+                        err_flag, stream);
 }
 
 int opx_ab3_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                   const opx_release_step *table, double dt, const int32_t *n_active, int32_t block, void *stream)
 {
-    return opx_ab3_fused_steps(s, g, win, table, 0, 1, dt, n_active, block, stream);
+    return opx_ab3_fused_steps(s, g, win, table, 0, 1, dt, n_active, block, nullptr, stream);
 }
 
 int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                             int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
-                            double dt, const int32_t *n_active, int32_t block, void *stream)
+                            double dt, const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_ab3_fused_dev"))) return rc;
@@ -2130,14 +2146,15 @@ int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const o
         return OPX_OK;
     opx_window unused{};
     return launch_fused(s, g, unused, win_dev, layout, dtype3, table, table_stride, nsteps, dt, n_active, block,
-                        stream);
+                        err_flag, stream);
 }
 
 int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                       int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
                       void *stream)
 {
-    return opx_ab3_fused_steps_dev(s, g, win_dev, dtype3, layout, table, 0, 1, dt, n_active, block, stream);
+    return opx_ab3_fused_steps_dev(s, g, win_dev, dtype3, layout, table, 0, 1, dt, n_active, block, nullptr,
+                                   stream);
 }
 
 int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,

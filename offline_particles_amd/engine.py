@@ -603,8 +603,12 @@ class AB3Engine:
             self._win_dev.copy_(_h2d(raw, torch.uint8, self.device), non_blocking=True)
             self._win_dirty = False
 
-    def _launch_steps(self, stream, m: int = 1):
-        """m steps on `stream`: device clocks, then one fused launch (both graph-capturable)."""
+    def _launch_steps(self, stream, m: int = 1, ring: bool = False):
+        """m steps on `stream`: device clocks, then one fused launch (both graph-capturable).
+
+        A launch of several steps with a packed layout reads only the packed
+        windows; ``ring`` runs it with the RING layout instead, for runs of
+        steps that need more snapshot pairs than the two packed windows hold."""
         rk2 = self.scheme == "rk2"
         if rk2 and m != 1:
             raise ValueError("the fused RK2 step runs one step per launch")
@@ -617,25 +621,26 @@ class AB3Engine:
                       self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self.alpha,
                       self._n_active_dev.data_ptr(), self.block, stream)
         else:
+            layout = _lib.OPX_LAYOUT_RING if (ring and m > 1) else self.layout
             _lib.call("opx_ab3_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
-                      self.window.dtype3, self.layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,
-                      self._n_active_dev.data_ptr(), self.block, stream)
+                      self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,
+                      self._n_active_dev.data_ptr(), self.block, self._err.data_ptr(), stream)
 
     def _launch_sizes(self, k):
         """Steps per launch for a run of k steps."""
         m = self.launch_steps if self.scheme == "ab3" else 1
         return [m] * (k // m) + ([k % m] if k % m else [])
 
-    def _graph(self, k):
+    def _graph(self, k, ring=False):
         """A captured graph of k steps for the current buffers and launch shape."""
-        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps)
+        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps, ring)
         g = self._graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 stream = torch.cuda.current_stream(self.device).cuda_stream
                 for m in self._launch_sizes(k):
-                    self._launch_steps(stream, m)
+                    self._launch_steps(stream, m, ring)
             self._graphs[key] = g
         return g
 
@@ -666,14 +671,16 @@ class AB3Engine:
             self._ensure_snapshots(pairs)
             self._ensure_windows(pairs)
             self._upload_window()
+            # at most two pairs are packed; runs that need more read the rings
+            ring = len({(a % self.nslots, b % self.nslots) for a, b in pairs}) > 2
             if self.graph_steps > 0 and self.timer is None and k > 1:
-                self._graph(k).replay()
+                self._graph(k, ring).replay()
             else:
                 for m in self._launch_sizes(k):
                     if self.timer is not None:
                         e0 = torch.cuda.Event(enable_timing=True)
                         e0.record()
-                    self._launch_steps(stream, m)
+                    self._launch_steps(stream, m, ring)
                     if self.timer is not None:
                         e1 = torch.cuda.Event(enable_timing=True)
                         e1.record()
@@ -693,27 +700,36 @@ class AB3Engine:
         self.timer = saved
         return ms
 
-    def autotune(self, *, xlows=(3, 4, 5), blocks=(128, 256), layouts=None, trial_steps: int = 3,
-                 cadence_probe: int = 40, max_cadence: int = 400) -> dict:
-        """Pick the bin shape, workgroup size and re-sort cadence by timing real steps.
+    def autotune(self, *, xlows=(3, 4, 5), blocks=(128, 256), layouts=None, launch_steps=None,
+                 trial_steps: int = 3, cadence_probe: int = 40, max_cadence: int = 400) -> dict:
+        """Pick the bin shape, stencil layout, workgroup size, steps per launch and
+        re-sort cadence by timing real steps.
 
         - bin shape: ``xlow`` low bits of the x cell kept contiguous in the sort
           key (Morton order above them), i.e. bins 2^xlow cells long in x;
         - stencil layout (ring / pair / quad / pair_lds / oct / ypair) and
-          workgroup size of the fused kernel;
+          workgroup size of the fused kernel (timed at one step per launch);
+        - AB3 steps per launch (``launch_steps`` candidates, default 1, 8, 16,
+          32): more steps keep the state in registers longer and reload fewer
+          stencils, but a launch needs the snapshots of all its steps;
         - re-sort cadence: after a rebin the step time grows as particles drift
           out of order; with rebin cost R and growth a per step, the average
-          cost per step is minimal at N = sqrt(2 R / a).
+          cost per step is minimal at N = sqrt(2 R / a), rounded up to whole
+          launches.
         Tuning advances the simulation (results do not depend on these
-        parameters); it needs ``len(xlows) * len(blocks) * trial_steps +
-        cadence_probe`` steps of the active releases.
+        parameters); it needs about ``len(xlows) * (len(layouts) * len(blocks))
+        * (trial_steps + 1) + 2 * sum(launch_steps) + cadence_probe`` steps of
+        the active releases.
         """
         import time as _time
 
         if layouts is None:
             layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
                        _lib.OPX_LAYOUT_OCT, _lib.OPX_LAYOUT_YPAIR)
+        if launch_steps is None:
+            launch_steps = (1, 8, 16, 32) if self.scheme == "ab3" else (1,)
         saved_every, self.rebin_every = self.rebin_every, 0
+        self.launch_steps = 1
         trials = {}
         for xlow in xlows:
             self.xlow = int(xlow)
@@ -727,28 +743,43 @@ class AB3Engine:
                     trials[(self.xlow, self.layout, self.block)] = self._time_steps(trial_steps)
         (self.xlow, layout, self.block), best = min(trials.items(), key=lambda kv: kv[1])
         self.set_layout(layout)
+        self.rebin()
+        per_launch = {}
+        for m in launch_steps:
+            self.launch_steps = int(m)
+            self._time_steps(int(m))
+            per_launch[int(m)] = self._time_steps(max(trial_steps, int(m)))
+        self.launch_steps, best = min(per_launch.items(), key=lambda kv: kv[1])
+        m = self.launch_steps
         torch.cuda.synchronize(self.device)
         t0 = _time.perf_counter()
         self.rebin()
         torch.cuda.synchronize(self.device)
         rebin_ms = 1e3 * (_time.perf_counter() - t0)
-        ts = np.array([self._time_steps(1) for _ in range(cadence_probe)])
-        k = np.arange(cadence_probe, dtype=np.float64)
-        slope = float(np.polyfit(k, ts, 1)[0]) if cadence_probe >= 3 else 0.0
+        nprobe = max(3, -(-cadence_probe // m))
+        ts = np.array([self._time_steps(m) for _ in range(nprobe)])
+        k = m * np.arange(nprobe, dtype=np.float64)
+        slope = float(np.polyfit(k, ts, 1)[0])
         cadence = int(np.clip(np.sqrt(2.0 * rebin_ms / slope), 5, max_cadence)) if slope > 0 else max_cadence
+        cadence = m * max(1, -(-cadence // m))
         self.rebin_every = cadence if saved_every != 0 else 0
-        self._since_rebin = cadence_probe
+        self._since_rebin = m * nprobe
         names = LAYOUT_NAMES
         self.tuning = {
-            "xlow": self.xlow, "layout": names[self.layout], "block": self.block, "rebin_every": self.rebin_every,
-            "kernel_ms": best, "rebin_ms": rebin_ms, "drift_ms_per_step": slope,
+            "xlow": self.xlow, "layout": names[self.layout], "block": self.block, "launch_steps": m,
+            "rebin_every": self.rebin_every, "kernel_ms_per_step": best, "rebin_ms": rebin_ms,
+            "drift_ms_per_step": slope,
             "trials": {f"xlow={x},{names[lay]},block={b}": round(v, 4) for (x, lay, b), v in trials.items()},
+            "launch_steps_trials": {str(k): round(v, 4) for k, v in per_launch.items()},
         }
         return self.tuning
 
     def check_clock(self) -> None:
         """Raise ValueError if a release's time left the time array (as the reference does)."""
-        if int(self._err.item()):
+        err = int(self._err.item())
+        if err == _lib.OPX_FLAG_UNPACKED_PAIR:
+            raise RuntimeError("a several-step launch met a snapshot pair that was not packed")
+        if err:
             raise ValueError("Time is out of bounds of the time array.")
 
     # ---------------------------------------------------------------- results

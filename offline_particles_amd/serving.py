@@ -117,7 +117,7 @@ class ReleaseServer:
                  stream_slots: int | None = 3, hbm_fraction: float = 0.8, max_particles: int | None = None,
                  max_releases: int = 1024, rebin_every: int = 40, xlow: int = 4, graph_steps: int = 8,
                  layout: int | None = None, block: int = 128, start_time: float | None = None,
-                 max_inflight_chunks: int = 3):
+                 max_inflight_chunks: int = 3, launch_steps: int = 1):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         # the window (rings and packed stencil windows) is allocated first, then
         # the particle buffers from what is left
@@ -129,6 +129,7 @@ class ReleaseServer:
             eng.set_layout(layout)
         eng.block = int(block)
         eng.graph_steps = int(graph_steps)
+        eng.launch_steps = int(launch_steps)
         bu, bv, bw, b2 = eng.packed_window_bytes()
         self.capacity = capacity_from_free_hbm(self.device, hbm_fraction=hbm_fraction,
                                                reserve_bytes=2 * (bu + bv + bw + b2) + (256 << 20),
