@@ -33,16 +33,22 @@ constexpr int kBlock = 256;
 constexpr int kXcds = 8; // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxC = 128; // largest NZ + 1 whose C is staged in LDS
 
+constexpr int kLevels = 2 * kMaxC + 1; // doubles of the staged level tables
+constexpr int kInvNZ = 2 * kMaxC;      // index of 1 / NZ in them
+constexpr double kInv12 = 1.0 / 12.0;  // the AB3 update's 1/12 (one multiply, not a divide)
+
 // The w-level tables every particle reads, staged in LDS by a workgroup:
-// L[k] = C[k] (Cs_w) and L[kMaxC + k] = k / NZ - 1, the w-level sigma of the
+// L[k] = C[k] (Cs_w), L[kMaxC + k] = k / NZ - 1, the w-level sigma of the
 // S-coordinate (_vertical_coordinate.pxd:19-35), the same FP64 values the
-// formula gives inline.
+// formula gives inline, and L[kInvNZ] = 1 / NZ.
 __device__ __forceinline__ void stage_levels(double *L, const double *C, int NZ, int t, int nt)
 {
     for (int k = t; k <= NZ; k += nt) {
         L[k] = C[k];
         L[kMaxC + k] = (double)k / NZ - 1.0;
     }
+    if (t == 0)
+        L[kInvNZ] = 1.0 / NZ;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -878,7 +884,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     const double hc = g.hc;
     const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
     const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
-    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
     const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
     double z = zeta0 + (zeta0 + h0) * S0;
 
@@ -886,9 +892,9 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     // the vertical uses this step's FP64 w
     const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
     const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
-    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) / 12.0;
-    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) / 12.0;
-    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) / 12.0;
+    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) * kInv12;
+    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) * kInv12;
+    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) * kInv12;
     ay = encode_axis(yn, NY);
     ax = encode_axis(xn, NX);
 
@@ -991,6 +997,12 @@ struct StencilCache {
     int s0;              // -1: no u, v, w nodes cached
     int s0_2d;           // -1: no 2-D corners cached
     Corners2 c2;
+    // h and zeta at the particle from the last post step, at the time with
+    // weight ft_hz of the pair with first slot s0_hz (-1: none): the next
+    // step's sample at the same position and time
+    double h, zeta;
+    float ft_hz;
+    int s0_hz;
 };
 
 // A field's stencil base (b0, b1, b2) packed like a cell: one compare per field.
@@ -1075,7 +1087,15 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     sample2<LAYOUT>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
     double h0, zeta0;
     const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
-    eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+    // the last post step evaluated h, zeta here at this time (same corners and
+    // weights); kept across steps for BF16 only, where the registers allow
+    constexpr bool kKeepHz = sizeof(T) == 2;
+    if (kKeepHz && c.s0_hz == rs.s0 && c.ft_hz == ft) {
+        h0 = c.h;
+        zeta0 = c.zeta;
+    } else {
+        eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+    }
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = u / dxv, dy0 = v / dyv, dz0 = (float)w;
 
@@ -1107,7 +1127,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const double hc = g.hc;
     const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
     const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
-    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
     const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
     double z = zeta0 + (zeta0 + h0) * S0;
 
@@ -1115,9 +1135,9 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     // the vertical uses this step's FP64 w
     const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
     const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
-    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) / 12.0;
-    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) / 12.0;
-    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) / 12.0;
+    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) * kInv12;
+    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) * kInv12;
+    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) * kInv12;
     ay = encode_axis(yn, NY);
     ax = encode_axis(xn, NX);
 
@@ -1131,6 +1151,12 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         sample2<LAYOUT>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
         eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
+        if constexpr (kKeepHz) {
+            c.h = h1;
+            c.zeta = zeta1;
+            c.ft_hz = rs.ft1;
+            c.s0_hz = rs.s0p;
+        }
         const double S = (z - zeta1) / (zeta1 + h1);
         // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
         // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
@@ -1152,6 +1178,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         azn = encode_axis(((double)k - 0.5) + (S - Sk) / (Sk1 - Sk), NZ);
     } else {
         azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
+        c.s0_hz = -1;
     }
 
     // ---- history shift (ab3_w_advection.pyx:173-178)
@@ -1190,6 +1217,7 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
     StencilCache<T, LAYOUT> c;
     c.s0 = -1;
     c.s0_2d = -1;
+    c.s0_hz = -1;
     bool loaded = false, moved = false;
 #pragma unroll 1
     for (int k = 0; k < nsteps; ++k) {
@@ -1270,7 +1298,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULT
     // stays valid when the host repacks windows between replays
     const opx_window win = IND ? *win_dev : win_arg;
     // the stretching curve C (NZ+1 <= kMaxC values) is read by every particle: keep it in LDS
-    __shared__ double Cw[2 * kMaxC];
+    __shared__ double Cw[kLevels];
     stage_levels(Cw, win.C, g.NZ, threadIdx.x, BLOCK);
     __syncthreads();
 
@@ -1382,7 +1410,7 @@ __global__ void __launch_bounds__(64)
 {
     constexpr int EPN = sizeof(typename Packed<T>::pair) / 4;
     const opx_window win = IND ? *win_dev : win_arg;
-    __shared__ double Cw[2 * kMaxC];
+    __shared__ double Cw[kLevels];
     __shared__ __attribute__((aligned(16))) uint32_t pool[kTile3 + 5 * kTile2];
     const int lane = (int)threadIdx.x;
     stage_levels(Cw, win.C, g.NZ, lane, 64);
@@ -1758,7 +1786,7 @@ __global__ void __launch_bounds__(BLOCK)
                 const int32_t *__restrict__ n_active)
 {
     const opx_window win = IND ? *win_dev : win_arg;
-    __shared__ double Cw[2 * kMaxC];
+    __shared__ double Cw[kLevels];
     stage_levels(Cw, win.C, g.NZ, threadIdx.x, BLOCK);
     __syncthreads();
 
