@@ -671,6 +671,78 @@ __global__ void k_clock(opx_release_clock *clocks, opx_release_step *table, int 
 
 __global__ void k_step_incr(int32_t *step, int nsteps) { *step += nsteps; }
 
+// ---------------------------------------------------------------- arithmetic shared by the fused steps
+
+// zidx of an FP64 z with h, zeta at the particle (_compute_zidx,
+// _vertical_coordinate.pxd:41-103): S = (z - zeta) / (zeta + h), the bracket k
+// with S_k <= S < S_{k+1} on the w-levels clamped to [0, NZ-1] (S_k increases
+// with k, so a walk from the hint k0 finds the reference's binary-search k),
+// and the linear inverse k - 1/2 + (S - S_k) / (S_{k+1} - S_k).
+__device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, double hc, int NZ, const double *Cw,
+                                              int k0)
+{
+    const double S = (z - zeta) / (zeta + h);
+    const double inv = 1.0 / (hc + h);
+    auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h * Cw[k]) * inv; };
+    int k = max(0, min(k0, NZ - 1));
+    double Sk = S_at(k);
+    while (k > 0 && S < Sk) {
+        --k;
+        Sk = S_at(k);
+    }
+    double Sk1 = S_at(k + 1);
+    while (k < NZ - 1 && S >= Sk1) {
+        ++k;
+        Sk = Sk1;
+        Sk1 = S_at(k + 1);
+    }
+    return ((double)k - 0.5) + (S - Sk) / (Sk1 - Sk);
+}
+
+// Euler -> AB2 -> AB3 start-up of one coordinate (ab3_w_advection.pyx:121-138):
+// the derivatives f_{n-1}, f_{n-2} the update uses, from this step's f0 and the
+// stored history, read only where the status needs it (h1(), h2()).
+template <typename H1, typename H2>
+__device__ __forceinline__ void ab3_startup(uint8_t st, float f0, H1 h1, H2 h2, float &f1, float &f2)
+{
+    if (st == OPX_MULTISTEP_1) {
+        f1 = f2 = f0;
+    } else {
+        f1 = h1();
+        f2 = st == OPX_MULTISTEP_2 ? -f0 + 2.0f * f1 : h2();
+    }
+}
+
+__device__ __forceinline__ uint8_t ab3_next_status(uint8_t st)
+{
+    return st == OPX_MULTISTEP_1 ? (uint8_t)OPX_MULTISTEP_2 : (st == OPX_MULTISTEP_2 ? (uint8_t)OPX_NORMAL : st);
+}
+
+// z at the particle (ab3_w_advection.pyx:144-165; Vtransform 2,
+// _vertical_coordinate.pxd:19-35) in FP64, C = Cs_w at w-levels (offset +0.5).
+__device__ __forceinline__ double z_at(const Axis &az, int ob, double h, double zeta, double hc, const double *Cw)
+{
+    const double wC = w64(az, ob, 0.5);
+    const double Cv = (1.0 - wC) * Cw[ob] + wC * Cw[ob + 1];
+    const double sigma = (az.pos() + 0.5) * Cw[kInvNZ] - 1.0;
+    const double S0 = (hc * sigma + h * Cv) / (hc + h);
+    return zeta + (zeta + h) * S0;
+}
+
+// The AB3 update of one coordinate (ab3_w_advection.pyx:168-170) in FP64.
+__device__ __forceinline__ double ab3_advance(double x, double dt, double f0, double f1, double f2)
+{
+    return x + dt * (23.0 * f0 - 16.0 * f1 + 5.0 * f2) * kInv12;
+}
+
+// The vertical update's f_{n-1}, f_{n-2} in FP64: this step's FP64 w where the
+// start-up uses f0 (f1, f2 are the FP32 start-up's results).
+__device__ __forceinline__ void ab3_startup_w(uint8_t st, double w, float f1, float f2, double &w1, double &w2)
+{
+    w1 = st == OPX_MULTISTEP_1 ? w : (double)f1;
+    w2 = st == OPX_MULTISTEP_1 ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * w1 : (double)f2);
+}
+
 // ---------------------------------------------------------------- fused AB3
 
 // A particle that passed the step's validation: decoded position, status and
@@ -856,47 +928,21 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = u / dxv, dy0 = v / dyv, dz0 = (float)w;
 
-    // ---- Euler -> AB2 -> AB3 start-up (ab3_w_advection.pyx:121-138)
+    // ---- Euler -> AB2 -> AB3 start-up (history read only where needed)
     float dx1, dy1, dz1, dx2, dy2, dz2;
-    uint8_t st_new = st;
-    if (st == OPX_MULTISTEP_1) {
-        dx1 = dx2 = dx0;
-        dy1 = dy2 = dy0;
-        dz1 = dz2 = dz0;
-        st_new = OPX_MULTISTEP_2;
-    } else {
-        dx1 = s.hx1[i];
-        dy1 = s.hy1[i];
-        dz1 = s.hz1[i];
-        if (st == OPX_MULTISTEP_2) {
-            dx2 = -dx0 + 2.0f * dx1;
-            dy2 = -dy0 + 2.0f * dy1;
-            dz2 = -dz0 + 2.0f * dz1;
-            st_new = OPX_NORMAL;
-        } else {
-            dx2 = s.hx2[i];
-            dy2 = s.hy2[i];
-            dz2 = s.hz2[i];
-        }
-    }
+    ab3_startup(st, dx0, [&] { return s.hx1[i]; }, [&] { return s.hx2[i]; }, dx1, dx2);
+    ab3_startup(st, dy0, [&] { return s.hy1[i]; }, [&] { return s.hy2[i]; }, dy1, dy2);
+    ab3_startup(st, dz0, [&] { return s.hz1[i]; }, [&] { return s.hz2[i]; }, dz1, dz2);
+    const uint8_t st_new = ab3_next_status(st);
 
-    // ---- z at x_n, t_n (ab3_w_advection.pyx:144-165), FP64
+    // ---- z at x_n, t_n, then the AB3 update in FP64 from FP32 derivatives; the
+    // vertical uses this step's FP64 w
     const double hc = g.hc;
-    const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
-    const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
-    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
-    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
-    double z = zeta0 + (zeta0 + h0) * S0;
-
-    // ---- AB3 update (ab3_w_advection.pyx:168-170) in FP64 from FP32 derivatives;
-    // the vertical uses this step's FP64 w
-    const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
-    const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
-    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) * kInv12;
-    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) * kInv12;
-    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) * kInv12;
-    ay = encode_axis(yn, NY);
-    ax = encode_axis(xn, NX);
+    double dz1d, dz2d;
+    ab3_startup_w(st, w, dz1, dz2, dz1d, dz2d);
+    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt, w, dz1d, dz2d);
+    ay = encode_axis(ab3_advance(yi, dt, (double)dy0, (double)dy1, (double)dy2), NY);
+    ax = encode_axis(ab3_advance(xi, dt, (double)dx0, (double)dx1, (double)dx2), NX);
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
     Axis azn;
@@ -925,25 +971,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
         }
         double h1, zeta1;
         eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
-        const double S = (z - zeta1) / (zeta1 + h1);
-        // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
-        // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
-        // with k, so a walk from the previous bracket finds the reference's k.
-        const double inv = 1.0 / (hc + h1);
-        auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h1 * Cw[k]) * inv; };
-        int k = ow.b;
-        double Sk = S_at(k);
-        while (k > 0 && S < Sk) {
-            --k;
-            Sk = S_at(k);
-        }
-        double Sk1 = S_at(k + 1);
-        while (k < NZ - 1 && S >= Sk1) {
-            ++k;
-            Sk = Sk1;
-            Sk1 = S_at(k + 1);
-        }
-        azn = encode_axis(((double)k - 0.5) + (S - Sk) / (Sk1 - Sk), NZ);
+        azn = encode_axis(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b), NZ);
     } else {
         azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
     }
@@ -1099,47 +1127,21 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = u / dxv, dy0 = v / dyv, dz0 = (float)w;
 
-    // ---- Euler -> AB2 -> AB3 start-up (ab3_w_advection.pyx:121-138)
+    // ---- Euler -> AB2 -> AB3 start-up, history from registers
     const uint8_t st = p.st;
     float dx1, dy1, dz1, dx2, dy2, dz2;
-    if (st == OPX_MULTISTEP_1) {
-        dx1 = dx2 = dx0;
-        dy1 = dy2 = dy0;
-        dz1 = dz2 = dz0;
-        p.st = OPX_MULTISTEP_2;
-    } else {
-        dx1 = p.hx1;
-        dy1 = p.hy1;
-        dz1 = p.hz1;
-        if (st == OPX_MULTISTEP_2) {
-            dx2 = -dx0 + 2.0f * dx1;
-            dy2 = -dy0 + 2.0f * dy1;
-            dz2 = -dz0 + 2.0f * dz1;
-            p.st = OPX_NORMAL;
-        } else {
-            dx2 = p.hx2;
-            dy2 = p.hy2;
-            dz2 = p.hz2;
-        }
-    }
+    ab3_startup(st, dx0, [&] { return p.hx1; }, [&] { return p.hx2; }, dx1, dx2);
+    ab3_startup(st, dy0, [&] { return p.hy1; }, [&] { return p.hy2; }, dy1, dy2);
+    ab3_startup(st, dz0, [&] { return p.hz1; }, [&] { return p.hz2; }, dz1, dz2);
+    p.st = ab3_next_status(st);
 
-    // ---- z at x_n, t_n (ab3_w_advection.pyx:144-165), FP64
+    // ---- z at x_n, t_n, then the AB3 update (as ab3_body)
     const double hc = g.hc;
-    const double wC = w64(az, ow.b, 0.5); // C = Cs_w at w-levels, offset +0.5
-    const double Cv = (1.0 - wC) * Cw[ow.b] + wC * Cw[ow.b + 1];
-    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
-    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
-    double z = zeta0 + (zeta0 + h0) * S0;
-
-    // ---- AB3 update (ab3_w_advection.pyx:168-170) in FP64 from FP32 derivatives;
-    // the vertical uses this step's FP64 w
-    const double dz1d = (st == OPX_MULTISTEP_1) ? w : (double)dz1;
-    const double dz2d = (st == OPX_MULTISTEP_1) ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * dz1d : (double)dz2);
-    z += dt * (23.0 * w - 16.0 * dz1d + 5.0 * dz2d) * kInv12;
-    const double yn = yi + dt * (23.0 * (double)dy0 - 16.0 * (double)dy1 + 5.0 * (double)dy2) * kInv12;
-    const double xn = xi + dt * (23.0 * (double)dx0 - 16.0 * (double)dx1 + 5.0 * (double)dx2) * kInv12;
-    ay = encode_axis(yn, NY);
-    ax = encode_axis(xn, NX);
+    double dz1d, dz2d;
+    ab3_startup_w(st, w, dz1, dz2, dz1d, dz2d);
+    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt, w, dz1d, dz2d);
+    ay = encode_axis(ab3_advance(yi, dt, (double)dy0, (double)dy1, (double)dy2), NY);
+    ax = encode_axis(ab3_advance(xi, dt, (double)dx0, (double)dx1, (double)dx2), NX);
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
     Axis azn;
@@ -1157,25 +1159,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
             c.ft_hz = rs.ft1;
             c.s0_hz = rs.s0p;
         }
-        const double S = (z - zeta1) / (zeta1 + h1);
-        // bracket k with S_k <= S < S_{k+1} on the w-levels, clamped to [0, NZ-1]
-        // (_compute_Cidx_from_S, _vertical_coordinate.pxd:50-84); S_k increases
-        // with k, so a walk from the previous bracket finds the reference's k.
-        const double inv = 1.0 / (hc + h1);
-        auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h1 * Cw[k]) * inv; };
-        int k = ow.b;
-        double Sk = S_at(k);
-        while (k > 0 && S < Sk) {
-            --k;
-            Sk = S_at(k);
-        }
-        double Sk1 = S_at(k + 1);
-        while (k < NZ - 1 && S >= Sk1) {
-            ++k;
-            Sk = Sk1;
-            Sk1 = S_at(k + 1);
-        }
-        azn = encode_axis(((double)k - 0.5) + (S - Sk) / (Sk1 - Sk), NZ);
+        azn = encode_axis(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b), NZ);
     } else {
         azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
         c.s0_hz = -1;
@@ -1662,30 +1646,6 @@ __device__ __forceinline__ void sample_2d(const opx_window &win, int s0, int s1,
         dxv = bil32(c2.dx, (float)cy.w, (float)cx.w);
         dyv = bil32(c2.dy, (float)cy.w, (float)cx.w);
     }
-}
-
-// zidx of depth z (compute_zidx, _vertical_coordinate.pxd:41-103): bracket
-// S_k <= S < S_{k+1} on the w-levels clamped to [0, NZ-1], found by a walk
-// from level k0 (S_k increases with k), then the linear inverse.
-__device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, double hc, int NZ, const double *Cw,
-                                              int k0)
-{
-    const double S = (z - zeta) / (zeta + h);
-    const double inv = 1.0 / (hc + h);
-    auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h * Cw[k]) * inv; };
-    int k = max(0, min(k0, NZ - 1));
-    double Sk = S_at(k);
-    while (k > 0 && S < Sk) {
-        --k;
-        Sk = S_at(k);
-    }
-    double Sk1 = S_at(k + 1);
-    while (k < NZ - 1 && S >= Sk1) {
-        ++k;
-        Sk = Sk1;
-        Sk1 = S_at(k + 1);
-    }
-    return ((double)k - 0.5) + (S - Sk) / (Sk1 - Sk);
 }
 
 // One particle's fused RK2 step: validation at t, stage 1 at (x_n, t), stage 2
