@@ -238,13 +238,12 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     if layout is not None:
         eng.set_layout(LAYOUTS[layout])
     eng.block = block
-    if scheme == "ab3":
-        eng.launch_steps = launch_steps
+    eng.launch_steps = launch_steps
     eng.rebin()
     eng.step(warmup)
     if tune:
         # steps per launch from 1, 8 and 16 (16 is the profiled launch size)
-        log("autotune:", eng.autotune(launch_steps=(1, 8, 16) if scheme == "ab3" else (1,)))
+        log("autotune:", eng.autotune(launch_steps=(1, 8, 16)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -627,7 +626,7 @@ def main():
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
-    ap.add_argument("--launch-steps", type=int, default=1, help="AB3 steps per fused launch (1..64)")
+    ap.add_argument("--launch-steps", type=int, default=1, help="fused steps per launch (1..64)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()

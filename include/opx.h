@@ -358,6 +358,23 @@ int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const o
 int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                   const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
                   void *stream);
+
+/* `nsteps` (1..64) fused RK2 steps in one launch, step k's release weights at
+ * table[k * table_stride + release] (from opx_clock_advance_steps with this
+ * alpha); the state stays in registers and stage 2 and the update blend the
+ * stencil nodes stage 1 read when they fall in the same cells and pair.  The
+ * same bits as nsteps one-step launches; with nsteps > 1 and a packed layout
+ * the pairs rule and *err_flag of opx_ab3_fused_steps apply.  No counterpart
+ * in the reference, which launches per stage. */
+int opx_rk2_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                        const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt, double alpha,
+                        const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);
+
+/* opx_rk2_fused_steps with the window descriptor in device memory. */
+int opx_rk2_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                            int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                            double dt, double alpha, const int32_t *n_active, int32_t block, int32_t *err_flag,
+                            void *stream);
 int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                       int32_t layout, const opx_release_step *table, double dt, double alpha,
                       const int32_t *n_active, int32_t block, void *stream);

@@ -142,6 +142,11 @@ _SIGNATURES.update({
     "opx_interp_trilinear": [ctypes.c_int64, _vp, _vp, _vp, _vp, _P(opx_field), _vp, _vp],
     "opx_rk2_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, ctypes.c_double, _vp,
                       ctypes.c_int32, _vp],
+    "opx_rk2_fused_steps": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32, _vp, _vp],
+    "opx_rk2_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
+                                _vp, _vp],
     "opx_ab3_fused_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
                           ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_ab3_fused": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_double, _vp, ctypes.c_int32,

@@ -1041,18 +1041,69 @@ __device__ __forceinline__ uint32_t base_key(int b0, int b1, int b2, const opx_g
 
 // The 2-D corners (h, zeta of pair (s0, s1), dx, dy) at base (by, bx), from
 // packed window j or the rings; kept if already cached.
-template <int LAYOUT>
+// RING_OK: a pair that is not packed (j < 0) reads the rings; without it the
+// caller has checked that j >= 0.
+template <int LAYOUT, bool RING_OK>
 __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, int s1, int NX, int by, int bx,
                                         Corners2 &c2, int &s0_2d)
 {
     if (s0_2d == s0 && c2.by == by && c2.bx == bx)
         return;
-    if constexpr (LAYOUT == OPX_LAYOUT_RING)
+    if (LAYOUT == OPX_LAYOUT_RING || (RING_OK && j < 0))
         load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
                    win.zeta + s1 * win.zeta_stride, NX, by, bx, true, c2);
     else
         load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, by, bx, true, c2);
     s0_2d = s0;
+}
+
+// The blended u, v, w stencil values of pair (s0, s1) at weights (gt, ft) and
+// stencil bases bz, by, bx (center), bxu, byv (inner x, y) and bzw (outer z).
+// A packed LAYOUT blends its cached raw nodes again and reloads a field only
+// when its bases or the pair changed; with RING_OK a pair that is not packed
+// (j < 0) reads the rings and leaves nothing cached, without it the caller has
+// checked that j >= 0.
+template <typename T, int LAYOUT, bool RING_OK>
+__device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid &g, int j, int s0, int s1, float gt,
+                                          float ft, int bz, int by, int bx, int bxu, int byv, int bzw,
+                                          StencilCache<T, LAYOUT> &c, float vu[8], float vv[8], float vw[8])
+{
+    const int NY = g.NY, NX = g.NX;
+    bool ring = LAYOUT == OPX_LAYOUT_RING;
+    if constexpr (LAYOUT != OPX_LAYOUT_RING) {
+        if (RING_OK && j < 0) {
+            ring = true;
+        } else {
+            const bool same = c.s0 == s0;
+            const uint32_t ku = base_key(bz, by, bxu, g), kv = base_key(bz, byv, bx, g), kw = base_key(bzw, by, bx, g);
+            if (!(same && c.ku == ku))
+                load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, bz, by, bxu, c.u);
+            if (!(same && c.kv == kv))
+                load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, bz, byv, bx, c.v);
+            if (!(same && c.kw == kw))
+                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, bzw, by, bx, c.w);
+            c.s0 = s0;
+            c.ku = ku;
+            c.kv = kv;
+            c.kw = kw;
+            blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
+            blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
+            blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
+        }
+    }
+    if constexpr (LAYOUT == OPX_LAYOUT_RING || RING_OK) {
+        if (ring) {
+            const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
+            const T *W = static_cast<const T *>(win.w);
+            corners<T, OPX_LAYOUT_RING>(U + s0 * win.u_stride, U + s1 * win.u_stride, gt, ft, NY, NX - 1, bz, by,
+                                        bxu, vu);
+            corners<T, OPX_LAYOUT_RING>(V + s0 * win.v_stride, V + s1 * win.v_stride, gt, ft, NY - 1, NX, bz, byv,
+                                        bx, vv);
+            corners<T, OPX_LAYOUT_RING>(W + s0 * win.w_stride, W + s1 * win.w_stride, gt, ft, NY, NX, bzw, by, bx,
+                                        vw);
+            c.s0 = -1;
+        }
+    }
 }
 
 // One AB3 step of a validated particle in registers (ab3_w_advection.pyx:20-229,
@@ -1075,44 +1126,17 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     float vu[8], vv[8], vw[8];
     const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1);
     const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
-    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
-        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
-        const T *W = static_cast<const T *>(win.w);
-        corners<T, OPX_LAYOUT_RING>(U + rs.s0 * win.u_stride, U + rs.s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b,
-                                    cy.b, ixu.b, vu);
-        corners<T, OPX_LAYOUT_RING>(V + rs.s0 * win.v_stride, V + rs.s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b,
-                                    iyv.b, cx.b, vv);
-        corners<T, OPX_LAYOUT_RING>(W + rs.s0 * win.w_stride, W + rs.s1 * win.w_stride, gt, ft, NY, NX, ow.b, cy.b,
-                                    cx.b, vw);
-    } else {
-        if (j < 0 || j1 < 0)
-            return false;
-        {
-            const bool same = c.s0 == rs.s0;
-            const uint32_t ku = base_key(cz.b, cy.b, ixu.b, g), kv = base_key(cz.b, iyv.b, cx.b, g);
-            const uint32_t kw = base_key(ow.b, cy.b, cx.b, g);
-            if (!(same && c.ku == ku))
-                load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, cz.b, cy.b, ixu.b, c.u);
-            if (!(same && c.kv == kv))
-                load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, cz.b, iyv.b, cx.b, c.v);
-            if (!(same && c.kw == kw))
-                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, ow.b, cy.b, cx.b, c.w);
-            c.s0 = rs.s0;
-            c.ku = ku;
-            c.kv = kv;
-            c.kw = kw;
-            blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
-            blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
-            blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
-        }
-    }
+    if (LAYOUT != OPX_LAYOUT_RING && (j < 0 || j1 < 0))
+        return false;
+    blend_uvw<T, LAYOUT, false>(win, g, j, rs.s0, rs.s1, gt, ft, cz.b, cy.b, cx.b, ixu.b, iyv.b, ow.b, c, vu, vv,
+                                vw);
     const float u = tri32(vu, cz.w, cy.w, ixu.w);
     const float v = tri32(vv, cz.w, iyv.w, cx.w);
     // w with FP64 weights: the vertical index moves through layers that can be
     // a fraction of a metre thick
     const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
     Corners2 &c2 = c.c2;
-    sample2<LAYOUT>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
+    sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
     double h0, zeta0;
     const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
     // the last post step evaluated h, zeta here at this time (same corners and
@@ -1150,7 +1174,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         // cell and the zeta snapshot pair does not change, so the corners read
         // above are reused
         const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
-        sample2<LAYOUT>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
+        sample2<LAYOUT, false>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
         eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
         if constexpr (kKeepHz) {
@@ -1180,15 +1204,25 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     return true;
 }
 
-// `nsteps` fused AB3 steps of particle i; step k reads its release's
-// weights from table[k * tstride + release].  The particle's state
-// is loaded once, stepped in registers and stored once: the same values as
-// nsteps one-step launches.  Each step validates first (validation.pyx:21-77,
-// inclusive bounds in FP64); a flagged particle stops.
-template <typename T, int LAYOUT>
-__device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
-                                        const opx_window &win, const opx_release_step *__restrict__ table,
-                                        int tstride, int nsteps, double dt, const double *Cw, int32_t *err)
+enum Scheme { kAB3 = 0, kRK2 = 1 };
+
+template <typename T, int LAYOUT, bool RING_OK>
+__device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
+                                         const opx_window &win, double dt, double alpha, const double *Cw,
+                                         float *zout);
+
+// `nsteps` fused steps of particle i (AB3 or RK2); step k reads its release's
+// weights from table[k * tstride + release].  The particle's state is loaded
+// once, stepped in registers and stored once: the same values as nsteps
+// one-step launches.  Each step validates first (validation.pyx:21-77,
+// inclusive bounds in FP64); a flagged particle stops.  A step that meets a
+// snapshot pair its packed window lacks (only without RING_OK) stops the
+// particle and sets *err.
+template <typename T, int LAYOUT, int SCHEME, bool RING_OK>
+__device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                          const opx_window &win, const opx_release_step *__restrict__ table,
+                                          int tstride, int nsteps, double dt, double alpha, const double *Cw,
+                                          int32_t *err)
 {
     if (i >= n)
         return;
@@ -1218,14 +1252,16 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
                 p.ay.q = s.fy[i];
                 p.ax.q = s.fx[i];
             }
-            if (p.st != OPX_MULTISTEP_1) {
-                p.hz1 = s.hz1[i];
-                p.hy1 = s.hy1[i];
-                p.hx1 = s.hx1[i];
-                if (p.st != OPX_MULTISTEP_2) {
-                    p.hz2 = s.hz2[i];
-                    p.hy2 = s.hy2[i];
-                    p.hx2 = s.hx2[i];
+            if constexpr (SCHEME == kAB3) {
+                if (p.st != OPX_MULTISTEP_1) {
+                    p.hz1 = s.hz1[i];
+                    p.hy1 = s.hy1[i];
+                    p.hx1 = s.hx1[i];
+                    if (p.st != OPX_MULTISTEP_2) {
+                        p.hz2 = s.hz2[i];
+                        p.hy2 = s.hy2[i];
+                        p.hx2 = s.hx2[i];
+                    }
                 }
             }
             loaded = true;
@@ -1240,7 +1276,13 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
             p.st = OPX_OUT_OF_DOMAIN;
             break;
         }
-        if (!ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, s.z ? s.z + i : nullptr)) {
+        float *zout = s.z ? s.z + i : nullptr;
+        bool ok;
+        if constexpr (SCHEME == kAB3)
+            ok = ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, zout);
+        else
+            ok = rk2_step<T, LAYOUT, RING_OK>(p, c, rs, g, win, dt, alpha, Cw, zout);
+        if (!ok) {
             if (err)
                 *err = OPX_FLAG_UNPACKED_PAIR;
             break;
@@ -1252,12 +1294,14 @@ __device__ __forceinline__ void ab3_run(int64_t i, int64_t n, const opx_split_st
         s.fz[i] = p.az.q;
         s.fy[i] = p.ay.q;
         s.fx[i] = p.ax.q;
-        s.hz2[i] = p.hz2;
-        s.hy2[i] = p.hy2;
-        s.hx2[i] = p.hx2;
-        s.hz1[i] = p.hz1;
-        s.hy1[i] = p.hy1;
-        s.hx1[i] = p.hx1;
+        if constexpr (SCHEME == kAB3) {
+            s.hz2[i] = p.hz2;
+            s.hy2[i] = p.hy2;
+            s.hx2[i] = p.hx2;
+            s.hz1[i] = p.hz1;
+            s.hy1[i] = p.hy1;
+            s.hx1[i] = p.hx1;
+        }
     }
     if (p.st != st0)
         s.status[i] = p.st;
@@ -1296,7 +1340,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULT
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
     if (c < c1) {
         if constexpr (MULTI)
-            ab3_run<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt, Cw, err);
+            fused_run<T, LAYOUT, kAB3, false>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt, 0.0,
+                                              Cw, err);
         else
             ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
     }
@@ -1596,101 +1641,63 @@ __device__ __forceinline__ AxD axd(double idx, double off, int size)
 }
 
 
-// u, v (FP32 trilinear) and w (FP64 weights) blended at slots (s0, s1) at an
-// FP64 position (rk2_w_advection.pyx:124-150).
-template <typename T, int LAYOUT>
-__device__ __forceinline__ void sample_uvw(const opx_window &win, const opx_grid &g, int s0, int s1, float gt,
-                                           float ft, double zi, double yi, double xi, float &u, float &v, double &w)
+// u, v (FP32 trilinear) and w (FP64 weights) of pair (s0, s1) at weights
+// (gt, ft) at an FP64 position (rk2_w_advection.pyx:124-150), through the
+// stencil cache.
+template <typename T, int LAYOUT, bool RING_OK>
+__device__ __forceinline__ void rk2_uvw(const opx_window &win, const opx_grid &g, int j, int s0, int s1, float gt,
+                                        float ft, double zi, double yi, double xi, StencilCache<T, LAYOUT> &c,
+                                        float &u, float &v, double &w)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const AxD cz = axd(zi, 0.0, NZ), cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
     const AxD ix = axd(xi, -0.5, NX - 1), iy = axd(yi, -0.5, NY - 1), oz = axd(zi, 0.5, NZ + 1);
     float vu[8], vv[8], vw[8];
-    const int j = packed_index<LAYOUT>(win, s0, s1);
-    if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
-        corners<T, LAYOUT>(pick(j, win.pu), nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ix.b, vu);
-        corners<T, LAYOUT>(pick(j, win.pv), nullptr, gt, ft, NY - 1, NX, cz.b, iy.b, cx.b, vv);
-        corners<T, LAYOUT>(pick(j, win.pw), nullptr, gt, ft, NY, NX, oz.b, cy.b, cx.b, vw);
-    } else {
-        const T *U = static_cast<const T *>(win.u), *V = static_cast<const T *>(win.v);
-        const T *W = static_cast<const T *>(win.w);
-        corners<T, OPX_LAYOUT_RING>(U + s0 * win.u_stride, U + s1 * win.u_stride, gt, ft, NY, NX - 1, cz.b, cy.b,
-                                    ix.b, vu);
-        corners<T, OPX_LAYOUT_RING>(V + s0 * win.v_stride, V + s1 * win.v_stride, gt, ft, NY - 1, NX, cz.b, iy.b,
-                                    cx.b, vv);
-        corners<T, OPX_LAYOUT_RING>(W + s0 * win.w_stride, W + s1 * win.w_stride, gt, ft, NY, NX, oz.b, cy.b, cx.b,
-                                    vw);
-    }
+    blend_uvw<T, LAYOUT, RING_OK>(win, g, j, s0, s1, gt, ft, cz.b, cy.b, cx.b, ix.b, iy.b, oz.b, c, vu, vv, vw);
     u = tri32(vu, (float)cz.w, (float)cy.w, (float)ix.w);
     v = tri32(vv, (float)cz.w, (float)iy.w, (float)cx.w);
     w = tri64(vw, oz.w, cy.w, cx.w);
 }
 
-// h and zeta (FP64 weights; zeta blended in FP32) and, if asked, dx and dy
-// (FP32) at an FP64 horizontal position.
-template <int LAYOUT>
-__device__ __forceinline__ void sample_2d(const opx_window &win, int s0, int s1, float gt, float ft, int NY, int NX,
-                                          double yi, double xi, bool want_dxdy, double &h, double &zeta, float &dxv,
-                                          float &dyv)
+// h and zeta (FP64 weights; zeta blended in FP32) and, with DXDY, dx and dy
+// (FP32) at an FP64 horizontal position, through the 2-D corner cache.
+template <int LAYOUT, bool RING_OK, bool DXDY, typename Cache>
+__device__ __forceinline__ void rk2_2d(const opx_window &win, int j, int s0, int s1, float gt, float ft, int NY,
+                                       int NX, double yi, double xi, Cache &c, double &h, double &zeta, float &dxv,
+                                       float &dyv)
 {
     const AxD cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
-    Corners2 c2;
-    const int j = packed_index<LAYOUT>(win, s0, s1);
-    if (LAYOUT != OPX_LAYOUT_RING && j >= 0)
-        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, cy.b, cx.b, want_dxdy, c2);
-    else
-        load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
-                   win.zeta + s1 * win.zeta_stride, NX, cy.b, cx.b, want_dxdy, c2);
-    eval2(c2, gt, ft, cy.w, cx.w, h, zeta);
-    if (want_dxdy) {
-        dxv = bil32(c2.dx, (float)cy.w, (float)cx.w);
-        dyv = bil32(c2.dy, (float)cy.w, (float)cx.w);
+    sample2<LAYOUT, RING_OK>(win, j, s0, s1, NX, cy.b, cx.b, c.c2, c.s0_2d);
+    eval2(c.c2, gt, ft, cy.w, cx.w, h, zeta);
+    if constexpr (DXDY) {
+        dxv = bil32(c.c2.dx, (float)cy.w, (float)cx.w);
+        dyv = bil32(c.c2.dy, (float)cy.w, (float)cx.w);
     }
 }
 
-// One particle's fused RK2 step: validation at t, stage 1 at (x_n, t), stage 2
-// at (x*, t + alpha dt), update and zidx at t + dt.
-template <typename T, int LAYOUT>
-__device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
-                                        const opx_window &win, const opx_release_step *__restrict__ table, double dt,
-                                        double alpha, const double *Cw)
+// One fused RK2 step of a validated particle in registers: stage 1 at
+// (x_n, t), stage 2 at (x*, t + alpha dt), update and zidx at t + dt.  Stage 2
+// and the update usually sample the cells (and pair) of stage 1, so they blend
+// its cached nodes at their own time weights.  Without RING_OK a packed LAYOUT
+// reads only its packed windows and returns false, leaving the particle
+// unchanged, if one of the step's pairs is not packed.
+template <typename T, int LAYOUT, bool RING_OK>
+__device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
+                                         const opx_window &win, double dt, double alpha, const double *Cw,
+                                         float *zout)
 {
-    if (i >= n)
-        return;
-    const uint8_t st = s.status[i];
-    if (st & OPX_INACTIVE)
-        return;
-    const int r = s.rel ? (int)s.rel[i] : 0;
-    const Step rs = load_release(table, r, win.nslots);
-    if (!rs.active)
-        return;
-
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
-    const uint32_t cell = (uint32_t)s.cell[i];
-    Axis az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
-    Axis ay = decode_axis(cell, g.xbits, g.ybits);
-    Axis ax = decode_axis(cell, 0, g.xbits);
-
-    // ---- validation (validation.pyx:21-77), the RK2 pre-step kernel
-    if (!(az.ok && ay.ok && ax.ok)) {
-        s.status[i] = OPX_NONFINITE;
-        return;
-    }
-    az.q = s.fz[i];
-    ay.q = s.fy[i];
-    ax.q = s.fx[i];
-    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
-    if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
-          xi <= g.xidx_max)) {
-        s.status[i] = OPX_OUT_OF_DOMAIN;
-        return;
-    }
+    const double zi = p.az.pos(), yi = p.ay.pos(), xi = p.ax.pos();
     const double hc = g.hc;
+    const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1), jm = packed_index<LAYOUT>(win, rs.s0m, rs.s1m);
+    const int jp = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
+    if (LAYOUT != OPX_LAYOUT_RING && !RING_OK && (j < 0 || jm < 0 || jp < 0))
+        return false;
 
     // ---- stage 1 (rk2_w_advection.pyx:31-153): z at x_n and k1 = (u/dx, v/dy, w)
     double h0, zeta0;
     float dxv, dyv;
-    sample_2d<LAYOUT>(win, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, true, h0, zeta0, dxv, dyv);
+    rk2_2d<LAYOUT, RING_OK, true>(win, j, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, c, h0, zeta0, dxv, dyv);
     const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
     const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
     const double sigma = (zi + 0.5) / NZ - 1.0;
@@ -1698,7 +1705,7 @@ __device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_st
     const double z = zeta0 + (zeta0 + h0) * S0;
     float u1, v1;
     double w1;
-    sample_uvw<T, LAYOUT>(win, g, rs.s0, rs.s1, rs.gt, rs.ft, zi, yi, xi, u1, v1, w1);
+    rk2_uvw<T, LAYOUT, RING_OK>(win, g, j, rs.s0, rs.s1, rs.gt, rs.ft, zi, yi, xi, c, u1, v1, w1);
     const double kx1 = (double)(u1 / dxv), ky1 = (double)(v1 / dyv);
 
     // ---- stage 2 (:155-275) at x* = x_n + alpha dt k1, time t + alpha dt
@@ -1706,11 +1713,11 @@ __device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_st
     const double zs = z + adt * w1, ys = yi + adt * ky1, xs = xi + adt * kx1;
     double hs, zetas;
     float dxs, dys;
-    sample_2d<LAYOUT>(win, rs.s0m, rs.s1m, rs.gtm, rs.ftm, NY, NX, ys, xs, true, hs, zetas, dxs, dys);
+    rk2_2d<LAYOUT, RING_OK, true>(win, jm, rs.s0m, rs.s1m, rs.gtm, rs.ftm, NY, NX, ys, xs, c, hs, zetas, dxs, dys);
     const double zis = zidx_from_z(zs, hs, zetas, hc, NZ, Cw, oz.b);
     float u2, v2;
     double w2;
-    sample_uvw<T, LAYOUT>(win, g, rs.s0m, rs.s1m, rs.gtm, rs.ftm, zis, ys, xs, u2, v2, w2);
+    rk2_uvw<T, LAYOUT, RING_OK>(win, g, jm, rs.s0m, rs.s1m, rs.gtm, rs.ftm, zis, ys, xs, c, u2, v2, w2);
     const double kx2 = (double)(u2 / dxs), ky2 = (double)(v2 / dys);
 
     // ---- update (:278-354): b1 = 1/(2 alpha) on stage 1, b2 = 1 - b1 on stage 2
@@ -1718,32 +1725,33 @@ __device__ __forceinline__ void rk2_one(int64_t i, int64_t n, const opx_split_st
     const double zn = z + b1 * dt * w1 + b2 * dt * w2;
     const double yn = yi + b1 * dt * ky1 + b2 * dt * ky2;
     const double xn = xi + b1 * dt * kx1 + b2 * dt * kx2;
-    ay = encode_axis(yn, NY);
-    ax = encode_axis(xn, NX);
+    const Axis ay = encode_axis(yn, NY), ax = encode_axis(xn, NX);
     Axis azn;
     if (ay.ok && ax.ok) {
         double h1, zeta1;
-        float unused0 = 0.f, unused1 = 0.f;
-        sample_2d<LAYOUT>(win, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, ay.pos(), ax.pos(), false, h1, zeta1, unused0,
-                          unused1);
-        int k0 = (int)floor(zis + 0.5);
+        float unused0, unused1;
+        rk2_2d<LAYOUT, RING_OK, false>(win, jp, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, ay.pos(), ax.pos(), c, h1,
+                                       zeta1, unused0, unused1);
+        const int k0 = (int)floor(zis + 0.5);
         azn = encode_axis(zidx_from_z(zn, h1, zeta1, hc, NZ, Cw, k0), NZ);
     } else {
         azn = encode_axis(__builtin_nan(""), NZ);
     }
-    s.cell[i] = (int32_t)pack_cell(azn, ay, ax, g);
-    s.fz[i] = azn.q;
-    s.fy[i] = ay.q;
-    s.fx[i] = ax.q;
-    if (s.z)
-        s.z[i] = (float)zn;
+    p.az = azn;
+    p.ay = ay;
+    p.ax = ax;
+    if (zout)
+        *zout = (float)zn;
+    return true;
 }
 
-template <typename T, int LAYOUT, int BLOCK, bool IND>
-__global__ void __launch_bounds__(BLOCK)
+// Fused RK2 steps, one particle per thread (launch shape as k_ab3_fused); the
+// one-step form (MULTI false) reads the rings for pairs that are not packed.
+template <typename T, int LAYOUT, int BLOCK, bool IND, bool MULTI>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULTI ? 3 : 1)))
     k_rk2_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
-                const opx_release_step *__restrict__ table, double dt, double alpha,
-                const int32_t *__restrict__ n_active)
+                const opx_release_step *__restrict__ table, int tstride, int nsteps, double dt, double alpha,
+                const int32_t *__restrict__ n_active, int32_t *err)
 {
     const opx_window win = IND ? *win_dev : win_arg;
     __shared__ double Cw[kLevels];
@@ -1757,7 +1765,8 @@ __global__ void __launch_bounds__(BLOCK)
     const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
     const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
     if (c < c1)
-        rk2_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, alpha, Cw);
+        fused_run<T, LAYOUT, kRK2, !MULTI>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, MULTI ? nsteps : 1,
+                                           dt, alpha, Cw, err);
 }
 
 // ---------------------------------------------------------------- re-binning
@@ -2015,8 +2024,8 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
 }
 
 int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
-               int layout, int dtype3, const opx_release_step *table, double dt, double alpha,
-               const int32_t *n_active, int block, void *stream)
+               int layout, int dtype3, const opx_release_step *table, int tstride, int nsteps, double dt, double alpha,
+               const int32_t *n_active, int block, int32_t *err, void *stream)
 {
     if (layout == OPX_LAYOUT_PAIR_LDS) // the RK2 step reads PAIR windows directly
         layout = OPX_LAYOUT_PAIR;
@@ -2024,9 +2033,14 @@ int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &wi
     hipStream_t st = as_stream(stream);
     const bool ind = win_dev != nullptr;
 #define OPX_RK2_LAUNCH(T, L, B, I)                                                                         \
-    if (block == B && layout == L && ind == I)                                                            \
-        hipLaunchKernelGGL((k_rk2_fused<T, L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, dt, alpha, \
-                           n_active);
+    if (block == B && layout == L && ind == I) {                                                          \
+        if (nsteps > 1)                                                                                   \
+            hipLaunchKernelGGL((k_rk2_fused<T, L, B, I, true>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
+                               table, tstride, nsteps, dt, alpha, n_active, err);                         \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_rk2_fused<T, L, B, I, false>), grid, blk, 0, st, *s, *g, win_arg, win_dev, \
+                               table, tstride, nsteps, dt, alpha, n_active, err);                         \
+    }
 #define OPX_RK2_LAUNCH_L(T, L)                                                                             \
     OPX_RK2_LAUNCH(T, L, 128, false) OPX_RK2_LAUNCH(T, L, 256, false) OPX_RK2_LAUNCH(T, L, 128, true)     \
         OPX_RK2_LAUNCH(T, L, 256, true)
@@ -2145,9 +2159,9 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
                                    stream);
 }
 
-int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
-                  const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
-                  void *stream)
+int opx_rk2_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                        const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt, double alpha,
+                        const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_rk2_fused", false))) return rc;
@@ -2155,18 +2169,29 @@ int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
     if ((rc = check_window(win, g))) return rc;
     OPX_REQUIRE(table != nullptr, "null release table");
     OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= 1, "table_stride must be >= 1 for several steps");
     if (s->n == 0)
         return OPX_OK;
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
     const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
-    return launch_rk2(s, g, *win, nullptr, layout, win->dtype3, table, dt, alpha, n_active, block, stream);
+    return launch_rk2(s, g, *win, nullptr, layout, win->dtype3, table, table_stride, nsteps, dt, alpha, n_active,
+                      block, err_flag, stream);
 }
 
-int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
-                      int32_t layout, const opx_release_step *table, double dt, double alpha,
-                      const int32_t *n_active, int32_t block, void *stream)
+int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                  const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
+                  void *stream)
+{
+    return opx_rk2_fused_steps(s, g, win, table, 0, 1, dt, alpha, n_active, block, nullptr, stream);
+}
+
+int opx_rk2_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                            int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                            double dt, double alpha, const int32_t *n_active, int32_t block, int32_t *err_flag,
+                            void *stream)
 {
     int rc;
     if ((rc = check_state(s, "opx_rk2_fused_dev", false))) return rc;
@@ -2175,13 +2200,24 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
     OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
     OPX_REQUIRE(alpha > 0.0, "alpha must be > 0");
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= 1, "table_stride must be >= 1 for several steps");
     if (block == 0)
         block = 128;
     OPX_REQUIRE(block == 128 || block == 256, "block size %d not in {128, 256}", block);
     if (s->n == 0)
         return OPX_OK;
     opx_window unused{};
-    return launch_rk2(s, g, unused, win_dev, layout, dtype3, table, dt, alpha, n_active, block, stream);
+    return launch_rk2(s, g, unused, win_dev, layout, dtype3, table, table_stride, nsteps, dt, alpha, n_active, block,
+                      err_flag, stream);
+}
+
+int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
+                      int32_t layout, const opx_release_step *table, double dt, double alpha,
+                      const int32_t *n_active, int32_t block, void *stream)
+{
+    return opx_rk2_fused_steps_dev(s, g, win_dev, dtype3, layout, table, 0, 1, dt, alpha, n_active, block, nullptr,
+                                   stream);
 }
 
 static int elem_bytes(int dtype3, int layout)

@@ -179,7 +179,7 @@ class AB3Engine:
         self._structs = None
         self._graphs = {}
         self.graph_steps = 0  # > 0: replay captured graphs of this many steps
-        # AB3 steps per fused launch (1..MAX_LAUNCH_STEPS): the state stays in
+        # steps per fused launch (1..MAX_LAUNCH_STEPS): the state stays in
         # registers between them and unchanged stencils are not read again
         self.launch_steps = 1
         # optional list collecting (start, end, steps) around each fused launch
@@ -610,25 +610,23 @@ class AB3Engine:
         windows; ``ring`` runs it with the RING layout instead, for runs of
         steps that need more snapshot pairs than the two packed windows hold."""
         rk2 = self.scheme == "rk2"
-        if rk2 and m != 1:
-            raise ValueError("the fused RK2 step runs one step per launch")
         _lib.call("opx_clock_advance_steps", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap, int(m),
                   self._rel_cap, self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots,
                   0, self._step_dev.data_ptr(), self.dt, self.alpha if rk2 else 0.0, self._err.data_ptr(), stream)
         s = self._launch_structs()
+        layout = _lib.OPX_LAYOUT_RING if (ring and m > 1) else self.layout
         if rk2:
-            _lib.call("opx_rk2_fused_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
-                      self.window.dtype3, self.layout, self._table.data_ptr(), self.dt, self.alpha,
-                      self._n_active_dev.data_ptr(), self.block, stream)
+            _lib.call("opx_rk2_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
+                      self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt, self.alpha,
+                      self._n_active_dev.data_ptr(), self.block, self._err.data_ptr(), stream)
         else:
-            layout = _lib.OPX_LAYOUT_RING if (ring and m > 1) else self.layout
             _lib.call("opx_ab3_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
                       self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,
                       self._n_active_dev.data_ptr(), self.block, self._err.data_ptr(), stream)
 
     def _launch_sizes(self, k):
         """Steps per launch for a run of k steps."""
-        m = self.launch_steps if self.scheme == "ab3" else 1
+        m = self.launch_steps
         return [m] * (k // m) + ([k % m] if k % m else [])
 
     def _graph(self, k, ring=False):
@@ -659,7 +657,7 @@ class AB3Engine:
         while left > 0:
             if self.rebin_every > 0 and self._since_rebin >= self.rebin_every:
                 self.rebin()
-            k = min(self.launch_steps if self.scheme == "ab3" else 1, left)
+            k = min(self.launch_steps, left)
             if self.graph_steps > 0 and self.timer is None:
                 k = min(max(self.graph_steps, k), left)
             if self.rebin_every > 0:
@@ -727,7 +725,7 @@ class AB3Engine:
             layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
                        _lib.OPX_LAYOUT_OCT, _lib.OPX_LAYOUT_YPAIR)
         if launch_steps is None:
-            launch_steps = (1, 8, 16, 32) if self.scheme == "ab3" else (1,)
+            launch_steps = (1, 8, 16, 32)
         saved_every, self.rebin_every = self.rebin_every, 0
         self.launch_steps = 1
         trials = {}

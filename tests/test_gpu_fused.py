@@ -238,12 +238,13 @@ def test_lds_tiles_staged_and_fallback(device, storage):
             assert np.array_equal(res[0][k], res[1][k], equal_nan=True), (rebin, k)
 
 
+@pytest.mark.parametrize("scheme", ["ab3", "rk2"])
 @pytest.mark.parametrize("storage", [torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("layout", ["ring", "pair", "quad", "oct", "ypair"])
-def test_several_steps_per_launch_identical(device, storage, layout):
-    """k AB3 steps per launch (state in registers, unchanged stencils blended again) give the bits of one
-    step per launch: across snapshot boundaries, re-bins and graph replays, with releases that start, end
-    and cross a time-array edge inside a launch, and particles that leave the domain."""
+def test_several_steps_per_launch_identical(device, storage, layout, scheme):
+    """k AB3 or RK2 steps per launch (state in registers, unchanged stencils blended again) give the bits of
+    one step per launch: across snapshot boundaries, re-bins and graph replays, with releases that start,
+    end and cross a time-array edge inside a launch, and particles that leave the domain."""
     from offline_particles_amd.engine import LAYOUT_NAMES
 
     code = {v: k for k, v in LAYOUT_NAMES.items()}[layout]
@@ -267,7 +268,7 @@ def test_several_steps_per_launch_identical(device, storage, layout):
     res = []
     for launch_steps, graph_steps in ((1, 0), (3, 0), (8, 0), (8, 8), (16, 5), (64, 0)):
         eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
-                        capacity=sum(n), rebin_every=23)
+                        capacity=sum(n), rebin_every=23, scheme=scheme)
         eng.set_layout(code)
         for p, t0, ns, s0 in zip(pos, starts, steps, first):
             eng.add_release(*p, start_time=t0, nsteps=ns, start_step=s0)
