@@ -442,6 +442,7 @@ _PROFILE_FOR_FORM = {
     ("fused_fp32", 16): "k_ab3_fused_steps16_f32_turbulent16M.md",
     ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 1): "k_rk2_fused_f32_turbulent16M.md",
+    ("fused_rk2_fp32", 16): "k_rk2_fused_steps16_f32_turbulent16M.md",
 }
 
 
