@@ -1080,7 +1080,9 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
                 load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, bz, by, bxu, c.u);
             if (!(same && c.kv == kv))
                 load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, bz, byv, bx, c.v);
-            if (!(same && c.kw == kw))
+            // FP32 w nodes are read every step: caching them would spill
+            constexpr bool kCacheW = sizeof(T) == 2;
+            if (kCacheW && !(same && c.kw == kw))
                 load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, bzw, by, bx, c.w);
             c.s0 = s0;
             c.ku = ku;
@@ -1088,7 +1090,13 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
             c.kw = kw;
             blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
             blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
-            blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
+            if constexpr (kCacheW) {
+                blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
+            } else {
+                Raw3<T, LAYOUT> rw;
+                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, bzw, by, bx, rw);
+                blend_raw<T, LAYOUT>(rw, gt, ft, vw);
+            }
         }
     }
     if constexpr (LAYOUT == OPX_LAYOUT_RING || RING_OK) {
