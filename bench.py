@@ -352,7 +352,7 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
 
 
 def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 50)):
-    """Batching form: uneven releases (own start time, duration, 10 km boxes) in one launch per step.
+    """Batching form: uneven releases (own start time, duration, 10 km boxes) stepped by shared launches.
 
     The bin shape, layout and workgroup size come from the autotuner run on a
     probe engine holding a sample of the releases; the main run then steps

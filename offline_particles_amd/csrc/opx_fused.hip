@@ -1315,7 +1315,8 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
         s.status[i] = p.st;
 }
 
-// Fused AB3 step, one particle per thread.  The grid covers s.n particles;
+// Fused AB3 steps (one, or nsteps with MULTI), one particle per thread.  The
+// grid covers s.n particles;
 // the particles actually stepped are the first min(s.n, *n_active), so one
 // captured graph serves any active count up to s.n (surplus blocks exit at
 // once).  XCD k (blocks k, k+8, ...) takes the k-th contiguous eighth of the

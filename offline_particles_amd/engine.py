@@ -1,10 +1,14 @@
 """Fused advection engine: the fast form of the hot path.
 
-``AB3Engine`` advances particles with one fused HIP kernel per step: AB3
-(``opx_ab3_fused``: validation -> ab3_w_advection -> ab3_post_w_advection,
-the reference's timesteppers.py:296-305 sequence) or, with ``scheme="rk2"``,
-RK2 (``opx_rk2_fused``: validation -> _rk2_step_1 -> _rk2_step_2 ->
-_rk2_update, timesteppers.py:241-257), over
+``AB3Engine`` advances particles with fused HIP kernels that take one or
+several (``launch_steps``) whole steps per launch: AB3 (``opx_ab3_fused_steps``:
+validation -> ab3_w_advection -> ab3_post_w_advection, the reference's
+timesteppers.py:296-305 sequence) or, with ``scheme="rk2"``, RK2
+(``opx_rk2_fused_steps``: validation -> _rk2_step_1 -> _rk2_step_2 ->
+_rk2_update, timesteppers.py:241-257).  Between the steps of a launch each
+particle's state stays in registers and the stencil nodes it read are blended
+again while its cells are unchanged; the results are the same bits as one step
+per launch.  The engine keeps
 
 - split positions (packed int32 cell + 32-bit fixed-point offsets, exact in
   FP64) and FP32 AB3 history;
