@@ -233,10 +233,13 @@ enum opx_layout {
 
 /* A ring of `nslots` snapshots per time-dependent field, contiguous with the
  * given element strides between slots; `hdd` packs the static (h, dx, dy, 0)
- * per rho point as float4.  Optionally up to two packed stencil windows
- * (built by opx_pack_window) for the slot pairs pslot[j] = {s0, s1}; a release
- * whose slots match packed window j reads it in `layout`, others read the
- * rings. */
+ * per rho point as float4.  With npacked == nslots the window also has the
+ * pair ring: for every slot s a packed stencil window (built by
+ * opx_pack_window) of the slot pair (s, (s + 1) mod nslots) in `layout`,
+ * window s at byte offset s * p*_stride from pu, pv, pw and p2.  A step that
+ * reads the pair with first slot s0 reads window s0; the caller keeps every
+ * window a launch reads packed from the current contents of its two slots.
+ * With npacked == 0 every step reads the rings. */
 typedef struct opx_window {
     const void *u, *v, *w;  /* dtype3 storage */
     const float *zeta;
@@ -245,16 +248,16 @@ typedef struct opx_window {
     int32_t nslots;
     const float *hdd; /* (NY, NX, 4) */
     const double *C;  /* (NZ + 1) */
-    int32_t layout;   /* opx_layout of the packed windows */
-    int32_t npacked;  /* 0, 1 or 2 */
-    int32_t pslot[2][2];
-    const void *pu[2], *pv[2], *pw[2];
-    const void *p2[2]; /* (NY, NX) float4 {h, zeta[s0], zeta[s1], dx} per packed window */
-    const float *dy;   /* (NY, NX) static dy, read with packed windows */
+    int32_t layout;   /* opx_layout of the pair ring */
+    int32_t npacked;  /* 0 (no pair ring) or nslots */
+    const void *pu, *pv, *pw;
+    const void *p2; /* (NY, NX) float4 {h, zeta[s0], zeta[s1], dx} per window */
+    uint64_t pu_stride, pv_stride, pw_stride, p2_stride; /* bytes from one window to the next */
+    const float *dy; /* (NY, NX) static dy, read with packed windows */
 } opx_window;
 
-/* Bytes of one packed window of u, v and w for a layout (PAIR or QUAD) and of
- * its 2-D part. */
+/* Bytes of one packed window of u, v and w for a layout (PAIR, QUAD, OCT or
+ * YPAIR; PAIR_LDS packs as PAIR) and of its 2-D part. */
 int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t *bytes_u, uint64_t *bytes_v,
                      uint64_t *bytes_w, uint64_t *bytes_2d);
 /* Build a packed window of ring slots (s0, s1) into caller-owned pu, pv, pw
@@ -273,7 +276,7 @@ typedef struct opx_release_clock {
 /* Values written to the device error flags (err_flag arguments). */
 enum opx_err_flag {
     OPX_FLAG_TIME_OUT_OF_RANGE = 1, /* a release's time left the time array */
-    OPX_FLAG_UNPACKED_PAIR = 2,     /* a several-step launch met a snapshot pair its window does not pack */
+    OPX_FLAG_UNPACKED_PAIR = 2,     /* a several-step launch of a packed layout found no pair ring */
 };
 
 /* Per-release weights of one step, written by opx_clock_advance (16 bytes,
@@ -333,11 +336,12 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
  * and 2-D stencil nodes it read are blended again at the next step's weights
  * while its cells and snapshot pair are unchanged: the results are those of
  * nsteps one-step launches (timesteppers.py:296-305 applied nsteps times),
- * bit for bit.  With nsteps > 1 and a packed layout, every snapshot pair the
- * steps read (at t and t + dt) must be one of the window's packed pairs: a
- * particle that meets another pair stops, unchanged from that step on, and
- * *err_flag (if not NULL) is set to OPX_FLAG_UNPACKED_PAIR; the RING layout
- * reads any pair.  No counterpart in the reference, which launches per step. */
+ * bit for bit.  With nsteps > 1 and a packed layout the steps read only the
+ * pair ring (npacked > 0), whose windows of every pair the steps read must be
+ * current; without a pair ring a particle stops, unchanged from that step on,
+ * and *err_flag (if not NULL) is set to OPX_FLAG_UNPACKED_PAIR.  The RING
+ * layout reads the rings.  No counterpart in the reference, which launches
+ * per step. */
 int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                         const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
                         const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);

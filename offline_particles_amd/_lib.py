@@ -113,8 +113,9 @@ class opx_window(ctypes.Structure):
                 ("u_stride", ctypes.c_int64), ("v_stride", ctypes.c_int64), ("w_stride", ctypes.c_int64),
                 ("zeta_stride", ctypes.c_int64), ("dtype3", ctypes.c_int32), ("nslots", ctypes.c_int32),
                 ("hdd", _vp), ("C", _vp), ("layout", ctypes.c_int32), ("npacked", ctypes.c_int32),
-                ("pslot", (ctypes.c_int32 * 2) * 2), ("pu", _vp * 2), ("pv", _vp * 2), ("pw", _vp * 2),
-                ("p2", _vp * 2), ("dy", _vp)]
+                ("pu", _vp), ("pv", _vp), ("pw", _vp), ("p2", _vp),
+                ("pu_stride", ctypes.c_uint64), ("pv_stride", ctypes.c_uint64), ("pw_stride", ctypes.c_uint64),
+                ("p2_stride", ctypes.c_uint64), ("dy", _vp)]
 
 
 class opx_release_clock(ctypes.Structure):

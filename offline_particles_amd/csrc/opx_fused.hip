@@ -515,14 +515,10 @@ __device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
     return (v[0] * gx + v[1] * fx) * gy + (v[2] * gx + v[3] * fx) * fy;
 }
 
-// Pointer a[1] if j else a[0], as a bit select: the compiler then keeps both
-// pointers in registers instead of forming a dynamically indexed (scratch)
-// copy of the window's pointer pair.
-__device__ __forceinline__ const void *pick(int j, const void *const a[2])
+// Packed window j of a pair ring at `base` (windows `stride` bytes apart).
+__device__ __forceinline__ const void *window(const void *base, uint64_t stride, int j)
 {
-    const uintptr_t p0 = reinterpret_cast<uintptr_t>(a[0]), p1 = reinterpret_cast<uintptr_t>(a[1]);
-    const uintptr_t m = (uintptr_t)0 - (uintptr_t)(j != 0);
-    return reinterpret_cast<const void *>(p0 ^ ((p0 ^ p1) & m));
+    return static_cast<const char *>(base) + (uint64_t)(uint32_t)j * stride;
 }
 
 // A release's step weights and ring slots, unpacked from its 16-byte table entry.
@@ -557,19 +553,16 @@ __device__ __forceinline__ Step load_release(const opx_release_step *__restrict_
     return t;
 }
 
-// Which of the (at most two) packed windows holds slot pair (s0, s1); -1: none.
+// The packed window of the slot pair with first slot s0 (-1: the rings): the
+// pair ring holds one window per slot.
 template <int LAYOUT>
 __device__ __forceinline__ int packed_index(const opx_window &win, int s0, int s1)
 {
-    if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+    (void)s1; // the pair's second slot is next_slot(s0)
+    if constexpr (LAYOUT == OPX_LAYOUT_RING)
         return -1;
-    } else {
-        if (win.npacked > 0 && win.pslot[0][0] == s0 && win.pslot[0][1] == s1)
-            return 0;
-        if (win.npacked > 1 && win.pslot[1][0] == s0 && win.pslot[1][1] == s1)
-            return 1;
-        return -1;
-    }
+    else
+        return win.npacked > 0 ? s0 : -1;
 }
 
 // ---------------------------------------------------------------- clock
@@ -878,22 +871,16 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     const Ax cz = ax_center(az, NZ), cy = ax_center(ay, NY), cx = ax_center(ax, NX);
     const Ax ixu = ax_inner(ax, NX), iyv = ax_inner(ay, NY), ow = ax_outer(az, NZ);
     float vu[8], vv[8], vw[8];
-    int j = -1;
-    if constexpr (LAYOUT != OPX_LAYOUT_RING) {
-        if (win.npacked > 0 && win.pslot[0][0] == rs.s0 && win.pslot[0][1] == rs.s1)
-            j = 0;
-        else if (win.npacked > 1 && win.pslot[1][0] == rs.s0 && win.pslot[1][1] == rs.s1)
-            j = 1;
-    }
+    const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1);
     const bool tiled = TILE && j == tile.j;
     if (tiled) {
         corners_tile<T>(pool, tile.u, cz.b, cy.b, ixu.b, gt, ft, vu);
         corners_tile<T>(pool, tile.v, cz.b, iyv.b, cx.b, gt, ft, vv);
         corners_tile<T>(pool, tile.w, ow.b, cy.b, cx.b, gt, ft, vw);
     } else if (LAYOUT != OPX_LAYOUT_RING && j >= 0) {
-        const void *pu = pick(j, win.pu);
-        const void *pv = pick(j, win.pv);
-        const void *pw = pick(j, win.pw);
+        const void *pu = window(win.pu, win.pu_stride, j);
+        const void *pv = window(win.pv, win.pv_stride, j);
+        const void *pw = window(win.pw, win.pw_stride, j);
         corners<T, LAYOUT>(pu, nullptr, gt, ft, NY, NX - 1, cz.b, cy.b, ixu.b, vu);
         corners<T, LAYOUT>(pv, nullptr, gt, ft, NY - 1, NX, cz.b, iyv.b, cx.b, vv);
         corners<T, LAYOUT>(pw, nullptr, gt, ft, NY, NX, ow.b, cy.b, cx.b, vw);
@@ -918,7 +905,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     if (tiled)
         load2_tile(pool, tile.t2, cy.b, cx.b, true, c2);
     else if (packed0)
-        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, cy.b, cx.b, true, c2);
+        load2_packed(static_cast<const float4 *>(window(win.p2, win.p2_stride, j)), win.dy, NX, cy.b, cx.b, true, c2);
     else
         load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
                    c2);
@@ -951,19 +938,13 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
         // cell and the zeta snapshot pair does not change, so the corners read
         // above are reused
         const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
-        int j1 = -1;
-        if constexpr (LAYOUT != OPX_LAYOUT_RING) {
-            if (win.npacked > 0 && win.pslot[0][0] == rs.s0p && win.pslot[0][1] == rs.s1p)
-                j1 = 0;
-            else if (win.npacked > 1 && win.pslot[1][0] == rs.s0p && win.pslot[1][1] == rs.s1p)
-                j1 = 1;
-        }
+        const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
         const bool same_pair = packed0 ? (j1 == j) : (j1 < 0 && rs.s0p == rs.s0 && rs.s1p == rs.s1);
         if (!(same_pair && by1 == c2.by && bx1 == c2.bx)) {
             if (tiled && j1 == j && in_box2(tile.t2, by1, bx1))
                 load2_tile(pool, tile.t2, by1, bx1, false, c2);
             else if (LAYOUT != OPX_LAYOUT_RING && j1 >= 0)
-                load2_packed(static_cast<const float4 *>(pick(j1, win.p2)), win.dy, NX, by1, bx1, false,
+                load2_packed(static_cast<const float4 *>(window(win.p2, win.p2_stride, j1)), win.dy, NX, by1, bx1, false,
                              c2);
             else
                 load2_ring(hdd, win.zeta + rs.s0p * win.zeta_stride, win.zeta + rs.s1p * win.zeta_stride, NX, by1,
@@ -1053,7 +1034,7 @@ __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, in
         load2_ring(reinterpret_cast<const float4 *>(win.hdd), win.zeta + s0 * win.zeta_stride,
                    win.zeta + s1 * win.zeta_stride, NX, by, bx, true, c2);
     else
-        load2_packed(static_cast<const float4 *>(pick(j, win.p2)), win.dy, NX, by, bx, true, c2);
+        load2_packed(static_cast<const float4 *>(window(win.p2, win.p2_stride, j)), win.dy, NX, by, bx, true, c2);
     s0_2d = s0;
 }
 
@@ -1077,13 +1058,13 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
             const bool same = c.s0 == s0;
             const uint32_t ku = base_key(bz, by, bxu, g), kv = base_key(bz, byv, bx, g), kw = base_key(bzw, by, bx, g);
             if (!(same && c.ku == ku))
-                load_raw<T, LAYOUT>(pick(j, win.pu), NY, NX - 1, bz, by, bxu, c.u);
+                load_raw<T, LAYOUT>(window(win.pu, win.pu_stride, j), NY, NX - 1, bz, by, bxu, c.u);
             if (!(same && c.kv == kv))
-                load_raw<T, LAYOUT>(pick(j, win.pv), NY - 1, NX, bz, byv, bx, c.v);
+                load_raw<T, LAYOUT>(window(win.pv, win.pv_stride, j), NY - 1, NX, bz, byv, bx, c.v);
             // FP32 w nodes are read every step: caching them would spill
             constexpr bool kCacheW = sizeof(T) == 2;
             if (kCacheW && !(same && c.kw == kw))
-                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, bzw, by, bx, c.w);
+                load_raw<T, LAYOUT>(window(win.pw, win.pw_stride, j), NY, NX, bzw, by, bx, c.w);
             c.s0 = s0;
             c.ku = ku;
             c.kv = kv;
@@ -1094,7 +1075,7 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
                 blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
             } else {
                 Raw3<T, LAYOUT> rw;
-                load_raw<T, LAYOUT>(pick(j, win.pw), NY, NX, bzw, by, bx, rw);
+                load_raw<T, LAYOUT>(window(win.pw, win.pw_stride, j), NY, NX, bzw, by, bx, rw);
                 blend_raw<T, LAYOUT>(rw, gt, ft, vw);
             }
         }
@@ -1497,10 +1478,10 @@ __global__ void __launch_bounds__(64)
         t.j = jmin;
         if (use) {
             const int jj = t.j;
-            stage_box<EPN>(pool, t.u, du0, pick(jj, win.pu), NY, NX - 1, lane);
-            stage_box<EPN>(pool, t.v, dv0, pick(jj, win.pv), NY - 1, NX, lane);
-            stage_box<EPN>(pool, t.w, dw0, pick(jj, win.pw), NY, NX, lane);
-            stage_box<4>(pool, t.t2, 1, pick(jj, win.p2), 1, NX, lane);
+            stage_box<EPN>(pool, t.u, du0, window(win.pu, win.pu_stride, jj), NY, NX - 1, lane);
+            stage_box<EPN>(pool, t.v, dv0, window(win.pv, win.pv_stride, jj), NY - 1, NX, lane);
+            stage_box<EPN>(pool, t.w, dw0, window(win.pw, win.pw_stride, jj), NY, NX, lane);
+            stage_box<4>(pool, t.t2, 1, window(win.p2, win.p2_stride, jj), 1, NX, lane);
             Box tdy = t.t2;
             tdy.off = kTile3 + 4 * (int)v2;
             stage_box<1>(pool, tdy, 1, win.dy, 1, NX, lane);
@@ -1941,12 +1922,18 @@ int check_window(const opx_window *w, const opx_grid *g)
                 w->dtype3);
     OPX_REQUIRE(w->nslots >= 2 && w->nslots <= kMaxSlots, "window needs 2 to %d slots (got %d)", kMaxSlots,
                 w->nslots);
-    OPX_REQUIRE(w->npacked >= 0 && w->npacked <= 2, "npacked %d not in [0, 2]", w->npacked);
-    OPX_REQUIRE(w->npacked == 0 || (w->layout >= OPX_LAYOUT_PAIR && w->layout <= OPX_LAYOUT_YPAIR),
-                "packed windows need layout PAIR, QUAD, PAIR_LDS, OCT or YPAIR (got %d)", w->layout);
-    for (int j = 0; j < w->npacked; ++j)
-        OPX_REQUIRE(w->pu[j] && w->pv[j] && w->pw[j] && w->p2[j], "packed window %d has a null pointer", j);
-    OPX_REQUIRE(w->npacked == 0 || w->dy != nullptr, "packed windows need the static dy array");
+    OPX_REQUIRE(w->npacked == 0 || w->npacked == w->nslots, "npacked %d must be 0 or nslots (%d)", w->npacked,
+                w->nslots);
+    if (w->npacked > 0) {
+        OPX_REQUIRE(w->layout >= OPX_LAYOUT_PAIR && w->layout <= OPX_LAYOUT_YPAIR,
+                    "packed windows need layout PAIR, QUAD, PAIR_LDS, OCT or YPAIR (got %d)", w->layout);
+        OPX_REQUIRE(w->pu && w->pv && w->pw && w->p2 && w->dy, "pair ring has a null pointer");
+        uint64_t bu, bv, bw, b2;
+        int rc;
+        if ((rc = opx_packed_bytes(g, w->dtype3, w->layout, &bu, &bv, &bw, &b2))) return rc;
+        OPX_REQUIRE(w->pu_stride >= bu && w->pv_stride >= bv && w->pw_stride >= bw && w->p2_stride >= b2,
+                    "pair ring strides below one packed window");
+    }
     OPX_REQUIRE(w->u_stride >= (int64_t)g->NZ * g->NY * (g->NX - 1) &&
                     w->v_stride >= (int64_t)g->NZ * (g->NY - 1) * g->NX &&
                     w->w_stride >= (int64_t)(g->NZ + 1) * g->NY * g->NX && w->zeta_stride >= (int64_t)g->NY * g->NX,

@@ -192,8 +192,11 @@ class AB3Engine:
         self.tuning = None
         # packed stencil windows (PAIR for FP32, QUAD for BF16 by default)
         self.layout = _lib.OPX_LAYOUT_PAIR if storage_dtype == torch.float32 else _lib.OPX_LAYOUT_QUAD
-        self._packed = []  # [(s0, s1, (pu, pv, pw, p2))], at most 2
-        self._spare_bufs = []  # packed-window buffers not in use
+        # the pair ring: one packed window per ring slot s (slots s, s + 1),
+        # with the snapshot pair each window was last packed from
+        self._pair_bufs = None
+        self._pair_strides = None
+        self._pair_key = None
         self._packed_bytes = {}
         # host mirror of the device clocks (same FP64 arithmetic), used to
         # know which snapshot pairs the next step reads
@@ -527,11 +530,8 @@ class AB3Engine:
         self._ring_ready[slot] = events
         self.snapshot_loads += 1
         self.snapshot_bytes += sum(self._src[n][src].nbytes for n in ("u", "v", "w", "zeta"))
-        # packed windows built from the slot's old contents are stale
-        stale = [p for p in self._packed if slot in (p[0], p[1])]
-        if stale:
-            self._packed = [p for p in self._packed if slot not in (p[0], p[1])]
-            self._spare_bufs.extend(p[2] for p in stale)
+        # pair-ring windows packed from the slot's old contents are repacked when
+        # next read (their packed snapshot pair no longer matches the slots)
 
     def _ensure_snapshots(self, pairs) -> None:
         """Make the snapshots of `pairs` resident; prefetch the one after them."""
@@ -557,48 +557,50 @@ class AB3Engine:
         if cur != nxt and (cur is None or cur < need[0]):
             self._load_slot(nxt)
 
-    def _window_buffers(self):
-        return tuple(torch.empty(b, dtype=torch.uint8, device=self.device) for b in self.packed_window_bytes())
+    def _pair_contents(self, s0):
+        """The snapshot pair that ring slots s0 and s0 + 1 hold now."""
+        s1 = (s0 + 1) % self.nslots
+        if self._src is None:  # every snapshot resident: slot s holds snapshot s
+            return (s0, s1)
+        return (self._ring_It[s0], self._ring_It[s1])
 
     def _ensure_windows(self, pairs):
-        """Pack the stencil windows of the snapshot pairs this step reads (at most two)."""
-        pairs = {(a % self.nslots, b % self.nslots) for a, b in pairs}
+        """Pack the pair-ring windows of the snapshot pairs the next steps read, where stale.
+
+        The pair ring holds one packed window per ring slot s, of the slot pair
+        (s, s + 1); a step reading the pair with first slot s0 reads window s0,
+        so every release reads packed stencils whatever its time."""
         w = self.window
         if self.layout == _lib.OPX_LAYOUT_RING:
             if w.npacked != 0:
                 w.npacked = 0
                 self._win_dirty = True
             return
-        have = [p for p in self._packed if (p[0], p[1]) in pairs]
-        missing = [pr for pr in sorted(pairs) if not any((p[0], p[1]) == pr for p in self._packed)]
-        spare = [p for p in self._packed if (p[0], p[1]) not in pairs]
-        for s0, s1 in missing[: 2 - len(have)]:
-            if spare:
-                bufs = spare.pop()[2]
-            elif self._spare_bufs:
-                bufs = self._spare_bufs.pop()
-            else:
-                bufs = self._window_buffers()
-            _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, int(s0), int(s1),
-                      bufs[0].data_ptr(), bufs[1].data_ptr(), bufs[2].data_ptr(), bufs[3].data_ptr(), self._stream())
-            have.append((s0, s1, bufs))
-        self._packed = have + spare[: 2 - len(have)]
-        new = (self.layout, tuple((int(a), int(b), tuple(x.data_ptr() for x in bufs)) for a, b, bufs in have))
-        if new != getattr(self, "_win_key", None):
-            self._win_key = new
+        if self._pair_bufs is None:
+            strides = [-(-b // 256) * 256 for b in self.packed_window_bytes()]
+            self._pair_bufs = [torch.empty(self.nslots * st, dtype=torch.uint8, device=self.device) for st in strides]
+            self._pair_strides = strides
+            self._pair_key = [None] * self.nslots
+            w.layout = self.layout
+            w.npacked = self.nslots
+            w.pu, w.pv, w.pw, w.p2 = (b.data_ptr() for b in self._pair_bufs)
+            w.pu_stride, w.pv_stride, w.pw_stride, w.p2_stride = strides
             self._win_dirty = True
-        w.layout = self.layout
-        w.npacked = len(have)
-        for j, (s0, s1, bufs) in enumerate(have):
-            w.pslot[j][0], w.pslot[j][1] = int(s0), int(s1)
-            w.pu[j], w.pv[j], w.pw[j], w.p2[j] = (b.data_ptr() for b in bufs)
+        for a, _ in sorted(pairs):
+            s0 = a % self.nslots
+            key = self._pair_contents(s0)
+            if self._pair_key[s0] != key:
+                bufs = [b.data_ptr() + s0 * st for b, st in zip(self._pair_bufs, self._pair_strides)]
+                _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, s0,
+                          (s0 + 1) % self.nslots, *bufs, self._stream())
+                self._pair_key[s0] = key
 
     def set_layout(self, layout: int) -> None:
         """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD, _PAIR_LDS, _OCT or _YPAIR)."""
         if layout != self.layout:
             self.layout = int(layout)
-            self._packed = []
-            self._spare_bufs = []
+            self._pair_bufs = None
+            self._pair_key = None
             self._win_dirty = True
 
     def _upload_window(self):
@@ -607,18 +609,14 @@ class AB3Engine:
             self._win_dev.copy_(_h2d(raw, torch.uint8, self.device), non_blocking=True)
             self._win_dirty = False
 
-    def _launch_steps(self, stream, m: int = 1, ring: bool = False):
-        """m steps on `stream`: device clocks, then one fused launch (both graph-capturable).
-
-        A launch of several steps with a packed layout reads only the packed
-        windows; ``ring`` runs it with the RING layout instead, for runs of
-        steps that need more snapshot pairs than the two packed windows hold."""
+    def _launch_steps(self, stream, m: int = 1):
+        """m steps on `stream`: device clocks, then one fused launch (both graph-capturable)."""
         rk2 = self.scheme == "rk2"
         _lib.call("opx_clock_advance_steps", self._clocks.data_ptr(), self._table.data_ptr(), self._rel_cap, int(m),
                   self._rel_cap, self._nrel_dev.data_ptr(), self._ta.data_ptr(), int(self._ta.numel()), self.nslots,
                   0, self._step_dev.data_ptr(), self.dt, self.alpha if rk2 else 0.0, self._err.data_ptr(), stream)
         s = self._launch_structs()
-        layout = _lib.OPX_LAYOUT_RING if (ring and m > 1) else self.layout
+        layout = self.layout
         if rk2:
             _lib.call("opx_rk2_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
                       self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt, self.alpha,
@@ -633,16 +631,16 @@ class AB3Engine:
         m = self.launch_steps
         return [m] * (k // m) + ([k % m] if k % m else [])
 
-    def _graph(self, k, ring=False):
+    def _graph(self, k):
         """A captured graph of k steps for the current buffers and launch shape."""
-        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps, ring)
+        key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps)
         g = self._graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 stream = torch.cuda.current_stream(self.device).cuda_stream
                 for m in self._launch_sizes(k):
-                    self._launch_steps(stream, m, ring)
+                    self._launch_steps(stream, m)
             self._graphs[key] = g
         return g
 
@@ -673,16 +671,14 @@ class AB3Engine:
             self._ensure_snapshots(pairs)
             self._ensure_windows(pairs)
             self._upload_window()
-            # at most two pairs are packed; runs that need more read the rings
-            ring = len({(a % self.nslots, b % self.nslots) for a, b in pairs}) > 2
             if self.graph_steps > 0 and self.timer is None and k > 1:
-                self._graph(k, ring).replay()
+                self._graph(k).replay()
             else:
                 for m in self._launch_sizes(k):
                     if self.timer is not None:
                         e0 = torch.cuda.Event(enable_timing=True)
                         e0.record()
-                    self._launch_steps(stream, m, ring)
+                    self._launch_steps(stream, m)
                     if self.timer is not None:
                         e1 = torch.cuda.Event(enable_timing=True)
                         e1.record()

@@ -131,8 +131,9 @@ class ReleaseServer:
         eng.graph_steps = int(graph_steps)
         eng.launch_steps = int(launch_steps)
         bu, bv, bw, b2 = eng.packed_window_bytes()
+        # the pair ring: one packed window per ring slot
         self.capacity = capacity_from_free_hbm(self.device, hbm_fraction=hbm_fraction,
-                                               reserve_bytes=2 * (bu + bv + bw + b2) + (256 << 20),
+                                               reserve_bytes=eng.nslots * (bu + bv + bw + b2) + (256 << 20),
                                                max_particles=max_particles)
         eng.reserve(self.capacity)
         self.queue = AdmissionQueue(self.capacity, max_releases)

@@ -49,3 +49,10 @@ def test_layout_names_match_engine():
     from offline_particles_amd.engine import LAYOUT_NAMES
 
     assert bench.LAYOUTS == {name: value for value, name in LAYOUT_NAMES.items()}
+
+
+def test_batched_only_run_heads_with_the_batching_form():
+    batching = {"form": "batched_bf16", "particle_steps_per_s": 3.5e10, "releases": 256}
+    out = bench.build_output(_args(), {}, (1024, 768, 50), None, None, batching, None)
+    assert out["value"] == 3.5e10 and out["config"]["form"] == "batched_bf16"
+    json.dumps(out)
