@@ -23,6 +23,12 @@ Forms measured in the same run:
   many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 BF16
   grid streamed through a 3-snapshot window, admission by free HBM; every
   request runs to completion; p50/p99 latency under ``serving``.
+- isopycnal (``--extras iso``): the isopycnal-following AB3 drop-in
+  (validation, ab3_isopycnal_following, post step) on the same workload with a
+  stratified, drifting density field; reported under ``isopycnal``.
+- output events (``--extras output``): compute_z plus the trilinear (u, v, w)
+  and bilinear (zeta, h) output kernels of the drop-in API over every particle
+  at a time index between snapshots; reported under ``output_events``.
 The headline ``value`` is the fastest 16M-particle form; every form is listed
 in ``forms``.
 
@@ -85,8 +91,9 @@ def touched_nodes(shape, offsets, pos, corners_3d=True):
     return int(mark.sum().item())
 
 
-def field_bytes_touched(wl, sim, storage_bytes_3d):
-    """Bytes of distinct field values one AB3 launch must read (2 snapshots for u, v, w, zeta)."""
+def field_bytes_touched(wl, sim, storage_bytes_3d, fields_3d=("u", "v", "w")):
+    """Bytes of distinct field values one AB3 launch must read (2 snapshots of
+    each 3-D field and of zeta)."""
     from offline_particles_amd.workloads import OFFSETS
 
     P = sim.particles
@@ -94,7 +101,7 @@ def field_bytes_touched(wl, sim, storage_bytes_3d):
     pos = [P[k][act] for k in ("zidx", "yidx", "xidx")]
     b2 = wl.fields["h"].dtype.itemsize
     total = 0
-    for name in ("u", "v", "w"):
+    for name in fields_3d:
         total += 2 * storage_bytes_3d * touched_nodes(wl.fields[name].shape[1:], OFFSETS[name], pos)
     for name, nsnap in (("h", 1), ("dx", 1), ("dy", 1), ("zeta", 2)):
         shp = wl.fields[name].shape[-2:]
@@ -107,6 +114,31 @@ def field_bytes_touched(wl, sim, storage_bytes_3d):
 # status, 3 positions, 6 history values (1 + 24 + 48 B) and writes z,
 # y, x, the 3 current derivatives and 6 history values (8 + 16 + 24 + 48 B).
 PLAIN_AB3_STATE_BYTES = 73 + 96
+# plain isopycnal-following AB3 kernel (ab3_isopycnal_following): the same
+# plus the particle's target density rho_t (8 B read)
+PLAIN_ISO_STATE_BYTES = PLAIN_AB3_STATE_BYTES + 8
+# output kernels: status and the indices read (1 + 8 per index), one FP64 value written
+OUTPUT_TRILINEAR_BYTES = 1 + 24 + 8
+OUTPUT_BILINEAR_BYTES = 1 + 16 + 8
+
+# isopycnal-following constants and targets of the isopycnal form: the density
+# of add_density spans 1026-1028 kg/m^3 over the column
+ISO_CONSTANTS = {"w_max": 5e-4, "rho_scale": 0.2}
+
+
+def iso_targets(n):
+    return np.random.default_rng(2).uniform(1026.2, 1027.8, n)
+
+
+def with_density(wl):
+    """A shallow copy of the workload with the time-dependent density field "rho" added."""
+    import copy
+
+    from offline_particles_amd.workloads import add_density
+
+    w2 = copy.copy(wl)
+    w2.fields = dict(wl.fields)
+    return add_density(w2)
 
 
 # ---------------------------------------------------------------------------
@@ -152,19 +184,15 @@ def active_count(sim):
     return int((sim.particles["status"] < 128).sum().item())
 
 
-def run_plain(wl, device, steps, warmup):
+def _time_plain(sim, tk, steps, warmup):
+    """Wall time of ``steps`` Simulation steps after ``warmup``, and the timed
+    kernel's mean duration; particle-steps count the active particles."""
     import torch
 
-    from offline_particles_amd.kernels.roms import ab3_w_advection_kernel
-    from offline_particles_amd.workloads import build_simulation
-
-    tk = _TimedKernel(ab3_w_advection_kernel)
-    sim = build_simulation(wl, "ab3", device=device, ab_kernel=tk.kernel)
     for _ in range(warmup):
         sim.step()
     torch.cuda.synchronize()
     a0 = active_count(sim)
-    fbytes = field_bytes_touched(wl, sim, wl.fields["u"].dtype.itemsize)
     tk.active = True
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -174,9 +202,19 @@ def run_plain(wl, device, steps, warmup):
     t1 = time.perf_counter()
     a1 = active_count(sim)
     tk.active = False
-    elapsed = t1 - t0
-    act = 0.5 * (a0 + a1)
-    kms = tk.mean_ms()
+    return t1 - t0, 0.5 * (a0 + a1), tk.mean_ms()
+
+
+def run_plain(wl, device, steps, warmup):
+    import torch
+
+    from offline_particles_amd.kernels.roms import ab3_w_advection_kernel
+    from offline_particles_amd.workloads import build_simulation
+
+    tk = _TimedKernel(ab3_w_advection_kernel)
+    sim = build_simulation(wl, "ab3", device=device, ab_kernel=tk.kernel)
+    fbytes = field_bytes_touched(wl, sim, wl.fields["u"].dtype.itemsize)
+    elapsed, act, kms = _time_plain(sim, tk, steps, warmup)
     algo = act * PLAIN_AB3_STATE_BYTES + fbytes
     res = {
         "form": "plain_fp32",
@@ -190,6 +228,116 @@ def run_plain(wl, device, steps, warmup):
         "achieved_GBs": algo / (kms * 1e-3) / 1e9,
     }
     del sim
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_plain_iso(wl, device, steps, warmup):
+    """The isopycnal-following AB3 drop-in (ab3_isopycnal_following_timestepper's
+    kernels: validation, ab3_isopycnal_following, its post step) on the workload
+    with a density field; FP32 fields, FP64 positions."""
+    import torch
+
+    from offline_particles_amd.kernels.roms import (ab3_isopycnal_following_kernel,
+                                                    ab3_post_isopycnal_following_kernel)
+    from offline_particles_amd.kernels.validation import validation_kernel
+    from offline_particles_amd.simulation import SimulationBuilder
+    from offline_particles_amd.timesteppers import ABTimestepper
+    from offline_particles_amd.workloads import build_fieldset
+
+    tk = _TimedKernel(ab3_isopycnal_following_kernel)
+    fs = build_fieldset(wl, device=device, constants=ISO_CONSTANTS)
+    ts = ABTimestepper(wl.time_array, np.float64(wl.dt), ab_kernel=tk.kernel, index_padding=5,
+                       pre_step_kernel=validation_kernel, post_step_kernel=ab3_post_isopycnal_following_kernel)
+    sim = SimulationBuilder(ts, fs).build_simulation(wl.nparticles, device=device)
+    sim.set_indices(*wl.positions)
+    sim.set_particle_field("status", 10)
+    sim.set_particle_field("rho_t", iso_targets(wl.nparticles))
+    fbytes = field_bytes_touched(wl, sim, wl.fields["u"].dtype.itemsize, fields_3d=("u", "v", "rho"))
+    elapsed, act, kms = _time_plain(sim, tk, steps, warmup)
+    algo = act * PLAIN_ISO_STATE_BYTES + fbytes
+    res = {
+        "form": "plain_iso_fp32",
+        "particle_steps_per_s": act * steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "active_particles": act,
+        "kernel": "k_ab3_iso (opx_ab3_isopycnal_following)",
+        "kernel_ms": kms,
+        "algo_bytes": algo,
+        "field_bytes_touched": fbytes,
+        "achieved_GBs": algo / (kms * 1e-3) / 1e9,
+        "roofline_frac": algo / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "constants": dict(ISO_CONSTANTS),
+    }
+    del sim, fs
+    torch.cuda.empty_cache()
+    return res
+
+
+OUTPUT_TIDX = 0.5  # output events between the first two snapshots
+
+
+def run_output(wl, device, rounds, warmup):
+    """Output events of the drop-in API: one round is compute_z plus trilinear u,
+    v, w and bilinear zeta, h for every particle (compute_z_kernel and the
+    linear-interpolation output kernels, one HIP kernel each), launched through
+    the Launcher at a time index between two snapshots."""
+    import torch
+
+    from offline_particles_amd.kernels import merge_particle_fields
+    from offline_particles_amd.kernels.output import (bilinear_interpolation_kernel,
+                                                      trilinear_interpolation_kernel)
+    from offline_particles_amd.kernels.roms import compute_z_kernel
+    from offline_particles_amd.launcher import Launcher
+    from offline_particles_amd.particles import Particles
+    from offline_particles_amd.workloads import OFFSETS, build_fieldset
+
+    kernels = {"z": compute_z_kernel("_z_out")}
+    for name in ("u", "v", "w"):
+        kernels[name] = trilinear_interpolation_kernel(name, f"_{name}_out")
+    for name in ("zeta", "h"):
+        kernels[name] = bilinear_interpolation_kernel(name, f"_{name}_out")
+    timed = {k: _TimedKernel(v) for k, v in kernels.items()}
+    fs = build_fieldset(wl, device=device)
+    launcher = Launcher(fs)
+    n = wl.nparticles
+    P = Particles(n, device=device, **merge_particle_fields([t.kernel for t in timed.values()]))
+    for k, a in zip(("zidx", "yidx", "xidx"), wl.positions):
+        P[k].copy_(torch.from_numpy(a))
+
+    def one_round():
+        for t in timed.values():
+            launcher.launch_kernel(t.kernel, P, OUTPUT_TIDX)
+
+    for _ in range(warmup):
+        one_round()
+    torch.cuda.synchronize()
+    for t in timed.values():
+        t.active = True
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        one_round()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kms = {k: t.mean_ms() for k, t in timed.items()}
+    pos = [P[k] for k in ("zidx", "yidx", "xidx")]
+    ub = wl.fields["u"].dtype.itemsize
+    # the trilinear kernel of u: its particle bytes plus the distinct u nodes of both snapshots
+    algo_u = n * OUTPUT_TRILINEAR_BYTES + 2 * ub * touched_nodes(wl.fields["u"].shape[1:], OFFSETS["u"], pos)
+    res = {
+        "form": "output_fp32",
+        "outputs_per_round": len(kernels),
+        "particle_outputs_per_s": n * len(kernels) * rounds / elapsed,
+        "ms_per_round": 1e3 * elapsed / rounds,
+        "particles": n,
+        "kernel_ms": kms,
+        "kernel": "k_interp_trilinear (opx_interp_trilinear), field u",
+        "algo_bytes": algo_u,
+        "achieved_GBs": algo_u / (kms["u"] * 1e-3) / 1e9,
+        "roofline_frac": algo_u / (kms["u"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "tidx": OUTPUT_TIDX,
+    }
+    del P, fs
     torch.cuda.empty_cache()
     return res
 
@@ -443,6 +591,8 @@ _PROFILE_FOR_FORM = {
     ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 1): "k_rk2_fused_f32_turbulent16M.md",
     ("fused_rk2_fp32", 16): "k_rk2_fused_steps16_f32_turbulent16M.md",
+    ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
+    ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
 }
 
 
@@ -467,21 +617,31 @@ def profiled_traffic(form, particles, grid, steps_per_launch=1):
 # CPU baseline: reference-structured FP64 oracle on the host cores
 # ---------------------------------------------------------------------------
 
-def cpu_baseline(wl, seconds, max_particles):
-    from oracle import OracleField, OracleSimulation
-    from offline_particles_amd.workloads import OFFSETS, TIME_DEPENDENT
-
+def _cpu_threads():
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         ncpu = os.cpu_count() or 1
-    threads = max(1, min(16, ncpu))
-    n = min(max_particles, wl.nparticles)
+    return max(1, min(16, ncpu))
+
+
+def _oracle_sim(wl, n, scheme, threads):
+    from oracle import OracleField, OracleSimulation
+    from offline_particles_amd.workloads import OFFSETS, TIME_DEPENDENT
+
     fields = {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in wl.fields.items()}
-    sim = OracleSimulation(fields, wl.time_array, wl.dt, scheme="ab3", nparticles=n, hc=wl.hc, NZ=wl.NZ,
-                           bounds=wl.bounds, threads=threads)
+    extra = dict(rho_t=iso_targets(wl.nparticles)[:n], **ISO_CONSTANTS) if scheme == "ab3_iso" else {}
+    sim = OracleSimulation(fields, wl.time_array, wl.dt, scheme=scheme, nparticles=n, hc=wl.hc, NZ=wl.NZ,
+                           bounds=wl.bounds, threads=threads, **extra)
     for k, a in zip(("zidx", "yidx", "xidx"), wl.positions):
         sim.p[k][:] = a[:n]
+    return sim
+
+
+def cpu_baseline(wl, seconds, max_particles, scheme="ab3", max_steps=20):
+    threads = _cpu_threads()
+    n = min(max_particles, wl.nparticles)
+    sim = _oracle_sim(wl, n, scheme, threads)
     sim.p["status"][:] = 10
     sim.step()  # warm-up (start-up step)
     steps, t0 = 0, time.perf_counter()
@@ -491,15 +651,43 @@ def cpu_baseline(wl, seconds, max_particles):
         sim.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or steps >= 20:
+        if el >= seconds or steps >= max_steps:
             break
+    what = "AB3" if scheme == "ab3" else "isopycnal-following AB3"
     return {
         "value": act / el,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} particles x {steps} AB3 steps of the same workload on the FP64 C/OpenMP oracle "
+        "sample": f"{n} particles x {steps} {what} steps of the same workload on the FP64 C/OpenMP oracle "
                   f"(full-grid time blend per launch, validation + AB3 + post per step), {el:.1f} s",
+    }
+
+
+def cpu_output_baseline(wl, seconds, max_particles):
+    """Output rounds of compute_z and trilinear u, v, w on the oracle, each the
+    launcher's full-grid blend of the field followed by the particle loop."""
+    threads = _cpu_threads()
+    n = min(max_particles, wl.nparticles)
+    sim = _oracle_sim(wl, n, "ab3", threads)
+    sim.tidx = OUTPUT_TIDX
+    out = np.zeros(n)
+    rounds, t0 = 0, time.perf_counter()
+    while True:
+        sim.output_compute_z(out)
+        for name in ("u", "v", "w"):
+            sim.output_trilinear(name, out)
+        rounds += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or rounds >= 5:
+            break
+    return {
+        "value": n * 4 * rounds / el,
+        "unit": "particle-outputs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} particles x {rounds} rounds of compute_z + trilinear u, v, w on the FP64 C/OpenMP "
+                  f"oracle (full-grid blend per output, then the particle loop), {el:.1f} s",
     }
 
 
@@ -552,7 +740,7 @@ def accuracy(wl, device, nsub, steps):
     return out
 
 
-def build_output(args, forms, grid, cpu, acc, batching, serving) -> dict:
+def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None, output_events=None) -> dict:
     """The JSON line: headline = the fastest AB3 form at the 16M-particle workload."""
     NX, NY, NZ = grid
     ab3 = [f for k, f in forms.items() if "rk2" not in k] or list(forms.values())
@@ -600,6 +788,8 @@ def build_output(args, forms, grid, cpu, acc, batching, serving) -> dict:
         "accuracy": acc,
         "batching": batching,
         "serving": serving,
+        "isopycnal": isopycnal,
+        "output_events": output_events,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
     }
@@ -634,6 +824,10 @@ def main():
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
     ap.add_argument("--launch-steps", type=int, default=1, help="fused steps per launch (1..64)")
+    ap.add_argument("--extras", default="iso,output",
+                    help="also measure the isopycnal-following AB3 drop-in (iso) and the output events "
+                         "(output) at the same workload; empty for none")
+    ap.add_argument("--output-rounds", type=int, default=10, help="timed output rounds of the output form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -678,6 +872,20 @@ def main():
                                     launch_steps=args.launch_steps)
             log(name, forms[name])
 
+    extras = set(filter(None, args.extras.split(",")))
+    isopycnal = output_events = wl_iso = None
+    if "iso" in extras:
+        t = time.perf_counter()
+        wl_iso = with_density(wl)
+        log(f"density field generated in {time.perf_counter() - t:.1f}s")
+        isopycnal = run_plain_iso(wl_iso, device, args.steps, args.warmup)
+        isopycnal["traffic"] = profiled_traffic("plain_iso_fp32", args.particles, (NX, NY, NZ))
+        log("isopycnal", isopycnal)
+    if "output" in extras:
+        output_events = run_output(wl, device, args.output_rounds, 2)
+        output_events["traffic"] = profiled_traffic("output_fp32", args.particles, (NX, NY, NZ))
+        log("output", output_events)
+
     batching = None
     if args.batched:
         batching = run_batched(device, args.batched, args.batched_max_size, torch.bfloat16, args.graph_steps)
@@ -695,6 +903,13 @@ def main():
         t = time.perf_counter()
         cpu = cpu_baseline(wl, args.cpu_seconds, args.cpu_particles)
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s:", cpu)
+        if isopycnal is not None:
+            isopycnal["cpu_baseline"] = cpu_baseline(wl_iso, args.cpu_seconds, args.cpu_particles // 4,
+                                                     scheme="ab3_iso", max_steps=5)
+            log("isopycnal cpu baseline:", isopycnal["cpu_baseline"])
+        if output_events is not None:
+            output_events["cpu_baseline"] = cpu_output_baseline(wl, args.cpu_seconds, args.cpu_particles)
+            log("output cpu baseline:", output_events["cpu_baseline"])
         if args.accuracy:
             wl_acc = wl
             if args.accuracy_steps + 5 > args.steps + args.warmup + 5:
@@ -703,7 +918,7 @@ def main():
             acc = accuracy(wl_acc, device, args.accuracy, args.accuracy_steps)
             del wl_acc
 
-    out = build_output(args, forms, (NX, NY, NZ), cpu, acc, batching, serving)
+    out = build_output(args, forms, (NX, NY, NZ), cpu, acc, batching, serving, isopycnal, output_events)
     print(json.dumps(out))
 
 

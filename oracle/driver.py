@@ -123,6 +123,10 @@ def load():
         getattr(L, name).restype = None
     L.orc_ab3_iso.argtypes = [P, S, F, ctypes.POINTER(_Field), _dp, ctypes.c_double, ctypes.c_double]
     L.orc_ab3_iso.restype = None
+    L.orc_output_trilinear.argtypes = [P, ctypes.POINTER(_Field), _dp]
+    L.orc_output_trilinear.restype = None
+    L.orc_output_compute_z.argtypes = [P, S, F, _dp]
+    L.orc_output_compute_z.restype = None
     L.orc_blend_f64.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp]
     L.orc_blend_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
                                 ctypes.c_float, _dp]
@@ -386,3 +390,19 @@ class OracleSimulation:
     def run(self, nsteps: int):
         for _ in range(nsteps):
             self.step()
+
+    # output events at the current time index: the launcher blends the field
+    # over the whole grid (fields.py:394-430), then the particle loop runs
+    def output_trilinear(self, name: str, out: np.ndarray) -> np.ndarray:
+        """trilinear_interpolation_kernel(name) (linear_interpolation.pyx:107-136) into ``out``."""
+        fld = self.fields[name]
+        arr = fld.field_data(self.tidx, self.L)
+        F = _Field(_ptr(arr), *arr.shape, *fld.offsets)
+        self.L.orc_output_trilinear(ctypes.byref(self._cp), ctypes.byref(F), _ptr(out))
+        return out
+
+    def output_compute_z(self, out: np.ndarray) -> np.ndarray:
+        """compute_z_kernel (_roms_core.pyx:17-73) into ``out``."""
+        F = self._fields(self.tidx, "vertical")
+        self.L.orc_output_compute_z(ctypes.byref(self._cp), ctypes.byref(self._scalars), ctypes.byref(F), _ptr(out))
+        return out

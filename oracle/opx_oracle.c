@@ -522,6 +522,43 @@ void orc_ab3_iso(orc_particles *p, const orc_scalars *s, const orc_fields *F, co
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* L1: output events (kernels/output/linear_interpolation.pyx,          */
+/*     kernels/roms/_roms_core.pyx)                                     */
+/* ------------------------------------------------------------------ */
+
+/* _trilinear_interpolation_kernel_function (linear_interpolation.pyx:107-136):
+ * out[i] = trilinear value of the blended field at the particle's indices plus
+ * the field's offsets; inactive particles keep their previous out[i]. */
+void orc_output_trilinear(const orc_particles *p, const orc_field *F, double *out)
+{
+    int64_t i, n = p->n;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; ++i) {
+        if (p->status[i] & ORC_INACTIVE)
+            continue;
+        out[i] = tril(F, p->zidx[i], p->yidx[i], p->xidx[i]);
+    }
+}
+
+/* _compute_z (_roms_core.pyx:17-73): z of each active particle from h, zeta
+ * (bilinear) and C (linear at zidx); inactive particles keep out[i]. */
+void orc_output_compute_z(const orc_particles *p, const orc_scalars *s, const orc_fields *F, double *out)
+{
+    int64_t i, n = p->n;
+    const double hc = s->hc;
+    const int NZ = s->NZ;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; ++i) {
+        if (p->status[i] & ORC_INACTIVE)
+            continue;
+        double h_value = bil(&F->h, p->yidx[i], p->xidx[i]);
+        double zeta_value = bil(&F->zeta, p->yidx[i], p->xidx[i]);
+        double C_value = orc_linear(F->C.a, F->C.n0, p->zidx[i] + F->C.o0);
+        out[i] = orc_compute_z(p->zidx[i], NZ, hc, h_value, C_value, zeta_value);
+    }
+}
+
 /* bounding box reductions (launcher.py:122-148); min/max of each index. */
 void orc_bbox(const orc_particles *p, double *out6)
 {

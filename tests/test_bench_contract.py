@@ -56,3 +56,25 @@ def test_batched_only_run_heads_with_the_batching_form():
     out = bench.build_output(_args(), {}, (1024, 768, 50), None, None, batching, None)
     assert out["value"] == 3.5e10 and out["config"]["form"] == "batched_bf16"
     json.dumps(out)
+
+
+def test_extra_rows_pass_through_and_cpu_legs_run():
+    """The isopycnal and output-event records appear in the line; their CPU legs
+    (oracle timings) run on a small workload; adding density leaves the shared
+    workload untouched."""
+    from offline_particles_amd.workloads import eddy_workload
+
+    iso = {"form": "plain_iso_fp32", "particle_steps_per_s": 1e9}
+    outev = {"form": "output_fp32", "particle_outputs_per_s": 1e10}
+    out = bench.build_output(_args(), {"plain_fp32": _form("plain_fp32", 1e9)}, (1024, 768, 50), None, None,
+                             None, None, iso, outev)
+    assert out["isopycnal"] is iso and out["output_events"] is outev
+    json.dumps(out)
+
+    wl = eddy_workload(steps=8, nparticles=2000)
+    wl_iso = bench.with_density(wl)
+    assert "rho" in wl_iso.fields and "rho" not in wl.fields
+    c = bench.cpu_baseline(wl_iso, 0.5, 2000, scheme="ab3_iso", max_steps=2)
+    assert c["value"] > 0 and "isopycnal" in c["sample"] and c["unit"] == "particle-steps/s"
+    o = bench.cpu_output_baseline(wl, 0.5, 2000)
+    assert o["value"] > 0 and o["unit"] == "particle-outputs/s"

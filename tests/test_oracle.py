@@ -284,6 +284,39 @@ def test_ab3_iso_converges_to_target_isopycnal():
     assert np.allclose(sim.p["zidx"], target_level, rtol=0, atol=1e-6)
 
 
+def test_output_events_closed_forms():
+    """The output-event loops (trilinear output, compute_z): an affine u blended
+    half-way between two snapshots is reproduced exactly at the u offsets; with
+    zeta = 0 and C = sigma, z = h sigma; inactive particles keep their value;
+    every active value equals the pinned scalar helpers."""
+    NZ, NY, NX, h = 8, 6, 7, 250.0
+    ta = np.arange(3) * 3600.0
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 3, h=h)
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX - 1), indexing="ij")
+    u0 = 0.5 + 0.125 * k - 0.25 * j + 0.375 * i
+    fields["u"] = D.OracleField(np.stack([u0, 3.0 * u0, u0]), fields["u"].offsets, True, cyclic=False)
+    rng = np.random.default_rng(5)
+    n = 400
+    pos = (rng.uniform(0, NZ - 1, n), rng.uniform(0, NY - 1, n), rng.uniform(0.5, NX - 1.5, n))
+    st = np.where(rng.uniform(size=n) < 0.1, 130, 0).astype(np.uint8)
+    sim = _sim(fields, ta, 300.0, "ab3", pos, NZ, NY, NX, status=st)
+    sim.tidx = 0.5
+    out = np.full(n, -7.0)
+    sim.output_trilinear("u", out)
+    act = st < 128
+    o = fields["u"].offsets
+    z, y, x = (a + b for a, b in zip(pos, o))
+    assert np.allclose(out[act], 2.0 * (0.5 + 0.125 * z - 0.25 * y + 0.375 * x)[act], rtol=0, atol=1e-12)
+    blended = fields["u"].field_data(0.5, D.load()).copy()
+    assert _same(out[act], D.trilinear(blended, z[act], y[act], x[act]))
+    assert np.all(out[~act] == -7.0)
+    zout = np.full(n, -7.0)
+    sim.output_compute_z(zout)
+    sigma = (pos[0] + 0.5) / NZ - 1.0
+    assert np.allclose(zout[act], h * sigma[act], rtol=0, atol=1e-10)
+    assert np.all(zout[~act] == -7.0)
+
+
 # ----------------------------------------------------------------- blend + time
 
 def test_blend_matches_numpy_dtype_semantics():
