@@ -23,9 +23,11 @@ Forms measured in the same run:
   many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 BF16
   grid streamed through a 3-snapshot window, admission by free HBM; every
   request runs to completion; p50/p99 latency under ``serving``.
-- isopycnal (``--extras iso``): the isopycnal-following AB3 drop-in
-  (validation, ab3_isopycnal_following, post step) on the same workload with a
-  stratified, drifting density field; reported under ``isopycnal``.
+- isopycnal (``--extras iso``): the isopycnal-following AB3 on the same
+  workload with a stratified, drifting density field, as the drop-in
+  (validation, ab3_isopycnal_following, post step) and as the fused engine
+  form (AB3Engine(scheme="ab3_iso"), FP32 storage, tuned like the fused
+  forms); reported under ``isopycnal``.
 - output events (``--extras output``): compute_z plus the trilinear (u, v, w)
   and bilinear (zeta, h) output kernels of the drop-in API over every particle
   at a time index between snapshots; reported under ``output_events``.
@@ -342,12 +344,13 @@ def run_output(wl, device, rounds, warmup):
     return res
 
 
-def field_bytes_touched_fused(wl, pos, storage_bytes_3d):
-    """Distinct field bytes one fused launch reads: u, v, w (2 snapshots), packed h/dx/dy, zeta (2), C."""
+def field_bytes_touched_fused(wl, pos, storage_bytes_3d, fields_3d=("u", "v", "w")):
+    """Distinct field bytes one fused launch reads: u, v, w (or rho; 2 snapshots),
+    packed h/dx/dy, zeta (2), C."""
     from offline_particles_amd.workloads import OFFSETS
 
     total = 0
-    for name in ("u", "v", "w"):
+    for name in fields_3d:
         total += 2 * storage_bytes_3d * touched_nodes(wl.fields[name].shape[1:], OFFSETS[name], pos)
     n2 = touched_nodes(wl.fields["h"].shape, OFFSETS["h"], pos)
     total += 16 * n2 + 2 * 4 * n2
@@ -363,6 +366,8 @@ FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
 # offsets (no history).  Its field term is counted at the step's start
 # position only (a lower bound: stage 2 reads around x*).
 FUSED_RK2_STATE_BYTES = 1 + 2 + 4 + 12 + 16
+# fused isopycnal-following AB3 kernel: the AB3 state plus the FP64 target density
+FUSED_ISO_STATE_BYTES = FUSED_STATE_BYTES + 8
 
 # stencil layout names -> OPX_LAYOUT_* values of the C-ABI
 LAYOUTS = {"ring": 0, "pair": 1, "quad": 2, "pair_lds": 3, "oct": 4, "ypair": 5}
@@ -375,14 +380,15 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     from offline_particles_amd.engine import AB3Engine
     from offline_particles_amd.workloads import build_fieldset
 
-    fs = build_fieldset(wl, device=device)
+    iso = scheme == "ab3_iso"
+    fs = build_fieldset(wl, device=device, constants=ISO_CONSTANTS if iso else None)
     eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                     capacity=wl.nparticles, rebin_every=rebin_every, xlow=xlow, scheme=scheme)
     del fs
     # the release runs as long as the time array allows; the timed steps must
     # all be real steps of every particle (checked below)
     nsteps_max = int((wl.time_array[-1] - wl.time_array[0]) // wl.dt) - 2
-    eng.add_release(*wl.positions, nsteps=nsteps_max)
+    eng.add_release(*wl.positions, nsteps=nsteps_max, **({"rho_t": iso_targets(wl.nparticles)} if iso else {}))
     if layout is not None:
         eng.set_layout(LAYOUTS[layout])
     eng.block = block
@@ -397,7 +403,8 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     z, y, x = eng.positions()
     act = eng.status() < 128
     sb = 2 if storage == torch.bfloat16 else 4
-    fbytes = field_bytes_touched_fused(wl, [z[act], y[act], x[act]], sb)
+    fbytes = field_bytes_touched_fused(wl, [z[act], y[act], x[act]], sb,
+                                       ("u", "v", "rho") if iso else ("u", "v", "w"))
     del z, y, x, act
     eng.timer = []
     torch.cuda.synchronize()
@@ -416,16 +423,20 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     elapsed = t1 - t0
     actn = 0.5 * (a0 + a1)
     # algorithmic bytes per particle-step times the particle-steps of one launch
-    algo = actn * (FUSED_STATE_BYTES if scheme == "ab3" else FUSED_RK2_STATE_BYTES) + fbytes
+    state_bytes = {"ab3": FUSED_STATE_BYTES, "rk2": FUSED_RK2_STATE_BYTES, "ab3_iso": FUSED_ISO_STATE_BYTES}[scheme]
+    algo = actn * state_bytes + fbytes
     algo_launch = m * algo
-    name = ("fused_" if scheme == "ab3" else "fused_rk2_") + ("bf16" if storage == torch.bfloat16 else "fp32")
+    name = {"ab3": "fused_", "rk2": "fused_rk2_", "ab3_iso": "fused_iso_"}[scheme] + ("bf16" if sb == 2 else "fp32")
+    kname = {"ab3": "k_ab3_fused", "rk2": "k_rk2_fused", "ab3_iso": "k_ab3_iso_fused"}[scheme]
+    entry = {"ab3": "opx_ab3_fused", "rk2": "opx_rk2_fused", "ab3_iso": "opx_ab3_iso_fused"}[scheme]
+    if m > 1 or iso:
+        entry += "_steps"
     res = {
         "form": name,
         "particle_steps_per_s": actn * steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
         "active_particles": actn,
-        "kernel": (f"k_ab3_fused<{'bf16' if sb == 2 else 'f32'}> (opx_ab3_fused)" if scheme == "ab3"
-                   else f"k_rk2_fused<{'bf16' if sb == 2 else 'f32'}> (opx_rk2_fused)"),
+        "kernel": f"{kname}<{'bf16' if sb == 2 else 'f32'}> ({entry})",
         "kernel_ms": kms,
         "steps_per_launch": m,
         "algo_bytes": algo_launch,
@@ -593,6 +604,7 @@ _PROFILE_FOR_FORM = {
     ("fused_rk2_fp32", 16): "k_rk2_fused_steps16_f32_turbulent16M.md",
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
+    ("fused_iso_fp32", 16): "k_ab3_iso_fused_steps16_turbulent16M.md",
 }
 
 
@@ -878,9 +890,17 @@ def main():
         t = time.perf_counter()
         wl_iso = with_density(wl)
         log(f"density field generated in {time.perf_counter() - t:.1f}s")
-        isopycnal = run_plain_iso(wl_iso, device, args.steps, args.warmup)
-        isopycnal["traffic"] = profiled_traffic("plain_iso_fp32", args.particles, (NX, NY, NZ))
-        log("isopycnal", isopycnal)
+        plain = run_plain_iso(wl_iso, device, args.steps, args.warmup)
+        plain["traffic"] = profiled_traffic("plain_iso_fp32", args.particles, (NX, NY, NZ))
+        log("isopycnal plain", plain)
+        fused = run_fused(wl_iso, device, args.steps, args.warmup, torch.float32, args.rebin_every, args.xlow,
+                          tune=not args.no_tune, layout=args.layout, block=args.block, scheme="ab3_iso",
+                          launch_steps=args.launch_steps)
+        fused["roofline_frac"] = fused["achieved_GBs"] / HBM_PEAK_GBS
+        fused["traffic"] = profiled_traffic("fused_iso_fp32", args.particles, (NX, NY, NZ),
+                                            fused["steps_per_launch"])
+        log("isopycnal fused", fused)
+        isopycnal = {"plain_iso_fp32": plain, "fused_iso_fp32": fused}
     if "output" in extras:
         output_events = run_output(wl, device, args.output_rounds, 2)
         output_events["traffic"] = profiled_traffic("output_fp32", args.particles, (NX, NY, NZ))

@@ -139,7 +139,7 @@ int opx_ab3_w_advection(const opx_particles *p, const opx_scalars *s, const opx_
 int opx_ab3_post_w_advection(const opx_particles *p, const opx_scalars *s, const opx_roms_fields *f,
                              void *stream);
 
-/* kernels/roms/ab3_isopycnal_following.pyx:26-173 (_ab3_isopycnal_following):
+/* kernels/roms/ab3_isopycnal_following.pyx:26-191 (_ab3_isopycnal_following):
  * the AB3 step of opx_ab3_w_advection with the vertical rate
  * dz0 = w_max * drho / (rho_scale + |drho|), drho = rho(x) - rho_t[i], where
  * `rho` is a 3-D (blended) density field and rho_t the per-particle target
@@ -196,7 +196,9 @@ int opx_interp_trilinear(int64_t n, const uint8_t *status, const double *zidx, c
  * f_{n-1}, f_{n-2} (the reference's _dz1.._dxidx2).  `rel` (release id per
  * particle) may be NULL (all release 0); `z` (FP32 output) may be NULL; `pid`
  * is permuted with the state by opx_rebin and may be NULL when rebinning is
- * not used. */
+ * not used.  `rho_t` (FP64 target density of the isopycnal-following step,
+ * the reference's particle field rho_t) is NULL for w advection and is
+ * permuted with the state by opx_rebin. */
 typedef struct opx_split_state {
     int64_t n;
     uint8_t *status;
@@ -206,6 +208,7 @@ typedef struct opx_split_state {
     int32_t *pid;
     uint16_t *rel;
     float *z;
+    double *rho_t;
 } opx_split_state;
 
 /* Grid sizes (rho points), cell-field widths with (1 << bits) >= N + 5 and
@@ -239,7 +242,9 @@ enum opx_layout {
  * window s at byte offset s * p*_stride from pu, pv, pw and p2.  A step that
  * reads the pair with first slot s0 reads window s0; the caller keeps every
  * window a launch reads packed from the current contents of its two slots.
- * With npacked == 0 every step reads the rings. */
+ * With npacked == 0 every step reads the rings.  The third 3-D field `w` has
+ * `wlevels` z-levels: NZ + 1 (0 means the same) for w at w-points, NZ for the
+ * density at rho points that the isopycnal-following step reads in its place. */
 typedef struct opx_window {
     const void *u, *v, *w;  /* dtype3 storage */
     const float *zeta;
@@ -254,6 +259,8 @@ typedef struct opx_window {
     const void *p2; /* (NY, NX) float4 {h, zeta[s0], zeta[s1], dx} per window */
     uint64_t pu_stride, pv_stride, pw_stride, p2_stride; /* bytes from one window to the next */
     const float *dy; /* (NY, NX) static dy, read with packed windows */
+    int32_t wlevels; /* z-levels of w: 0 or NZ + 1 (w), NZ (density, isopycnal step) */
+    int32_t reserved;
 } opx_window;
 
 /* Bytes of one packed window of u, v and w for a layout (PAIR, QUAD, OCT or
@@ -350,6 +357,27 @@ int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_w
 int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev, int32_t dtype3,
                             int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
                             double dt, const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);
+
+/* `nsteps` (1..64) fused isopycnal-following AB3 steps in one launch: the
+ * steps of opx_ab3_fused_steps with the vertical rate
+ * w = w_max * drho / (rho_scale + |drho|), drho = rho(x_n, t_n) - rho_t[i]
+ * (ab3_isopycnal_following.pyx:26-191, post step :193-242; launch order of
+ * ab3_isopycnal_following_timestepper, models/roms/__init__.py:109-140), rho
+ * read from the window's third field (wlevels = NZ, FP32 storage, rho points)
+ * and trilinearly interpolated in FP64.  `s->rho_t` is required.  Replaces
+ * the reference's per-step launches of validation_kernel,
+ * ab3_isopycnal_following_kernel and its post kernel.  PAIR_LDS reads PAIR
+ * windows. */
+int opx_ab3_iso_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                            const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
+                            double w_max, double rho_scale, const int32_t *n_active, int32_t block,
+                            int32_t *err_flag, void *stream);
+
+/* opx_ab3_iso_fused_steps with the window descriptor in device memory. */
+int opx_ab3_iso_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev,
+                                int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                                double dt, double w_max, double rho_scale, const int32_t *n_active, int32_t block,
+                                int32_t *err_flag, void *stream);
 
 /* One fused RK2 step (validation -> _rk2_step_1 at t -> _rk2_step_2 at
  * t + alpha dt -> _rk2_update at t + dt, rk2_w_advection.pyx:31-354 in the

@@ -92,7 +92,7 @@ _SIGNATURES = {
 class opx_split_state(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("status", _vp), ("cell", _vp), ("fz", _vp), ("fy", _vp), ("fx", _vp),
                 ("hz1", _vp), ("hy1", _vp), ("hx1", _vp), ("hz2", _vp), ("hy2", _vp), ("hx2", _vp),
-                ("pid", _vp), ("rel", _vp), ("z", _vp)]
+                ("pid", _vp), ("rel", _vp), ("z", _vp), ("rho_t", _vp)]
 
 
 class opx_grid(ctypes.Structure):
@@ -115,7 +115,8 @@ class opx_window(ctypes.Structure):
                 ("hdd", _vp), ("C", _vp), ("layout", ctypes.c_int32), ("npacked", ctypes.c_int32),
                 ("pu", _vp), ("pv", _vp), ("pw", _vp), ("p2", _vp),
                 ("pu_stride", ctypes.c_uint64), ("pv_stride", ctypes.c_uint64), ("pw_stride", ctypes.c_uint64),
-                ("p2_stride", ctypes.c_uint64), ("dy", _vp)]
+                ("p2_stride", ctypes.c_uint64), ("dy", _vp), ("wlevels", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class opx_release_clock(ctypes.Structure):
@@ -156,6 +157,12 @@ _SIGNATURES.update({
                             ctypes.c_double, _vp, ctypes.c_int32, _vp, _vp],
     "opx_ab3_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp,
                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, ctypes.c_int32, _vp, _vp],
+    "opx_ab3_iso_fused_steps": [_P(opx_split_state), _P(opx_grid), _P(opx_window), _vp, ctypes.c_int32,
+                                ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
+                                _vp, _vp],
+    "opx_ab3_iso_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, _vp, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp,
+                                    ctypes.c_int32, _vp, _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],

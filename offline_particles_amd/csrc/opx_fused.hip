@@ -528,6 +528,12 @@ struct Step {
     bool active;
 };
 
+// Constants of the isopycnal-following vertical rate (the reference's scalars
+// w_max and rho_scale, ab3_isopycnal_following.pyx:54-55).
+struct IsoRate {
+    double w_max, rho_scale;
+};
+
 constexpr int kMaxSlots = 1023; // slot fields of opx_release_step are 10 bits
 constexpr int kMaxLaunchSteps = 64; // steps one fused launch may take
 
@@ -1099,9 +1105,13 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
 // the same operations as ab3_body).  A packed LAYOUT reads only its packed
 // windows: it returns false, leaving the particle unchanged, if a snapshot pair
 // the step needs is not packed (the caller runs such steps with RING).
-template <typename T, int LAYOUT>
+// ISO: the isopycnal-following step (ab3_isopycnal_following.pyx:26-242): the
+// window's third field is the density at rho points and the vertical rate is
+// iso.w_max * drho / (iso.rho_scale + |drho|), drho = rho - rho_t.
+template <typename T, int LAYOUT, bool ISO = false>
 __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
-                                         const opx_window &win, double dt, const double *Cw, float *zout)
+                                         const opx_window &win, double dt, const double *Cw, float *zout,
+                                         IsoRate iso = IsoRate{}, double rho_t = 0.0)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const Axis az = p.az;
@@ -1117,13 +1127,22 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
     if (LAYOUT != OPX_LAYOUT_RING && (j < 0 || j1 < 0))
         return false;
-    blend_uvw<T, LAYOUT, false>(win, g, j, rs.s0, rs.s1, gt, ft, cz.b, cy.b, cx.b, ixu.b, iyv.b, ow.b, c, vu, vv,
+    // the third field's z base: w-points (outer) for w, rho points for density
+    const int bz3 = ISO ? cz.b : ow.b;
+    blend_uvw<T, LAYOUT, false>(win, g, j, rs.s0, rs.s1, gt, ft, cz.b, cy.b, cx.b, ixu.b, iyv.b, bz3, c, vu, vv,
                                 vw);
     const float u = tri32(vu, cz.w, cy.w, ixu.w);
     const float v = tri32(vv, cz.w, iyv.w, cx.w);
     // w with FP64 weights: the vertical index moves through layers that can be
     // a fraction of a metre thick
-    const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    double w;
+    if constexpr (ISO) {
+        const double rho = tri64(vw, w64(az, cz.b, 0.0), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+        const double drho = rho - rho_t;
+        w = iso.w_max * drho / (iso.rho_scale + fabs(drho));
+    } else {
+        w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    }
     Corners2 &c2 = c.c2;
     sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
     double h0, zeta0;
@@ -1193,7 +1212,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     return true;
 }
 
-enum Scheme { kAB3 = 0, kRK2 = 1 };
+enum Scheme { kAB3 = 0, kRK2 = 1, kAB3ISO = 2 };
 
 template <typename T, int LAYOUT, bool RING_OK>
 __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
@@ -1211,7 +1230,7 @@ template <typename T, int LAYOUT, int SCHEME, bool RING_OK>
 __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
                                           const opx_window &win, const opx_release_step *__restrict__ table,
                                           int tstride, int nsteps, double dt, double alpha, const double *Cw,
-                                          int32_t *err)
+                                          int32_t *err, IsoRate iso = IsoRate{})
 {
     if (i >= n)
         return;
@@ -1225,6 +1244,10 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
     c.s0 = -1;
     c.s0_2d = -1;
     c.s0_hz = -1;
+    // the target density, read once per launch (ab3_isopycnal_following.pyx:131)
+    double rho_t = 0.0;
+    if constexpr (SCHEME == kAB3ISO)
+        rho_t = s.rho_t[i];
     bool loaded = false, moved = false;
 #pragma unroll 1
     for (int k = 0; k < nsteps; ++k) {
@@ -1241,7 +1264,7 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
                 p.ay.q = s.fy[i];
                 p.ax.q = s.fx[i];
             }
-            if constexpr (SCHEME == kAB3) {
+            if constexpr (SCHEME != kRK2) {
                 if (p.st != OPX_MULTISTEP_1) {
                     p.hz1 = s.hz1[i];
                     p.hy1 = s.hy1[i];
@@ -1269,6 +1292,8 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
         bool ok;
         if constexpr (SCHEME == kAB3)
             ok = ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, zout);
+        else if constexpr (SCHEME == kAB3ISO)
+            ok = ab3_step<T, LAYOUT, true>(p, c, rs, g, win, dt, Cw, zout, iso, rho_t);
         else
             ok = rk2_step<T, LAYOUT, RING_OK>(p, c, rs, g, win, dt, alpha, Cw, zout);
         if (!ok) {
@@ -1283,7 +1308,7 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
         s.fz[i] = p.az.q;
         s.fy[i] = p.ay.q;
         s.fx[i] = p.ax.q;
-        if constexpr (SCHEME == kAB3) {
+        if constexpr (SCHEME != kRK2) {
             s.hz2[i] = p.hz2;
             s.hy2[i] = p.hy2;
             s.hx2[i] = p.hx2;
@@ -1335,6 +1360,32 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MULT
         else
             ab3_one<T, LAYOUT>(c * BLOCK + threadIdx.x, n, s, g, win, table, dt, Cw);
     }
+}
+
+// Fused isopycnal-following AB3 steps (FP32 storage; the window's third field
+// holds the density at rho points), one particle per thread with the launch
+// shape of k_ab3_fused; one step per launch runs the several-step path with
+// nsteps = 1 (the same arithmetic).
+template <int LAYOUT, int BLOCK, bool IND>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+    k_ab3_iso_fused(opx_split_state s, opx_grid g, opx_window win_arg, const opx_window *__restrict__ win_dev,
+                    const opx_release_step *__restrict__ table, int tstride, int nsteps, double dt, IsoRate iso,
+                    const int32_t *__restrict__ n_active, int32_t *err)
+{
+    const opx_window win = IND ? *win_dev : win_arg;
+    __shared__ double Cw[kLevels];
+    stage_levels(Cw, win.C, g.NZ, threadIdx.x, BLOCK);
+    __syncthreads();
+
+    int64_t n = s.n;
+    if (n_active)
+        n = min(n, (int64_t)*n_active);
+    const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+    const uint32_t xcd = blockIdx.x % kXcds, lb = blockIdx.x / kXcds;
+    const int64_t c = chunks * xcd / kXcds + lb, c1 = chunks * (xcd + 1) / kXcds;
+    if (c < c1)
+        fused_run<float, LAYOUT, kAB3ISO, false>(c * BLOCK + threadIdx.x, n, s, g, win, table, tstride, nsteps, dt,
+                                                 0.0, Cw, err, iso);
 }
 
 // ---------------------------------------------------------------- LDS-tiled fused AB3 (one wave per workgroup)
@@ -1882,6 +1933,8 @@ __global__ void k_permute(opx_split_state src, opx_split_state dst, const uint32
         dst.rel[i] = src.rel[j];
     if (src.z)
         dst.z[i] = src.z[j];
+    if (src.rho_t)
+        dst.rho_t[i] = src.rho_t[j];
 }
 
 __global__ void k_zero_int(int32_t *p) { *p = 0; }
@@ -1914,9 +1967,18 @@ int check_grid(const opx_grid *g)
     return OPX_OK;
 }
 
-int check_window(const opx_window *w, const opx_grid *g)
+// Levels of the window's third 3-D field (w: NZ + 1, density: NZ).
+inline int w_levels(const opx_window *w, const opx_grid *g) { return w->wlevels ? w->wlevels : g->NZ + 1; }
+
+// iso: the isopycnal-following step, whose third field is the density (NZ
+// levels, FP32 storage); otherwise w (NZ + 1 levels).
+int check_window(const opx_window *w, const opx_grid *g, bool iso = false)
 {
     OPX_REQUIRE(w != nullptr, "null window");
+    OPX_REQUIRE(w_levels(w, g) == (iso ? g->NZ : g->NZ + 1), "window's third field has %d levels; the %s step "
+                "needs %d", w_levels(w, g), iso ? "isopycnal-following" : "w-advection", iso ? g->NZ : g->NZ + 1);
+    OPX_REQUIRE(!iso || w->dtype3 == OPX_F32, "the isopycnal-following step needs FP32 storage (got %d)",
+                w->dtype3);
     OPX_REQUIRE(w->u && w->v && w->w && w->zeta && w->hdd && w->C, "window has a null field pointer");
     OPX_REQUIRE(w->dtype3 == OPX_F32 || w->dtype3 == OPX_BF16, "fused step supports F32/BF16 u,v,w (got %d)",
                 w->dtype3);
@@ -1936,7 +1998,8 @@ int check_window(const opx_window *w, const opx_grid *g)
     }
     OPX_REQUIRE(w->u_stride >= (int64_t)g->NZ * g->NY * (g->NX - 1) &&
                     w->v_stride >= (int64_t)g->NZ * (g->NY - 1) * g->NX &&
-                    w->w_stride >= (int64_t)(g->NZ + 1) * g->NY * g->NX && w->zeta_stride >= (int64_t)g->NY * g->NX,
+                    w->w_stride >= (int64_t)w_levels(w, g) * g->NY * g->NX &&
+                    w->zeta_stride >= (int64_t)g->NY * g->NX,
                 "window slot strides smaller than the field sizes");
     return OPX_OK;
 }
@@ -2017,6 +2080,32 @@ int launch_fused(const opx_split_state *s, const opx_grid *g, const opx_window &
 #undef OPX_FUSED_LAUNCH_L
 #undef OPX_FUSED_LAUNCH
     return check_launch("opx_ab3_fused");
+}
+
+int launch_iso(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
+               int layout, const opx_release_step *table, int tstride, int nsteps, double dt, IsoRate iso,
+               const int32_t *n_active, int block, int32_t *err, void *stream)
+{
+    if (layout == OPX_LAYOUT_PAIR_LDS) // the isopycnal step reads PAIR windows directly
+        layout = OPX_LAYOUT_PAIR;
+    const dim3 grid(grid_xcd(s->n, block)), blk(block);
+    hipStream_t st = as_stream(stream);
+    const bool ind = win_dev != nullptr;
+#define OPX_ISO_LAUNCH(L, B, I)                                                                              \
+    if (block == B && layout == L && ind == I)                                                              \
+        hipLaunchKernelGGL((k_ab3_iso_fused<L, B, I>), grid, blk, 0, st, *s, *g, win_arg, win_dev, table, tstride, \
+                           nsteps, dt, iso, n_active, err);
+#define OPX_ISO_LAUNCH_L(L)                                                                                  \
+    OPX_ISO_LAUNCH(L, 128, false) OPX_ISO_LAUNCH(L, 256, false) OPX_ISO_LAUNCH(L, 128, true)                 \
+        OPX_ISO_LAUNCH(L, 256, true)
+    OPX_ISO_LAUNCH_L(OPX_LAYOUT_RING)
+    OPX_ISO_LAUNCH_L(OPX_LAYOUT_PAIR)
+    OPX_ISO_LAUNCH_L(OPX_LAYOUT_QUAD)
+    OPX_ISO_LAUNCH_L(OPX_LAYOUT_OCT)
+    OPX_ISO_LAUNCH_L(OPX_LAYOUT_YPAIR)
+#undef OPX_ISO_LAUNCH_L
+#undef OPX_ISO_LAUNCH
+    return check_launch("opx_ab3_iso_fused");
 }
 
 int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &win_arg, const opx_window *win_dev,
@@ -2155,6 +2244,56 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
                                    stream);
 }
 
+static int check_iso_args(const opx_split_state *s, const opx_release_step *table, int32_t table_stride,
+                          int32_t nsteps, double w_max, double rho_scale, int32_t *block)
+{
+    OPX_REQUIRE(table != nullptr, "null release table");
+    OPX_REQUIRE(nsteps >= 1 && nsteps <= kMaxLaunchSteps, "nsteps %d not in [1, %d]", nsteps, kMaxLaunchSteps);
+    OPX_REQUIRE(nsteps == 1 || table_stride >= 1, "table_stride must be >= 1 for several steps");
+    OPX_REQUIRE(s->n == 0 || s->rho_t != nullptr, "the isopycnal-following step needs rho_t");
+    OPX_REQUIRE(std::isfinite(w_max) && std::isfinite(rho_scale) && rho_scale > 0.0,
+                "w_max must be finite and rho_scale > 0 (got %g, %g)", w_max, rho_scale);
+    if (*block == 0)
+        *block = 128;
+    OPX_REQUIRE(*block == 128 || *block == 256, "block size %d not in {128, 256}", *block);
+    return OPX_OK;
+}
+
+int opx_ab3_iso_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
+                            const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt,
+                            double w_max, double rho_scale, const int32_t *n_active, int32_t block,
+                            int32_t *err_flag, void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_ab3_iso_fused"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    if ((rc = check_window(win, g, true))) return rc;
+    if ((rc = check_iso_args(s, table, table_stride, nsteps, w_max, rho_scale, &block))) return rc;
+    if (s->n == 0)
+        return OPX_OK;
+    const int layout = win->npacked > 0 ? win->layout : OPX_LAYOUT_RING;
+    return launch_iso(s, g, *win, nullptr, layout, table, table_stride, nsteps, dt, IsoRate{w_max, rho_scale},
+                      n_active, block, err_flag, stream);
+}
+
+int opx_ab3_iso_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const opx_window *win_dev,
+                                int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
+                                double dt, double w_max, double rho_scale, const int32_t *n_active, int32_t block,
+                                int32_t *err_flag, void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_ab3_iso_fused_dev"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(win_dev != nullptr, "null window");
+    OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
+    if ((rc = check_iso_args(s, table, table_stride, nsteps, w_max, rho_scale, &block))) return rc;
+    if (s->n == 0)
+        return OPX_OK;
+    opx_window unused{};
+    return launch_iso(s, g, unused, win_dev, layout, table, table_stride, nsteps, dt, IsoRate{w_max, rho_scale},
+                      n_active, block, err_flag, stream);
+}
+
 int opx_rk2_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_window *win,
                         const opx_release_step *table, int32_t table_stride, int32_t nsteps, double dt, double alpha,
                         const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
@@ -2271,7 +2410,9 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
                        static_cast<float4 *>(p2), n2d);
     if ((rc = check_launch("opx_pack_window 2d"))) return rc;
     const int64_t nu = (int64_t)g->NZ * g->NY * (g->NX - 1), nv = (int64_t)g->NZ * (g->NY - 1) * g->NX;
-    const int64_t nw = (int64_t)(g->NZ + 1) * g->NY * g->NX;
+    const int wl = w_levels(win, g);
+    OPX_REQUIRE(wl == g->NZ || wl == g->NZ + 1, "third field levels %d not NZ or NZ + 1", wl);
+    const int64_t nw = (int64_t)wl * g->NY * g->NX;
     struct F {
         const void *base;
         int64_t stride, n;
@@ -2378,7 +2519,7 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
     OPX_REQUIRE(xlow >= 0 && xlow <= g->xbits, "xlow %d out of [0, %d]", xlow, g->xbits);
     OPX_REQUIRE(clocks == nullptr || nrel >= 0, "negative release count");
     OPX_REQUIRE((src->pid == nullptr) == (dst->pid == nullptr) && (src->rel == nullptr) == (dst->rel == nullptr) &&
-                    (src->z == nullptr) == (dst->z == nullptr),
+                    (src->z == nullptr) == (dst->z == nullptr) && (src->rho_t == nullptr) == (dst->rho_t == nullptr),
                 "src/dst optional arrays differ");
     const int64_t n = src->n;
     uint64_t need = 0;

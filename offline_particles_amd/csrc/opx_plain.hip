@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_rk2_update(opx_particles p, opx_scal
 // ---------------- AB3 (ab3_w_advection.pyx) ----------------
 
 // ISO: the vertical rate is dz0 = w_max * drho / (rho_scale + |drho|) with
-// drho = rho(x) - rho_t (ab3_isopycnal_following.pyx:126-134) instead of w.
+// drho = rho(x) - rho_t (ab3_isopycnal_following.pyx:125-132) instead of w.
 struct IsoArgs {
     opx_field rho;
     const double *rho_t;

@@ -3,9 +3,14 @@
 ``AB3Engine`` advances particles with fused HIP kernels that take one or
 several (``launch_steps``) whole steps per launch: AB3 (``opx_ab3_fused_steps``:
 validation -> ab3_w_advection -> ab3_post_w_advection, the reference's
-timesteppers.py:296-305 sequence) or, with ``scheme="rk2"``, RK2
+timesteppers.py:296-305 sequence), with ``scheme="rk2"`` RK2
 (``opx_rk2_fused_steps``: validation -> _rk2_step_1 -> _rk2_step_2 ->
-_rk2_update, timesteppers.py:241-257).  Between the steps of a launch each
+_rk2_update, timesteppers.py:241-257), or with ``scheme="ab3_iso"`` the
+isopycnal-following AB3 (``opx_ab3_iso_fused_steps``: validation ->
+ab3_isopycnal_following -> its post step, models/roms/__init__.py:109-140),
+which reads the fieldset's density field "rho" in place of w, the constants
+``w_max`` and ``rho_scale`` from the fieldset and each particle's target
+density ``rho_t`` from ``add_release``.  Between the steps of a launch each
 particle's state stays in registers and the stencil nodes it read are blended
 again while its cells are unchanged; the results are the same bits as one step
 per launch.  The engine keeps
@@ -93,7 +98,7 @@ class SplitState:
     # status 1 + cell 4 + offsets 3 x 4 + FP32 history 6 x 4 + pid 4 + rel 2
     BYTES_PER_PARTICLE = 47
 
-    def __init__(self, capacity: int, device, *, with_z: bool = False):
+    def __init__(self, capacity: int, device, *, with_z: bool = False, with_rho_t: bool = False):
         n = int(capacity)
         self.capacity = n
         self.status = torch.full((n,), ParticleStatus.INACTIVE, dtype=torch.uint8, device=device)
@@ -105,6 +110,8 @@ class SplitState:
         self.pid = torch.zeros(n, dtype=torch.int32, device=device)
         self.rel = torch.zeros(n, dtype=torch.int16, device=device)  # read as uint16
         self.z = torch.zeros(n, dtype=torch.float32, device=device) if with_z else None
+        # target density of the isopycnal-following step (FP64, 8 B per particle)
+        self.rho_t = torch.zeros(n, dtype=torch.float64, device=device) if with_rho_t else None
 
     def struct(self, n: int, start: int = 0) -> _lib.opx_split_state:
         s = _lib.opx_split_state()
@@ -120,6 +127,7 @@ class SplitState:
         s.pid = p(self.pid, 4)
         s.rel = p(self.rel, 2)
         s.z = p(self.z, 4)
+        s.rho_t = p(self.rho_t, 8)
         return s
 
 
@@ -134,9 +142,21 @@ class AB3Engine:
         if storage_dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
         self.storage_dtype = storage_dtype
-        if scheme not in ("ab3", "rk2"):
-            raise ValueError(f"scheme must be 'ab3' or 'rk2', not {scheme!r}")
+        if scheme not in ("ab3", "rk2", "ab3_iso"):
+            raise ValueError(f"scheme must be 'ab3', 'rk2' or 'ab3_iso', not {scheme!r}")
         self.scheme = scheme
+        if scheme == "ab3_iso":
+            # the isopycnal-following step (ab3_isopycnal_following.pyx) reads
+            # its scalars from the fieldset constants, as the reference's
+            # launcher does; the density needs FP32 storage (BF16 resolves
+            # 1026 kg/m^3 only to 8)
+            if storage_dtype != torch.float32:
+                raise TypeError("the isopycnal-following step needs storage_dtype=torch.float32")
+            for k in ("w_max", "rho_scale"):
+                if k not in fieldset.constants:
+                    raise KeyError(k)
+            self.w_max = float(fieldset.constants["w_max"])
+            self.rho_scale = float(fieldset.constants["rho_scale"])
         self.alpha = float(alpha)
         if scheme == "rk2" and not self.alpha > 0:
             raise ValueError("RK2 needs alpha > 0")
@@ -152,7 +172,8 @@ class AB3Engine:
             raise ValueError("a streamed window needs at least 3 slots")
         self._build_window(fieldset)
         self._with_z = with_z
-        self._state = SplitState(capacity, self.device, with_z=with_z)
+        self._with_rho_t = scheme == "ab3_iso"
+        self._state = self._new_state(capacity)
         self._spare = None
         self.n = 0
         # device-resident values the step reads, so captured graphs stay valid:
@@ -207,12 +228,19 @@ class AB3Engine:
     # ---------------------------------------------------------------- setup
 
     def _build_window(self, fs):
-        for name, st in _ROMS_STAGGERS.items():
+        # the third 3-D field: w at w-points, or for the isopycnal-following
+        # step the density at rho points (kept in the same ring)
+        third = "rho" if self.scheme == "ab3_iso" else "w"
+        staggers = dict(_ROMS_STAGGERS)
+        if third == "rho":
+            del staggers["w"]
+            staggers["rho"] = (Stagger.CENTER, Stagger.CENTER, Stagger.CENTER)
+        for name, st in staggers.items():
             f = fs[name]
             if tuple(f.stagger) != st:
                 raise ValueError(f"field '{name}' has staggering {tuple(map(str, f.stagger))}; the fused step "
                                  f"needs the ROMS C-grid {tuple(map(str, st))}")
-        for name in ("u", "v", "w", "zeta"):
+        for name in ("u", "v", third, "zeta"):
             if not isinstance(fs[name], TimeDependentField):
                 raise TypeError(f"field '{name}' must be time dependent")
         for name in ("h", "dx", "dy", "C"):
@@ -220,12 +248,12 @@ class AB3Engine:
                 raise TypeError(f"field '{name}' must be static")
         NZ, NY, NX = fs.z_size, fs.y_size, fs.x_size
         self.NZ, self.NY, self.NX = NZ, NY, NX
-        snaps = {n: fs[n].data for n in ("u", "v", "w", "zeta")}
+        snaps = {n: fs[third if n == "w" else n].data for n in ("u", "v", "w", "zeta")}
         K = {int(t.shape[0]) for t in snaps.values()}
         if len(K) != 1:
             raise ValueError(f"u, v, w, zeta hold different snapshot counts {K}")
         self.nsnap = K.pop()
-        cyc = {fs[n].cyclic for n in ("u", "v", "w", "zeta")}
+        cyc = {fs[n].cyclic for n in ("u", "v", third, "zeta")}
         if cyc == {False} and self.nsnap != len(self.time_array):
             raise ValueError("non-cyclic fields need one snapshot per time_array entry")
         self.nslots = self.nsnap
@@ -273,7 +301,11 @@ class AB3Engine:
         w.hdd = _ptr(self._hdd)
         w.C = _ptr(self._C)
         w.dy = _ptr(self._dy)
+        w.wlevels = NZ if third == "rho" else NZ + 1
         self.window = w
+
+    def _new_state(self, capacity):
+        return SplitState(capacity, self.device, with_z=self._with_z, with_rho_t=self._with_rho_t)
 
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
@@ -282,7 +314,7 @@ class AB3Engine:
         """Allocate the particle buffers, the re-bin target and the re-bin workspace
         for `capacity` particles up front (no allocation while stepping after this)."""
         self._ensure_capacity(int(capacity), exact=True)
-        self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
+        self._spare = self._new_state(self._state.capacity)
         self._alloc_work(self._state.capacity)
         self._graphs = {}
 
@@ -306,9 +338,10 @@ class AB3Engine:
         if n <= self._state.capacity:
             return
         cap = n if exact else max(n, 2 * self._state.capacity)
-        new = SplitState(cap, self.device, with_z=self._with_z)
+        new = self._new_state(cap)
         old = self._state
-        for name in ("status", "cell", "pid", "rel") + _STATE_F32 + (("z",) if self._with_z else ()):
+        for name in (("status", "cell", "pid", "rel") + _STATE_F32 + (("z",) if self._with_z else ())
+                     + (("rho_t",) if self._with_rho_t else ())):
             getattr(new, name)[: self.n].copy_(getattr(old, name)[: self.n])
         self._state = new
         self._spare = None
@@ -316,16 +349,21 @@ class AB3Engine:
         self._graphs = {}
 
     def add_release(self, zidx, yidx, xidx, *, start_time=None, nsteps: int, start_step: int = 0,
-                    status: int | None = None) -> int:
+                    status: int | None = None, rho_t=None) -> int:
         """Add a group of particles (a release); returns its id.
 
         The release's clock starts at ``start_time`` (default time_array[0])
         and it advances at global steps ``start_step`` .. while its
         ``nsteps`` steps last.  Particles start with ``status`` (default
         MULTISTEP_1 for AB3: Euler -> AB2 -> AB3 start-up; NORMAL for RK2).
+        ``rho_t`` (a value or one per particle) is the target density of the
+        isopycnal-following scheme, required there and refused otherwise.
         """
         if status is None:
-            status = ParticleStatus.MULTISTEP_1 if self.scheme == "ab3" else ParticleStatus.NORMAL
+            status = ParticleStatus.NORMAL if self.scheme == "rk2" else ParticleStatus.MULTISTEP_1
+        if (rho_t is None) == self._with_rho_t:
+            raise ValueError("rho_t is required by scheme 'ab3_iso'" if self._with_rho_t
+                             else f"rho_t is only used by scheme 'ab3_iso', not {self.scheme!r}")
         r = self._free_ids.pop() if self._free_ids else len(self._clocks_host)
         if r >= 1 << 15:
             raise ValueError("too many releases")
@@ -347,6 +385,12 @@ class AB3Engine:
             S.rel[start:start + m] = r
             for name in _STATE_F32[3:]:
                 getattr(S, name)[start:start + m] = 0.0
+            if self._with_rho_t:
+                rt = _h2d(rho_t if torch.is_tensor(rho_t) else np.asarray(rho_t, np.float64), torch.float64,
+                          self.device).reshape(-1)
+                if rt.numel() not in (1, m):
+                    raise ValueError(f"rho_t has {rt.numel()} values for {m} particles")
+                S.rho_t[start:start + m].copy_(rt.expand(m))
         self.n = start + m
         self._n_active_dev.fill_(self.n)
         self._release_pids[r] = (self._next_pid, m)
@@ -447,7 +491,7 @@ class AB3Engine:
         if self.n == 0:
             return
         if self._spare is None or self._spare.capacity != self._state.capacity:
-            self._spare = SplitState(self._state.capacity, self.device, with_z=self._with_z)
+            self._spare = self._new_state(self._state.capacity)
             self._graphs = {}
         if self._work is None or self._work_n < self.n:
             self._alloc_work(self.n)
@@ -621,6 +665,11 @@ class AB3Engine:
             _lib.call("opx_rk2_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
                       self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt, self.alpha,
                       self._n_active_dev.data_ptr(), self.block, self._err.data_ptr(), stream)
+        elif self.scheme == "ab3_iso":
+            _lib.call("opx_ab3_iso_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid),
+                      self._win_dev.data_ptr(), layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,
+                      self.w_max, self.rho_scale, self._n_active_dev.data_ptr(), self.block, self._err.data_ptr(),
+                      stream)
         else:
             _lib.call("opx_ab3_fused_steps_dev", ctypes.byref(s), ctypes.byref(self.grid), self._win_dev.data_ptr(),
                       self.window.dtype3, layout, self._table.data_ptr(), self._rel_cap, int(m), self.dt,

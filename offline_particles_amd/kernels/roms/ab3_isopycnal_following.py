@@ -41,7 +41,7 @@ def ab3_isopycnal_following(particles, scalars, fielddata) -> None:
 
 
 def ab3_post_isopycnal_following(particles, scalars, fielddata) -> None:
-    """Post-advection step to update particle zidx from z (ab3_isopycnal_following.pyx:175-229)."""
+    """Post-advection step to update particle zidx from z (ab3_isopycnal_following.pyx:193-242)."""
     p = particles_struct(particles, ("z",))
     s = scalars_struct(scalars, ("hc", "NZ"))
     f = fields_struct(fielddata, _VERT)

@@ -465,10 +465,10 @@ void orc_ab3_post(orc_particles *p, const orc_scalars *s, const orc_fields *F)
 /* L1: AB3 isopycnal following (kernels/roms/ab3_isopycnal_following.pyx) */
 /* ------------------------------------------------------------------ */
 
-/* _ab3_isopycnal_following (ab3_isopycnal_following.pyx:26-173): the AB3
+/* _ab3_isopycnal_following (ab3_isopycnal_following.pyx:26-191): the AB3
  * step of orc_ab3 with the vertical rate dz0 = w_max * drho / (rho_scale +
- * |drho|), drho = rho(x) - rho_t (:126-134), in place of w.  The post step is
- * orc_ab3_post (:175-229 is the same computation). */
+ * |drho|), drho = rho(x) - rho_t (:125-132), in place of w.  The post step is
+ * orc_ab3_post (:193-242 is the same computation). */
 void orc_ab3_iso(orc_particles *p, const orc_scalars *s, const orc_fields *F, const orc_field *rho,
                  const double *rho_t, double w_max, double rho_scale)
 {

@@ -178,3 +178,44 @@ def test_kernel_rejects_host_tensors():
     p = Particles(4, device="cpu", status=np.uint8, zidx=np.float64, yidx=np.float64, xidx=np.float64)
     with pytest.raises(ValueError):
         finite_indices(p, {}, {})
+
+
+def test_iso_fused_entry_point_rejects_bad_arguments(lib):
+    """opx_ab3_iso_fused_steps checks its window and arguments on the host and
+    returns an error before any launch: a w window (NZ + 1 levels), BF16
+    storage, a missing rho_t, rho_scale <= 0.  (Every case fails a check, so
+    nothing reaches the device.)"""
+    NZ, NY, NX = 10, 8, 9
+    g = _lib.opx_grid()
+    g.NZ, g.NY, g.NX = NZ, NY, NX
+    g.xbits, g.ybits, g.zbits = 4, 4, 4
+    g.hc = 10.0
+    buf = ctypes.create_string_buffer(64)
+    p = ctypes.addressof(buf)
+    s = _lib.opx_split_state()
+    s.n = 1
+    for name in ("status", "cell", "fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2", "rho_t"):
+        setattr(s, name, p)
+    w = _lib.opx_window()
+    w.u = w.v = w.w = w.zeta = w.hdd = w.C = p
+    w.u_stride = w.v_stride = w.w_stride = w.zeta_stride = (NZ + 1) * NY * NX
+    w.dtype3, w.nslots = _lib.OPX_F32, 2
+    w.wlevels = NZ + 1
+
+    def rc(rho_scale=0.2):
+        return lib.opx_ab3_iso_fused_steps(ctypes.byref(s), ctypes.byref(g), ctypes.byref(w), p, 0, 1, 120.0, 5e-4,
+                                           rho_scale, None, 128, None, None)
+
+    assert rc() == _lib.OPX_ERR_INVALID and b"levels" in lib.opx_last_error()
+    w.wlevels = NZ
+    w.dtype3 = _lib.OPX_BF16
+    assert rc() == _lib.OPX_ERR_INVALID and b"FP32" in lib.opx_last_error()
+    w.dtype3 = _lib.OPX_F32
+    s.rho_t = None
+    assert rc() == _lib.OPX_ERR_INVALID and b"rho_t" in lib.opx_last_error()
+    s.rho_t = p
+    assert rc(0.0) == _lib.OPX_ERR_INVALID and b"rho_scale" in lib.opx_last_error()
+    # the w-advection step refuses a density window
+    assert lib.opx_ab3_fused_steps(ctypes.byref(s), ctypes.byref(g), ctypes.byref(w), p, 0, 1, 120.0, None, 128,
+                                   None, None) == _lib.OPX_ERR_INVALID
+    assert b"levels" in lib.opx_last_error()
