@@ -30,7 +30,9 @@ Forms measured in the same run:
   forms); reported under ``isopycnal``.
 - output events (``--extras output``): compute_z plus the trilinear (u, v, w)
   and bilinear (zeta, h) output kernels of the drop-in API over every particle
-  at a time index between snapshots; reported under ``output_events``.
+  at a time index between snapshots, and the same six outputs from the fused
+  engine's state (AB3Engine.sample, one kernel over the re-binned particles);
+  reported under ``output_events``.
 The headline ``value`` is the fastest 16M-particle form; every form is listed
 in ``forms``.
 
@@ -344,6 +346,63 @@ def run_output(wl, device, rounds, warmup):
     return res
 
 
+def run_output_fused(wl, device, rounds, warmup):
+    """Output events of the fused form: AB3Engine.sample of u, v, w (trilinear),
+    zeta, h (bilinear) and z (compute_z) for every particle in one kernel
+    (opx_sample_split) over the engine's re-binned state, at the engine's time
+    after 16 steps."""
+    import torch
+
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.workloads import OFFSETS, build_fieldset
+
+    fs = build_fieldset(wl, device=device)
+    n = wl.nparticles
+    eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=n, rebin_every=64, xlow=3)
+    eng.add_release(*wl.positions, nsteps=32)
+    eng.launch_steps = 16
+    eng.rebin()
+    eng.step(16)
+    names = ("u", "v", "w", "zeta", "h")
+    for _ in range(warmup):
+        eng.sample(fs, names, depth="z")
+    torch.cuda.synchronize()
+    eng.timer = []
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        eng.sample(fs, names, depth="z")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kms = float(np.mean([a.elapsed_time(b) for a, b, _ in eng.timer]))
+    eng.timer = None
+    act_mask = eng.status() < 128
+    act = int(act_mask.sum().item())
+    z, y, x = eng.positions()
+    pos = [z[act_mask], y[act_mask], x[act_mask]]
+    # state read (status, cell, 3 offsets, release id, pid = 23 B) and six FP64
+    # outputs written per active particle, plus the distinct field nodes read
+    fb = sum(2 * 4 * touched_nodes(wl.fields[k].shape[1:], OFFSETS[k], pos) for k in ("u", "v", "w"))
+    n2 = touched_nodes(wl.fields["h"].shape, OFFSETS["h"], pos)
+    fb += (2 * 4 + 4) * n2 + wl.fields["C"].nbytes
+    algo = act * (23 + 6 * 8) + fb
+    res = {
+        "form": "output_fused_fp32",
+        "outputs_per_round": 6,
+        "particle_outputs_per_s": act * 6 * rounds / elapsed,
+        "ms_per_round": 1e3 * elapsed / rounds,
+        "particles": n,
+        "active_particles": act,
+        "kernel": "k_sample (opx_sample_split)",
+        "kernel_ms": kms,
+        "algo_bytes": algo,
+        "achieved_GBs": algo / (kms * 1e-3) / 1e9,
+        "roofline_frac": algo / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    del eng, fs
+    torch.cuda.empty_cache()
+    return res
+
+
 def field_bytes_touched_fused(wl, pos, storage_bytes_3d, fields_3d=("u", "v", "w")):
     """Distinct field bytes one fused launch reads: u, v, w (or rho; 2 snapshots),
     packed h/dx/dy, zeta (2), C."""
@@ -605,6 +664,7 @@ _PROFILE_FOR_FORM = {
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
     ("fused_iso_fp32", 16): "k_ab3_iso_fused_steps16_turbulent16M.md",
+    ("output_fused_fp32", 1): "k_sample_turbulent16M.md",
 }
 
 
@@ -902,9 +962,13 @@ def main():
         log("isopycnal fused", fused)
         isopycnal = {"plain_iso_fp32": plain, "fused_iso_fp32": fused}
     if "output" in extras:
-        output_events = run_output(wl, device, args.output_rounds, 2)
-        output_events["traffic"] = profiled_traffic("output_fp32", args.particles, (NX, NY, NZ))
-        log("output", output_events)
+        plain = run_output(wl, device, args.output_rounds, 2)
+        plain["traffic"] = profiled_traffic("output_fp32", args.particles, (NX, NY, NZ))
+        log("output plain", plain)
+        fused = run_output_fused(wl, device, args.output_rounds, 2)
+        fused["traffic"] = profiled_traffic("output_fused_fp32", args.particles, (NX, NY, NZ))
+        log("output fused", fused)
+        output_events = {"output_fp32": plain, "output_fused_fp32": fused}
 
     batching = None
     if args.batched:

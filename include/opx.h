@@ -411,6 +411,42 @@ int opx_rk2_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
                       int32_t layout, const opx_release_step *table, double dt, double alpha,
                       const int32_t *n_active, int32_t block, void *stream);
 
+/* Output events of the fused form.  Each particle's samples form one record
+ * of `out_stride` FP64 slots at out + o * out_stride, o = pid[i] (or i when
+ * pid is NULL), the order of opx_split_to_f64: slot k holds sample k of an
+ * active particle (status without the INACTIVE bit), every other slot and
+ * every slot of an inactive particle holds NaN.  out_stride is 1, 2, 4 or 8
+ * (>= nsamples; `out` 16-byte aligned above 1): a record of 4 or 8 slots is
+ * one 32- or 64-byte piece of memory written whole, so the scattered records
+ * cost no partial-line writes. */
+enum opx_sample_kind {
+    OPX_SAMPLE_LINEAR = 1,    /* 1-D field over zidx (linear_interpolation.pyx:34-68) */
+    OPX_SAMPLE_BILINEAR = 2,  /* 2-D field over (yidx, xidx) (linear_interpolation.pyx:70-105) */
+    OPX_SAMPLE_TRILINEAR = 3, /* 3-D field (linear_interpolation.pyx:107-136) */
+    OPX_SAMPLE_COMPUTE_Z = 4, /* z from h, zeta (bilinear) and C (linear) (_roms_core.pyx:17-73) */
+};
+
+typedef struct opx_sample {
+    int32_t kind;   /* opx_sample_kind */
+    int32_t field;  /* index of the field in a release's row; h for OPX_SAMPLE_COMPUTE_Z */
+    int32_t field2; /* zeta for OPX_SAMPLE_COMPUTE_Z */
+    int32_t field3; /* C for OPX_SAMPLE_COMPUTE_Z */
+} opx_sample;
+
+/* Evaluate `nsamples` samples at the split positions of `s` in one launch.
+ * `fields_dev` (device memory) holds one row of `nfields` opx_field per
+ * release: row r is the fields' snapshot pairs and blend weights at release
+ * r's time index (a static field has next == NULL); `samples_dev` (device
+ * memory) lists the samples (at most 8).  The FP64 index of a split position is exact, so
+ * each value is bit for bit what opx_interp_* / opx_compute_z give at the
+ * indices opx_split_to_f64 exports; the gathers follow the state's (re-binned)
+ * order instead of the caller's.  Replaces, for the fused form, the
+ * reference's output kernels launched per field and compute_z_kernel
+ * (kernels/output/linear_interpolation.pyx:34-136, kernels/roms/_roms_core.pyx:17-73). */
+int opx_sample_split(const opx_split_state *s, const opx_grid *g, const opx_field *fields_dev, int32_t nfields,
+                     int32_t nrel, const opx_sample *samples_dev, int32_t nsamples, double hc, double *out,
+                     int32_t out_stride, void *stream);
+
 /* Split <-> FP64 index conversion.  opx_split_from_f64 writes the state of
  * particle i from idx arrays (and status/rel untouched); opx_split_to_f64
  * writes zidx/yidx/xidx[pid[i]] (or [i] if pid is NULL). */

@@ -119,6 +119,14 @@ class opx_window(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
+OPX_SAMPLE_LINEAR, OPX_SAMPLE_BILINEAR, OPX_SAMPLE_TRILINEAR, OPX_SAMPLE_COMPUTE_Z = 1, 2, 3, 4
+
+
+class opx_sample(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("field", ctypes.c_int32), ("field2", ctypes.c_int32),
+                ("field3", ctypes.c_int32)]
+
+
 class opx_release_clock(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("start_step", ctypes.c_int32), ("steps_left", ctypes.c_int32)]
 
@@ -163,6 +171,8 @@ _SIGNATURES.update({
     "opx_ab3_iso_fused_steps_dev": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, _vp, ctypes.c_int32,
                                     ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp,
                                     ctypes.c_int32, _vp, _vp],
+    "opx_sample_split": [_P(opx_split_state), _P(opx_grid), _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+                         ctypes.c_double, _vp, ctypes.c_int32, _vp],
     "opx_split_from_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_split_to_f64": [_P(opx_split_state), _P(opx_grid), _vp, _vp, _vp, _vp],
     "opx_rebin_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],

@@ -24,6 +24,7 @@
 #include <cstdint>
 
 #include "opx.h"
+#include "opx_numerics.hpp"
 #include "opx_runtime.hpp"
 
 namespace opx {
@@ -1655,6 +1656,71 @@ __global__ void k_split_to_f64(opx_split_state s, opx_grid g, double *zidx, doub
     xidx[o] = x.ok ? x.pos() : nan;
 }
 
+// ---------------------------------------------------------------- output events at split positions
+
+constexpr int kMaxSamples = 8;
+
+// The reference's output kernels and compute_z (linear_interpolation.pyx:34-136,
+// _roms_core.pyx:17-73) at the exact FP64 index of each split position, with
+// the drop-in's FP64 arithmetic (opx_numerics.hpp); one thread per particle in
+// the state's order, all samples of a particle in one pass, written as one
+// whole record (NaN-padded) so that the scatter by pid writes full pieces.
+// ONE_REL: a single release, whose field row every lane reads alike.
+template <bool ONE_REL>
+__global__ void __launch_bounds__(kBlock)
+    k_sample(opx_split_state s, opx_grid g, const opx_field *__restrict__ fields, int nfields,
+             const opx_sample *__restrict__ samples, int nsamples, double hc, double *__restrict__ out, int stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n)
+        return;
+    const double nan = __builtin_nan("");
+    double v[kMaxSamples];
+#pragma unroll
+    for (int k = 0; k < kMaxSamples; ++k)
+        v[k] = nan;
+    if (!(s.status[i] & OPX_INACTIVE)) {
+        const uint32_t cell = (uint32_t)s.cell[i];
+        Axis z = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+        Axis y = decode_axis(cell, g.xbits, g.ybits);
+        Axis x = decode_axis(cell, 0, g.xbits);
+        z.q = s.fz[i];
+        y.q = s.fy[i];
+        x.q = s.fx[i];
+        const double zi = z.ok ? z.pos() : nan, yi = y.ok ? y.pos() : nan, xi = x.ok ? x.pos() : nan;
+        const opx_field *row = ONE_REL ? fields : fields + (int64_t)(s.rel ? (int)s.rel[i] : 0) * nfields;
+#pragma unroll
+        for (int k = 0; k < kMaxSamples; ++k) {
+            if (k >= nsamples)
+                break;
+            const opx_sample sm = samples[k];
+            if (sm.kind == OPX_SAMPLE_LINEAR) {
+                v[k] = linear_f64(row[sm.field], zi);
+            } else if (sm.kind == OPX_SAMPLE_BILINEAR) {
+                v[k] = bilinear_f64(row[sm.field], yi, xi);
+            } else if (sm.kind == OPX_SAMPLE_TRILINEAR) {
+                v[k] = trilinear_f64(row[sm.field], zi, yi, xi);
+            } else {
+                const double h = bilinear_f64(row[sm.field], yi, xi);
+                const double zeta = bilinear_f64(row[sm.field2], yi, xi);
+                const double C = linear_f64(row[sm.field3], zi);
+                v[k] = compute_z(zi, g.NZ, hc, h, C, zeta);
+            }
+        }
+    }
+    double *rec = out + (s.pid ? (int64_t)s.pid[i] : i) * stride;
+    if (stride == 1) {
+        rec[0] = v[0];
+        return;
+    }
+    // stride 2, 4 or 8 with a 16-byte aligned `out`: 16-byte stores
+    double2 *r2 = reinterpret_cast<double2 *>(rec);
+#pragma unroll
+    for (int k = 0; k < kMaxSamples / 2; ++k)
+        if (2 * k < stride)
+            r2[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
 // ---------------------------------------------------------------- fused RK2
 
 // `(Py_ssize_t)p` clamped to [0, hi] (linear.pxd:3-9); NaN and values outside
@@ -2464,6 +2530,31 @@ int opx_split_from_f64(const opx_split_state *s, const opx_grid *g, const double
     hipLaunchKernelGGL(k_split_from_f64, dim3(grid_for(s->n)), dim3(kBlock), 0, as_stream(stream), *s, *g, zidx,
                        yidx, xidx);
     return check_launch("opx_split_from_f64");
+}
+
+int opx_sample_split(const opx_split_state *s, const opx_grid *g, const opx_field *fields_dev, int32_t nfields,
+                     int32_t nrel, const opx_sample *samples_dev, int32_t nsamples, double hc, double *out,
+                     int32_t out_stride, void *stream)
+{
+    int rc;
+    if ((rc = check_state(s, "opx_sample_split", false))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(nfields >= 1 && nrel >= 1, "need at least one field and release");
+    OPX_REQUIRE(nsamples >= 1 && nsamples <= kMaxSamples, "nsamples %d not in [1, %d]", nsamples, kMaxSamples);
+    OPX_REQUIRE(out_stride >= nsamples && (out_stride == 1 || out_stride == 2 || out_stride == 4 || out_stride == 8),
+                "out_stride %d must be 1, 2, 4 or 8 and at least nsamples (%d)", out_stride, nsamples);
+    OPX_REQUIRE(fields_dev != nullptr && samples_dev != nullptr && out != nullptr, "null table or output");
+    OPX_REQUIRE(out_stride == 1 || reinterpret_cast<uintptr_t>(out) % 16 == 0, "out must be 16-byte aligned");
+    if (s->n == 0)
+        return OPX_OK;
+    const dim3 grid(grid_for(s->n)), blk(kBlock);
+    if (nrel == 1)
+        hipLaunchKernelGGL(k_sample<true>, grid, blk, 0, as_stream(stream), *s, *g, fields_dev, (int)nfields,
+                           samples_dev, (int)nsamples, hc, out, (int)out_stride);
+    else
+        hipLaunchKernelGGL(k_sample<false>, grid, blk, 0, as_stream(stream), *s, *g, fields_dev, (int)nfields,
+                           samples_dev, (int)nsamples, hc, out, (int)out_stride);
+    return check_launch("opx_sample_split");
 }
 
 int opx_split_to_f64(const opx_split_state *s, const opx_grid *g, double *zidx, double *yidx, double *xidx,

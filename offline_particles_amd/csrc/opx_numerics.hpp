@@ -59,6 +59,67 @@ __device__ __forceinline__ double load_blend(const opx_field &f, int64_t i)
     }
 }
 
+// Values i and i + 1 of a snapshot, read with one load: 8 bytes at 4-byte
+// alignment (FP32) or 16 bytes at 8-byte alignment (FP64).
+typedef uint32_t num_u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t num_u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+
+__device__ __forceinline__ void load_pair(const float *p, float &a, float &b)
+{
+    const num_u32x2_a4 t = *reinterpret_cast<const num_u32x2_a4 *>(p);
+    const uint32_t x = t.x, y = t.y;
+    a = __uint_as_float(x);
+    b = __uint_as_float(y);
+}
+
+__device__ __forceinline__ void load_pair(const double *p, double &a, double &b)
+{
+    const num_u32x4_a8 t = *reinterpret_cast<const num_u32x4_a8 *>(p);
+    const uint32_t x0 = t.x, x1 = t.y, y0 = t.z, y1 = t.w;
+    a = __hiloint2double((int)x1, (int)x0);
+    b = __hiloint2double((int)y1, (int)y0);
+}
+
+// load_blend of stencil values i and i + 1 (the same values, in the same
+// arithmetic), reading each snapshot's neighbours with one load for FP32 and
+// FP64 fields.
+__device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, double &va, double &vb)
+{
+    if (f.dtype == OPX_F64) {
+        double p0, p1;
+        load_pair(static_cast<const double *>(f.prev) + i, p0, p1);
+        if (f.next == nullptr) {
+            va = p0;
+            vb = p1;
+            return;
+        }
+        double n0, n1;
+        load_pair(static_cast<const double *>(f.next) + i, n0, n1);
+        double a0 = p0 * f.gt, b0 = n0 * f.ft;
+        double a1 = p1 * f.gt, b1 = n1 * f.ft;
+        va = a0 + b0;
+        vb = a1 + b1;
+    } else if (f.dtype == OPX_F32) {
+        float p0, p1;
+        load_pair(static_cast<const float *>(f.prev) + i, p0, p1);
+        if (f.next == nullptr) {
+            va = (double)p0;
+            vb = (double)p1;
+            return;
+        }
+        float n0, n1;
+        load_pair(static_cast<const float *>(f.next) + i, n0, n1);
+        const float gt = (float)f.gt, ft = (float)f.ft;
+        float a0 = p0 * gt, b0 = n0 * ft;
+        float a1 = p1 * gt, b1 = n1 * ft;
+        va = (double)(a0 + b0);
+        vb = (double)(a1 + b1);
+    } else {
+        va = load_blend(f, i);
+        vb = load_blend(f, i + 1);
+    }
+}
+
 // linear_interpolation (linear.pxd:11-17) on a 1-D field at rho index idx.
 __device__ __forceinline__ double linear_f64(const opx_field &f, double idx)
 {
@@ -66,7 +127,9 @@ __device__ __forceinline__ double linear_f64(const opx_field &f, double idx)
     int64_t I0 = truncate_index(idx, f.shape[0] - 2);
     double f0 = idx - (double)I0;
     double g0 = 1.0 - f0;
-    return g0 * load_blend(f, I0) + f0 * load_blend(f, I0 + 1);
+    double v0, v1;
+    load_blend2(f, I0, v0, v1);
+    return g0 * v0 + f0 * v1;
 }
 
 // bilinear_interpolation (linear.pxd:19-33) on a 2-D field at rho (y, x).
@@ -82,10 +145,9 @@ __device__ __forceinline__ double bilinear_f64(const opx_field &f, double y, dou
     double g0 = 1.0 - f0;
     double g1 = 1.0 - f1;
     int64_t b = I0 * n1 + I1;
-    double v00 = load_blend(f, b);
-    double v01 = load_blend(f, b + 1);
-    double v10 = load_blend(f, b + n1);
-    double v11 = load_blend(f, b + n1 + 1);
+    double v00, v01, v10, v11;
+    load_blend2(f, b, v00, v01);
+    load_blend2(f, b + n1, v10, v11);
     return g0 * g1 * v00 + g0 * f1 * v01 + f0 * g1 * v10 + f0 * f1 * v11;
 }
 
@@ -109,10 +171,11 @@ __device__ __forceinline__ double trilinear_f64(const opx_field &f, double z, do
     int64_t b01 = b00 + n2;
     int64_t b10 = b00 + n1 * n2;
     int64_t b11 = b10 + n2;
-    double v000 = load_blend(f, b00), v001 = load_blend(f, b00 + 1);
-    double v010 = load_blend(f, b01), v011 = load_blend(f, b01 + 1);
-    double v100 = load_blend(f, b10), v101 = load_blend(f, b10 + 1);
-    double v110 = load_blend(f, b11), v111 = load_blend(f, b11 + 1);
+    double v000, v001, v010, v011, v100, v101, v110, v111;
+    load_blend2(f, b00, v000, v001);
+    load_blend2(f, b01, v010, v011);
+    load_blend2(f, b10, v100, v101);
+    load_blend2(f, b11, v110, v111);
     return (g0 * g1 * g2 * v000 +
             g0 * g1 * f2 * v001 +
             g0 * f1 * g2 * v010 +

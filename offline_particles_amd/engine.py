@@ -208,6 +208,7 @@ class AB3Engine:
         # registers between them and unchanged stencils are not read again
         self.launch_steps = 1
         # optional list collecting (start, end, steps) around each fused launch
+        # (steps 0: an output launch of sample())
         self.timer = None
         self.block = 128  # workgroup size of the fused kernel
         self.tuning = None
@@ -848,6 +849,86 @@ class AB3Engine:
         out = torch.empty(self.n, dtype=torch.uint8, device=self.device)
         pid = self._state.pid[: self.n].long()
         out[pid] = self._state.status[: self.n]
+        return out
+
+    def sample(self, fieldset, fields=(), *, depth: str | None = None) -> dict:
+        """Output event of the fused form: FP64 values at every particle, in the
+        order of positions().
+
+        ``fields`` names fields of ``fieldset`` (the drop-in Fieldset, e.g. the
+        one the engine was built from): a 3-D field is interpolated trilinearly,
+        a 2-D one bilinearly over (y, x) and a 1-D one linearly over z, like
+        trilinear/bilinear/linear_interpolation_kernel
+        (kernels/output/linear_interpolation.pyx:34-136); ``depth`` names an
+        extra output holding z (compute_z_kernel, _roms_core.pyx:17-73).  Each
+        release is sampled at its own current time (the time index of
+        Timestepper.get_time_index, blend weights of the field's dtype).
+        Inactive particles get NaN.  One kernel evaluates every output of a
+        particle, in the re-binned order, so the gathers stay local; the values
+        equal the drop-in kernels' at positions() bit for bit.  At most 8
+        outputs per call; they are column views of one (n, 1/2/4/8) record
+        tensor (each particle's outputs written together).
+        """
+        from .kernels._marshal import field_struct
+        from .launcher import _NO_BBOX
+
+        if self._compacted:
+            raise RuntimeError("releases were retired: sample() follows positions(), which needs every release")
+        names = list(dict.fromkeys(fields))
+        if depth is not None:
+            for k in ("h", "zeta", "C"):
+                if k not in names:
+                    names.append(k)
+            for k in ("hc", "NZ"):
+                if k not in fieldset.constants:
+                    raise KeyError(k)
+        if not names:
+            raise ValueError("nothing to sample")
+        nrel = len(self._clocks_host)
+        rows = (_lib.opx_field * (max(1, nrel) * len(names)))()
+        for r in range(nrel):
+            t = self._t[r]
+            ta = self.time_array
+            if not (ta[0] <= t <= ta[-1]):
+                raise ValueError("Time is out of bounds of the time array.")
+            idx = np.searchsorted(ta, t, side="right") - 1
+            tidx = idx + (t - ta[idx]) / (ta[idx + 1] - ta[idx])
+            for k, name in enumerate(names):
+                rows[r * len(names) + k] = field_struct(fieldset[name].get_field_data(tidx, _NO_BBOX), name)
+        index = {name: k for k, name in enumerate(names)}
+        kinds = {1: _lib.OPX_SAMPLE_LINEAR, 2: _lib.OPX_SAMPLE_BILINEAR, 3: _lib.OPX_SAMPLE_TRILINEAR}
+        outs, samples = [], []
+        for name in dict.fromkeys(fields):
+            ndim = rows[index[name]].ndim
+            if ndim not in kinds:
+                raise ValueError(f"field '{name}' has {ndim} dimensions")
+            outs.append(name)
+            samples.append(_lib.opx_sample(kinds[ndim], index[name], 0, 0))
+        if depth is not None:
+            outs.append(depth)
+            samples.append(_lib.opx_sample(_lib.OPX_SAMPLE_COMPUTE_Z, index["h"], index["zeta"], index["C"]))
+        if len(samples) > 8:
+            raise ValueError(f"{len(samples)} outputs; one sample() call writes at most 8")
+        # one record of 1, 2, 4 or 8 FP64 slots per particle, written whole
+        width = next(w for w in (1, 2, 4, 8) if w >= len(samples))
+        rec = torch.empty((self.n, width), dtype=torch.float64, device=self.device)
+        if self.n and nrel:
+            rows_dev = _h2d(np.frombuffer(bytes(rows), dtype=np.uint8), torch.uint8, self.device)
+            smp = (_lib.opx_sample * len(samples))(*samples)
+            smp_dev = _h2d(np.frombuffer(bytes(smp), dtype=np.uint8), torch.uint8, self.device)
+            if self.timer is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            _lib.call("opx_sample_split", ctypes.byref(self._state.struct(self.n)), ctypes.byref(self.grid),
+                      rows_dev.data_ptr(), len(names), nrel, smp_dev.data_ptr(), len(samples),
+                      float(fieldset.constants.get("hc", self.grid.hc)), rec.data_ptr(), width, self._stream())
+            if self.timer is not None:
+                e1 = torch.cuda.Event(enable_timing=True)
+                e1.record()
+                self.timer.append((e0, e1, 0))
+            # the tables live until the launch has read them
+            self._sample_tables = (rows_dev, smp_dev)
+        out = {name: rec[:, k] for k, name in enumerate(outs)}
         return out
 
     def release_slice(self, r: int) -> slice:

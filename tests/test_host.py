@@ -49,6 +49,7 @@ _STRUCTS = {
     "opx_window": _lib.opx_window,
     "opx_release_clock": _lib.opx_release_clock,
     "opx_release_step": _lib.opx_release_step,
+    "opx_sample": _lib.opx_sample,
 }
 
 
