@@ -49,6 +49,7 @@ import torch
 from . import _lib
 from .fields import StaticField, TimeDependentField
 from .kernels.status import ParticleStatus
+from .snapshots import SnapshotSource
 from .spatial_arrays import Stagger
 
 _ROMS_STAGGERS = {
@@ -249,8 +250,11 @@ class AB3Engine:
                 raise TypeError(f"field '{name}' must be static")
         NZ, NY, NX = fs.z_size, fs.y_size, fs.x_size
         self.NZ, self.NY, self.NX = NZ, NY, NX
-        snaps = {n: fs[third if n == "w" else n].data for n in ("u", "v", "w", "zeta")}
-        K = {int(t.shape[0]) for t in snaps.values()}
+        flds = {n: fs[third if n == "w" else n] for n in ("u", "v", "w", "zeta")}
+        lazy = [n for n, f in flds.items() if f.source is not None]
+        if lazy and self._stream_slots is None:
+            raise ValueError(f"fields {lazy} are read lazily from their source: give stream_slots to stream them")
+        K = {int(f.num_snapshots) for f in flds.values()}
         if len(K) != 1:
             raise ValueError(f"u, v, w, zeta hold different snapshot counts {K}")
         self.nsnap = K.pop()
@@ -259,17 +263,21 @@ class AB3Engine:
             raise ValueError("non-cyclic fields need one snapshot per time_array entry")
         self.nslots = self.nsnap
         dts = {"u": self.storage_dtype, "v": self.storage_dtype, "w": self.storage_dtype, "zeta": torch.float32}
+        self._dts = dts
         if self._stream_slots is None:
-            self._u, self._v, self._w, self._zeta = (snaps[n].to(self.device, dts[n]).contiguous()
+            self._u, self._v, self._w, self._zeta = (flds[n].data.to(self.device, dts[n]).contiguous()
                                                      for n in ("u", "v", "w", "zeta"))
             self._src = None
         else:
-            # host source (pinned, storage dtype) and a ring of device slots;
-            # snapshot It lives in slot It % nslots while it is needed
+            # host source (pinned, storage dtype, or a SnapshotSource decoded on
+            # demand) and a ring of device slots; snapshot It lives in slot
+            # It % nslots while it is needed
             self.nslots = self._stream_slots
-            self._src = {n: snaps[n].to("cpu", dts[n]).contiguous().pin_memory() for n in ("u", "v", "w", "zeta")}
+            self._src = {n: (flds[n].source if flds[n].source is not None
+                             else flds[n].data.to("cpu", dts[n]).contiguous().pin_memory())
+                         for n in ("u", "v", "w", "zeta")}
             self._u, self._v, self._w, self._zeta = (
-                torch.empty((self.nslots,) + tuple(snaps[n].shape[1:]), dtype=dts[n], device=self.device)
+                torch.empty((self.nslots,) + tuple(flds[n].spatial_shape), dtype=dts[n], device=self.device)
                 for n in ("u", "v", "w", "zeta"))
             self._ring_It = [None] * self.nslots
             self._ring_ready = [None] * self.nslots
@@ -561,22 +569,38 @@ class AB3Engine:
         compute = torch.cuda.current_stream(self.device)
         # one side stream per 3-D field: the copies run on separate DMA queues
         events = []
+        nbytes = 0
         for k, (dst, names) in enumerate(((self._u, ("u",)), (self._v, ("v",)), (self._w, ("w", "zeta")))):
             cs = self._copy_streams[k]
             cs.wait_stream(compute)
             with torch.cuda.stream(cs):
                 for name in names:
                     ring = self._zeta if name == "zeta" else dst
-                    ring[slot].copy_(self._src[name][src], non_blocking=True)
+                    host = self._host_snapshot(name, src)
+                    ring[slot].copy_(host, non_blocking=True)
+                    nbytes += host.nbytes
                 ev = torch.cuda.Event()
                 ev.record(cs)
             events.append(ev)
         self._ring_It[slot] = It
         self._ring_ready[slot] = events
         self.snapshot_loads += 1
-        self.snapshot_bytes += sum(self._src[n][src].nbytes for n in ("u", "v", "w", "zeta"))
+        self.snapshot_bytes += nbytes
         # pair-ring windows packed from the slot's old contents are repacked when
         # next read (their packed snapshot pair no longer matches the slots)
+
+    def _host_snapshot(self, name: str, src: int) -> torch.Tensor:
+        """Snapshot `src` of a field in pinned host memory: a slice of the pinned copy,
+        or decoded from the field's SnapshotSource (the next one is then decoded
+        ahead on a host thread)."""
+        s = self._src[name]
+        if not isinstance(s, SnapshotSource):
+            return s[src]
+        host = torch.from_numpy(np.ascontiguousarray(s.read(src)))
+        pinned = torch.empty(host.shape, dtype=self._dts[name], pin_memory=True)
+        pinned.copy_(host)
+        s.prefetch(src + 1)
+        return pinned
 
     def _ensure_snapshots(self, pairs) -> None:
         """Make the snapshots of `pairs` resident; prefetch the one after them."""

@@ -23,7 +23,8 @@ import numpy as np
 import torch
 
 from ._lib import OPX_BF16, OPX_F32, OPX_F64, storage_code
-from .spatial_arrays import BBox, DeviceArray, SpatialArray, Stagger, as_device_tensor
+from .snapshots import SnapshotSource, ZarrSnapshots
+from .spatial_arrays import BBox, DeviceArray, SpatialArray, Stagger, as_device_tensor, default_device, to_torch_dtype
 
 FieldData = collections.namedtuple("FieldData", ["array", "offsets"])
 
@@ -172,11 +173,18 @@ class StaticField(Field):
 class TimeDependentField(Field):
     """A field given as snapshots on a time axis, resident in HBM (fields.py:237-458).
 
-    ``data`` is a (T, *spatial) host or device array.  With ``cyclic=True``
-    snapshot ``It`` is ``data[It % T]`` so a short set of periodic snapshots
-    serves an arbitrarily long time axis; otherwise ``It`` must lie in
-    ``[0, T-2]`` as in the reference.
+    ``data`` is a (T, *spatial) host or device array, or a ``SnapshotSource``
+    (e.g. ``ZarrSnapshots``) read lazily: then only the snapshots in use stay
+    on the device (the last few, the next one decoded ahead on a host thread),
+    as the reference's dask-backed fields load what a launch needs
+    (fields.py:443-458).  With ``cyclic=True`` snapshot ``It`` is
+    ``data[It % T]`` so a short set of periodic snapshots serves an arbitrarily
+    long time axis; otherwise ``It`` must lie in ``[0, T-2]`` as in the
+    reference.
     """
+
+    # snapshots of a lazily read field kept on the device
+    LAZY_CACHE = 3
 
     def __init__(self, data, z_stagger, y_stagger, x_stagger, spatial_array_factory=None,
                  output_dtype=np.float64, *, cyclic: bool = False, device=None, storage_dtype=None):
@@ -185,19 +193,45 @@ class TimeDependentField(Field):
             raise ValueError("TimeDependentField requires at least 2 dimensions (time + spatial).")
         if data.shape[0] < 2 and not cyclic:
             raise ValueError("TimeDependentField requires at least 2 time steps.")
-        self._snapshots = as_device_tensor(data, device, storage_dtype)
-        self._num_timesteps = int(self._snapshots.shape[0])
+        if isinstance(data, SnapshotSource):
+            self._source = data
+            self._snapshots = None
+            self._device = torch.device(device) if device is not None else default_device()
+            self._storage = to_torch_dtype(storage_dtype if storage_dtype is not None else data.dtype)
+            self._cache = collections.OrderedDict()
+            self._shape = tuple(data.shape)
+        else:
+            self._source = None
+            self._snapshots = as_device_tensor(data, device, storage_dtype)
+            self._shape = tuple(self._snapshots.shape)
+        self._num_timesteps = int(self._shape[0])
         self._cyclic = bool(cyclic)
         self._output_dtype = output_dtype
         self._It = 0
 
+    @classmethod
+    def from_zarr(cls, store, name: str, z_stagger, y_stagger, x_stagger, *, preload: bool = False,
+                  cyclic: bool = False, device=None, storage_dtype=None, workers: int = 8) -> "TimeDependentField":
+        """A field from a (T, ...) Zarr v3 array of a local store: every snapshot
+        loaded now (``preload``) or read lazily through ``ZarrSnapshots``."""
+        src = ZarrSnapshots(store, name, workers=workers)
+        data = src.read_all() if preload else src
+        return cls(data, z_stagger, y_stagger, x_stagger, cyclic=cyclic, device=device, storage_dtype=storage_dtype)
+
+    @property
+    def source(self):
+        """The SnapshotSource of a lazily read field (None when every snapshot is resident)."""
+        return self._source
+
     @property
     def data(self) -> torch.Tensor:
+        if self._source is not None:
+            raise TypeError("this field reads its snapshots lazily: use snapshot(It) or .source")
         return self._snapshots
 
     @property
     def dtype(self):
-        return self._snapshots.dtype
+        return self._storage if self._source is not None else self._snapshots.dtype
 
     @property
     def cyclic(self) -> bool:
@@ -209,10 +243,10 @@ class TimeDependentField(Field):
 
     @property
     def spatial_shape(self) -> tuple:
-        return tuple(self._snapshots.shape[1:])
+        return tuple(self._shape[1:])
 
     def __repr__(self) -> str:
-        return (f"TimeDependentField(shape={tuple(self._snapshots.shape)}, z_stagger={self.z_stagger}, "
+        return (f"TimeDependentField(shape={self._shape}, z_stagger={self.z_stagger}, "
                 f"y_stagger={self.y_stagger}, x_stagger={self.x_stagger}, cyclic={self._cyclic})")
 
     def validate_shape(self, simulation_shape) -> None:
@@ -224,9 +258,24 @@ class TimeDependentField(Field):
 
     def snapshot(self, It: int) -> torch.Tensor:
         """Device view of snapshot It."""
-        if self._cyclic:
-            return self._snapshots[It % self._num_timesteps]
-        return self._snapshots[It]
+        idx = It % self._num_timesteps if self._cyclic else It
+        if self._source is None:
+            return self._snapshots[idx]
+        if not 0 <= idx < self._num_timesteps:
+            raise IndexError(f"snapshot {It} out of range")
+        t = self._cache.get(idx)
+        if t is None:
+            host = self._source.read(idx)
+            t = torch.from_numpy(np.ascontiguousarray(host)).to(self._device, self._storage)
+            self._cache[idx] = t
+            while len(self._cache) > self.LAZY_CACHE:
+                self._cache.popitem(last=False)
+            nxt = idx + 1
+            if self._cyclic or nxt < self._num_timesteps:
+                self._source.prefetch(nxt)
+        else:
+            self._cache.move_to_end(idx)
+        return t
 
     @property
     def previous_time_slice(self) -> torch.Tensor:
@@ -259,7 +308,7 @@ class TimeDependentField(Field):
         self._It = It
 
     def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
-        It, ft, gt = blend_weights(time_index, self._snapshots.dtype)
+        It, ft, gt = blend_weights(time_index, self.dtype)
         self.set_time_index(It)
         window = BlendWindow(self.snapshot(It), self.snapshot(It + 1), ft, gt, It)
         return FieldData(window, self.active_offsets)

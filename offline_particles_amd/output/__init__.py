@@ -9,7 +9,7 @@ same rows in host memory.
 from ._linear_interpolation import linearly_interpolate_fields
 from ._memory_writer import MemoryOutputBuilder, MemoryOutputWriter
 from ._output import AbstractOutputWriter, AbstractOutputWriterBuilder, Output
-from ._zarr import ZarrArray, open_array
+from .._zarr import ZarrArray, open_array
 from ._zarr_writer import DEFAULT_CHUNKSIZE, ZarrOutputBuilder, ZarrOutputWriter
 
 __all__ = [

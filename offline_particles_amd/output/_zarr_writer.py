@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from ._output import AbstractOutputWriter, AbstractOutputWriterBuilder, Output
-from ._zarr import ZarrArray
+from .._zarr import ZarrArray
 
 DEFAULT_CHUNKSIZE = 10_000
 

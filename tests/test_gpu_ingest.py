@@ -1,0 +1,70 @@
+"""Snapshot ingest from Zarr v3 stores (snapshots.ZarrSnapshots,
+TimeDependentField.from_zarr): fields read lazily from disk give the same bits
+as fields resident in HBM, on the drop-in path (Simulation.step) and on the
+fused engine's streamed window (decode on host threads, copy on side streams)."""
+
+import numpy as np
+import pytest
+import torch
+
+from offline_particles_amd.engine import AB3Engine
+from offline_particles_amd.fields import StaticField, TimeDependentField
+from offline_particles_amd.fieldset import Fieldset
+from offline_particles_amd.snapshots import write_zarr_snapshots
+from offline_particles_amd.workloads import STAGGERS, TIME_DEPENDENT, build_fieldset, build_simulation, eddy_workload, \
+    turbulent_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _zarr_fieldset(wl, root, device, compressor=None):
+    fields = {}
+    for name, arr in wl.fields.items():
+        zs, ys, xs = STAGGERS[name]
+        if name in TIME_DEPENDENT:
+            write_zarr_snapshots(root, name, arr, compressor=compressor)
+            fields[name] = TimeDependentField.from_zarr(root, name, zs, ys, xs, cyclic=True, device=device)
+        else:
+            fields[name] = StaticField.from_numpy(arr, zs, ys, xs, device=device)
+    return Fieldset(len(wl.time_array), wl.NZ, wl.NY, wl.NX, constants={"hc": wl.hc, "NZ": wl.NZ}, **fields)
+
+
+def test_drop_in_from_zarr_equals_resident(device, tmp_path):
+    wl = eddy_workload(steps=60, nparticles=3000, dtype=np.float32)
+    fs = _zarr_fieldset(wl, str(tmp_path / "eddy.zarr"), device, compressor="zstd")
+    assert fs["u"].source is not None
+    from offline_particles_amd import SimulationBuilder
+    from offline_particles_amd.models import roms
+
+    ts = roms.rk2_w_advection_timestepper(wl.time_array, np.float64(wl.dt))
+    lazy = SimulationBuilder(ts, fs).build_simulation(wl.nparticles, device=device)
+    lazy.set_indices(*wl.positions)
+    ref = build_simulation(wl, "rk2", device=device)
+    for _ in range(60):  # 5 h of 300 s steps: crosses several snapshots
+        lazy.step()
+        ref.step()
+    for k in ("status", "zidx", "yidx", "xidx"):
+        assert np.array_equal(lazy.particles.to_numpy(k), ref.particles.to_numpy(k), equal_nan=True), k
+
+
+def test_engine_streams_from_zarr(device, tmp_path):
+    wl = turbulent_workload(NX=96, NY=80, NZ=12, nparticles=8000, steps=120, nsnap_unique=6)
+    fs_lazy = _zarr_fieldset(wl, str(tmp_path / "turb.zarr"), device)
+    fs = build_fieldset(wl, device=device)
+    res = []
+    for f, slots in ((fs, None), (fs_lazy, 3)):
+        eng = AB3Engine(f, wl.time_array, np.float64(wl.dt), device=device, capacity=wl.nparticles, rebin_every=13,
+                        stream_slots=slots)
+        eng.launch_steps = 8
+        eng.add_release(*wl.positions, nsteps=120)
+        eng.step(120)  # 4 h: the 3-slot window streams 5 snapshots
+        eng.check_clock()
+        torch.cuda.synchronize()
+        if slots:
+            assert eng.snapshot_loads >= 5
+        z, y, x = (t.cpu().numpy() for t in eng.positions())
+        res.append((z, y, x, eng.status().cpu().numpy()))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b, equal_nan=True)
+    with pytest.raises(ValueError):
+        AB3Engine(fs_lazy, wl.time_array, np.float64(wl.dt), device=device, capacity=10)  # lazy needs streaming

@@ -141,3 +141,48 @@ def test_array_reads_other_chunkings(tmp_path):
     assert np.array_equal(b[:, :, :], data)
     assert np.array_equal(b[2], data[2])
     assert np.array_equal(b[1:3, 2:5, 3:6], data[1:3, 2:5, 3:6])
+
+
+def test_zarr_snapshots_read_back_and_prefetch(tmp_path):
+    from offline_particles_amd.snapshots import ZarrSnapshots, write_zarr_snapshots
+
+    rng = np.random.default_rng(1)
+    data = rng.normal(size=(5, 4, 6, 9)).astype(np.float32)
+    write_zarr_snapshots(tmp_path / "s.zarr", "u", data, chunks=(1, 2, 4, 5))
+    write_zarr_snapshots(tmp_path / "s.zarr", "v", data, compressor="zstd")
+    for name in ("u", "v"):
+        src = ZarrSnapshots(tmp_path / "s.zarr", name)
+        assert src.shape == data.shape and src.dtype == np.float32 and len(src) == 5
+        src.prefetch(3)
+        assert np.array_equal(src.read(3), data[3])
+        assert np.array_equal(src.read(7), data[2])  # index mod T
+        assert np.array_equal(src.read_all(), data)
+
+
+def test_lazy_time_dependent_field_equals_resident(tmp_path):
+    from offline_particles_amd.fields import TimeDependentField
+    from offline_particles_amd.launcher import _NO_BBOX
+    from offline_particles_amd.snapshots import ZarrSnapshots, write_zarr_snapshots
+
+    rng = np.random.default_rng(2)
+    data = rng.normal(size=(6, 3, 5, 7)).astype(np.float32)
+    write_zarr_snapshots(tmp_path / "f.zarr", "u", data)
+    eager = TimeDependentField(data, "center", "center", "inner", cyclic=True, device="cpu")
+    lazy = TimeDependentField(ZarrSnapshots(tmp_path / "f.zarr", "u"), "center", "center", "inner", cyclic=True,
+                              device="cpu")
+    pre = TimeDependentField.from_zarr(tmp_path / "f.zarr", "u", "center", "center", "inner", preload=True,
+                                       cyclic=True, device="cpu")
+    assert lazy.source is not None and pre.source is None
+    with pytest.raises(TypeError):
+        lazy.data
+    assert lazy.dtype == eager.dtype and lazy.spatial_shape == eager.spatial_shape
+    for tidx in (0.25, 1.5, 4.75, 9.125, 2.0):
+        a, b, c = (f.get_field_data(np.float64(tidx), _NO_BBOX).array for f in (eager, lazy, pre))
+        for w in (b, c):
+            assert (a.ft, a.gt, a.It) == (w.ft, w.gt, w.It)
+            assert torch.equal(a.prev, w.prev) and torch.equal(a.next, w.next)
+    assert len(lazy._cache) <= TimeDependentField.LAZY_CACHE
+    strict = TimeDependentField(ZarrSnapshots(tmp_path / "f.zarr", "u"), "center", "center", "inner",
+                                device="cpu")
+    with pytest.raises(IndexError):
+        strict.snapshot(6)
