@@ -33,6 +33,10 @@ Forms measured in the same run:
   at a time index between snapshots, and the same six outputs from the fused
   engine's state (AB3Engine.sample, one kernel over the re-binned particles);
   reported under ``output_events``.
+- io (``--extras io``, not in the default run): the workload's snapshots staged
+  to a Zarr v3 store on local disk, the fused engine streaming them from that
+  store beside streaming from pinned host memory, and the Zarr trajectory
+  writer; reported under ``io``.
 The headline ``value`` is the fastest 16M-particle form; every form is listed
 in ``forms``.
 
@@ -54,6 +58,7 @@ import json
 import os
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -401,6 +406,90 @@ def run_output_fused(wl, device, rounds, warmup):
     del eng, fs
     torch.cuda.empty_cache()
     return res
+
+
+def run_io(wl, device, steps=60, rounds=4):
+    """The data formats either side of the path, on local disk (a temporary
+    directory): the workload's FP32 snapshots staged to a Zarr v3 store, the
+    fused engine (FP32, 16M particles, 3-slot streamed window) fed from that
+    store (host-thread decode, pinned staging, side-stream copies) beside the
+    same engine fed from pinned host memory, and the Zarr trajectory writer
+    writing rounds of three FP64 outputs of every particle."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from offline_particles_amd.engine import AB3Engine
+    from offline_particles_amd.fields import StaticField, TimeDependentField
+    from offline_particles_amd.fieldset import Fieldset
+    from offline_particles_amd.kernels.roms import compute_z_kernel
+    from offline_particles_amd.output import Output, ZarrOutputBuilder
+    from offline_particles_amd.snapshots import write_zarr_snapshots
+    from offline_particles_amd.workloads import STAGGERS, TIME_DEPENDENT
+
+    tmp = tempfile.mkdtemp(prefix="opx_io_")
+    try:
+        root = os.path.join(tmp, "snapshots.zarr")
+        t0 = time.perf_counter()
+        staged = 0
+        lazy, host = {}, {}
+        for name, arr in wl.fields.items():
+            zs, ys, xs = STAGGERS[name]
+            if name in TIME_DEPENDENT:
+                write_zarr_snapshots(root, name, arr)
+                staged += arr.nbytes
+                lazy[name] = TimeDependentField.from_zarr(root, name, zs, ys, xs, cyclic=True, device=device)
+                host[name] = TimeDependentField(arr, zs, ys, xs, cyclic=True, device="cpu")
+            else:
+                lazy[name] = host[name] = StaticField.from_numpy(arr, zs, ys, xs, device=device)
+        stage_s = time.perf_counter() - t0
+        consts = {"hc": wl.hc, "NZ": wl.NZ}
+        res = {"staged_GB": staged / 1e9, "stage_write_GBs": staged / stage_s / 1e9}
+        for label, flds in (("zarr", lazy), ("pinned_host", host)):
+            fs = Fieldset(len(wl.time_array), wl.NZ, wl.NY, wl.NX, constants=consts, **flds)
+            eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=wl.nparticles,
+                            rebin_every=64, xlow=3, stream_slots=3)
+            eng.launch_steps = 16
+            eng.add_release(*wl.positions, nsteps=steps + 16)
+            eng.rebin()
+            eng.step(16)
+            torch.cuda.synchronize()
+            loads0, bytes0 = eng.snapshot_loads, eng.snapshot_bytes
+            t0 = time.perf_counter()
+            eng.step(steps)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            res[f"engine_from_{label}"] = {
+                "particle_steps_per_s": eng.active_count() * steps / el, "steps": steps,
+                "snapshot_loads": eng.snapshot_loads - loads0,
+                "snapshot_GBs": (eng.snapshot_bytes - bytes0) / el / 1e9, "seconds": el}
+            del eng, fs
+            torch.cuda.empty_cache()
+        # trajectory writer: rounds of three FP64 outputs of every particle
+        n = wl.nparticles
+        b = ZarrOutputBuilder("zarr", os.path.join(tmp, "traj.zarr"))
+        for k in ("z", "y", "x"):
+            b.add_output(Output(k, compute_z_kernel(f"_{k}"), particle_field=f"_{k}"))
+        w = b.build(n)
+        parts = {f"_{k}": torch.from_numpy(np.ascontiguousarray(a)).to(device) for k, a in zip("zyx", wl.positions)}
+        state = types.SimpleNamespace(time=0.0, particles=parts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            state.time = float(r)
+            w.write_time(state)
+            for k in ("z", "y", "x"):
+                w.write_output(k, state)
+            w.finalise_write_round(state)
+        el = time.perf_counter() - t0
+        w.close()
+        res["trajectory_writer"] = {"rounds": rounds, "outputs_per_round": 3, "particles": n,
+                                    "GB_written": rounds * 3 * n * 8 / 1e9, "GBs": rounds * 3 * n * 8 / el / 1e9,
+                                    "seconds_per_round": el / rounds, "chunk_files_per_round": 3 * -(-n // 10_000)}
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def field_bytes_touched_fused(wl, pos, storage_bytes_3d, fields_3d=("u", "v", "w")):
@@ -897,8 +986,9 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
     ap.add_argument("--launch-steps", type=int, default=1, help="fused steps per launch (1..64)")
     ap.add_argument("--extras", default="iso,output",
-                    help="also measure the isopycnal-following AB3 drop-in (iso) and the output events "
-                         "(output) at the same workload; empty for none")
+                    help="also measure the isopycnal-following AB3 (iso), the output events (output) and, "
+                         "on request, the Zarr snapshot ingest and trajectory writer on local disk (io) at the "
+                         "same workload; empty for none")
     ap.add_argument("--output-rounds", type=int, default=10, help="timed output rounds of the output form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale reference for rocprof FETCH/WRITE_SIZE)")
@@ -969,6 +1059,10 @@ def main():
         fused["traffic"] = profiled_traffic("output_fused_fp32", args.particles, (NX, NY, NZ))
         log("output fused", fused)
         output_events = {"output_fp32": plain, "output_fused_fp32": fused}
+    io = None
+    if "io" in extras:
+        io = run_io(wl, device)
+        log("io", io)
 
     batching = None
     if args.batched:
@@ -1003,6 +1097,8 @@ def main():
             del wl_acc
 
     out = build_output(args, forms, (NX, NY, NZ), cpu, acc, batching, serving, isopycnal, output_events)
+    if io is not None:
+        out["io"] = io
     print(json.dumps(out))
 
 

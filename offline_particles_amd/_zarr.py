@@ -157,7 +157,23 @@ class ZarrArray:
             data = _zstd().decompress(data, decompressed_size=n, asbytes=True)
         if len(data) != n:
             raise ValueError(f"chunk of {len(data)} bytes, expected {n}")
-        return np.frombuffer(data, dtype=self._dtype_disk).astype(self.dtype).reshape(self.chunks)
+        return np.frombuffer(data, dtype=self._dtype_disk).astype(self.dtype, copy=False).reshape(self.chunks)
+
+    def read_chunk_into(self, idx: Sequence[int], out: np.ndarray) -> bool:
+        """Read an uncompressed, native-order chunk file straight into ``out`` (C-contiguous,
+        the chunk's shape and dtype); False if the chunk needs decoding (then nothing is read)."""
+        if self._zstd or self._dtype_disk != self.dtype or not out.flags.c_contiguous:
+            return False
+        path = self.chunk_path(idx)
+        if not os.path.exists(path):
+            out[...] = self.fill_value
+            return True
+        buf = memoryview(out.reshape(-1).view(np.uint8))
+        with open(path, "rb", buffering=0) as f:
+            got = f.readinto(buf)
+        if got != out.nbytes:
+            raise ValueError(f"chunk {path}: {got} bytes, expected {out.nbytes}")
+        return True
 
     def write_chunk(self, idx: Sequence[int], block: np.ndarray) -> None:
         """Write chunk ``idx`` from ``block`` (its part inside the array; the rest is fill)."""

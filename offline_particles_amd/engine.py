@@ -281,6 +281,8 @@ class AB3Engine:
                 for n in ("u", "v", "w", "zeta"))
             self._ring_It = [None] * self.nslots
             self._ring_ready = [None] * self.nslots
+            self._ring_shape = {n: tuple(flds[n].spatial_shape) for n in ("u", "v", "w", "zeta")}
+            self._staged = {}
             self._copy_streams = [torch.cuda.Stream(self.device) for _ in range(3)]
             self.snapshot_loads = 0
             self.snapshot_bytes = 0
@@ -589,18 +591,42 @@ class AB3Engine:
         # pair-ring windows packed from the slot's old contents are repacked when
         # next read (their packed snapshot pair no longer matches the slots)
 
+    def _stage(self, name: str, src: int):
+        """Start reading snapshot `src` of a lazily read field into pinned staging
+        memory on the source's host threads (once); returns (future, pinned tensor)."""
+        key = (name, src)
+        job = self._staged.get(key)
+        if job is None:
+            s = self._src[name]
+            pinned = torch.empty(self._ring_shape[name], dtype=self._dts[name], pin_memory=True)
+            if pinned.dtype == torch.bfloat16 or np.dtype(s.dtype) != pinned.numpy().dtype:
+                fut = s.submit(lambda: pinned.copy_(torch.from_numpy(s.read(src))))
+            else:
+                fut = s.submit(s.read_into, src, pinned.numpy())
+            job = self._staged[key] = (fut, pinned)
+        return job
+
     def _host_snapshot(self, name: str, src: int) -> torch.Tensor:
         """Snapshot `src` of a field in pinned host memory: a slice of the pinned copy,
-        or decoded from the field's SnapshotSource (the next one is then decoded
-        ahead on a host thread)."""
+        or read from the field's SnapshotSource into pinned staging memory (the
+        next snapshot is then staged ahead on a host thread)."""
         s = self._src[name]
         if not isinstance(s, SnapshotSource):
             return s[src]
-        host = torch.from_numpy(np.ascontiguousarray(s.read(src)))
-        pinned = torch.empty(host.shape, dtype=self._dts[name], pin_memory=True)
-        pinned.copy_(host)
-        s.prefetch(src + 1)
+        fut, pinned = self._stage(name, src)
+        fut.result()
+        del self._staged[(name, src)]
+        self._stage(name, (src + 1) % self.nsnap)
         return pinned
+
+    def _snapshot_staged(self, It: int) -> bool:
+        """Whether every lazily read field has snapshot It in pinned memory (starting
+        the reads that have not started)."""
+        ok = True
+        for name, s in self._src.items():
+            if isinstance(s, SnapshotSource):
+                ok &= self._stage(name, It % self.nsnap)[0].done()
+        return ok
 
     def _ensure_snapshots(self, pairs) -> None:
         """Make the snapshots of `pairs` resident; prefetch the one after them."""
@@ -623,7 +649,9 @@ class AB3Engine:
         nxt = need[-1] + 1
         slot = nxt % self.nslots
         cur = self._ring_It[slot]
-        if cur != nxt and (cur is None or cur < need[0]):
+        # the look-ahead load waits for nothing: a snapshot still being read from
+        # its source is loaded by a later call
+        if cur != nxt and (cur is None or cur < need[0]) and self._snapshot_staged(nxt):
             self._load_slot(nxt)
 
     def _pair_contents(self, s0):

@@ -43,8 +43,22 @@ class SnapshotSource:
     def __len__(self) -> int:
         return self.shape[0]
 
-    def _read(self, i: int) -> np.ndarray:  # pragma: no cover - abstract
+    def _read(self, i: int) -> np.ndarray:
+        out = np.empty(self.shape[1:], dtype=self.dtype)
+        self._read_into(i, out)
+        return out
+
+    def _read_into(self, i: int, out: np.ndarray) -> None:  # pragma: no cover - abstract
         raise NotImplementedError
+
+    def submit(self, fn, *args) -> cf.Future:
+        """Run fn(*args) on the source's host threads."""
+        return self._pool.submit(fn, *args)
+
+    def read_into(self, i: int, out: np.ndarray) -> None:
+        """Snapshot i (mod T) into ``out`` (e.g. pinned staging memory), on the calling thread."""
+        self._read_into(i % len(self), out)
+        self.reads += 1
 
     def prefetch(self, i: int) -> None:
         """Start decoding snapshot i (mod T) on a host thread if it is not already."""
@@ -72,24 +86,26 @@ class ZarrSnapshots(SnapshotSource):
 
     def __init__(self, store, name: str, *, workers: int = 8) -> None:
         self._array = ZarrArray(os.fspath(store), name)
-        super().__init__(self._array.shape, self._array.dtype, workers=2)
+        super().__init__(self._array.shape, self._array.dtype, workers=4)
         if len(self._array.shape) < 2:
             raise ValueError(f"{name}: a snapshot array needs a time axis and a spatial one")
         self._chunk_pool = cf.ThreadPoolExecutor(max_workers=max(1, int(workers)))
 
-    def _read(self, i: int) -> np.ndarray:
+    def _read_into(self, i: int, out: np.ndarray) -> None:
         a = self._array
-        out = np.empty(a.shape[1:], dtype=a.dtype)
         c0 = a.chunks[0]
-        ranges = [range(-(-n // c)) for n, c in zip(a.shape[1:], a.chunks[1:])]
+        if c0 == 1 and tuple(a.chunks[1:]) == tuple(a.shape[1:]) and out.dtype == a.dtype:
+            # one chunk per snapshot: the file goes straight into `out`
+            if a.read_chunk_into((i,) + (0,) * (len(a.shape) - 1), out.reshape((1,) + out.shape)):
+                return
+        counts = [-(-n // c) for n, c in zip(a.shape[1:], a.chunks[1:])]
 
         def one(idx):
             chunk = a.read_chunk((i // c0,) + idx)[i % c0]
             dst = tuple(slice(k * c, min((k + 1) * c, n)) for k, c, n in zip(idx, a.chunks[1:], a.shape[1:]))
             out[dst] = chunk[tuple(slice(0, s.stop - s.start) for s in dst)]
 
-        list(self._chunk_pool.map(one, np.ndindex(*[len(r) for r in ranges])))
-        return out
+        list(self._chunk_pool.map(one, np.ndindex(*counts)))
 
 
 def write_zarr_snapshots(store, name: str, data: np.ndarray, *, chunks=None, compressor: str | None = None,
