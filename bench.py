@@ -589,6 +589,9 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
         "steps_per_launch": m,
         "algo_bytes": algo_launch,
         "algo_bytes_per_step": algo,
+        # logical gather bytes: what the particles' stencils name, before any cache reuse (u, v, w:
+        # 8 nodes x 2 snapshots each; h, zeta x 2, dx, dy corners and the two C levels ~ 32 x 4 B)
+        "gather_bytes_per_particle_step": 48 * sb + 32 * 4,
         "field_bytes_touched": fbytes,
         "achieved_GBs": algo_launch / (kms * 1e-3) / 1e9,
         "rebin_every": eng.rebin_every,
@@ -778,6 +781,16 @@ def profiled_traffic(form, particles, grid, steps_per_launch=1):
 # CPU baseline: reference-structured FP64 oracle on the host cores
 # ---------------------------------------------------------------------------
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return None
+
+
 def _cpu_threads():
     try:
         ncpu = len(os.sched_getaffinity(0))
@@ -819,6 +832,7 @@ def cpu_baseline(wl, seconds, max_particles, scheme="ab3", max_steps=20):
         "value": act / el,
         "unit": "particle-steps/s",
         "cores": threads,
+        "cpu_model": _cpu_model(),
         "kind": "port",
         "sample": f"{n} particles x {steps} {what} steps of the same workload on the FP64 C/OpenMP oracle "
                   f"(full-grid time blend per launch, validation + AB3 + post per step), {el:.1f} s",
@@ -846,6 +860,7 @@ def cpu_output_baseline(wl, seconds, max_particles):
         "value": n * 4 * rounds / el,
         "unit": "particle-outputs/s",
         "cores": threads,
+        "cpu_model": _cpu_model(),
         "kind": "port",
         "sample": f"{n} particles x {rounds} rounds of compute_z + trilinear u, v, w on the FP64 C/OpenMP "
                   f"oracle (full-grid blend per output, then the particle loop), {el:.1f} s",
@@ -944,6 +959,9 @@ def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None,
                                         head.get("steps_per_launch", 1)),
             "algorithmic_bytes": head["algo_bytes"],
             "steps_per_launch": head.get("steps_per_launch", 1),
+            "logical_gather_bytes": (head["gather_bytes_per_particle_step"] * head["active_particles"]
+                                     * head.get("steps_per_launch", 1)
+                                     if "gather_bytes_per_particle_step" in head else None),
         },
         "cpu_baseline": cpu,
         "accuracy": acc,
