@@ -871,7 +871,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     const Step &rs = p.rs;
     const uint8_t st = p.st;
     Axis az = p.az, ay = p.ay, ax = p.ax;
-    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+    const double yi = ay.pos(), xi = ax.pos();
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
     const float gt = rs.gt, ft = rs.ft;
@@ -1117,7 +1117,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const Axis az = p.az;
     Axis ay = p.ay, ax = p.ax;
-    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
+    const double yi = ay.pos(), xi = ax.pos();
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
     const float gt = rs.gt, ft = rs.ft;
