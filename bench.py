@@ -544,8 +544,8 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.rebin()
     eng.step(warmup)
     if tune:
-        # steps per launch from 1, 8 and 16 (16 is the profiled launch size)
-        log("autotune:", eng.autotune(launch_steps=(1, 8, 16)))
+        # steps per launch from 1, 8, 16 and 32 (16 and 32 are the profiled launch sizes)
+        log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -753,6 +753,9 @@ _PROFILE_FOR_FORM = {
     ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 1): "k_rk2_fused_f32_turbulent16M.md",
     ("fused_rk2_fp32", 16): "k_rk2_fused_steps16_f32_turbulent16M.md",
+    ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M.md",
+    ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M.md",
+    ("fused_rk2_fp32", 32): "k_rk2_fused_steps32_f32_turbulent16M.md",
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
     ("fused_iso_fp32", 16): "k_ab3_iso_fused_steps16_turbulent16M.md",
