@@ -759,6 +759,7 @@ _PROFILE_FOR_FORM = {
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
     ("fused_iso_fp32", 16): "k_ab3_iso_fused_steps16_turbulent16M.md",
+    ("fused_iso_fp32", 32): "k_ab3_iso_fused_steps32_turbulent16M.md",
     ("output_fused_fp32", 1): "k_sample_turbulent16M.md",
 }
 
