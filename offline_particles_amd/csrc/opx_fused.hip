@@ -1228,17 +1228,11 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
 // snapshot pair its packed window lacks (only without RING_OK) stops the
 // particle and sets *err.
 template <typename T, int LAYOUT, int SCHEME, bool RING_OK>
-__device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+__device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, const opx_grid &g,
                                           const opx_window &win, const opx_release_step *__restrict__ table,
                                           int tstride, int nsteps, double dt, double alpha, const double *Cw,
-                                          int32_t *err, IsoRate iso = IsoRate{})
+                                          int32_t *err, IsoRate iso, uint8_t st0, int r)
 {
-    if (i >= n)
-        return;
-    const uint8_t st0 = s.status[i];
-    if (st0 & OPX_INACTIVE)
-        return;
-    const int r = s.rel ? (int)s.rel[i] : 0;
     PState p;
     p.st = st0;
     StencilCache<T, LAYOUT> c;
@@ -1320,6 +1314,29 @@ __device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_
     }
     if (p.st != st0)
         s.status[i] = p.st;
+}
+
+template <typename T, int LAYOUT, int SCHEME, bool RING_OK>
+__device__ __forceinline__ void fused_run(int64_t i, int64_t n, const opx_split_state &s, const opx_grid &g,
+                                          const opx_window &win, const opx_release_step *__restrict__ table,
+                                          int tstride, int nsteps, double dt, double alpha, const double *Cw,
+                                          int32_t *err, IsoRate iso = IsoRate{})
+{
+    if (i >= n)
+        return;
+    const uint8_t st0 = s.status[i];
+    if (st0 & OPX_INACTIVE)
+        return;
+    const int r = s.rel ? (int)s.rel[i] : 0;
+    // a wave whose live particles all belong to one release steps it with the
+    // release index, its table entries and window addresses in scalar registers
+    const int r0 = __builtin_amdgcn_readfirstlane(r);
+    if (__all(r == r0))
+        fused_loop<T, LAYOUT, SCHEME, RING_OK>(i, s, g, win, table, tstride, nsteps, dt, alpha, Cw, err, iso, st0,
+                                               r0);
+    else
+        fused_loop<T, LAYOUT, SCHEME, RING_OK>(i, s, g, win, table, tstride, nsteps, dt, alpha, Cw, err, iso, st0,
+                                               r);
 }
 
 // Fused AB3 steps (one, or nsteps with MULTI), one particle per thread.  The
