@@ -126,11 +126,17 @@ class ZarrOutputWriter(AbstractOutputWriter):
     def finalise_write_round(self, state) -> None:
         """Confirm that every output was written this round, then commit the round (zarr_writer.py:80-97)."""
         expected = self._output_count + 1
+        problem = None
         if self._time_count != expected:
-            raise RuntimeError(f"Time output has {self._time_count} entries, expected {expected}.")
+            problem = f"Time output has {self._time_count} entries, expected {expected}."
         for name, count in self._row_counts.items():
-            if count != expected:
-                raise RuntimeError(f"Output '{name}' has {count} time entries, expected {expected}.")
+            if problem is None and count != expected:
+                problem = f"Output '{name}' has {count} time entries, expected {expected}."
+        if problem is not None:
+            # the chunk writes already submitted this round finish before the
+            # error reaches the caller, so no host thread still writes the store
+            cf.wait(self._round)
+            raise RuntimeError(problem)
         futures, self._round = self._round, []
         arrays = [self._time_array] + [a for _, a in self._outputs.values()]
         commit = self._committer.submit(self._commit, futures, arrays, expected)
