@@ -2079,6 +2079,16 @@ int check_grid(const opx_grid *g)
                 g->xbits);
     // largest 3-D field (w: (NZ+1) NY NX) must be indexable with 32 bits
     OPX_REQUIRE((int64_t)(g->NZ + 1) * g->NY * g->NX < ((int64_t)1 << 32), "grid too large for 32-bit offsets");
+    // Stored cells span [-kBias, N+1]; a position beyond them is clamped to that
+    // range, which the domain check still flags only while every bound lies in
+    // (-kBias, N+1] on its axis.
+    const double lo[3] = {g->zidx_min, g->yidx_min, g->xidx_min};
+    const double hi[3] = {g->zidx_max, g->yidx_max, g->xidx_max};
+    const int n[3] = {g->NZ, g->NY, g->NX};
+    for (int d = 0; d < 3; ++d)
+        OPX_REQUIRE(lo[d] > -kBias && lo[d] <= n[d] + 1 && hi[d] > -kBias && hi[d] <= n[d] + 1,
+                    "index bounds [%g, %g] of axis %d lie outside (%d, %d], the range the fused state holds",
+                    lo[d], hi[d], d, -kBias, n[d] + 1);
     return OPX_OK;
 }
 

@@ -250,7 +250,15 @@ class AB3Engine:
                 raise TypeError(f"field '{name}' must be static")
         NZ, NY, NX = fs.z_size, fs.y_size, fs.x_size
         self.NZ, self.NY, self.NX = NZ, NY, NX
-        flds = {n: fs[third if n == "w" else n] for n in ("u", "v", "w", "zeta")}
+        # the split state stores cells -2 .. N+1 per axis and clamps positions
+        # beyond them; the domain check keeps flagging those only while every
+        # bound lies in (-2, N+1]
+        for axis, (lo, hi), n in (("zidx", fs.zidx_bounds, NZ), ("yidx", fs.yidx_bounds, NY),
+                                  ("xidx", fs.xidx_bounds, NX)):
+            if not all(-2.0 < float(b) <= n + 1 for b in (lo, hi)):
+                raise ValueError(f"{axis} bounds ({float(lo)}, {float(hi)}) lie outside (-2, {n + 1}], the index "
+                                 f"range the fused engine's split positions hold")
+        flds ={n: fs[third if n == "w" else n] for n in ("u", "v", "w", "zeta")}
         lazy = [n for n, f in flds.items() if f.source is not None]
         if lazy and self._stream_slots is None:
             raise ValueError(f"fields {lazy} are read lazily from their source: give stream_slots to stream them")

@@ -14,6 +14,7 @@ from ...kernels.roms import (
     ab3_post_isopycnal_following_kernel,
     ab3_post_w_advection_kernel,
     ab3_w_advection_kernel,
+    compute_z_kernel,
     rk2_w_advection_step_1_kernel,
     rk2_w_advection_step_2_kernel,
     rk2_w_advection_update_kernel,
@@ -22,6 +23,7 @@ from ...kernels.validation import validation_kernel
 from ...timesteppers import ABTimestepper, RK2Timestepper
 
 __all__ = [
+    "compute_z_kernel",
     "ab3_w_advection_kernel", "ab3_post_w_advection_kernel", "ab3_w_advection_timestepper",
     "rk2_w_advection_step_1_kernel", "rk2_w_advection_step_2_kernel", "rk2_w_advection_update_kernel",
     "rk2_w_advection_timestepper",

@@ -44,14 +44,15 @@ def _create_kwargs(kw: dict) -> dict:
         out["fill_value"] = kw["fill_value"]
     if "attributes" in kw:
         out["attributes"] = kw["attributes"]
-    if "compressors" in kw:
-        c = kw["compressors"]
-        if c in (None, (), []):
-            out["compressor"] = None
-        elif c == "zstd" or (isinstance(c, (list, tuple)) and list(c) == ["zstd"]):
-            out["compressor"] = "zstd"
-        else:
-            raise ValueError(f"compressors must be None or 'zstd', got {c!r}")
+    # zarr.create_array's default compressors="auto" gives numeric v3 arrays
+    # the zstd codec (level 0), so the default here is zstd as well
+    c = kw.get("compressors", "auto")
+    if c is None or (isinstance(c, (list, tuple)) and not c):
+        out["compressor"] = None
+    elif c in ("auto", "zstd") or (isinstance(c, (list, tuple)) and list(c) == ["zstd"]):
+        out["compressor"] = "zstd"
+    else:
+        raise ValueError(f"compressors must be None, 'auto' or 'zstd', got {c!r}")
     return out
 
 

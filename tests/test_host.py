@@ -220,3 +220,44 @@ def test_iso_fused_entry_point_rejects_bad_arguments(lib):
     assert lib.opx_ab3_fused_steps(ctypes.byref(s), ctypes.byref(g), ctypes.byref(w), p, 0, 1, 120.0, None, 128,
                                    None, None) == _lib.OPX_ERR_INVALID
     assert b"levels" in lib.opx_last_error()
+
+
+def test_fused_engine_rejects_bounds_beyond_split_range(lib):
+    """The split state stores cells -2 .. N+1 per axis and clamps positions beyond
+    them, so bounds outside (-2, N+1] would let a clamped particle pass the
+    domain check: the engine raises ValueError and the C-ABI refuses the grid."""
+    from offline_particles_amd import Fieldset, StaticField, TimeDependentField
+    from offline_particles_amd.engine import AB3Engine
+
+    NZ, NY, NX = 4, 5, 6
+    T = 3
+    td = {"u": ((NZ, NY, NX - 1), ("center", "center", "inner")),
+          "v": ((NZ, NY - 1, NX), ("center", "inner", "center")),
+          "w": ((NZ + 1, NY, NX), ("outer", "center", "center")),
+          "zeta": ((NY, NX), ("invariant", "center", "center"))}
+    st = {"h": ((NY, NX), ("invariant", "center", "center")), "dx": ((NY, NX), ("invariant", "center", "center")),
+          "dy": ((NY, NX), ("invariant", "center", "center")), "C": ((NZ + 1,), ("outer", "invariant", "invariant"))}
+
+    def fieldset(**bounds):
+        f = {k: TimeDependentField(np.zeros((T,) + s, np.float32), *g, device="cpu") for k, (s, g) in td.items()}
+        f.update({k: StaticField.from_numpy(np.ones(s, np.float32), *g, device="cpu") for k, (s, g) in st.items()})
+        return Fieldset(T, NZ, NY, NX, constants={"hc": 10.0}, **bounds, **f)
+
+    ta = np.arange(T, dtype=np.float64)
+    for bounds in ({"xidx_bounds": (-5, NX + 4)}, {"xidx_bounds": (0, NX + 2)}, {"yidx_bounds": (-2, NY - 1)},
+                   {"zidx_bounds": (0, NZ + 1.5)}):
+        with pytest.raises(ValueError, match="bounds"):
+            AB3Engine(fieldset(**bounds), ta, 0.1, device="cpu")
+
+    # the C-ABI checks the same range before any launch
+    g = _lib.opx_grid()
+    g.NZ, g.NY, g.NX = NZ, NY, NX
+    g.xbits, g.ybits, g.zbits = 4, 4, 4
+    g.hc = 10.0
+    g.zidx_max, g.yidx_max, g.xidx_max = NZ - 1, NY - 1, NX + 4
+    g.xidx_min = -5
+    s = _lib.opx_split_state()
+    assert lib.opx_split_from_f64(ctypes.byref(s), ctypes.byref(g), None, None, None, None) == _lib.OPX_ERR_INVALID
+    assert b"bounds" in lib.opx_last_error()
+    g.xidx_min, g.xidx_max = -1.5, NX + 1
+    assert lib.opx_split_from_f64(ctypes.byref(s), ctypes.byref(g), None, None, None, None) == _lib.OPX_OK

@@ -28,7 +28,7 @@ def _outputs():
 
 def test_store_layout_follows_the_v3_spec(tmp_path):
     root = str(tmp_path / "traj.zarr")
-    b = ZarrOutputBuilder("zarr", root, chunksize=4)
+    b = ZarrOutputBuilder("zarr", root, chunksize=4, array_kwargs={"compressors": None})
     b.add_output(_outputs())
     w = b.build(10)
     rows = [np.arange(10, dtype=np.float64) * (r + 1) for r in range(3)]
@@ -65,6 +65,26 @@ def test_store_layout_follows_the_v3_spec(tmp_path):
     assert a.shape == (3, 10) and np.array_equal(a[:, :], np.stack(rows))
     assert np.array_equal(a[1, 3:9], rows[1][3:9]) and a[2, 9] == rows[2][9]
     assert np.array_equal(open_array(root, "time")[:], [0.0, 100.0, 200.0])
+
+
+def test_default_codecs_are_bytes_then_zstd(tmp_path):
+    """zarr.create_array's default compressors="auto" gives a numeric v3 array
+    the codecs [bytes (little endian), zstd (level 0)]; the writer does the same
+    when array_kwargs names no compressors."""
+    root = str(tmp_path / "d.zarr")
+    b = ZarrOutputBuilder("zarr", root, chunksize=4)
+    b.add_output(_outputs())
+    w = b.build(6)
+    row = np.arange(6, dtype=np.float64)
+    w.write_time(_state(1.0, _depth=row))
+    w.write_output("depth", _state(1.0, _depth=row))
+    w.finalise_write_round(None)
+    for name in ("depth", "time"):
+        meta = json.load(open(os.path.join(root, name, "zarr.json")))
+        assert meta["codecs"] == [{"name": "bytes", "configuration": {"endian": "little"}},
+                                  {"name": "zstd", "configuration": {"level": 0, "checksum": False}}]
+    assert open(os.path.join(root, "depth", "c", "0", "0"), "rb").read()[:4] == bytes.fromhex("28b52ffd")
+    assert np.array_equal(open_array(root, "depth")[0], row)
 
 
 def test_zstd_chunks_and_fill_value(tmp_path):
