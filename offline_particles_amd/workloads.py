@@ -28,6 +28,7 @@ is long enough for ``steps`` steps of ``dt``.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -84,8 +85,25 @@ def layer_thickness(h: np.ndarray, C: np.ndarray, hc: float = HC) -> np.ndarray:
     return np.diff(z_w, axis=0)
 
 
+class _ContinuityGeometry:
+    """The snapshot-independent parts of ``_w_from_continuity`` for one grid:
+    layer thicknesses at u and v faces and the slopes of the fixed s-surfaces
+    at the NZ+1 w-levels (computed once, shared by every snapshot)."""
+
+    def __init__(self, h: np.ndarray, C: np.ndarray, dx: float, dy: float, hc: float = HC):
+        NZ = C.shape[0] - 1
+        Hz = layer_thickness(h, C, hc).astype(np.float32)
+        self.Hu = 0.5 * (Hz[:, :, 1:] + Hz[:, :, :-1])
+        self.Hv = 0.5 * (Hz[:, 1:, :] + Hz[:, :-1, :])
+        del Hz
+        sigma_w = (np.arange(NZ + 1, dtype=np.float64) - NZ) / NZ
+        Sw = ((hc * sigma_w[:, None, None] + h[None] * C[:, None, None]) / (hc + h[None])).astype(np.float32)
+        # d z_w / dy, d z_w / dx per w-level
+        self.slopes = [np.gradient((h * Sw[k]).astype(np.float32), dy, dx) for k in range(NZ + 1)]
+
+
 def _w_from_continuity(u: np.ndarray, v: np.ndarray, h: np.ndarray, C: np.ndarray, dx: float,
-                       dy: float, hc: float = HC) -> np.ndarray:
+                       dy: float, hc: float = HC, geom: _ContinuityGeometry | None = None) -> np.ndarray:
     """Physical w at the NZ+1 w-levels: omega from continuity plus u.grad(z_w).
 
     omega (the flux through sigma surfaces) is integrated upward from zero at
@@ -93,14 +111,14 @@ def _w_from_continuity(u: np.ndarray, v: np.ndarray, h: np.ndarray, C: np.ndarra
     integrated transport is divergence free it also vanishes at the surface.
     The fixed s-surfaces z_w(x, y) slope, so the vertical velocity of a fluid
     parcel is w = omega + u.grad(z_w), which equals -u.grad(h) at the seabed.
+    ``geom`` holds the grid's snapshot-independent terms (built here if absent).
     """
     NZ = C.shape[0] - 1
-    Hz = layer_thickness(h, C, hc).astype(np.float32)
+    if geom is None:
+        geom = _ContinuityGeometry(h, C, dx, dy, hc)
     NY, NX = h.shape
-    Hu = 0.5 * (Hz[:, :, 1:] + Hz[:, :, :-1])
-    Hv = 0.5 * (Hz[:, 1:, :] + Hz[:, :-1, :])
-    U = (u * Hu).astype(np.float32)
-    V = (v * Hv).astype(np.float32)
+    U = (u * geom.Hu).astype(np.float32)
+    V = (v * geom.Hv).astype(np.float32)
     div = np.zeros((NZ, NY, NX), np.float32)
     div[:, :, :-1] += U / dx
     div[:, :, 1:] -= U / dx
@@ -117,11 +135,8 @@ def _w_from_continuity(u: np.ndarray, v: np.ndarray, h: np.ndarray, C: np.ndarra
     vr = np.zeros((NZ, NY, NX), np.float32)
     vr[:, 1:-1, :] = 0.5 * (v[:, :-1, :] + v[:, 1:, :])
     vr[:, 0, :], vr[:, -1, :] = v[:, 0, :], v[:, -1, :]
-    sigma_w = (np.arange(NZ + 1, dtype=np.float64) - NZ) / NZ
-    Sw = ((hc * sigma_w[:, None, None] + h[None] * C[:, None, None]) / (hc + h[None])).astype(np.float32)
     for k in range(NZ + 1):
-        zk = (h * Sw[k]).astype(np.float32)
-        gy, gx = np.gradient(zk, dy, dx)
+        gy, gx = geom.slopes[k]
         if k == 0:
             uk, vk = ur[0], vr[0]
         elif k == NZ:
@@ -276,6 +291,7 @@ def turbulent_workload(*, NX=1024, NY=768, NZ=50, nparticles=4_000_000, steps=10
 
     us, vs, ws, zs = [], [], [], []
     scale = None
+    geom = _ContinuityGeometry(h, C, dxm, dym)
     for n in range(nsnap_unique):
         c = coef * np.exp(2j * np.pi * freq * n / nsnap_unique)
         # transport streamfunction at psi (corner) points (j + 1/2, i + 1/2)
@@ -294,15 +310,27 @@ def turbulent_workload(*, NX=1024, NY=768, NZ=50, nparticles=4_000_000, steps=10
             scale = urms / rms
         u = prof[:, None, None] * (u2 * scale).astype(np.float32)[None]
         v = prof[:, None, None] * (v2 * scale).astype(np.float32)[None]
-        w = _w_from_continuity(u, v, h, C, dxm, dym)
-        w[:, land] = 0.0
         zeta = (psi_r * scale * 1e-4 / 9.81).astype(np.float64)
         zeta[land] = 0.0
-        us.append(u.astype(dtype))
-        vs.append(v.astype(dtype))
-        ws.append(w.astype(dtype))
+        us.append(u)
+        vs.append(v)
         zs.append(zeta.astype(dtype))
-        del u, v, w
+        del u, v
+
+    # w of each snapshot from continuity, on host threads (numpy's loops run
+    # without the GIL; each snapshot's arithmetic is unchanged)
+    def w_of(n):
+        w = _w_from_continuity(us[n], vs[n], h, C, dxm, dym, geom=geom)
+        w[:, land] = 0.0
+        return w.astype(dtype)
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max(1, min(4, nsnap_unique, os.cpu_count() or 1))) as pool:
+        ws = list(pool.map(w_of, range(nsnap_unique)))
+    del geom
+    us = [u.astype(dtype) for u in us]
+    vs = [v.astype(dtype) for v in vs]
     nsnap = int(math.ceil(steps * dt / snapshot_dt)) + 3
     fields = {
         "u": np.stack(us),
