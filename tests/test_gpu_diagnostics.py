@@ -90,10 +90,21 @@ def test_zarr_writer_matches_memory_writer(device, tmp_path):
         z = ZarrOutputBuilder(f"zarr{int(mode)}", roots[mode], chunksize=1000, async_writes=mode)
         z.add_output(*outs, depth)
         b.add_output_writer(z, n=4)
-    sim = b.build_simulation(len(wl.positions[0]), device=device)
+    n = len(wl.positions[0])
+    sim = b.build_simulation(n, device=device)
+    # the oracle's own run of the same particles: its positions and time index
+    # at each write round give the rows the store must hold
+    from ._helpers import run_oracle
+
+    sim_o = run_oracle(wl, "rk2", 0)
+    rounds = [(float(sim_o.tidx), np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n, np.uint8))]
     sim.set_indices(*wl.positions)
     for _ in range(20):
         sim.step()
+        sim_o.step()
+        if sim.iteration % 4 == 0:
+            rounds.append((float(sim_o.tidx),) + tuple(sim_o.p[k].copy() for k in ("zidx", "yidx", "xidx",
+                                                                                     "status")))
     sim.output_writers["zarr1"].flush()
     exp = sim.output_writers["mem"].arrays()
     for mode, root in roots.items():
@@ -102,3 +113,12 @@ def test_zarr_writer_matches_memory_writer(device, tmp_path):
             got = open_array(root, name)
             assert got.shape == exp[name].shape, (mode, name)
             assert np.array_equal(got[:, :], exp[name]), (mode, name)
+    # the Zarr rows against the oracle's outputs at the oracle's positions
+    got = {name: open_array(roots[False], name)[:, :] for name in ("u", "zeta", "depth")}
+    assert got["u"].shape[0] == len(rounds)
+    for r, (tidx, z, y, x, st) in enumerate(rounds):
+        act = st < 128
+        for name in ("u", "zeta"):
+            assert np.array_equal(got[name][r][act], _expected(wl, name, tidx, z[act], y[act], x[act])), (name, r)
+        h, zeta, C = (_expected(wl, k, tidx, z[act], y[act], x[act]) for k in ("h", "zeta", "C"))
+        assert np.array_equal(got["depth"][r][act], od.compute_z(z[act], wl.NZ, wl.hc, h, C, zeta)), ("depth", r)

@@ -112,6 +112,37 @@ def test_releases_alone_equal_mix(device):
         off += k
 
 
+@pytest.mark.parametrize("scheme", ["ab3", "rk2"])
+def test_off_grid_start_vs_oracle(device, scheme):
+    """A release starting off the dt grid (its own FP64 clock: blend weights and
+    snapshot pairs at t0 + n dt, RK2's stage-2 time t + alpha dt), alone and
+    stepped in a mix with a release on the grid, against the oracle started at
+    the same time."""
+    wl = _turb()
+    rng = np.random.default_rng(21)
+    pos = tuple(rng.uniform(1, m - 2, 6000) for m in (wl.NZ, wl.NY, wl.NX))
+    other = tuple(rng.uniform(1, m - 2, 2000) for m in (wl.NZ, wl.NY, wl.NX))
+    t0, steps = 1234.5, 70  # crosses the snapshot boundaries at 3600 and 7200 s
+    status = 0 if scheme == "rk2" else 10
+    sim_o = run_oracle(wl, scheme, steps, positions=pos, status=np.full(6000, status, np.uint8), time=t0)
+    fs = build_fieldset(wl, device=device)
+    for mixed in (False, True):
+        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), device=device, capacity=8000, rebin_every=11,
+                        scheme=scheme)
+        eng.launch_steps = 8
+        if mixed:
+            eng.add_release(*other, start_time=0.0, nsteps=steps + 5)
+        r = eng.add_release(*pos, start_time=t0, nsteps=steps, start_step=3 if mixed else 0)
+        eng.step(steps + (3 if mixed else 0))
+        eng.check_clock()
+        sl = eng.release_slice(r)
+        g = dict(zip(POS, (t[sl].cpu().numpy() for t in eng.positions())))
+        assert np.array_equal(eng.status()[sl].cpu().numpy(), sim_o.p["status"]), (scheme, mixed)
+        devs = {k: max_dev(g[k], sim_o.p[k]) for k in POS}
+        print(f"{scheme} release at t0 = {t0} s ({'mixed' if mixed else 'alone'}): max deviation {devs}", flush=True)
+        assert max(devs.values()) <= 1e-5
+
+
 def test_clock_leaves_time_array(device):
     wl = eddy_workload(steps=4, nparticles=100, dtype=np.float32)
     eng = _engine(wl, device, nsteps=1000)

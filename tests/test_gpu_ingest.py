@@ -1,7 +1,9 @@
 """Snapshot ingest from Zarr v3 stores (snapshots.ZarrSnapshots,
 TimeDependentField.from_zarr): fields read lazily from disk give the same bits
 as fields resident in HBM, on the drop-in path (Simulation.step) and on the
-fused engine's streamed window (decode on host threads, copy on side streams)."""
+fused engine's streamed window (decode on host threads, copy on side streams);
+both runs fed from Zarr are also checked against the oracle's (bit-exact for the
+drop-in, within 1e-5 for the fused form)."""
 
 import numpy as np
 import pytest
@@ -13,6 +15,8 @@ from offline_particles_amd.fieldset import Fieldset
 from offline_particles_amd.snapshots import write_zarr_snapshots
 from offline_particles_amd.workloads import STAGGERS, TIME_DEPENDENT, build_fieldset, build_simulation, eddy_workload, \
     turbulent_workload
+
+from ._helpers import POS, max_dev, run_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +49,10 @@ def test_drop_in_from_zarr_equals_resident(device, tmp_path):
         ref.step()
     for k in ("status", "zidx", "yidx", "xidx"):
         assert np.array_equal(lazy.particles.to_numpy(k), ref.particles.to_numpy(k), equal_nan=True), k
+    # and the oracle's run on the same arrays (the drop-in is bit-exact)
+    sim_o = run_oracle(wl, "rk2", 60)
+    for k in ("status", "zidx", "yidx", "xidx"):
+        assert np.array_equal(lazy.particles.to_numpy(k), sim_o.p[k], equal_nan=True), k
 
 
 def test_engine_streams_from_zarr(device, tmp_path):
@@ -66,5 +74,11 @@ def test_engine_streams_from_zarr(device, tmp_path):
         res.append((z, y, x, eng.status().cpu().numpy()))
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b, equal_nan=True)
+    # the Zarr-fed run against the oracle: status bit-exact, positions within 1e-5
+    sim_o = run_oracle(wl, "ab3", 120)
+    assert np.array_equal(res[1][3], sim_o.p["status"])
+    dev = max(max_dev(a, sim_o.p[k]) for a, k in zip(res[1][:3], POS))
+    print(f"Zarr-fed fused AB3, 120 steps: max deviation from the oracle {dev:.3g}", flush=True)
+    assert dev <= 1e-5
     with pytest.raises(ValueError):
         AB3Engine(fs_lazy, wl.time_array, np.float64(wl.dt), device=device, capacity=10)  # lazy needs streaming

@@ -136,6 +136,31 @@ def test_serving_results_alone_equal_mix(device):
         assert req.latency >= 0
 
 
+def test_served_requests_vs_oracle_off_grid_clock(device):
+    """A service clock started off the dt grid: every served request (FP32
+    storage, streamed window) against the oracle run of its particles from the
+    request's start time."""
+    from ._helpers import max_dev, run_oracle
+
+    wl = _wl()
+    rng = np.random.default_rng(16)
+    sizes, steps, arrive = [1200, 500, 900], [50, 35, 40], [0, 7, 19]
+    reqs = [Request(*_release(wl, rng, m), nsteps=ns, arrival_step=a) for m, ns, a in zip(sizes, steps, arrive)]
+    fs = build_fieldset(wl, device="cpu")
+    srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.float32, device=device,
+                        stream_slots=3, max_particles=2000, max_releases=4, rebin_every=9, graph_steps=6,
+                        start_time=777.25)
+    srv.run(reqs, realtime=False)
+    for req in reqs:
+        assert req.start_time != round(req.start_time / wl.dt) * wl.dt
+        sim_o = run_oracle(wl, "ab3", req.nsteps, positions=(req.zidx, req.yidx, req.xidx), time=req.start_time)
+        got = {k: v.numpy() for k, v in req.result.items()}
+        assert np.array_equal(got["status"], sim_o.p["status"]), req.start_step
+        dev = max(max_dev(got[k], sim_o.p[k]) for k in POS)
+        print(f"served request of {req.size} from t = {req.start_time} s: max deviation {dev:.3g}", flush=True)
+        assert dev <= 1e-5
+
+
 def test_serving_realtime_latency(device):
     # requests may run back to back: a time axis for 12 x 80 steps
     wl = turbulent_workload(NX=96, NY=80, NZ=12, nparticles=1, steps=1200, nsnap_unique=4)

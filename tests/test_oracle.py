@@ -125,16 +125,19 @@ def test_vertical_coordinate_linear_stretching():
         assert np.allclose(back, k, rtol=0, atol=1e-9)
 
 
-def _uniform_fields(NZ, NY, NX, u_of_t, nsnap, h=100.0, w=0.0, C=None, dxm=1000.0):
-    """Spatially uniform u(t) snapshots, v = 0, uniform w, flat h, zeta = 0."""
+def _uniform_fields(NZ, NY, NX, u_of_t, nsnap, h=100.0, w=0.0, C=None, dxm=1000.0, zeta_of_k=None):
+    """Spatially uniform u(t) snapshots, v = 0, uniform w (or w given per
+    w-level), flat h, zeta = 0 (or uniform zeta_of_k(snapshot))."""
     if C is None:
         C = (np.arange(NZ + 1) - NZ) / NZ
     u = np.stack([np.full((NZ, NY, NX - 1), u_of_t(k)) for k in range(nsnap)])
     v = np.zeros((nsnap, NZ, NY - 1, NX))
-    ww = np.full((nsnap, NZ + 1, NY, NX), w)
+    w = np.asarray(w, np.float64)
+    ww = np.broadcast_to(w.reshape((-1, 1, 1)) if w.ndim else w, (nsnap, NZ + 1, NY, NX)).copy()
+    zeta = np.stack([np.full((NY, NX), 0.0 if zeta_of_k is None else zeta_of_k(k)) for k in range(nsnap)])
     from offline_particles_amd.workloads import OFFSETS
 
-    arr = {"u": u, "v": v, "w": ww, "zeta": np.zeros((nsnap, NY, NX)), "h": np.full((NY, NX), h),
+    arr = {"u": u, "v": v, "w": ww, "zeta": zeta, "h": np.full((NY, NX), h),
            "dx": np.full((NY, NX), dxm), "dy": np.full((NY, NX), dxm), "C": C}
     return {k: D.OracleField(a, OFFSETS[k], k in ("u", "v", "w", "zeta"), cyclic=False) for k, a in arr.items()}
 
@@ -228,6 +231,80 @@ def test_uniform_w_flat_bed(scheme):
     for _ in range(6):
         sim.step()
     assert np.allclose(sim.p["zidx"], z0 + 6 * NZ * w * dt / h, rtol=0, atol=1e-10)
+
+
+def _zidx_of_z(z, zeta, h, NZ):
+    """Inverse Vtransform 2 for C = sigma: S = sigma = (z - zeta) / (zeta + h)."""
+    return NZ * ((z - zeta) / (zeta + h) + 1.0) - 0.5
+
+
+def _z_of_zidx(zidx, zeta, h, NZ):
+    return zeta + (zeta + h) * ((zidx + 0.5) / NZ - 1.0)
+
+
+@pytest.mark.parametrize("scheme", ["rk2", "ab3"])
+def test_zeta_time_level_fixed_depth(scheme):
+    """Uniform zeta(t) = z0 + z1 t, w = 0, flat h, C = sigma: a particle keeps its
+    depth z while the free surface moves, so after N steps
+    zidx_N = NZ ((z - zeta(t_N)) / (zeta(t_N) + h) + 1) - 1/2.  This needs zeta at
+    t_{n+1} in the AB3 post step and in the RK2 update (zeta read at t_n there
+    would leave zidx unchanged)."""
+    NZ, NY, NX, h, dt, snap = 10, 5, 6, 100.0, 300.0, 3600.0
+    ta = np.arange(4) * snap
+    z0, z1 = 0.5, 2e-4
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4, h=h, zeta_of_k=lambda k: z0 + z1 * ta[k])
+    k0 = np.array([1.0, 4.5, 7.25])
+    sim = _sim(fields, ta, dt, scheme, (k0.copy(), np.full(3, 2.0), np.full(3, 3.0)), NZ, NY, NX,
+               status=10 if scheme == "ab3" else 0)
+    N = 20
+    for _ in range(N):
+        sim.step()
+    depth = _z_of_zidx(k0, z0, h, NZ)
+    tN = N * dt
+    expect = _zidx_of_z(depth, z0 + z1 * tN, h, NZ)
+    assert np.allclose(sim.p["zidx"], expect, rtol=0, atol=1e-10)
+    assert np.allclose(sim.p["z"], depth, rtol=0, atol=1e-10)  # z of the last step's start: the same depth
+    assert np.min(np.abs(expect - k0)) > 0.01  # the surface moved 1.2 m: zeta at t_n would leave zidx at k0
+
+
+def test_rk2_stage_2_reads_zeta_at_intermediate_time():
+    """Uniform zeta(t) = z0 + z1 t, w = b k at w-level k (so w at a particle is
+    b (zidx + 1/2)), flat h, C = sigma, u = v = 0.  One RK2 step from (zidx_n, t_n):
+    z_n = zeta_n + (zeta_n + h) sigma_n, k1 = b (zidx_n + 1/2), z* = z_n + alpha dt k1,
+    zidx* from z* with zeta(t_n + alpha dt), k2 = b (zidx* + 1/2),
+    z_{n+1} = z_n + dt (k1 / (2 alpha) + (1 - 1 / (2 alpha)) k2), zidx_{n+1} from z_{n+1}
+    with zeta(t_{n+1}).  Reading stage 2's zeta at t_n or t_{n+1} instead gives a
+    different answer (checked)."""
+    NZ, NY, NX, h, dt, snap, alpha = 10, 5, 6, 100.0, 300.0, 3600.0, 2 / 3
+    ta = np.arange(4) * snap
+    z0, z1, b = 0.5, 4e-3, 2e-4
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4, h=h, w=b * np.arange(NZ + 1),
+                             zeta_of_k=lambda k: z0 + z1 * ta[k])
+    k0 = np.array([1.0, 4.5, 6.25])
+    sim = _sim(fields, ta, dt, "rk2", (k0.copy(), np.full(3, 2.0), np.full(3, 3.0)), NZ, NY, NX, status=0)
+    N = 8
+    for _ in range(N):
+        sim.step()
+
+    def model(stage2_time):
+        k = k0.copy()
+        for n in range(N):
+            t = n * dt
+            zeta = z0 + z1 * t
+            zn = _z_of_zidx(k, zeta, h, NZ)
+            k1 = b * (k + 0.5)
+            zs = zn + alpha * dt * k1
+            ks = _zidx_of_z(zs, z0 + z1 * stage2_time(t), h, NZ)
+            k2 = b * (ks + 0.5)
+            b1 = 1.0 / (2.0 * alpha)
+            zn1 = zn + dt * (b1 * k1 + (1.0 - b1) * k2)
+            k = _zidx_of_z(zn1, z0 + z1 * (t + dt), h, NZ)
+        return k
+
+    expect = model(lambda t: t + alpha * dt)
+    assert np.allclose(sim.p["zidx"], expect, rtol=0, atol=1e-10)
+    for wrong in (lambda t: t, lambda t: t + dt):
+        assert np.max(np.abs(model(wrong) - expect)) > 1e-6
 
 
 def _iso_sim(fields, rho, ta, dt, pos, NZ, NY, NX, rho_t, w_max, rho_scale):
