@@ -16,13 +16,16 @@ Forms measured in the same run:
   picked by the engine's autotuner during warm-up, and the periodic re-binning
   is inside the timed region.
 - batching (``--batched NREL``, default 256): uneven releases (sizes log-uniform
-  1k-1M, start over 24 h, 6-72 h long, 10 km boxes) on the same grid in BF16,
-  stepped together from captured HIP graphs with the autotuned bin shape,
-  layout and re-sort cadence; reported under ``batching``.
+  1k-1M, start over 24 h, 6-72 h long, 10 km boxes) on the same grid with FP32
+  and with BF16 storage (``--mix-storage``), stepped together from captured HIP
+  graphs with the autotuned bin shape, layout and re-sort cadence; one release
+  of each mix (the one nearest 1000 steps, 5000 of its particles) is checked
+  against the oracle from its own start time; reported under ``batching``.
 - serving (``--serve SECONDS``, default 5): Poisson arrivals at 20/s for that
-  many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 BF16
-  grid streamed through a 3-snapshot window, admission by free HBM; every
-  request runs to completion; p50/p99 latency under ``serving``.
+  many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 grid
+  (FP32 and BF16 storage, the same requests) streamed through a 3-snapshot
+  window, admission by free HBM; every request runs to completion; p50/p99
+  latency under ``serving``.
 - isopycnal (``--extras iso``): the isopycnal-following AB3 on the same
   workload with a stratified, drifting density field, as the drop-in
   (validation, ab3_isopycnal_following, post step) and as the fused engine
@@ -37,15 +40,22 @@ Forms measured in the same run:
   to a Zarr v3 store on local disk, the fused engine streaming them from that
   store beside streaming from pinned host memory, and the Zarr trajectory
   writer; reported under ``io``.
-The headline ``value`` is the fastest 16M-particle form; every form is listed
-in ``forms``.
+The headline ``value`` is the fastest 16M-particle AB3 form whose positions
+after 1000 steps stay within 1e-3 grid-index units of the FP64 oracle (the
+accuracy run; without one, FP32-storage forms only); every form is listed in
+``forms`` with its deviation.
 
 Prints ONE JSON line.  ``value`` is particle-steps/s of the headline form with
 all inputs resident in HBM; particle-steps count the particles active during
-the timed steps (average of the active counts before and after).
+the timed steps (average of the active counts before and after).  The fused
+forms' timed window is the requested steps rounded up to whole re-sort cycles
+of whole launches (``config.timed_steps``), starting with a re-bin.
 ``roofline`` is computed for the dominant kernel from algorithmic bytes
 (compulsory particle state + distinct field nodes touched) over its average
-duration measured with HIP events on its stream.  The CPU leg times the
+duration measured with HIP events on its stream; ``roofline_valu`` gives the
+several-step kernels' VALU issue rate (the instruction count per wave-step
+from the committed counter summary, over the live duration), which is what
+bounds them.  The CPU leg times the
 reference-structured FP64 CPU oracle on the host cores over a bounded sample
 (``cpu_baseline``) and runs it as the checker of the GPU forms' positions
 after 1000 steps on a 100k-particle subsample (``accuracy``).
@@ -67,6 +77,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue ceiling: 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock, one
+# wave64 VALU instruction per SIMD every 4 cycles (16 lanes per SIMD; the
+# counter SQ_ACTIVE_INST_VALU counts these 4-cycle slots)
+VALU_PEAK_INSTS = 256 * 4 * 2.4e9 / 4
+# accuracy bar of the headline: max position deviation from the FP64 oracle
+# after the accuracy run's steps (1000), in grid-index units
+ACCURACY_BAR = 1e-3
 
 
 def log(*a):
@@ -554,10 +571,18 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     fbytes = field_bytes_touched_fused(wl, [z[act], y[act], x[act]], sb,
                                        ("u", "v", "rho") if iso else ("u", "v", "w"))
     del z, y, x, act
+    # time whole re-sort cycles of whole launches: the requested steps rounded
+    # up to a multiple of the re-bin cadence (itself whole launches), starting
+    # with a re-bin, so the window holds the re-bins at their tuned rate
+    m_tuned = max(1, eng.launch_steps)
+    cycle = eng.rebin_every if eng.rebin_every > 0 else m_tuned
+    timed = cycle * max(1, -(-steps // cycle))
+    if eng.rebin_every > 0:
+        eng._since_rebin = eng.rebin_every
     eng.timer = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    eng.step(steps)
+    eng.step(timed)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     # per launch: the launches that took the full step count (a re-bin can cut one short)
@@ -581,8 +606,10 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
         entry += "_steps"
     res = {
         "form": name,
-        "particle_steps_per_s": actn * steps / elapsed,
-        "ms_per_step": 1e3 * elapsed / steps,
+        "particle_steps_per_s": actn * timed / elapsed,
+        "ms_per_step": 1e3 * elapsed / timed,
+        "timed_steps": timed,
+        "rebins_timed": timed // eng.rebin_every if eng.rebin_every > 0 else 0,
         "active_particles": actn,
         "kernel": f"{kname}<{'bf16' if sb == 2 else 'f32'}> ({entry})",
         "kernel_ms": kms,
@@ -606,7 +633,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
 
 def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_size=2_000_000, graph_steps=8,
-                tune=True, max_particles=256_000_000):
+                tune=True, max_particles=256_000_000, storage=None, wl=None, reqs=None):
     """Serving form: Poisson arrivals (``rate``/s for ``seconds`` s), sizes log-uniform on
     [1k, max_size], 1-10 simulated days (720-7200 steps of 120 s), BF16 fields streamed
     through a 3-slot window, admission by free HBM; every request runs to completion."""
@@ -617,22 +644,28 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
     from offline_particles_amd.workloads import build_fieldset, turbulent_workload
 
     NX, NY, NZ = grid
+    storage = torch.bfloat16 if storage is None else storage
     t = time.perf_counter()
     days = 10
-    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1, steps=int(days * 86400 / 120) * 2 + 40,
-                            nsnap_unique=nsnap, dtype=np.float32)
-    reqs = poisson_requests(wl, rate=rate, duration=seconds, size_range=(1_000, max_size), steps_range=(720, 7200))
+    if wl is None:
+        wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1, steps=int(days * 86400 / 120) * 2 + 40,
+                                nsnap_unique=nsnap, dtype=np.float32)
+    if reqs is None:
+        reqs = poisson_requests(wl, rate=rate, duration=seconds, size_range=(1_000, max_size),
+                                steps_range=(720, 7200))
+    else:
+        for r in reqs:  # fresh copies of the same requests (the server fills in results)
+            r.release, r.start_step, r.result = -1, -1, None
     log(f"serving: {len(reqs)} requests, {sum(r.size for r in reqs)} particles, "
         f"{sum(r.size * r.nsteps for r in reqs):.3e} particle-steps; workload in {time.perf_counter() - t:.1f}s")
     t = time.perf_counter()
-    fs = build_fieldset(wl, device="cpu", storage_dtype=torch.bfloat16)
-    del wl.fields["u"], wl.fields["v"], wl.fields["w"]
+    fs = build_fieldset(wl, device="cpu", storage_dtype=storage)
     tuning = None
     layout, xlow, block, rebin_every, launch_steps = None, 4, 128, 40, 1
     if tune:
         # autotune on a probe: the 40 largest requests started together
         probe = sorted(reqs, key=lambda r: -r.size)[:40]
-        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
+        eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                         capacity=sum(r.size for r in probe), max_releases=64, stream_slots=3)
         for r in probe:
             eng.add_release(r.zidx, r.yidx, r.xidx, nsteps=400)
@@ -647,18 +680,18 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
         launch_steps = tuning["launch_steps"]
         del eng
         torch.cuda.empty_cache()
-    srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=torch.bfloat16, device=device,
+    srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                         stream_slots=3, max_releases=4096, rebin_every=rebin_every, xlow=xlow, layout=layout,
                         block=block, graph_steps=graph_steps, max_particles=max_particles,
                         launch_steps=launch_steps)
     log(f"serving: server ready in {time.perf_counter() - t:.1f}s, capacity {srv.capacity} particles")
     srv.run(reqs)
     res = dict(srv.stats)
-    res.update({"form": "served_bf16", "grid": [NX, NY, NZ], "arrival_rate_per_s": rate, "arrival_seconds": seconds,
-                "tuning": tuning})
+    res.update({"form": "served_" + ("bf16" if storage == torch.bfloat16 else "fp32"), "grid": [NX, NY, NZ],
+                "arrival_rate_per_s": rate, "arrival_seconds": seconds, "tuning": tuning})
     del srv, fs
     torch.cuda.empty_cache()
-    return res
+    return res, wl, reqs
 
 
 def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 50)):
@@ -710,10 +743,12 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     t0 = time.perf_counter()
     nxt = 0
     g = 0
+    rel_ids = {}
     while g < last:
         while nxt < nrel and start_step[order[nxt]] <= g:
             r = rels[order[nxt]]
-            eng.add_release(*r.positions, start_time=r.start_time, nsteps=r.nsteps, start_step=start_step[order[nxt]])
+            rel_ids[int(order[nxt])] = eng.add_release(*r.positions, start_time=r.start_time, nsteps=r.nsteps,
+                                                       start_step=start_step[order[nxt]])
             nxt += 1
         upcoming = start_step[order[nxt]] if nxt < nrel else last
         k = max(1, min(upcoming, last) - g)
@@ -723,6 +758,16 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     elapsed = time.perf_counter() - t0
     eng.check_clock()
     exited = float((eng.status() >= 128).float().mean().item())
+    # the release whose run is nearest 1000 steps, read back for the accuracy check
+    # against the oracle (its first 5000 particles)
+    pick = int(np.argmin([abs(r.nsteps - 1000) for r in rels]))
+    sl = eng.release_slice(rel_ids[pick])
+    sample = {"release": pick, "particles": min(5000, rels[pick].size),
+              "start_time": rels[pick].start_time, "nsteps": rels[pick].nsteps}
+    n_s = sample["particles"]
+    pos = [t_[sl][:n_s].cpu().numpy() for t_ in eng.positions()]
+    sample["result"] = dict(zip(("zidx", "yidx", "xidx"), pos), status=eng.status()[sl][:n_s].cpu().numpy())
+    sample["positions"] = tuple(a[:n_s] for a in rels[pick].positions)
     res = {
         "form": "batched_" + ("bf16" if storage == torch.bfloat16 else "fp32"),
         "releases": nrel,
@@ -737,7 +782,37 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     }
     del eng, fs
     torch.cuda.empty_cache()
-    return res
+    return res, sample, wl
+
+
+def batched_accuracy(wl, samples):
+    """Max / RMS deviation of one release of the batching mix (read back from the mix)
+    from the FP64 oracle run of the same particles from the release's start time
+    over its steps; one oracle run serves every storage form."""
+    from oracle import OracleField, OracleSimulation
+    from offline_particles_amd.workloads import OFFSETS, TIME_DEPENDENT
+
+    s0 = next(iter(samples.values()))
+    t = time.perf_counter()
+    fields = {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in wl.fields.items()}
+    n = s0["particles"]
+    sim = OracleSimulation(fields, wl.time_array, wl.dt, scheme="ab3", nparticles=n, hc=wl.hc, NZ=wl.NZ,
+                           bounds=wl.bounds, threads=_cpu_threads(), time=s0["start_time"])
+    for k, a in zip(("zidx", "yidx", "xidx"), s0["positions"]):
+        sim.p[k][:] = a
+    sim.p["status"][:] = 10
+    sim.run(s0["nsteps"])
+    log(f"batched accuracy: oracle {n} x {s0['nsteps']} steps in {time.perf_counter() - t:.1f}s")
+    out = {}
+    for name, smp in samples.items():
+        g = smp["result"]
+        both = (g["status"] < 128) & (sim.p["status"] < 128)
+        d = np.stack([np.abs(g[k] - sim.p[k]) for k in ("zidx", "yidx", "xidx")])[:, both].max(axis=0)
+        out[name] = {"release": smp["release"], "particles": n, "steps": smp["nsteps"],
+                     "start_time": smp["start_time"], "max": float(d.max()) if d.size else 0.0,
+                     "rms": float(np.sqrt((d**2).mean())) if d.size else 0.0,
+                     "status_mismatch": int((g["status"] != sim.p["status"]).sum())}
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -764,21 +839,74 @@ _PROFILE_FOR_FORM = {
 }
 
 
-def profiled_traffic(form, particles, grid, steps_per_launch=1):
-    """HBM bytes per launch of the form's dominant kernel from its committed rocprofv3
-    summary, when this run is that profile's workload (16M particles, 1024x768x50,
-    the same steps per launch)."""
-    key = (form, steps_per_launch)
-    if particles != 16_000_000 or tuple(grid) != (1024, 768, 50) or key not in _PROFILE_FOR_FORM:
-        return None
-    path = os.path.join(ROOT, "profiles", _PROFILE_FOR_FORM[key])
+def _profile_file(table, form, particles, grid, steps_per_launch):
+    """The committed summary for (form, steps per launch) at the profiled workload
+    (16M particles, 1024x768x50): the exact launch size, else the nearest
+    profiled one of the form.  Returns (path, profiled steps per launch)."""
+    if particles != 16_000_000 or tuple(grid) != (1024, 768, 50):
+        return None, None
+    sizes = [m for f, m in table if f == form]
+    if not sizes:
+        return None, None
+    m = min(sizes, key=lambda k: (abs(np.log(k) - np.log(max(1, steps_per_launch))), -k))
+    return os.path.join(ROOT, "profiles", table[(form, m)]), m
+
+
+def _profile_value(path, key):
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         return None
     return None
+
+
+def profiled_traffic(form, particles, grid, steps_per_launch=1):
+    """HBM bytes per launch of the form's dominant kernel from its committed rocprofv3
+    summary at this workload (16M particles, 1024x768x50).  A launch size with no
+    summary of its own takes the per-step traffic of the nearest profiled size of
+    the form times its steps (the state is read and written once per launch, so
+    this slightly overstates longer launches and understates shorter ones)."""
+    path, m = _profile_file(_PROFILE_FOR_FORM, form, particles, grid, steps_per_launch)
+    if path is None:
+        return None
+    v = _profile_value(path, "traffic_bytes_per_launch")
+    return None if v is None else v * steps_per_launch / m
+
+
+def profiled_traffic_source(form, particles, grid, steps_per_launch=1):
+    path, m = _profile_file(_PROFILE_FOR_FORM, form, particles, grid, steps_per_launch)
+    if path is None:
+        return None
+    src = os.path.relpath(path, ROOT)
+    return src if m == steps_per_launch else f"{src} (per step, {m}-step launches, scaled to {steps_per_launch})"
+
+
+# (form, steps per launch) -> shader-counter summary holding valu_insts_per_wave_step
+_COUNTERS_FOR_FORM = {
+    ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M_counters.md",
+    ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M_counters.md",
+    ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M_counters.md",
+}
+
+
+def valu_roofline(form, particles, grid, steps_per_launch, active, kernel_ms):
+    """VALU issue rate of the several-step kernel: the wave64 VALU instructions per
+    wave-step that rocprofv3 counted (SQ_INSTS_VALU / (SQ_WAVES x steps)) for this
+    form, times the wave-steps of the live launch, over its HIP-event duration;
+    peak = 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction."""
+    path, m = _profile_file(_COUNTERS_FOR_FORM, form, particles, grid, steps_per_launch)
+    if path is None or not kernel_ms or kernel_ms != kernel_ms:
+        return None
+    x = _profile_value(path, "valu_insts_per_wave_step")
+    if x is None:
+        return None
+    achieved = x * (active / 64.0) * steps_per_launch / (kernel_ms * 1e-3)
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_INSTS,
+            "unit": "wave64 VALU instructions/s", "frac": achieved / VALU_PEAK_INSTS,
+            "valu_insts_per_wave_step": x, "source": os.path.relpath(path, ROOT)
+            + ("" if m == steps_per_launch else f" ({m}-step launches)")}
 
 
 # ---------------------------------------------------------------------------
@@ -786,21 +914,38 @@ def profiled_traffic(form, particles, grid, steps_per_launch=1):
 # ---------------------------------------------------------------------------
 
 def _cpu_model():
+    """Model name plus family / model / stepping and the logical CPUs the host has."""
+    info = {}
+    ncpu = 0
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "processor":
+                ncpu += 1
+            elif k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
     except OSError:  # pragma: no cover
-        pass
-    return None
+        return None
+    if not info:
+        return None
+    return (f"{info.get('model name', '?')} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}; {ncpu} logical CPUs on the host, "
+            f"{len(os.sched_getaffinity(0))} in this process's affinity)")
 
 
 def _cpu_threads():
+    """Host threads for the CPU legs: the cores this process may run on
+    (sched_getaffinity), capped by OMP_NUM_THREADS when the machine sets it (the
+    GPU host gives one GPU's job a share of 16 cores and sets it to 16)."""
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         ncpu = os.cpu_count() or 1
-    return max(1, min(16, ncpu))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        ncpu = min(ncpu, int(omp))
+    return max(1, ncpu)
 
 
 def _oracle_sim(wl, n, scheme, threads):
@@ -882,7 +1027,7 @@ def accuracy(wl, device, nsub, steps):
     pos = tuple(a[:nsub] for a in wl.positions)
     t = time.perf_counter()
     fields = {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in wl.fields.items()}
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = _cpu_threads()
     sim = OracleSimulation(fields, wl.time_array, wl.dt, scheme="ab3", nparticles=nsub, hc=wl.hc, NZ=wl.NZ,
                            bounds=wl.bounds, threads=threads)
     for k, a in zip(("zidx", "yidx", "xidx"), pos):
@@ -920,17 +1065,48 @@ def accuracy(wl, device, nsub, steps):
     return out
 
 
+_DTYPE = {
+    "plain_fp32": ("f64", "FP32 storage of every field; FP64 positions, blends, interpolation and vertical "
+                          "transform (the drop-in's arithmetic)"),
+    "fused_fp32": ("f32", "FP32 storage of u, v, w; FP32 interpolation of u and v; FP64 blends of w, h, zeta, "
+                          "FP64 vertical transform and clocks; positions as packed cells plus 32-bit fixed-point "
+                          "offsets"),
+    "fused_bf16": ("bf16", "BF16 storage of u, v, w, widened to FP32 for interpolation; otherwise as fused_fp32"),
+}
+
+
+def _meets_bar(name, acc):
+    """Whether a form's accuracy run met the bar; without one, only FP32-storage forms
+    qualify (BF16 storage misses the bar on this flow, see the accuracy report)."""
+    if acc is None:
+        return "bf16" not in name
+    a = acc.get(name)
+    return a is not None and a["max"] <= ACCURACY_BAR
+
+
 def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None, output_events=None) -> dict:
-    """The JSON line: headline = the fastest AB3 form at the 16M-particle workload."""
+    """The JSON line: headline = the fastest AB3 form at the 16M-particle workload
+    among those whose accuracy run meets the bar."""
     NX, NY, NZ = grid
-    ab3 = [f for k, f in forms.items() if "rk2" not in k] or list(forms.values())
+    for k, f in forms.items():
+        if acc is not None and k in acc:
+            f["max_dev_vs_oracle"] = acc[k]["max"]
+            f["meets_accuracy_bar"] = acc[k]["max"] <= ACCURACY_BAR
+    ab3_all = [f for k, f in forms.items() if "rk2" not in k]
+    ab3 = [f for f in ab3_all if _meets_bar(f["form"], acc)]
+    bar_met = bool(ab3)
+    if not ab3:
+        ab3 = ab3_all or list(forms.values())
     if not ab3:
         # no 16M form was asked for: the batching form heads the line
-        if batching is None:
+        if not batching:
             raise SystemExit("bench.py: no form was measured (give --forms or --batched)")
-        ab3 = [dict(batching, kernel="k_ab3_fused (batched releases)", ms_per_step=float("nan"), achieved_GBs=None,
+        b = next(iter(batching.values()))
+        ab3 = [dict(b, kernel="k_ab3_fused (batched releases)", ms_per_step=float("nan"), achieved_GBs=None,
                     algo_bytes=None)]
     head = max(ab3, key=lambda f: f["particle_steps_per_s"])
+    m = head.get("steps_per_launch", 1)
+    dtype, detail = _DTYPE.get(head["form"], ("f32", None))
 
     out = {
         "metric": "particle-steps/s (16M, AB3) plain vs batched/BF16; p50/p99 release latency",
@@ -943,7 +1119,8 @@ def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head["form"].startswith("plain") else "f32",
+        "dtype": dtype,
+        "dtype_detail": detail,
         "data": "synthetic (seeded turbulent ROMS fields, uniform random particles)",
         "config": {
             "workload": f"{'RK2' if 'rk2' in head['form'] else 'AB3'} dt=120s, {args.particles} particles, "
@@ -951,6 +1128,11 @@ def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None,
             "form": head["form"],
             "particles": args.particles,
             "grid": [NX, NY, NZ],
+            "timed_steps": head.get("timed_steps", args.steps),
+            "headline_rule": (f"fastest AB3 form whose max deviation from the FP64 oracle after "
+                              f"{acc['steps'] if acc else '-'} steps is <= {ACCURACY_BAR:g} grid-index units"
+                              + ("" if acc else " (no accuracy run: FP32-storage forms only)")
+                              + ("" if bar_met else "; NO form met it, fastest form shown")),
         },
         "roofline": {
             "bound": "hbm",
@@ -959,14 +1141,15 @@ def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": head["achieved_GBs"] / HBM_PEAK_GBS if head["achieved_GBs"] is not None else None,
-            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ),
-                                        head.get("steps_per_launch", 1)),
+            "traffic": profiled_traffic(head["form"], args.particles, (NX, NY, NZ), m),
+            "traffic_source": profiled_traffic_source(head["form"], args.particles, (NX, NY, NZ), m),
             "algorithmic_bytes": head["algo_bytes"],
-            "steps_per_launch": head.get("steps_per_launch", 1),
-            "logical_gather_bytes": (head["gather_bytes_per_particle_step"] * head["active_particles"]
-                                     * head.get("steps_per_launch", 1)
+            "steps_per_launch": m,
+            "logical_gather_bytes": (head["gather_bytes_per_particle_step"] * head["active_particles"] * m
                                      if "gather_bytes_per_particle_step" in head else None),
         },
+        "roofline_valu": (valu_roofline(head["form"], args.particles, (NX, NY, NZ), m, head["active_particles"],
+                                        head.get("kernel_ms")) if m > 1 else None),
         "cpu_baseline": cpu,
         "accuracy": acc,
         "batching": batching,
@@ -976,6 +1159,11 @@ def build_output(args, forms, grid, cpu, acc, batching, serving, isopycnal=None,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in forms.items()},
     }
+    for k, f in out["forms"].items():
+        sl = f.get("steps_per_launch", 1)
+        if sl > 1:
+            f["roofline_valu"] = valu_roofline(k, args.particles, (NX, NY, NZ), sl, forms[k]["active_particles"],
+                                               forms[k].get("kernel_ms"))
     return out
 
 
@@ -1002,6 +1190,8 @@ def main():
     ap.add_argument("--batched", type=int, default=256, metavar="NREL",
                     help="also run the batching form with NREL releases (256 for the full case)")
     ap.add_argument("--batched-max-size", type=int, default=1_000_000)
+    ap.add_argument("--mix-storage", default="fp32,bf16",
+                    help="field storage of the batching and serving forms (comma-separated: fp32, bf16)")
     ap.add_argument("--serve", type=float, default=5.0, metavar="SECONDS",
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
@@ -1030,8 +1220,10 @@ def main():
     device = torch.device("cuda", 0)
     NX, NY, NZ = (int(s) for s in args.grid.split("x"))
     t = time.perf_counter()
-    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles, steps=args.steps + args.warmup + 600,
-                            dtype=np.float32)
+    # time axis: warm-up, the autotuner's trial steps (~350) and the timed window
+    # (the requested steps rounded up to whole re-sort cycles of at most 400 steps)
+    wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles,
+                            steps=args.steps + args.warmup + 1000, dtype=np.float32)
     log(f"workload generated in {time.perf_counter() - t:.1f}s: grid {NX}x{NY}x{NZ}, {args.particles} particles")
 
     if args.copy_calibration:
@@ -1086,15 +1278,27 @@ def main():
         io = run_io(wl, device)
         log("io", io)
 
+    storages = {"fp32": torch.float32, "bf16": torch.bfloat16}
     batching = None
+    samples, wl_b = {}, None
     if args.batched:
-        batching = run_batched(device, args.batched, args.batched_max_size, torch.bfloat16, args.graph_steps)
-        log("batched", batching)
+        batching = {}
+        for st in args.mix_storage.split(","):
+            res, samples["batched_" + st], wl_b = run_batched(device, args.batched, args.batched_max_size,
+                                                             storages[st], args.graph_steps)
+            batching[res["form"]] = res
+            log("batched", res)
     serving = None
     if args.serve > 0:
         sg = tuple(int(v) for v in args.serve_grid.lower().split("x"))
-        serving = run_serving(device, args.serve, grid=sg, graph_steps=args.graph_steps, tune=not args.no_tune)
-        log("serving", serving)
+        serving = {}
+        wl_s = reqs = None
+        for st in args.mix_storage.split(","):
+            res, wl_s, reqs = run_serving(device, args.serve, grid=sg, graph_steps=args.graph_steps,
+                                          tune=not args.no_tune, storage=storages[st], wl=wl_s, reqs=reqs)
+            serving[res["form"]] = res
+            log("serving", res)
+        del wl_s, reqs
     # CPU leg: the FP64 oracle is timed on a bounded sample of the workload
     # (the baseline), and run as the checker of every GPU form's positions on
     # a subsample over 1000 steps (the accuracy report)
@@ -1112,11 +1316,17 @@ def main():
             log("output cpu baseline:", output_events["cpu_baseline"])
         if args.accuracy:
             wl_acc = wl
-            if args.accuracy_steps + 5 > args.steps + args.warmup + 5:
+            if wl.time_array[-1] - wl.time_array[0] < (args.accuracy_steps + 5) * wl.dt:
                 wl_acc = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.accuracy,
                                             steps=args.accuracy_steps + 5, dtype=np.float32)
             acc = accuracy(wl_acc, device, args.accuracy, args.accuracy_steps)
             del wl_acc
+        if samples:
+            for name, a in batched_accuracy(wl_b, samples).items():
+                batching[name]["accuracy"] = a
+                batching[name]["meets_accuracy_bar"] = a["max"] <= ACCURACY_BAR
+            log("batched accuracy:", {k: v.get("accuracy") for k, v in batching.items()})
+    del wl_b, samples
 
     out = build_output(args, forms, (NX, NY, NZ), cpu, acc, batching, serving, isopycnal, output_events)
     if io is not None:

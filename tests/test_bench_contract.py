@@ -20,11 +20,18 @@ def _args(**kw):
 def test_headline_is_fastest_ab3_form_and_keys_present():
     forms = {"plain_fp32": _form("plain_fp32", 1e9), "fused_fp32": _form("fused_fp32", 1.7e10),
              "fused_bf16": _form("fused_bf16", 1.9e10), "fused_rk2_fp32": _form("fused_rk2_fp32", 5e10)}
-    out = bench.build_output(_args(), forms, (1024, 768, 50), {"value": 3e7, "unit": "particle-steps/s",
-                                                               "cores": 16, "kind": "port", "sample": "s"},
-                             None, None, None)
+    cpu = {"value": 3e7, "unit": "particle-steps/s", "cores": 16, "kind": "port", "sample": "s"}
+    # with an accuracy run, the headline is the fastest AB3 form within the bar
+    acc = {"steps": 1000, "plain_fp32": {"max": 0.0}, "fused_fp32": {"max": 7e-6}, "fused_bf16": {"max": 0.98}}
+    out = bench.build_output(_args(), forms, (1024, 768, 50), cpu, acc, None, None)
     json.dumps(out)
-    assert out["value"] == 1.9e10 and out["config"]["form"] == "fused_bf16"  # RK2 is not the headline
+    assert out["value"] == 1.7e10 and out["config"]["form"] == "fused_fp32"  # BF16 misses the bar, RK2 is not AB3
+    assert out["forms"]["fused_bf16"]["meets_accuracy_bar"] is False
+    acc["fused_bf16"]["max"] = 5e-4
+    assert bench.build_output(_args(), forms, (1024, 768, 50), cpu, acc, None, None)["config"]["form"] == "fused_bf16"
+    # without one, only FP32-storage forms qualify
+    out = bench.build_output(_args(), forms, (1024, 768, 50), cpu, None, None, None)
+    assert out["config"]["form"] == "fused_fp32"
     for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
               "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
         assert k in out, k
@@ -51,8 +58,20 @@ def test_layout_names_match_engine():
     assert bench.LAYOUTS == {name: value for value, name in LAYOUT_NAMES.items()}
 
 
+def test_traffic_scaled_from_nearest_profiled_launch_and_valu_roofline():
+    f = _form("fused_fp32", 4e10)
+    f.update(steps_per_launch=28, kernel_ms=8.0)
+    out = bench.build_output(_args(), {"fused_fp32": f}, (1024, 768, 50), None, None, None, None)
+    r = out["roofline"]
+    t32 = bench.profiled_traffic("fused_fp32", 16_000_000, (1024, 768, 50), 32)
+    assert r["traffic"] == t32 * 28 / 32 and "scaled to 28" in r["traffic_source"]  # nearest profiled: 32
+    v = out["roofline_valu"]
+    if v is not None:  # needs the committed counter summary of the form
+        assert v["bound"] == "valu" and 0 < v["frac"] and v["peak"] == bench.VALU_PEAK_INSTS
+
+
 def test_batched_only_run_heads_with_the_batching_form():
-    batching = {"form": "batched_bf16", "particle_steps_per_s": 3.5e10, "releases": 256}
+    batching = {"batched_bf16": {"form": "batched_bf16", "particle_steps_per_s": 3.5e10, "releases": 256}}
     out = bench.build_output(_args(), {}, (1024, 768, 50), None, None, batching, None)
     assert out["value"] == 3.5e10 and out["config"]["form"] == "batched_bf16"
     json.dumps(out)
