@@ -472,6 +472,18 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
               const opx_release_clock *clocks, int32_t nrel, void *work, uint64_t work_bytes, int32_t *n_active,
               int32_t *n_live, void *stream);
 
+/* opx_rebin without the `work` argument: the keys, values and sort storage live
+ * in a static 64 MiB device buffer of the library (one per device, held until
+ * the process ends), so the call allocates nothing.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Refuses, with OPX_ERR_INVALID, a
+ * state whose opx_rebin_workspace_bytes(n) exceeds 64 MiB (16 bytes per
+ * particle plus the radix sort's temporary storage: about 3.9 million
+ * particles).  Calls that overlap on one device share the buffer and race. */
+int opx_rebin_noalloc(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+                      const opx_release_clock *clocks, int32_t nrel, int32_t *n_active, int32_t *n_live,
+                      void *stream);
+
 /* Results of one release: every particle i < s->n with rel[i] == r writes its
  * FP64 indices (NaN for a non-finite index) and status at position
  * pid[i] - pid_base (uint32 arithmetic), i.e. the particles of a release

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "opx.h"
 #include "opx_numerics.hpp"
@@ -2273,6 +2274,107 @@ int launch_rk2(const opx_split_state *s, const opx_grid *g, const opx_window &wi
 }
 
 } // namespace
+
+namespace {
+
+// ---- re-bin body shared by opx_rebin (caller's workspace) and
+// opx_rebin_noalloc (the static scratch buffer below)
+
+int check_rebin_args(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+                     const opx_release_clock *clocks, int32_t nrel, const int32_t *n_active)
+{
+    int rc;
+    if ((rc = check_state(src, "opx_rebin src"))) return rc;
+    if ((rc = check_state(dst, "opx_rebin dst"))) return rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(dst->n == src->n, "src/dst sizes differ");
+    OPX_REQUIRE(n_active != nullptr, "null n_active");
+    OPX_REQUIRE(xlow >= 0 && xlow <= g->xbits, "xlow %d out of [0, %d]", xlow, g->xbits);
+    OPX_REQUIRE(clocks == nullptr || nrel >= 0, "negative release count");
+    OPX_REQUIRE((src->pid == nullptr) == (dst->pid == nullptr) && (src->rel == nullptr) == (dst->rel == nullptr) &&
+                    (src->z == nullptr) == (dst->z == nullptr) && (src->rho_t == nullptr) == (dst->rho_t == nullptr),
+                "src/dst optional arrays differ");
+    return OPX_OK;
+}
+
+// `work` holds at least opx_rebin_workspace_bytes(src->n) bytes (checked by the callers).
+int rebin(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+          const opx_release_clock *clocks, int32_t nrel, void *work, uint64_t work_bytes, int32_t *n_active,
+          int32_t *n_live, void *stream)
+{
+    int rc;
+    const int64_t n = src->n;
+    hipStream_t st = as_stream(stream);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
+        if (n_live)
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_live);
+        return check_launch("opx_rebin");
+    }
+    const size_t arr = align256((size_t)n * sizeof(uint32_t));
+    char *base = static_cast<char *>(work);
+    uint32_t *k_in = reinterpret_cast<uint32_t *>(base);
+    uint32_t *k_out = reinterpret_cast<uint32_t *>(base + arr);
+    uint32_t *v_in = reinterpret_cast<uint32_t *>(base + 2 * arr);
+    uint32_t *v_out = reinterpret_cast<uint32_t *>(base + 3 * arr);
+    void *temp = base + 4 * arr;
+    size_t temp_bytes = (size_t)(work_bytes - 4 * arr);
+    const int key_bits = g->xbits + g->ybits + g->zbits;
+    const uint32_t inactive_key = 1u << key_bits;
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, clocks, (int)nrel,
+                       inactive_key, k_in, v_in);
+    if ((rc = check_launch("opx_rebin keys"))) return rc;
+    // keys span [0, inactive_key + 1] = key_bits + 1 bits
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
+                                                      key_bits + 1, st);
+    if (e != hipSuccess) {
+        set_error("opx_rebin: radix sort failed: %s", hipGetErrorString(e));
+        return OPX_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(1), 0, st, k_out, n, inactive_key, n_active, n_live);
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *dst, v_out);
+    return check_launch("opx_rebin permute");
+}
+
+// ---- the allocation-free re-bin's scratch: one file-scope device array, so
+// opx_rebin_noalloc needs no allocation.  Each device holds its own copy at
+// its own address, found once per device with hipGetSymbolAddress; host code
+// uses that address, never the array's name (a host-side stand-in).  Every
+// call on a device shares the device's copy: calls that overlap on one device
+// race.  Nothing in it needs to start at zero (every key, value and the sort's
+// temporary storage are written before they are read).
+constexpr uint64_t kNoAllocScratchBytes = uint64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("opx_rebin_noalloc: hipGetDevice failed: %s", hipGetErrorString(e));
+        return OPX_ERR_HIP;
+    }
+    OPX_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "opx_rebin_noalloc: device %d beyond the %d supported", dev,
+                kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("opx_rebin_noalloc: hipGetSymbolAddress of the scratch buffer failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return OPX_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OPX_OK;
+}
+
+} // namespace
 } // namespace opx
 
 using namespace opx;
@@ -2661,51 +2763,32 @@ int opx_rebin(const opx_split_state *src, const opx_split_state *dst, const opx_
               int32_t *n_live, void *stream)
 {
     int rc;
-    if ((rc = check_state(src, "opx_rebin src"))) return rc;
-    if ((rc = check_state(dst, "opx_rebin dst"))) return rc;
-    if ((rc = check_grid(g))) return rc;
-    OPX_REQUIRE(dst->n == src->n, "src/dst sizes differ");
-    OPX_REQUIRE(n_active != nullptr, "null n_active");
-    OPX_REQUIRE(xlow >= 0 && xlow <= g->xbits, "xlow %d out of [0, %d]", xlow, g->xbits);
-    OPX_REQUIRE(clocks == nullptr || nrel >= 0, "negative release count");
-    OPX_REQUIRE((src->pid == nullptr) == (dst->pid == nullptr) && (src->rel == nullptr) == (dst->rel == nullptr) &&
-                    (src->z == nullptr) == (dst->z == nullptr) && (src->rho_t == nullptr) == (dst->rho_t == nullptr),
-                "src/dst optional arrays differ");
-    const int64_t n = src->n;
+    if ((rc = opx::check_rebin_args(src, dst, g, xlow, clocks, nrel, n_active))) return rc;
     uint64_t need = 0;
-    opx_rebin_workspace_bytes(n, &need);
+    opx_rebin_workspace_bytes(src->n, &need);
     OPX_REQUIRE(work != nullptr && work_bytes >= need, "rebin workspace too small (%llu < %llu bytes)",
                 (unsigned long long)work_bytes, (unsigned long long)need);
-    hipStream_t st = as_stream(stream);
-    if (n == 0) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_active);
-        if (n_live)
-            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, n_live);
-        return check_launch("opx_rebin");
-    }
-    const size_t arr = align256((size_t)n * sizeof(uint32_t));
-    char *base = static_cast<char *>(work);
-    uint32_t *k_in = reinterpret_cast<uint32_t *>(base);
-    uint32_t *k_out = reinterpret_cast<uint32_t *>(base + arr);
-    uint32_t *v_in = reinterpret_cast<uint32_t *>(base + 2 * arr);
-    uint32_t *v_out = reinterpret_cast<uint32_t *>(base + 3 * arr);
-    void *temp = base + 4 * arr;
-    size_t temp_bytes = (size_t)(work_bytes - 4 * arr);
-    const int key_bits = g->xbits + g->ybits + g->zbits;
-    const uint32_t inactive_key = 1u << key_bits;
-    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *g, xlow, clocks, (int)nrel,
-                       inactive_key, k_in, v_in);
-    if ((rc = check_launch("opx_rebin keys"))) return rc;
-    // keys span [0, inactive_key + 1] = key_bits + 1 bits
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
-                                                      key_bits + 1, st);
-    if (e != hipSuccess) {
-        set_error("opx_rebin: radix sort failed: %s", hipGetErrorString(e));
-        return OPX_ERR_HIP;
-    }
-    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(1), 0, st, k_out, n, inactive_key, n_active, n_live);
-    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, st, *src, *dst, v_out);
-    return check_launch("opx_rebin permute");
+    return opx::rebin(src, dst, g, xlow, clocks, nrel, work, work_bytes, n_active, n_live, stream);
+}
+
+int opx_rebin_noalloc(const opx_split_state *src, const opx_split_state *dst, const opx_grid *g, int32_t xlow,
+                      const opx_release_clock *clocks, int32_t nrel, int32_t *n_active, int32_t *n_live,
+                      void *stream)
+{
+    int rc;
+    if ((rc = opx::check_rebin_args(src, dst, g, xlow, clocks, nrel, n_active))) return rc;
+    // 16 bytes per particle alone exceed the buffer: refuse before querying the sort
+    OPX_REQUIRE((uint64_t)src->n * 16u <= opx::kNoAllocScratchBytes,
+                "opx_rebin_noalloc: %lld particles need more than the %llu-byte static scratch buffer",
+                (long long)src->n, (unsigned long long)opx::kNoAllocScratchBytes);
+    uint64_t need = 0;
+    opx_rebin_workspace_bytes(src->n, &need);
+    OPX_REQUIRE(need <= opx::kNoAllocScratchBytes,
+                "opx_rebin_noalloc: %lld particles need %llu bytes of scratch, above the %llu-byte static buffer",
+                (long long)src->n, (unsigned long long)need, (unsigned long long)opx::kNoAllocScratchBytes);
+    void *work = nullptr;
+    if ((rc = opx::noalloc_scratch(&work))) return rc;
+    return opx::rebin(src, dst, g, xlow, clocks, nrel, work, opx::kNoAllocScratchBytes, n_active, n_live, stream);
 }
 
 } // extern "C"
