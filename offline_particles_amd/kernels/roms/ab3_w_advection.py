@@ -15,7 +15,8 @@ from ... import _lib
 from .._kernels import ParticleKernel
 from .._marshal import byref, fields_struct, particles_struct, scalars_struct, stream_handle
 
-__all__ = ["ab3_w_advection", "ab3_post_w_advection", "ab3_w_advection_kernel", "ab3_post_w_advection_kernel"]
+__all__ = ["ab3_w_advection", "ab3_post_w_advection", "ab3_w_advection_kernel", "ab3_post_w_advection_kernel",
+           "rk2_startup_history_kernel"]
 
 _FULL = ("u", "v", "w", "dx", "dy", "h", "C", "zeta")
 _VERT = ("h", "C", "zeta")
@@ -52,3 +53,28 @@ ab3_post_w_advection_kernel = ParticleKernel(
     scalars={"hc": np.float64, "NZ": np.int32},
     simulation_fields=list(_VERT),
 )
+
+
+def _rk2_startup_history(particles, scalars, fielddata, stage) -> None:
+    """History bookkeeping of the RK2 start-up (timesteppers.ABRK2StartupTimestepper):
+    "save" keeps the first step's stage-1 derivatives f(x0, t0) (left in _d*1) in
+    _d*0; "restore" moves them to _d*2 after the second step, whose stage 1 left
+    f(x1, t1) in _d*1.  Device-to-device copies of whole particle arrays."""
+    for axis in ("z", "yidx", "xidx"):
+        d0, d1, d2 = (particles[f"_d{axis}{k}"] for k in (0, 1, 2))
+        if stage == "save":
+            d0.copy_(d1)
+        else:
+            d2.copy_(d0)
+
+
+def rk2_startup_history_kernel(stage: str) -> ParticleKernel:
+    """The history step of the RK2 start-up of AB3 ("save" after the first RK2 step,
+    "restore" after the second); no counterpart in the reference."""
+    if stage not in ("save", "restore"):
+        raise ValueError(f"stage must be 'save' or 'restore', not {stage!r}")
+    import functools
+
+    fn = functools.partial(_rk2_startup_history, stage=stage)
+    fn.__qualname__ = f"rk2_startup_history_{stage}"
+    return ParticleKernel(fn, particle_fields={k: _F8 for k in _HIST}, scalars={}, simulation_fields=[])

@@ -20,7 +20,7 @@ from ...kernels.roms import (
     rk2_w_advection_update_kernel,
 )
 from ...kernels.validation import validation_kernel
-from ...timesteppers import ABTimestepper, RK2Timestepper
+from ...timesteppers import ABRK2StartupTimestepper, ABTimestepper, RK2Timestepper
 
 __all__ = [
     "compute_z_kernel",
@@ -42,11 +42,25 @@ def rk2_w_advection_timestepper(time_array, dt, *, time=None, time_unit=None, it
 
 
 def ab3_w_advection_timestepper(time_array, dt, *, time=None, time_unit=None, iteration: int = 0,
-                                index_padding: int = 5) -> ABTimestepper:
-    """AB3 timestepper with the ROMS w-advection kernels."""
-    return ABTimestepper(time_array, dt, ab_kernel=ab3_w_advection_kernel, time=time, time_unit=time_unit,
-                         iteration=iteration, index_padding=index_padding, pre_step_kernel=validation_kernel,
-                         post_step_kernel=ab3_post_w_advection_kernel)
+                                index_padding: int = 5, startup: str = "multistep",
+                                alpha: float = 2 / 3) -> ABTimestepper:
+    """AB3 timestepper with the ROMS w-advection kernels.
+
+    ``startup="multistep"`` (the reference's only form) starts each particle
+    through its MULTISTEP status codes (Euler, AB2); ``startup="rk2"`` takes the
+    first two steps with the RK2 w-advection kernels (parameter ``alpha``) and
+    starts AB3 from their stage-1 derivatives (ABRK2StartupTimestepper)."""
+    if startup == "multistep":
+        return ABTimestepper(time_array, dt, ab_kernel=ab3_w_advection_kernel, time=time, time_unit=time_unit,
+                             iteration=iteration, index_padding=index_padding, pre_step_kernel=validation_kernel,
+                             post_step_kernel=ab3_post_w_advection_kernel)
+    if startup == "rk2":
+        return ABRK2StartupTimestepper(time_array, dt, ab3_w_advection_kernel, rk2_w_advection_step_1_kernel,
+                                       rk2_w_advection_step_2_kernel, rk2_w_advection_update_kernel, alpha=alpha,
+                                       time=time, time_unit=time_unit, iteration=iteration,
+                                       index_padding=index_padding, pre_step_kernel=validation_kernel,
+                                       post_step_kernel=ab3_post_w_advection_kernel)
+    raise ValueError(f"startup must be 'multistep' or 'rk2', not {startup!r}")
 
 
 def ab3_isopycnal_following_timestepper(time_array, dt, *, time=None, time_unit=None, iteration: int = 0,

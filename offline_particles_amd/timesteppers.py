@@ -171,3 +171,58 @@ class ABTimestepper(Timestepper):
         self.advance_time()
         if self._post_step_kernel is not None:
             launcher.launch_kernel(self._post_step_kernel, particles, self._tidx)
+
+
+class ABRK2StartupTimestepper(ABTimestepper):
+    """AB3 whose first two steps are RK2 steps: a start-up variant with no
+    counterpart in the reference, whose ABTimestepper starts through the
+    MULTISTEP status codes (Euler, then AB2, ab3_w_advection.pyx:121-138).
+
+    The first two steps run the RK2 sequence of RK2Timestepper
+    (timesteppers.py:241-257: pre-step, stage 1 at t, stage 2 at
+    t + alpha dt, advance, update at t + dt).  The RK2 and AB3 kernels share
+    the derivative fields ``_d*1`` and ``_d*2`` (rk2_w_advection.pyx:378-380,
+    ab3_w_advection.pyx:250-258), so the history is kept in them: after the
+    first step stage 1's f(x0, t0) is saved in ``_d*0``; after the second,
+    ``_d*2`` = f(x0, t0) while stage 1 left ``_d*1`` = f(x1, t1), the history
+    AB3 needs at its first step from (x2, t2).  Particles start with status
+    NORMAL (a MULTISTEP status would restart their history with Euler).
+    """
+
+    _alpha_scalar = ScalarSource("_RK2_alpha", lambda self, tidx: self._alpha)
+
+    def __init__(self, time_array, dt, ab_kernel, rk_step_1_kernel, rk_step_2_kernel, rk_update_kernel, *,
+                 alpha: float = 2 / 3, time_unit=None, time=None, iteration: int = 0, index_padding: int = 0,
+                 pre_step_kernel=None, post_step_kernel=None) -> None:
+        super().__init__(time_array, dt, ab_kernel, time_unit=time_unit, time=time, iteration=iteration,
+                         index_padding=index_padding, pre_step_kernel=pre_step_kernel,
+                         post_step_kernel=post_step_kernel)
+        self._rk = (rk_step_1_kernel, rk_step_2_kernel, rk_update_kernel)
+        self._alpha = alpha
+        self._rk_steps_left = 2
+        from .kernels.roms.ab3_w_advection import rk2_startup_history_kernel
+
+        self._history = (rk2_startup_history_kernel("save"), rk2_startup_history_kernel("restore"))
+
+    @property
+    def alpha(self) -> float:
+        return self._alpha
+
+    @property
+    def kernels(self) -> tuple:
+        return super().kernels + self._rk + self._history
+
+    def timestep_particles(self, particles, launcher) -> None:
+        if self._rk_steps_left == 0:
+            super().timestep_particles(particles, launcher)
+            return
+        rk1, rk2, update = self._rk
+        if self._pre_step_kernel is not None:
+            launcher.launch_kernel(self._pre_step_kernel, particles, self._tidx)
+        launcher.launch_kernel(rk1, particles, self._tidx)
+        intermediate_tidx = self.get_time_index(self.time + self._alpha * self.dt)
+        launcher.launch_kernel(rk2, particles, intermediate_tidx)
+        self.advance_time()
+        launcher.launch_kernel(update, particles, self._tidx)
+        self._rk_steps_left -= 1
+        launcher.launch_kernel(self._history[0 if self._rk_steps_left else 1], particles, self._tidx)

@@ -394,8 +394,9 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
                      status=None, ab_kernel=None):
     """A drop-in Simulation for the workload with particles placed.
 
-    ``scheme`` is "rk2" or "ab3"; AB3 particles start from MULTISTEP_1 (Euler
-    -> AB2 -> AB3) unless ``status`` is given.  ``ab_kernel`` replaces the AB3
+    ``scheme`` is "rk2", "ab3" or "ab3_rk2" (AB3 whose first two steps are RK2
+    steps); AB3 particles start from MULTISTEP_1 (Euler -> AB2 -> AB3) unless
+    ``status`` is given, RK2 and "ab3_rk2" ones from NORMAL.  ``ab_kernel`` replaces the AB3
     kernel (same declarations), e.g. to time it.
     """
     from .kernels.roms import ab3_post_w_advection_kernel, ab3_w_advection_kernel
@@ -409,6 +410,8 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
     dt = np.float64(wl.dt)
     if scheme == "rk2":
         ts = roms.rk2_w_advection_timestepper(wl.time_array, dt)
+    elif scheme == "ab3_rk2":
+        ts = roms.ab3_w_advection_timestepper(wl.time_array, dt, startup="rk2")
     elif scheme == "ab3":
         ts = ABTimestepper(wl.time_array, dt, ab_kernel=ab_kernel or ab3_w_advection_kernel, index_padding=5,
                            pre_step_kernel=validation_kernel, post_step_kernel=ab3_post_w_advection_kernel)

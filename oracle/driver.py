@@ -45,6 +45,9 @@ PINNING = {
     "uniform-w vertical closed form",
     "orc_ab3/orc_ab3_post": "closed form for u = a + b t across Euler -> AB2 -> AB3 start-up "
     "(x_N - x_0 = a N dt + b (t_N^2 - t_0^2)/2 - b dt^2/2) + uniform-w vertical closed form",
+    "ab3_rk2 (RK2 start-up of AB3, composed)": "closed forms: u = a + b t gives a N dt + b dt^2 N (N - 1) / 2 + "
+    "b dt^2 (2 (alpha - 1/2) + (N - 2) / 2) after N steps; dx/dt = lambda x gives the RK2 amplification for two "
+    "steps, then AB3 from f(x0), f(x1); uniform w over a flat bed",
     "driver (get_time_index, launch order, FP32 blend weights)": "restated from timesteppers.py / fields.py; "
     "numpy semantics checked directly; composition pinned by the RK2/AB3 closed forms with "
     "time-dependent fields",
@@ -268,7 +271,8 @@ class OracleSimulation:
     """Reference-structured simulation on the CPU oracle.
 
     ``scheme`` is ``"rk2"`` (rk2_w_advection_timestepper), ``"ab3"``
-    (ab3_w_advection_timestepper) or ``"ab3_iso"``
+    (ab3_w_advection_timestepper), ``"ab3_rk2"`` (the same with its first two
+    steps taken by the RK2 kernels) or ``"ab3_iso"``
     (ab3_isopycnal_following_timestepper: ``fields`` has "rho", and
     ``rho_t``, ``w_max``, ``rho_scale`` are given); all use the validation
     pre-step.
@@ -279,8 +283,15 @@ class OracleSimulation:
     def __init__(self, fields: dict, time_array, dt, *, scheme: str, nparticles: int, hc: float,
                  NZ: int, bounds, alpha: float = 2 / 3, time=None, threads: int | None = None,
                  bbox: bool = True, rho_t=None, w_max: float = 0.0, rho_scale: float = 1.0):
-        if scheme not in ("rk2", "ab3", "ab3_iso"):
+        if scheme not in ("rk2", "ab3", "ab3_iso", "ab3_rk2"):
             raise ValueError(scheme)
+        # "ab3_rk2": AB3 whose first two steps are RK2 steps (a start-up variant
+        # composed from the reference's kernels; no reference mode): the RK2 and
+        # AB3 kernels share the derivative fields _d*1/_d*2
+        # (rk2_w_advection.pyx:378-380, ab3_w_advection.pyx:250-258), so f(x0, t0)
+        # from step 1's stage 1 is saved and put in _d*2 after step 2, whose stage
+        # 1 leaves f(x1, t1) in _d*1
+        self._rk_steps_left = 2 if scheme == "ab3_rk2" else 0
         self.L = load()
         if threads:
             self.L.orc_set_num_threads(int(threads))
@@ -354,17 +365,29 @@ class OracleSimulation:
         self.L.orc_finite_indices(ctypes.byref(self._cp))
         self.L.orc_domain_bounds(ctypes.byref(self._cp), ctypes.byref(self._scalars))
 
+    def _rk2_step(self):
+        # timesteppers.py:241-257
+        L = self.L
+        self._validate()
+        self._launch(L.orc_rk2_step_1, self.tidx, "full")
+        intermediate_time = self.time + self.alpha * self.dt
+        intermediate_tidx = self.get_time_index(intermediate_time)
+        self._launch(L.orc_rk2_step_2, intermediate_tidx, "full")
+        self.advance_time()
+        self._launch(L.orc_rk2_update, self.tidx, "vertical")
+
     def step(self):
         L = self.L
-        if self.scheme == "rk2":
-            # timesteppers.py:241-257
-            self._validate()
-            self._launch(L.orc_rk2_step_1, self.tidx, "full")
-            intermediate_time = self.time + self.alpha * self.dt
-            intermediate_tidx = self.get_time_index(intermediate_time)
-            self._launch(L.orc_rk2_step_2, intermediate_tidx, "full")
-            self.advance_time()
-            self._launch(L.orc_rk2_update, self.tidx, "vertical")
+        if self._rk_steps_left:
+            self._rk2_step()
+            self._rk_steps_left -= 1
+            for axis in ("z", "yidx", "xidx"):
+                if self._rk_steps_left:
+                    self.p[f"d{axis}0"][:] = self.p[f"d{axis}1"]
+                else:
+                    self.p[f"d{axis}2"][:] = self.p[f"d{axis}0"]
+        elif self.scheme == "rk2":
+            self._rk2_step()
         elif self.scheme == "ab3_iso":
             # ABTimestepper (timesteppers.py:296-305) with the isopycnal kernels
             # (models/roms/__init__.py:109-140)

@@ -3,8 +3,10 @@
 1024 x 768 x 50 grid, 8 hourly FP32 snapshots cycled (turbulent k^-3 flow,
 2% land), AB3 from MULTISTEP_1, dt = 120 s:
 
-- plain FP32 drop-in, 4M particles x 100 steps: a seeded 100k subsample
-  equals the oracle's run of those 100k bit for bit, status included;
+- plain FP32 drop-in, 4M particles x 100 steps, AB3 from MULTISTEP_1 and AB3
+  with an RK2 start-up (the configuration's "AB3 with RK2 startup"): a seeded
+  100k subsample equals the oracle's run of those 100k bit for bit, status
+  included;
 - fused FP32, 16M particles x 100 steps (32 steps per launch, re-binning):
   the subsample is within 1e-5 grid-index units of the oracle with status
   bit-exact, and equal bit for bit to the fused run of those 100k alone
@@ -58,13 +60,23 @@ def grid1024():
     return wl, sub
 
 
+_ORACLE_RUNS = {}
+
+
+def _oracle_sub(grid1024, scheme="ab3"):
+    if scheme not in _ORACLE_RUNS:
+        wl, sub = grid1024
+        t = time.perf_counter()
+        sim = run_oracle(wl, scheme, STEPS, positions=tuple(a[sub] for a in wl.positions))
+        _log(f"oracle: {N_SUB} particles x {STEPS} steps ({scheme}) in {time.perf_counter() - t:.1f}s")
+        _ORACLE_RUNS[scheme] = {k: sim.p[k].copy() for k in ("status",) + POS}
+    return _ORACLE_RUNS[scheme]
+
+
 @pytest.fixture(scope="module")
 def oracle_sub(grid1024):
-    wl, sub = grid1024
-    t = time.perf_counter()
-    sim = run_oracle(wl, "ab3", STEPS, positions=tuple(a[sub] for a in wl.positions))
-    _log(f"oracle: {N_SUB} particles x {STEPS} AB3 steps in {time.perf_counter() - t:.1f}s")
-    return {k: sim.p[k].copy() for k in ("status",) + POS}
+    yield _oracle_sub(grid1024)
+    _ORACLE_RUNS.clear()
 
 
 def _fused(wl, device, pos, storage, *, launch_steps=32, rebin_every=32):
@@ -89,21 +101,24 @@ def _same(a, b, what, idx=None):
 
 
 @pytest.mark.timeout(300)
-def test_plain_fp32_4M_subsample_bit_exact(device, grid1024, oracle_sub):
+@pytest.mark.parametrize("scheme", ["ab3", "ab3_rk2"])
+def test_plain_fp32_4M_subsample_bit_exact(device, grid1024, oracle_sub, scheme):
+    """AB3 from MULTISTEP_1, and AB3 whose first two steps are RK2 steps."""
     wl, sub = grid1024
+    oracle_sub = _oracle_sub(grid1024, scheme)
     pos = tuple(a[:N_PLAIN] for a in wl.positions)
-    sim = build_simulation(wl, "ab3", device=device, positions=pos)
+    sim = build_simulation(wl, scheme, device=device, positions=pos)
     t = time.perf_counter()
     for _ in range(STEPS):
         sim.step()
     torch.cuda.synchronize()
-    _log(f"plain FP32 drop-in: {N_PLAIN} particles x {STEPS} steps in {time.perf_counter() - t:.2f}s")
+    _log(f"plain FP32 drop-in ({scheme}): {N_PLAIN} particles x {STEPS} steps in {time.perf_counter() - t:.2f}s")
     g = {k: sim.particles.to_numpy(k) for k in ("status",) + POS}
     del sim
     torch.cuda.empty_cache()
     assert np.array_equal(g["status"][sub], oracle_sub["status"]), "status differs"
     devs = {k: max_dev(g[k][sub], oracle_sub[k]) for k in POS}
-    _log("plain FP32 4M: max deviation of the 100k subsample from the oracle:", devs)
+    _log(f"plain FP32 4M ({scheme}): max deviation of the 100k subsample from the oracle:", devs)
     assert max(devs.values()) == 0.0
     # the run moved the particles and some left the domain
     assert np.mean(np.abs(g["xidx"][sub] - pos[2][sub])) > 0.1

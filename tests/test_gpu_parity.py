@@ -1,7 +1,9 @@
 """Parity of the HIP drop-in kernels with the CPU oracle (same seeded inputs).
 
 Gate: FP64 fields within 1e-9 grid-index units of max position deviation after
-100 steps, FP32 fields within 1e-5, status codes bit-exact.  The drop-in keeps
+100 steps, FP32 fields within 1e-5, status codes bit-exact.  Schemes: RK2, AB3
+(MULTISTEP start-up) and AB3 whose first two steps are RK2 steps ("ab3_rk2",
+against the oracle's composition of the same kernels).  The drop-in keeps
 the reference's FP64 operation order and is built without FMA contraction, so
 the observed deviation is expected to be exactly zero; the test reports it.
 """
@@ -26,7 +28,7 @@ def _compare(sim_o, sim_g, tol):
     return devs
 
 
-@pytest.mark.parametrize("scheme", ["rk2", "ab3"])
+@pytest.mark.parametrize("scheme", ["rk2", "ab3", "ab3_rk2"])
 def test_eddy_fp64_100_steps(device, scheme):
     wl = eddy_workload(steps=100)
     sim_o = run_oracle(wl, scheme, 100)
@@ -37,7 +39,7 @@ def test_eddy_fp64_100_steps(device, scheme):
     assert (sim_o.p["status"] == 130).any()  # some particles leave through the surface/bed
 
 
-@pytest.mark.parametrize("scheme", ["rk2", "ab3"])
+@pytest.mark.parametrize("scheme", ["rk2", "ab3", "ab3_rk2"])
 def test_turbulent_fp32_100_steps(device, scheme):
     wl = turbulent_workload(NX=96, NY=80, NZ=12, nparticles=20_000, steps=100, nsnap_unique=4)
     sim_o = run_oracle(wl, scheme, 100)

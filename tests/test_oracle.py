@@ -219,7 +219,60 @@ def test_ab3_startup_closed_form():
     assert np.max(np.abs(sim2.p["xidx"] - sim.p["xidx"])) > 1e-3
 
 
-@pytest.mark.parametrize("scheme", ["rk2", "ab3"])
+def test_ab3_rk2_startup_closed_form():
+    """u = a + b t, AB3 whose first two steps are RK2 steps: each RK2 step moves
+    a dt + b t dt + (alpha - 1/2) b dt^2, and AB3 from exact samples f(t0), f(t1)
+    of the linear u is exact, so after N steps
+    x_N - x_0 = a N dt + b dt^2 N (N - 1) / 2 + b dt^2 (2 (alpha - 1/2) + (N - 2) / 2).
+    Saving stage 2's derivatives instead of stage 1's, or starting with Euler,
+    gives a different answer (checked)."""
+    a, b, dxm, dt, snap, alpha = -0.1, 2e-5, 1000.0, 120.0, 3600.0, 2 / 3
+    NZ, NY, NX = 4, 6, 300
+    ta = np.arange(4) * snap
+    fields = _uniform_fields(NZ, NY, NX, lambda k: a + b * ta[k], 4, dxm=dxm)
+    x0 = np.array([100.0, 150.25])
+    sim = _sim(fields, ta, dt, "ab3_rk2", (np.full(2, 2.0), np.full(2, 3.0), x0), NZ, NY, NX, status=0)
+    N = 40
+    for n in range(N):
+        sim.step()
+        assert (sim.p["status"] == 0).all()
+    exact = (a * N * dt + b * dt**2 * N * (N - 1) / 2 + b * dt**2 * (2 * (alpha - 0.5) + (N - 2) / 2)) / dxm
+    assert np.allclose(sim.p["xidx"] - x0, exact, rtol=0, atol=1e-10)
+    euler = (a * N * dt + b * (N * dt) ** 2 / 2 - b * dt**2 / 2) / dxm  # the MULTISTEP start-up
+    assert abs(euler - exact) > 1e-5
+    # history from stage 2 (f at t + alpha dt) instead of stage 1: the first AB3 step sees both samples
+    # b alpha dt too large (-16 + 5 of them), the second one sample (+5): b alpha dt^2 / 2 less in all
+    wrong = exact - b * alpha * dt**2 / 2 / dxm
+    assert abs(wrong - exact) > 1e-5 and not np.allclose(sim.p["xidx"] - x0, wrong, rtol=0, atol=1e-10)
+
+
+def test_ab3_rk2_startup_linear_field():
+    """dx/dt = lambda (x - xc): two RK2 steps multiply x - xc by
+    g = 1 + lambda dt + (alpha - 1/2) (lambda dt)^2 each; AB3 then runs on the
+    history f(x0) = lambda X0, f(x1) = lambda X1 (X = x - xc): the linear
+    recurrence X_{n+1} = X_n + lambda dt (23 X_n - 16 X_{n-1} + 5 X_{n-2}) / 12."""
+    lam, dt, xc, dxm, alpha = 2e-5, 600.0, 50.0, 1000.0, 2 / 3
+    NZ, NY, NX = 3, 5, 120
+    ta = np.arange(4) * 7200.0
+    i = np.arange(NX - 1) + 0.5
+    from offline_particles_amd.workloads import OFFSETS
+
+    fields = _uniform_fields(NZ, NY, NX, lambda k: 0.0, 4)
+    fields["u"] = D.OracleField(np.broadcast_to(lam * (i - xc) * dxm, (4, NZ, NY, NX - 1)).copy(), OFFSETS["u"],
+                                True, cyclic=False)
+    x = np.array([20.0, 49.0, 70.25, 90.0])
+    sim = _sim(fields, ta, dt, "ab3_rk2", (np.full(4, 0.7), np.full(4, 2.0), x), NZ, NY, NX, status=0)
+    N = 12
+    for _ in range(N):
+        sim.step()
+    g = 1 + lam * dt + (alpha - 0.5) * (lam * dt) ** 2
+    X = [x - xc, (x - xc) * g, (x - xc) * g * g]
+    for _ in range(N - 2):
+        X.append(X[-1] + lam * dt * (23 * X[-1] - 16 * X[-2] + 5 * X[-3]) / 12)
+    assert np.allclose(sim.p["xidx"], xc + X[N], rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("scheme", ["rk2", "ab3", "ab3_rk2"])
 def test_uniform_w_flat_bed(scheme):
     """Flat h, zeta = 0, C = sigma, uniform w: dzidx = NZ w dt / h per step."""
     NZ, NY, NX, h, w, dt = 10, 5, 6, 100.0, 1e-3, 300.0
