@@ -9,7 +9,10 @@ validation, the AB3 stage and the post-step zidx update.
 Forms measured in the same run:
 - plain_fp32: the drop-in behind the reference API (Simulation.step ->
   validation, ab3_w_advection, ab3_post_w_advection HIP kernels), FP32 fields,
-  FP64 positions and FP64 interpolation.
+  FP64 positions and FP64 interpolation, particles in the caller's order.
+- plain_fp32_ordered: the same with Simulation(spatial_order_every=20): the
+  particle arrays permuted into the Morton order of their cells every 20 steps
+  (inside the timed loop); the user sees the same bits in the same order.
 - fused_fp32 / fused_bf16: AB3Engine, one fused HIP kernel per step over
   split positions (packed int32 cell + 32-bit fixed-point offsets) with u, v,
   w stored in FP32 or BF16; bin shape, workgroup size and re-sort cadence are
@@ -231,19 +234,24 @@ def _time_plain(sim, tk, steps, warmup):
     return t1 - t0, 0.5 * (a0 + a1), tk.mean_ms()
 
 
-def run_plain(wl, device, steps, warmup):
+def run_plain(wl, device, steps, warmup, order_every=0):
+    """The drop-in through Simulation.step; with ``order_every`` > 0 the
+    simulation keeps its particle arrays in the Morton order of the cells,
+    refreshed every that many steps inside the timed loop (form
+    plain_fp32_ordered)."""
     import torch
 
     from offline_particles_amd.kernels.roms import ab3_w_advection_kernel
     from offline_particles_amd.workloads import build_simulation
 
     tk = _TimedKernel(ab3_w_advection_kernel)
-    sim = build_simulation(wl, "ab3", device=device, ab_kernel=tk.kernel)
+    sim = build_simulation(wl, "ab3", device=device, ab_kernel=tk.kernel, spatial_order_every=order_every)
     fbytes = field_bytes_touched(wl, sim, wl.fields["u"].dtype.itemsize)
     elapsed, act, kms = _time_plain(sim, tk, steps, warmup)
     algo = act * PLAIN_AB3_STATE_BYTES + fbytes
     res = {
-        "form": "plain_fp32",
+        "form": "plain_fp32_ordered" if order_every else "plain_fp32",
+        "spatial_order_every": order_every,
         "particle_steps_per_s": act * steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
         "active_particles": act,
@@ -1047,11 +1055,12 @@ def accuracy(wl, device, nsub, steps):
                 "status_mismatch": int((st != ref["status"]).sum()), "compared": int(both.sum())}
 
     out = {"particles": nsub, "steps": steps}
-    g = build_simulation(wl, "ab3", device=device, positions=pos)
-    for _ in range(steps):
-        g.step()
-    out["plain_fp32"] = report("plain", *(g.particles.to_numpy(k) for k in ("zidx", "yidx", "xidx", "status")))
-    del g
+    for name, every in (("plain_fp32", 0), ("plain_fp32_ordered", 20)):
+        g = build_simulation(wl, "ab3", device=device, positions=pos, spatial_order_every=every)
+        for _ in range(steps):
+            g.step()
+        out[name] = report(name, *(g.particles.to_numpy(k) for k in ("zidx", "yidx", "xidx", "status")))
+        del g
     fs = build_fieldset(wl, device=device)
     for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
         eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=dt, device=device, capacity=nsub)
@@ -1068,6 +1077,7 @@ def accuracy(wl, device, nsub, steps):
 _DTYPE = {
     "plain_fp32": ("f64", "FP32 storage of every field; FP64 positions, blends, interpolation and vertical "
                           "transform (the drop-in's arithmetic)"),
+    "plain_fp32_ordered": ("f64", "as plain_fp32, particle arrays in the Morton order of their cells"),
     "fused_fp32": ("f32", "FP32 storage of u, v, w; FP32 interpolation of u and v; FP64 blends of w, h, zeta, "
                           "FP64 vertical transform and clocks; positions as packed cells plus 32-bit fixed-point "
                           "offsets"),
@@ -1177,7 +1187,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-particles", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain_fp32,fused_fp32,fused_bf16,fused_rk2_fp32")
+    ap.add_argument("--forms", default="plain_fp32,plain_fp32_ordered,fused_fp32,fused_bf16,fused_rk2_fp32")
+    ap.add_argument("--order-every", type=int, default=20,
+                    help="steps between spatial reorders of the plain_fp32_ordered form")
     ap.add_argument("--rebin-every", type=int, default=25)
     ap.add_argument("--xlow", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --xlow/--rebin-every)")
@@ -1240,6 +1252,9 @@ def main():
     if "plain_fp32" in wanted:
         forms["plain_fp32"] = run_plain(wl, device, args.steps, args.warmup)
         log("plain_fp32", forms["plain_fp32"])
+    if "plain_fp32_ordered" in wanted:
+        forms["plain_fp32_ordered"] = run_plain(wl, device, args.steps, args.warmup, order_every=args.order_every)
+        log("plain_fp32_ordered", forms["plain_fp32_ordered"])
     for name, dt, scheme in (("fused_fp32", torch.float32, "ab3"), ("fused_bf16", torch.bfloat16, "ab3"),
                              ("fused_rk2_fp32", torch.float32, "rk2"), ("fused_rk2_bf16", torch.bfloat16, "rk2")):
         if name in wanted:

@@ -171,6 +171,21 @@ int opx_interp_bilinear(int64_t n, const uint8_t *status, const double *idx0, co
 int opx_interp_trilinear(int64_t n, const uint8_t *status, const double *zidx, const double *yidx,
                          const double *xidx, const opx_field *f, double *out, void *stream);
 
+/* Spatial ordering of the drop-in's particle arrays (opt-in; no counterpart
+ * in the reference, which processes particles in the caller's order,
+ * ab3_w_advection.pyx:80).  opx_spatial_order writes perm[j] = the particle
+ * placed at j: particles sorted by a Morton key of their cell floor(index)
+ * (clamped to the grid; the low `xlow` x bits kept contiguous), those with the
+ * INACTIVE bit or a non-finite index last; stable.  `work` holds at least
+ * opx_spatial_order_workspace_bytes(n) bytes.  opx_gather applies a
+ * permutation to one array: dst[j] = src[perm[j]] for elements of 1, 2, 4 or 8
+ * bytes (src != dst).  Each particle's results are independent of the order. */
+int opx_spatial_order_workspace_bytes(int64_t n, uint64_t *bytes);
+int opx_spatial_order(int64_t n, const uint8_t *status, const double *zidx, const double *yidx, const double *xidx,
+                      int32_t NZ, int32_t NY, int32_t NX, int32_t xlow, void *work, uint64_t work_bytes,
+                      int32_t *perm, void *stream);
+int opx_gather(int64_t n, const int32_t *perm, const void *src, void *dst, int32_t elem_bytes, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Fused form: one kernel per AB3 step over split positions.           */
 /*                                                                     */

@@ -32,6 +32,7 @@ COMMON = [
 SOURCES = {
     "opx_plain.hip": ["-ffp-contract=off"],
     "opx_fused.hip": ["-ffp-contract=off"],
+    "opx_order.hip": [],
 }
 
 

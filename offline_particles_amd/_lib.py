@@ -182,6 +182,10 @@ _SIGNATURES.update({
                         _vp, _vp],
     "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, _vp,
                   ctypes.c_uint64, _vp, _vp, _vp],
+    "opx_spatial_order_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],
+    "opx_spatial_order": [ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_int32, _vp, ctypes.c_uint64, _vp, _vp],
+    "opx_gather": [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int32, _vp],
     "opx_rebin_noalloc": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp,
                           ctypes.c_int32, _vp, _vp, _vp],
     "opx_extract_release": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, _vp,

@@ -83,13 +83,35 @@ class Particles(_FrozenArrayMapping):
 
 
 class ParticlesView(_FrozenArrayMapping):
-    """Read-only access to a Particles object's arrays."""
+    """Read-only access to a Particles object's arrays.
 
-    __slots__ = ("_length",)
+    With ``order`` (a callable returning None or an int32 device tensor whose
+    entry j is the particle held in slot j), the arrays are kept in a
+    spatial order and each field is returned in the particles' own order: a
+    new tensor, out[order[j]] = slot j (without an order, the arrays
+    themselves)."""
 
-    def __init__(self, parent: Particles) -> None:
+    __slots__ = ("_length", "_order")
+
+    def __init__(self, parent: Particles, order=None) -> None:
         object.__setattr__(self, "_length", len(parent))
+        object.__setattr__(self, "_order", order)
         super().__init__(**dict(parent.arrays))
 
     def __len__(self) -> int:
         return object.__getattribute__(self, "_length")
+
+    def __getitem__(self, name: str) -> torch.Tensor:
+        a = super().__getitem__(name)
+        order_fn = object.__getattribute__(self, "_order")
+        order = order_fn() if order_fn is not None else None
+        if order is None:
+            return a
+        out = torch.empty_like(a)
+        out[order.long()] = a
+        return out
+
+    def __getattr__(self, name: str) -> torch.Tensor:
+        if name in object.__getattribute__(self, "_arrays"):
+            return self[name]
+        raise AttributeError(f"'{self.__class__.__name__}' object has no attribute '{name}'")

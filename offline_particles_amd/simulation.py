@@ -121,7 +121,7 @@ class Simulation:
     """A particle simulation on one device."""
 
     def __init__(self, nparticles: int, timestepper, fieldset, iteration_scheduler=None, time_scheduler=None,
-                 output_writers=None, *, device=None) -> None:
+                 output_writers=None, *, device=None, spatial_order_every: int = 0, spatial_order_xlow: int = 3) -> None:
         self._timestepper = timestepper
         self._fieldset = fieldset
         self._iteration_scheduler = iteration_scheduler or IterationScheduler()
@@ -137,7 +137,18 @@ class Simulation:
             kernels.extend(event.kernels)
         particle_fields = merge_particle_fields(kernels)
         self._particles = Particles(nparticles, device=device, **particle_fields)
-        self._particles_view = ParticlesView(self._particles)
+        # spatial ordering (opt-in): every `spatial_order_every` steps the
+        # particle arrays are permuted into the Morton order of the particles'
+        # cells, so the stencil gathers of neighbouring threads share cache
+        # lines; self._order[j] is the particle held in slot j (None: identity)
+        if spatial_order_every < 0:
+            raise ValueError("spatial_order_every must be non-negative")
+        self._order_every = int(spatial_order_every)
+        self._order_xlow = int(spatial_order_xlow)
+        self._order = None
+        self._since_order = 0
+        self._order_work = None
+        self._particles_view = ParticlesView(self._particles, lambda: self._order)
         self._invoke_events()
 
     timestepper = property(lambda self: self._timestepper)
@@ -173,6 +184,10 @@ class Simulation:
             t = torch.from_numpy(np.ascontiguousarray(arr)).to(target.device)
         return torch.broadcast_to(t, target.shape)
 
+    def _to_slots(self, t):
+        """Values given in the particles' order, arranged in slot order."""
+        return t if self._order is None else t[self._order.long()]
+
     def set_indices(self, zidx=None, yidx=None, xidx=None) -> None:
         """Set particle indices (simulation.py:178-207); all inputs are checked before any write."""
         vals = {}
@@ -180,15 +195,61 @@ class Simulation:
             if v is not None:
                 vals[name] = self._as_field(name, v)
         for name, t in vals.items():
-            self._particles[name].copy_(t)
+            self._particles[name].copy_(self._to_slots(t))
 
     def set_particle_field(self, field_name: str, values) -> None:
         """Set one particle field (simulation.py:209-223)."""
-        self._particles[field_name].copy_(self._as_field(field_name, values))
+        self._particles[field_name].copy_(self._to_slots(self._as_field(field_name, values)))
+
+    @property
+    def spatial_order(self):
+        """None, or the int32 device tensor whose entry j is the particle held in slot j."""
+        return self._order
+
+    def reorder(self) -> None:
+        """Permute every particle array into the Morton order of the particles'
+        cells now (opx_spatial_order, opx_gather); what the particles hold, and
+        what ``particles``, ``set_indices``, ``set_particle_field`` and the output
+        writers see, does not change."""
+        import ctypes
+
+        from . import _lib
+        from .kernels._marshal import stream_handle
+
+        P = self._particles
+        n = len(P)
+        self._since_order = 0
+        if n == 0:
+            return
+        dev = P["zidx"].device
+        if self._order_work is None:
+            need = ctypes.c_uint64()
+            _lib.call("opx_spatial_order_workspace_bytes", n, ctypes.byref(need))
+            self._order_work = torch.empty(int(need.value), dtype=torch.uint8, device=dev)
+        fs = self._fieldset
+        perm = torch.empty(n, dtype=torch.int32, device=dev)
+        st = stream_handle()
+        _lib.call("opx_spatial_order", n, P["status"].data_ptr(), P["zidx"].data_ptr(), P["yidx"].data_ptr(),
+                  P["xidx"].data_ptr(), fs.z_size, fs.y_size, fs.x_size, self._order_xlow,
+                  self._order_work.data_ptr(), self._order_work.numel(), perm.data_ptr(), st)
+        tmp = {}
+        for name, a in P.arrays.items():
+            b = tmp.get(a.dtype)
+            if b is None:
+                b = tmp[a.dtype] = torch.empty_like(a)
+            _lib.call("opx_gather", n, perm.data_ptr(), a.data_ptr(), b.data_ptr(), a.element_size(), st)
+            a.copy_(b)
+        order = self._order if self._order is not None else torch.arange(n, dtype=torch.int32, device=dev)
+        new = torch.empty_like(order)
+        _lib.call("opx_gather", n, perm.data_ptr(), order.data_ptr(), new.data_ptr(), 4, st)
+        self._order = new
 
     def step(self) -> None:
         """Advance the particles by one timestep, then run scheduled events."""
+        if self._order_every and self._since_order % self._order_every == 0:
+            self.reorder()
         self._timestepper.timestep_particles(self._particles, self._launcher)
+        self._since_order += 1
         self._invoke_events()
 
 
@@ -234,7 +295,8 @@ class SimulationBuilder:
             raise ValueError(f"Output writer '{name}' already exists.")
         self._output_writers[name] = (builder, {"n": n, "dt": dt, "first": first})
 
-    def build_simulation(self, nparticles: int, *, device=None) -> Simulation:
+    def build_simulation(self, nparticles: int, *, device=None, spatial_order_every: int = 0,
+                         spatial_order_xlow: int = 3) -> Simulation:
         writers = {}
         time_type = self._timestepper.time_array.dtype
         for name, (builder, kwargs) in self._output_writers.items():
@@ -242,4 +304,5 @@ class SimulationBuilder:
             for event in writers[name].create_events():
                 self.add_event(event, **kwargs)
         return Simulation(nparticles, self._timestepper, self._fieldset, self._iteration_scheduler,
-                          self._time_scheduler, writers, device=device)
+                          self._time_scheduler, writers, device=device, spatial_order_every=spatial_order_every,
+                          spatial_order_xlow=spatial_order_xlow)

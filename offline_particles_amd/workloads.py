@@ -391,12 +391,15 @@ def build_fieldset(wl: Workload, *, device=None, storage_dtype=None, cyclic: boo
 
 
 def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=None, positions=None,
-                     status=None, ab_kernel=None):
+                     status=None, ab_kernel=None, spatial_order_every: int = 0):
     """A drop-in Simulation for the workload with particles placed.
 
     ``scheme`` is "rk2", "ab3" or "ab3_rk2" (AB3 whose first two steps are RK2
     steps); AB3 particles start from MULTISTEP_1 (Euler -> AB2 -> AB3) unless
-    ``status`` is given, RK2 and "ab3_rk2" ones from NORMAL.  ``ab_kernel`` replaces the AB3
+    ``status`` is given, RK2 and "ab3_rk2" ones from NORMAL.
+    ``spatial_order_every`` > 0 keeps the particle arrays in the Morton order
+    of the particles' cells, refreshed every that many steps (results and what
+    the user sees are unchanged).  ``ab_kernel`` replaces the AB3
     kernel (same declarations), e.g. to time it.
     """
     from .kernels.roms import ab3_post_w_advection_kernel, ab3_w_advection_kernel
@@ -417,7 +420,8 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
                            pre_step_kernel=validation_kernel, post_step_kernel=ab3_post_w_advection_kernel)
     else:
         raise ValueError(scheme)
-    sim = SimulationBuilder(ts, fs).build_simulation(len(pos[0]), device=device)
+    sim = SimulationBuilder(ts, fs).build_simulation(len(pos[0]), device=device,
+                                                     spatial_order_every=spatial_order_every)
     sim.set_indices(*pos)
     if status is not None:
         sim.set_particle_field("status", status)

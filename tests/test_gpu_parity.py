@@ -95,3 +95,54 @@ def test_isopycnal_following_100_steps(device, dtype, tol):
     for _ in range(100):
         sim_g.step()
     _compare(sim_o, sim_g, tol)
+
+
+@pytest.mark.parametrize("scheme", ["ab3", "rk2"])
+def test_spatial_order_changes_nothing_the_user_sees(device, scheme):
+    """Simulation(spatial_order_every=N) permutes every particle array into the
+    Morton order of the cells every N steps: positions, status, every particle
+    field and the output writer's rows, read in the particles' own order, are
+    the bits of the run in the caller's order; writes after a reorder land on
+    the right particles."""
+    from offline_particles_amd import SimulationBuilder
+    from offline_particles_amd.kernels.roms import compute_z_kernel
+    from offline_particles_amd.models import roms
+    from offline_particles_amd.output import MemoryOutputBuilder, Output, linearly_interpolate_fields
+    from offline_particles_amd.workloads import build_fieldset
+
+    wl = turbulent_workload(NX=96, NY=80, NZ=12, nparticles=20_000, steps=100, nsnap_unique=4)
+    fs = build_fieldset(wl, device=device)
+    sims = []
+    for every in (0, 7):
+        ts = (roms.ab3_w_advection_timestepper if scheme == "ab3" else roms.rk2_w_advection_timestepper)(
+            wl.time_array, np.float64(wl.dt))
+        b = SimulationBuilder(ts, fs)
+        mem = MemoryOutputBuilder("mem")
+        mem.add_output(*linearly_interpolate_fields(fs, "u", "zeta"),
+                       Output("depth", compute_z_kernel("_depth"), particle_field="_depth"))
+        b.add_output_writer(mem, n=5)
+        sim = b.build_simulation(wl.nparticles, device=device, spatial_order_every=every)
+        sim.set_indices(*wl.positions)
+        if scheme == "ab3":
+            sim.set_particle_field("status", 10)
+        sims.append(sim)
+    for k in range(60):
+        for sim in sims:
+            sim.step()
+        if k == 40:
+            # a write after several reorders: particle 123 made inactive in both runs
+            for sim in sims:
+                st = sim.particles.to_numpy("status")
+                st[123] = 128
+                sim.set_particle_field("status", st)
+    plain, ordered = sims
+    order = ordered.spatial_order.cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(wl.nparticles)) and not np.array_equal(order,
+                                                                                        np.arange(wl.nparticles))
+    for name in plain.particles.arrays:
+        a, b = plain.particles.to_numpy(name), ordered.particles.to_numpy(name)
+        assert np.array_equal(a, b, equal_nan=True), name
+    assert plain.particles.to_numpy("status")[123] == 128
+    ra, rb = plain.output_writers["mem"].arrays(), ordered.output_writers["mem"].arrays()
+    for name in ra:
+        assert np.array_equal(ra[name], rb[name], equal_nan=True), name
