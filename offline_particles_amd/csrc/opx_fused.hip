@@ -224,18 +224,34 @@ __device__ __forceinline__ double w64(const Axis &a, int b, double o) { return (
 
 // The time blend of one stencil value, p*gt + q*ft with each operation rounded
 // in FP32 (no contraction): the same arithmetic as the reference's FP32
-// full-grid blend (fields.py:426-428), and identical in every stencil layout.
+// full-grid blend (fields.py:426-428).  The fused forms keep it for the
+// isopycnal-following step's density, whose rate takes the difference of the
+// density and a target close to it and so magnifies any change in its last
+// bits.
 __device__ __forceinline__ float blend_rn(float p, float q, float gt, float ft)
 {
 #pragma clang fp contract(off)
     return p * gt + q * ft;
 }
 
-template <typename T>
+// The fused forms' time blend of velocities and the free surface: p + (q - p)*ft
+// as one difference and one fused multiply-add (two instructions instead of
+// three; within an FP32 rounding or two of blend_rn).  RN selects blend_rn.
+// Every stencil layout and launch form blends with the same operations.
+template <bool RN = false>
+__device__ __forceinline__ float blend_t(float p, float q, float gt, float ft)
+{
+    if constexpr (RN)
+        return blend_rn(p, q, gt, ft);
+    else
+        return fmaf(q - p, ft, p);
+}
+
+template <typename T, bool RN = false>
 __device__ __forceinline__ float blend(const T *__restrict__ p, const T *__restrict__ q, uint32_t i, float gt,
                                        float ft)
 {
-    return blend_rn(ld3(p, i), ld3(q, i), gt, ft);
+    return blend_t<RN>(ld3(p, i), ld3(q, i), gt, ft);
 }
 
 // ---------------------------------------------------------------- stencil corners
@@ -329,44 +345,44 @@ __device__ __forceinline__ void load_unit(const void *__restrict__ pp, const uin
 // Blends unit k of raw nodes into stencil values v (order 000, 001, 010, 011,
 // 100, 101, 110, 111 over (axis0, axis1, axis2)): p*gt + q*ft in FP32, the
 // same operations in every layout.
-template <typename T, int LAYOUT>
+template <typename T, int LAYOUT, bool RN = false>
 __device__ __forceinline__ void blend_unit(const Raw3<T, LAYOUT> &r, int k, float gt, float ft, float v[8])
 {
     if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
         const float2 e0 = unpair(r.n[2 * k]), e1 = unpair(r.n[2 * k + 1]);
-        v[2 * k] = blend_rn(e0.x, e0.y, gt, ft);
-        v[2 * k + 1] = blend_rn(e1.x, e1.y, gt, ft);
+        v[2 * k] = blend_t<RN>(e0.x, e0.y, gt, ft);
+        v[2 * k + 1] = blend_t<RN>(e1.x, e1.y, gt, ft);
     } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
         const float4 e = unquad(r.n[k]);
-        v[2 * k] = blend_rn(e.x, e.z, gt, ft);
-        v[2 * k + 1] = blend_rn(e.y, e.w, gt, ft);
+        v[2 * k] = blend_t<RN>(e.x, e.z, gt, ft);
+        v[2 * k + 1] = blend_t<RN>(e.y, e.w, gt, ft);
     } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
         if constexpr (sizeof(T) == 4) {
             const float8 &e = r.n[k];
-            v[4 * k + 0] = blend_rn(e.a.x, e.b.x, gt, ft);
-            v[4 * k + 1] = blend_rn(e.a.y, e.b.y, gt, ft);
-            v[4 * k + 2] = blend_rn(e.a.z, e.b.z, gt, ft);
-            v[4 * k + 3] = blend_rn(e.a.w, e.b.w, gt, ft);
+            v[4 * k + 0] = blend_t<RN>(e.a.x, e.b.x, gt, ft);
+            v[4 * k + 1] = blend_t<RN>(e.a.y, e.b.y, gt, ft);
+            v[4 * k + 2] = blend_t<RN>(e.a.z, e.b.z, gt, ft);
+            v[4 * k + 3] = blend_t<RN>(e.a.w, e.b.w, gt, ft);
         } else {
             const uint4 &e = r.n[k];
-            v[4 * k + 0] = blend_rn(bf_lo(e.x), bf_lo(e.z), gt, ft);
-            v[4 * k + 1] = blend_rn(bf_hi(e.x), bf_hi(e.z), gt, ft);
-            v[4 * k + 2] = blend_rn(bf_lo(e.y), bf_lo(e.w), gt, ft);
-            v[4 * k + 3] = blend_rn(bf_hi(e.y), bf_hi(e.w), gt, ft);
+            v[4 * k + 0] = blend_t<RN>(bf_lo(e.x), bf_lo(e.z), gt, ft);
+            v[4 * k + 1] = blend_t<RN>(bf_hi(e.x), bf_hi(e.z), gt, ft);
+            v[4 * k + 2] = blend_t<RN>(bf_lo(e.y), bf_lo(e.w), gt, ft);
+            v[4 * k + 3] = blend_t<RN>(bf_hi(e.y), bf_hi(e.w), gt, ft);
         }
     } else {
         if constexpr (sizeof(T) == 4) {
             const float4 &a = r.n[2 * k], &c = r.n[2 * k + 1];
-            v[4 * k + 0] = blend_rn(a.x, a.y, gt, ft);
-            v[4 * k + 1] = blend_rn(c.x, c.y, gt, ft);
-            v[4 * k + 2] = blend_rn(a.z, a.w, gt, ft);
-            v[4 * k + 3] = blend_rn(c.z, c.w, gt, ft);
+            v[4 * k + 0] = blend_t<RN>(a.x, a.y, gt, ft);
+            v[4 * k + 1] = blend_t<RN>(c.x, c.y, gt, ft);
+            v[4 * k + 2] = blend_t<RN>(a.z, a.w, gt, ft);
+            v[4 * k + 3] = blend_t<RN>(c.z, c.w, gt, ft);
         } else {
             const uint2 &a = r.n[2 * k], &c = r.n[2 * k + 1];
-            v[4 * k + 0] = blend_rn(bf_lo(a.x), bf_hi(a.x), gt, ft);
-            v[4 * k + 1] = blend_rn(bf_lo(c.x), bf_hi(c.x), gt, ft);
-            v[4 * k + 2] = blend_rn(bf_lo(a.y), bf_hi(a.y), gt, ft);
-            v[4 * k + 3] = blend_rn(bf_lo(c.y), bf_hi(c.y), gt, ft);
+            v[4 * k + 0] = blend_t<RN>(bf_lo(a.x), bf_hi(a.x), gt, ft);
+            v[4 * k + 1] = blend_t<RN>(bf_lo(c.x), bf_hi(c.x), gt, ft);
+            v[4 * k + 2] = blend_t<RN>(bf_lo(a.y), bf_hi(a.y), gt, ft);
+            v[4 * k + 3] = blend_t<RN>(bf_lo(c.y), bf_hi(c.y), gt, ft);
         }
     }
 }
@@ -395,18 +411,18 @@ __device__ __forceinline__ void load_raw(const void *__restrict__ pp, int n1, in
         load_unit<T, LAYOUT>(pp, rows, k, r);
 }
 
-template <typename T, int LAYOUT>
+template <typename T, int LAYOUT, bool RN = false>
 __device__ __forceinline__ void blend_raw(const Raw3<T, LAYOUT> &r, float gt, float ft, float v[8])
 {
 #pragma unroll
     for (int k = 0; k < kRawUnits<LAYOUT>; ++k)
-        blend_unit<T, LAYOUT>(r, k, gt, ft, v);
+        blend_unit<T, LAYOUT, RN>(r, k, gt, ft, v);
 }
 
 // The 8 blended stencil values of a field with dims (., n1, n2) and base
 // (b0, b1, b2).  RING reads slot pointers p and q; the packed layouts read
 // window p, blending each unit as soon as it is loaded.
-template <typename T, int LAYOUT>
+template <typename T, int LAYOUT, bool RN = false>
 __device__ __forceinline__ void corners(const void *__restrict__ pp, const void *__restrict__ qq, float gt, float ft,
                                         int n1, int n2, int b0, int b1, int b2, float v[8])
 {
@@ -416,15 +432,15 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
         const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            v[2 * r] = blend(p, q, rows[r], gt, ft);
-            v[2 * r + 1] = blend(p, q, rows[r] + 1, gt, ft);
+            v[2 * r] = blend<T, RN>(p, q, rows[r], gt, ft);
+            v[2 * r + 1] = blend<T, RN>(p, q, rows[r] + 1, gt, ft);
         }
     } else {
         Raw3<T, LAYOUT> r;
 #pragma unroll
         for (int k = 0; k < kRawUnits<LAYOUT>; ++k) {
             load_unit<T, LAYOUT>(pp, rows, k, r);
-            blend_unit<T, LAYOUT>(r, k, gt, ft, v);
+            blend_unit<T, LAYOUT, RN>(r, k, gt, ft, v);
         }
     }
 }
@@ -533,14 +549,13 @@ __device__ __forceinline__ void load2_ring(const float4 *__restrict__ hdd, const
     (void)want_dxdy;
 }
 
-// h and zeta (blended in FP32 like the reference, then widened) bilinear
-// with FP64 weights.
+// h and zeta (blended in FP32, then widened) bilinear with FP64 weights.
 __device__ __forceinline__ void eval2(const Corners2 &c, float gt, float ft, double wy, double wx, double &h,
                                       double &zeta)
 {
     h = bil64(c.h[0], c.h[1], c.h[2], c.h[3], wy, wx);
-    zeta = bil64(blend_rn(c.z0[0], c.z1[0], gt, ft), blend_rn(c.z0[1], c.z1[1], gt, ft),
-                 blend_rn(c.z0[2], c.z1[2], gt, ft), blend_rn(c.z0[3], c.z1[3], gt, ft), wy, wx);
+    zeta = bil64(blend_t(c.z0[0], c.z1[0], gt, ft), blend_t(c.z0[1], c.z1[1], gt, ft),
+                 blend_t(c.z0[2], c.z1[2], gt, ft), blend_t(c.z0[3], c.z1[3], gt, ft), wy, wx);
 }
 
 __device__ __forceinline__ float bil32(const float v[4], float fy, float fx)
@@ -714,9 +729,10 @@ __global__ void k_step_incr(int32_t *step, int nsteps) { *step += nsteps; }
 __device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, double hc, int NZ, const double *Cw,
                                               int k0)
 {
-    const double S = (z - zeta) / (zeta + h);
-    const double inv = 1.0 / (hc + h);
-    auto S_at = [&](int k) { return (hc * Cw[kMaxC + k] + h * Cw[k]) * inv; };
+    // S and the level values S_k = (hc sigma_k + h C_k) / (hc + h), all scaled
+    // by hc + h > 0 (order and ratios unchanged): two divisions instead of three
+    const double S = (z - zeta) * (hc + h) / (zeta + h);
+    auto S_at = [&](int k) { return hc * Cw[kMaxC + k] + h * Cw[k]; };
     int k = max(0, min(k0, NZ - 1));
     double Sk = S_at(k);
     while (k > 0 && S < Sk) {
@@ -852,16 +868,16 @@ __device__ __forceinline__ void corners_tile(const uint32_t *pool, const Box &b,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const float2 e0 = P[rows[r]], e1 = P[rows[r] + 1];
-            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
-            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
+            v[2 * r] = blend_t(e0.x, e0.y, gt, ft);
+            v[2 * r + 1] = blend_t(e1.x, e1.y, gt, ft);
         }
     } else {
         const uint32_t *P = pool + b.off;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const float2 e0 = unpair(P[rows[r]]), e1 = unpair(P[rows[r] + 1]);
-            v[2 * r] = blend_rn(e0.x, e0.y, gt, ft);
-            v[2 * r + 1] = blend_rn(e1.x, e1.y, gt, ft);
+            v[2 * r] = blend_t(e0.x, e0.y, gt, ft);
+            v[2 * r + 1] = blend_t(e1.x, e1.y, gt, ft);
         }
     }
 }
@@ -936,9 +952,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     }
     const float u = tri32(vu, cz.w, cy.w, ixu.w);
     const float v = tri32(vv, cz.w, iyv.w, cx.w);
-    // w with FP64 weights: the vertical index moves through layers that can be
-    // a fraction of a metre thick
-    const double w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+    const double w = (double)tri32(vw, ow.w, cy.w, cx.w);
     const float4 *hdd = reinterpret_cast<const float4 *>(win.hdd);
     Corners2 c2;
     const bool packed0 = LAYOUT != OPX_LAYOUT_RING && j >= 0;
@@ -1083,8 +1097,9 @@ __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, in
 // A packed LAYOUT blends its cached raw nodes again and reloads a field only
 // when its bases or the pair changed; with RING_OK a pair that is not packed
 // (j < 0) reads the rings and leaves nothing cached, without it the caller has
-// checked that j >= 0.
-template <typename T, int LAYOUT, bool RING_OK>
+// checked that j >= 0.  RN3: the third field (the isopycnal-following
+// step's density) is blended with blend_rn.
+template <typename T, int LAYOUT, bool RING_OK, bool RN3 = false>
 __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid &g, int j, int s0, int s1, float gt,
                                           float ft, int bz, int by, int bx, int bxu, int byv, int bzw,
                                           StencilCache<T, LAYOUT> &c, float vu[8], float vv[8], float vw[8])
@@ -1112,11 +1127,11 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
             blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
             blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
             if constexpr (kCacheW) {
-                blend_raw<T, LAYOUT>(c.w, gt, ft, vw);
+                blend_raw<T, LAYOUT, RN3>(c.w, gt, ft, vw);
             } else {
                 Raw3<T, LAYOUT> rw;
                 load_raw<T, LAYOUT>(window(win.pw, win.pw_stride, j), NY, NX, bzw, by, bx, rw);
-                blend_raw<T, LAYOUT>(rw, gt, ft, vw);
+                blend_raw<T, LAYOUT, RN3>(rw, gt, ft, vw);
             }
         }
     }
@@ -1128,8 +1143,8 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
                                         bxu, vu);
             corners<T, OPX_LAYOUT_RING>(V + s0 * win.v_stride, V + s1 * win.v_stride, gt, ft, NY - 1, NX, bz, byv,
                                         bx, vv);
-            corners<T, OPX_LAYOUT_RING>(W + s0 * win.w_stride, W + s1 * win.w_stride, gt, ft, NY, NX, bzw, by, bx,
-                                        vw);
+            corners<T, OPX_LAYOUT_RING, RN3>(W + s0 * win.w_stride, W + s1 * win.w_stride, gt, ft, NY, NX, bzw, by,
+                                             bx, vw);
             c.s0 = -1;
         }
     }
@@ -1163,19 +1178,19 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         return false;
     // the third field's z base: w-points (outer) for w, rho points for density
     const int bz3 = ISO ? cz.b : ow.b;
-    blend_uvw<T, LAYOUT, false>(win, g, j, rs.s0, rs.s1, gt, ft, cz.b, cy.b, cx.b, ixu.b, iyv.b, bz3, c, vu, vv,
-                                vw);
+    blend_uvw<T, LAYOUT, false, ISO>(win, g, j, rs.s0, rs.s1, gt, ft, cz.b, cy.b, cx.b, ixu.b, iyv.b, bz3, c, vu,
+                                     vv, vw);
     const float u = tri32(vu, cz.w, cy.w, ixu.w);
     const float v = tri32(vv, cz.w, iyv.w, cx.w);
-    // w with FP64 weights: the vertical index moves through layers that can be
-    // a fraction of a metre thick
+    // w in FP32 like u and v; the density of the isopycnal rate with FP64
+    // weights and separately rounded terms (the rate magnifies its last bits)
     double w;
     if constexpr (ISO) {
         const double rho = tri64_rn(vw, w64(az, cz.b, 0.0), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
         const double drho = rho - rho_t;
         w = iso.w_max * drho / (iso.rho_scale + fabs(drho));
     } else {
-        w = tri64(vw, w64(az, ow.b, 0.5), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+        w = (double)tri32(vw, ow.w, cy.w, cx.w);
     }
     Corners2 &c2 = c.c2;
     sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
@@ -1276,36 +1291,38 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
     double rho_t = 0.0;
     if constexpr (SCHEME == kAB3ISO)
         rho_t = s.rho_t[i];
-    bool loaded = false, moved = false;
+    // the state, loaded before the steps (the loop then needs none of the
+    // state's pointers)
+    {
+        const uint32_t cell = (uint32_t)s.cell[i];
+        p.az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+        p.ay = decode_axis(cell, g.xbits, g.ybits);
+        p.ax = decode_axis(cell, 0, g.xbits);
+        if (p.az.ok && p.ay.ok && p.ax.ok) {
+            p.az.q = s.fz[i];
+            p.ay.q = s.fy[i];
+            p.ax.q = s.fx[i];
+        }
+        if constexpr (SCHEME != kRK2) {
+            if (p.st != OPX_MULTISTEP_1) {
+                p.hz1 = s.hz1[i];
+                p.hy1 = s.hy1[i];
+                p.hx1 = s.hx1[i];
+                if (p.st != OPX_MULTISTEP_2) {
+                    p.hz2 = s.hz2[i];
+                    p.hy2 = s.hy2[i];
+                    p.hx2 = s.hx2[i];
+                }
+            }
+        }
+    }
+    bool moved = false;
+    float zlast = 0.0f;
 #pragma unroll 1
     for (int k = 0; k < nsteps; ++k) {
         const Step rs = load_release(table + (int64_t)k * tstride, r, win.nslots);
         if (!rs.active)
             continue;
-        if (!loaded) {
-            const uint32_t cell = (uint32_t)s.cell[i];
-            p.az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
-            p.ay = decode_axis(cell, g.xbits, g.ybits);
-            p.ax = decode_axis(cell, 0, g.xbits);
-            if (p.az.ok && p.ay.ok && p.ax.ok) {
-                p.az.q = s.fz[i];
-                p.ay.q = s.fy[i];
-                p.ax.q = s.fx[i];
-            }
-            if constexpr (SCHEME != kRK2) {
-                if (p.st != OPX_MULTISTEP_1) {
-                    p.hz1 = s.hz1[i];
-                    p.hy1 = s.hy1[i];
-                    p.hx1 = s.hx1[i];
-                    if (p.st != OPX_MULTISTEP_2) {
-                        p.hz2 = s.hz2[i];
-                        p.hy2 = s.hy2[i];
-                        p.hx2 = s.hx2[i];
-                    }
-                }
-            }
-            loaded = true;
-        }
         if (!(p.az.ok && p.ay.ok && p.ax.ok)) {
             p.st = OPX_NONFINITE;
             break;
@@ -1316,7 +1333,7 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
             p.st = OPX_OUT_OF_DOMAIN;
             break;
         }
-        float *zout = s.z ? s.z + i : nullptr;
+        float *zout = &zlast; // the last step's z, stored after the loop
         bool ok;
         if constexpr (SCHEME == kAB3)
             ok = ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, zout);
@@ -1332,6 +1349,8 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         moved = true;
     }
     if (moved) {
+        if (s.z)
+            s.z[i] = zlast;
         s.cell[i] = (int32_t)pack_cell(p.az, p.ay, p.ax, g);
         s.fz[i] = p.az.q;
         s.fy[i] = p.ay.q;
@@ -1798,7 +1817,7 @@ __device__ __forceinline__ AxD axd(double idx, double off, int size)
 }
 
 
-// u, v (FP32 trilinear) and w (FP64 weights) of pair (s0, s1) at weights
+// u, v, w (FP32 trilinear) of pair (s0, s1) at weights
 // (gt, ft) at an FP64 position (rk2_w_advection.pyx:124-150), through the
 // stencil cache.
 template <typename T, int LAYOUT, bool RING_OK>
@@ -1813,7 +1832,7 @@ __device__ __forceinline__ void rk2_uvw(const opx_window &win, const opx_grid &g
     blend_uvw<T, LAYOUT, RING_OK>(win, g, j, s0, s1, gt, ft, cz.b, cy.b, cx.b, ix.b, iy.b, oz.b, c, vu, vv, vw);
     u = tri32(vu, (float)cz.w, (float)cy.w, (float)ix.w);
     v = tri32(vv, (float)cz.w, (float)iy.w, (float)cx.w);
-    w = tri64(vw, oz.w, cy.w, cx.w);
+    w = (double)tri32(vw, (float)oz.w, (float)cy.w, (float)cx.w);
 }
 
 // h and zeta (FP64 weights; zeta blended in FP32) and, with DXDY, dx and dy
@@ -1857,7 +1876,7 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     rk2_2d<LAYOUT, RING_OK, true>(win, j, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, c, h0, zeta0, dxv, dyv);
     const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
     const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
-    const double sigma = (zi + 0.5) / NZ - 1.0;
+    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
     const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
     const double z = zeta0 + (zeta0 + h0) * S0;
     float u1, v1;
