@@ -569,8 +569,8 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.rebin()
     eng.step(warmup)
     if tune:
-        # steps per launch from 1, 8, 16 and 32 (16 and 32 are the profiled launch sizes)
-        log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32)))
+        # steps per launch from 1, 8, 16, 32 and 64 (1 and 32 are the profiled launch sizes)
+        log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32, 64)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -830,18 +830,15 @@ def batched_accuracy(wl, samples):
 # (form, steps per launch) -> summary
 _PROFILE_FOR_FORM = {
     ("plain_fp32", 1): "k_ab3_plain_turbulent16M.md",
+    ("plain_fp32_ordered", 1): "k_ab3_plain_ordered_turbulent16M.md",
     ("fused_fp32", 1): "k_ab3_fused_f32_turbulent16M.md",
     ("fused_bf16", 1): "k_ab3_fused_bf16_turbulent16M.md",
-    ("fused_fp32", 16): "k_ab3_fused_steps16_f32_turbulent16M.md",
-    ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 1): "k_rk2_fused_f32_turbulent16M.md",
-    ("fused_rk2_fp32", 16): "k_rk2_fused_steps16_f32_turbulent16M.md",
     ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M.md",
     ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 32): "k_rk2_fused_steps32_f32_turbulent16M.md",
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
-    ("fused_iso_fp32", 16): "k_ab3_iso_fused_steps16_turbulent16M.md",
     ("fused_iso_fp32", 32): "k_ab3_iso_fused_steps32_turbulent16M.md",
     ("output_fused_fp32", 1): "k_sample_turbulent16M.md",
 }
@@ -893,9 +890,8 @@ def profiled_traffic_source(form, particles, grid, steps_per_launch=1):
 
 # (form, steps per launch) -> shader-counter summary holding valu_insts_per_wave_step
 _COUNTERS_FOR_FORM = {
-    ("fused_bf16", 16): "k_ab3_fused_steps16_bf16_turbulent16M_counters.md",
-    ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M_counters.md",
-    ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M_counters.md",
+    ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M.md",
+    ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M.md",
 }
 
 
@@ -1232,10 +1228,10 @@ def main():
     device = torch.device("cuda", 0)
     NX, NY, NZ = (int(s) for s in args.grid.split("x"))
     t = time.perf_counter()
-    # time axis: warm-up, the autotuner's trial steps (~350) and the timed window
-    # (the requested steps rounded up to whole re-sort cycles of at most 400 steps)
+    # time axis: warm-up, the autotuner's trial steps (~600) and the timed window
+    # (the requested steps rounded up to whole re-sort cycles of at most 448 steps)
     wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=args.particles,
-                            steps=args.steps + args.warmup + 1000, dtype=np.float32)
+                            steps=args.steps + args.warmup + 1600, dtype=np.float32)
     log(f"workload generated in {time.perf_counter() - t:.1f}s: grid {NX}x{NY}x{NZ}, {args.particles} particles")
 
     if args.copy_calibration:
