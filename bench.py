@@ -1058,14 +1058,36 @@ def accuracy(wl, device, nsub, steps):
         out[name] = report(name, *(g.particles.to_numpy(k) for k in ("zidx", "yidx", "xidx", "status")))
         del g
     fs = build_fieldset(wl, device=device)
+    res_bf16 = None
     for name, dt in (("fused_fp32", torch.float32), ("fused_bf16", torch.bfloat16)):
         eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=dt, device=device, capacity=nsub)
         eng.add_release(*pos, nsteps=steps)
         eng.step(steps)
         eng.check_clock()
         z, y, x = (t_.cpu().numpy() for t_ in eng.positions())
-        out[name] = report(name, z, y, x, eng.status().cpu().numpy())
+        st = eng.status().cpu().numpy()
+        out[name] = report(name, z, y, x, st)
+        if dt == torch.bfloat16:
+            res_bf16 = (z, y, x, st)
         del eng
+    # BF16 storage error vs kernel arithmetic: the oracle again on u, v, w rounded
+    # to BF16 (held in FP32), against which the BF16 form differs only by its
+    # arithmetic
+    t = time.perf_counter()
+    rounded = dict(wl.fields)
+    for k in ("u", "v", "w"):
+        rounded[k] = torch.from_numpy(wl.fields[k]).to(torch.bfloat16).to(torch.float32).numpy()
+    fields_r = {k: OracleField(v, OFFSETS[k], k in TIME_DEPENDENT) for k, v in rounded.items()}
+    sim = OracleSimulation(fields_r, wl.time_array, wl.dt, scheme="ab3", nparticles=nsub, hc=wl.hc, NZ=wl.NZ,
+                           bounds=wl.bounds, threads=threads)
+    for k, a in zip(("zidx", "yidx", "xidx"), pos):
+        sim.p[k][:] = a
+    sim.p["status"][:] = 10
+    sim.run(steps)
+    del fields_r, rounded
+    log(f"accuracy: oracle on BF16-rounded fields {nsub} x {steps} steps in {time.perf_counter() - t:.1f}s")
+    ref = {k: sim.p[k] for k in ("zidx", "yidx", "xidx", "status")}
+    out["fused_bf16_vs_bf16_rounded_fields"] = report("fused_bf16_rounded", *res_bf16)
     log("accuracy:", out)
     return out
 
