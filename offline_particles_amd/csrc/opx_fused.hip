@@ -721,6 +721,20 @@ __global__ void k_step_incr(int32_t *step, int nsteps) { *step += nsteps; }
 
 // ---------------------------------------------------------------- arithmetic shared by the fused steps
 
+// a / b for the fused forms' vertical transform: the hardware reciprocal of
+// b refined by two Newton-Raphson steps, the quotient, and one correction of
+// the quotient from its fused residual (eight instructions instead of the IEEE
+// divide's eleven; within an ulp or so of it).  b is a positive normal number
+// well inside the FP64 range (a depth, a layer-level difference).
+__device__ __forceinline__ double div64(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+
 // zidx of an FP64 z with h, zeta at the particle (_compute_zidx,
 // _vertical_coordinate.pxd:41-103): S = (z - zeta) / (zeta + h), the bracket k
 // with S_k <= S < S_{k+1} on the w-levels clamped to [0, NZ-1] (S_k increases
@@ -731,7 +745,7 @@ __device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, d
 {
     // S and the level values S_k = (hc sigma_k + h C_k) / (hc + h), all scaled
     // by hc + h > 0 (order and ratios unchanged): two divisions instead of three
-    const double S = (z - zeta) * (hc + h) / (zeta + h);
+    const double S = div64((z - zeta) * (hc + h), zeta + h);
     auto S_at = [&](int k) { return hc * Cw[kMaxC + k] + h * Cw[k]; };
     int k = max(0, min(k0, NZ - 1));
     double Sk = S_at(k);
@@ -745,7 +759,7 @@ __device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, d
         Sk = Sk1;
         Sk1 = S_at(k + 1);
     }
-    return ((double)k - 0.5) + (S - Sk) / (Sk1 - Sk);
+    return ((double)k - 0.5) + div64(S - Sk, Sk1 - Sk);
 }
 
 // Euler -> AB2 -> AB3 start-up of one coordinate (ab3_w_advection.pyx:121-138):
@@ -774,22 +788,19 @@ __device__ __forceinline__ double z_at(const Axis &az, int ob, double h, double 
     const double wC = w64(az, ob, 0.5);
     const double Cv = (1.0 - wC) * Cw[ob] + wC * Cw[ob + 1];
     const double sigma = (az.pos() + 0.5) * Cw[kInvNZ] - 1.0;
-    const double S0 = (hc * sigma + h * Cv) / (hc + h);
+    const double S0 = div64(hc * sigma + h * Cv, hc + h);
     return zeta + (zeta + h) * S0;
 }
 
-// The AB3 update of one coordinate (ab3_w_advection.pyx:168-170) in FP64.
-__device__ __forceinline__ double ab3_advance(double x, double dt, double f0, double f1, double f2)
+// The AB3 update of one coordinate (ab3_w_advection.pyx:168-170) in FP64 from
+// the FP32 rates (this step's and the FP32 history): the combination
+// 23 f0 - 16 f1 + 5 f2 with fused multiply-adds (formed in FP32 it would lose
+// bits to the cancellation between its terms), added to the coordinate times
+// dt / 12 with one more.
+__device__ __forceinline__ double ab3_advance(double x, double dt12, float f0, float f1, float f2)
 {
-    return x + dt * (23.0 * f0 - 16.0 * f1 + 5.0 * f2) * kInv12;
-}
-
-// The vertical update's f_{n-1}, f_{n-2} in FP64: this step's FP64 w where the
-// start-up uses f0 (f1, f2 are the FP32 start-up's results).
-__device__ __forceinline__ void ab3_startup_w(uint8_t st, double w, float f1, float f2, double &w1, double &w2)
-{
-    w1 = st == OPX_MULTISTEP_1 ? w : (double)f1;
-    w2 = st == OPX_MULTISTEP_1 ? w : (st == OPX_MULTISTEP_2 ? -w + 2.0 * w1 : (double)f2);
+    const double c = fma(5.0, (double)f2, fma(-16.0, (double)f1, 23.0 * (double)f0));
+    return fma(c, dt12, x);
 }
 
 // ---------------------------------------------------------------- fused AB3
@@ -976,14 +987,11 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     ab3_startup(st, dz0, [&] { return s.hz1[i]; }, [&] { return s.hz2[i]; }, dz1, dz2);
     const uint8_t st_new = ab3_next_status(st);
 
-    // ---- z at x_n, t_n, then the AB3 update in FP64 from FP32 derivatives; the
-    // vertical uses this step's FP64 w
-    const double hc = g.hc;
-    double dz1d, dz2d;
-    ab3_startup_w(st, w, dz1, dz2, dz1d, dz2d);
-    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt, w, dz1d, dz2d);
-    ay = encode_axis(ab3_advance(yi, dt, (double)dy0, (double)dy1, (double)dy2), NY);
-    ax = encode_axis(ab3_advance(xi, dt, (double)dx0, (double)dx1, (double)dx2), NX);
+    // ---- z at x_n, t_n, then the AB3 update from the FP32 rates
+    const double hc = g.hc, dt12 = dt * kInv12;
+    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    ay = encode_axis(ab3_advance(yi, dt12, dy0, dy1, dy2), NY);
+    ax = encode_axis(ab3_advance(xi, dt12, dx0, dx1, dx2), NX);
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
     Axis azn;
@@ -1217,12 +1225,10 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     p.st = ab3_next_status(st);
 
     // ---- z at x_n, t_n, then the AB3 update (as ab3_body)
-    const double hc = g.hc;
-    double dz1d, dz2d;
-    ab3_startup_w(st, w, dz1, dz2, dz1d, dz2d);
-    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt, w, dz1d, dz2d);
-    ay = encode_axis(ab3_advance(yi, dt, (double)dy0, (double)dy1, (double)dy2), NY);
-    ax = encode_axis(ab3_advance(xi, dt, (double)dx0, (double)dx1, (double)dx2), NX);
+    const double hc = g.hc, dt12 = dt * kInv12;
+    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    ay = encode_axis(ab3_advance(yi, dt12, dy0, dy1, dy2), NY);
+    ax = encode_axis(ab3_advance(xi, dt12, dx0, dx1, dx2), NX);
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
     Axis azn;
@@ -1877,7 +1883,7 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
     const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
     const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
-    const double S0 = (hc * sigma + h0 * Cv) / (hc + h0);
+    const double S0 = div64(hc * sigma + h0 * Cv, hc + h0);
     const double z = zeta0 + (zeta0 + h0) * S0;
     float u1, v1;
     double w1;
