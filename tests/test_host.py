@@ -261,3 +261,25 @@ def test_fused_engine_rejects_bounds_beyond_split_range(lib):
     assert b"bounds" in lib.opx_last_error()
     g.xidx_min, g.xidx_max = -1.5, NX + 1
     assert lib.opx_split_from_f64(ctypes.byref(s), ctypes.byref(g), None, None, None, None) == _lib.OPX_OK
+
+
+def test_rk2_startup_timestepper_declarations():
+    """AB3 with an RK2 start-up declares the union of the AB3, RK2 and history
+    kernels' particle fields (the RK2 and AB3 derivative names coincide, so the
+    merge is conflict-free), the RK2 parameter as a scalar, and only the
+    reference's two start-up names are accepted."""
+    from offline_particles_amd.kernels import merge_particle_fields, merge_scalars
+    from offline_particles_amd.models import roms
+    from offline_particles_amd.timesteppers import ABRK2StartupTimestepper, ABTimestepper
+
+    ta = np.arange(4, dtype=np.float64) * 3600.0
+    ts = roms.ab3_w_advection_timestepper(ta, np.float64(120.0), startup="rk2")
+    assert isinstance(ts, ABRK2StartupTimestepper) and ts.alpha == 2 / 3
+    fields = merge_particle_fields(list(ts.kernels))
+    for name in ("status", "zidx", "yidx", "xidx", "z", "_dz0", "_dz1", "_dz2", "_dyidx0", "_dyidx1", "_dyidx2",
+                 "_dxidx0", "_dxidx1", "_dxidx2"):
+        assert name in fields, name
+    assert "_RK2_alpha" in merge_scalars(list(ts.kernels))
+    assert type(roms.ab3_w_advection_timestepper(ta, np.float64(120.0))) is ABTimestepper
+    with pytest.raises(ValueError):
+        roms.ab3_w_advection_timestepper(ta, np.float64(120.0), startup="euler")
