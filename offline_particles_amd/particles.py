@@ -115,3 +115,13 @@ class ParticlesView(_FrozenArrayMapping):
         if name in object.__getattribute__(self, "_arrays"):
             return self[name]
         raise AttributeError(f"'{self.__class__.__name__}' object has no attribute '{name}'")
+
+    @property
+    def arrays(self):
+        """The fields in the particles' order (new tensors while the arrays are
+        kept in a spatial order, else the arrays themselves)."""
+        raw = object.__getattribute__(self, "_arrays")
+        order_fn = object.__getattribute__(self, "_order")
+        if order_fn is None or order_fn() is None:
+            return raw
+        return types.MappingProxyType({name: self[name] for name in raw})
