@@ -569,7 +569,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.rebin()
     eng.step(warmup)
     if tune:
-        # steps per launch from 1, 8, 16, 32 and 64 (1 and 32 are the profiled launch sizes)
+        # steps per launch from 1, 8, 16, 32 and 64 (1, 32 and 64 are the profiled launch sizes)
         log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32, 64)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
@@ -837,6 +837,9 @@ _PROFILE_FOR_FORM = {
     ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M.md",
     ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M.md",
     ("fused_rk2_fp32", 32): "k_rk2_fused_steps32_f32_turbulent16M.md",
+    ("fused_fp32", 64): "k_ab3_fused_steps64_f32_turbulent16M.md",
+    ("fused_bf16", 64): "k_ab3_fused_steps64_bf16_turbulent16M.md",
+    ("fused_rk2_fp32", 64): "k_rk2_fused_steps64_f32_turbulent16M.md",
     ("plain_iso_fp32", 1): "k_ab3_iso_plain_turbulent16M.md",
     ("output_fp32", 1): "k_interp_trilinear_turbulent16M.md",
     ("fused_iso_fp32", 32): "k_ab3_iso_fused_steps32_turbulent16M.md",
@@ -892,6 +895,8 @@ def profiled_traffic_source(form, particles, grid, steps_per_launch=1):
 _COUNTERS_FOR_FORM = {
     ("fused_bf16", 32): "k_ab3_fused_steps32_bf16_turbulent16M.md",
     ("fused_fp32", 32): "k_ab3_fused_steps32_f32_turbulent16M.md",
+    ("fused_bf16", 64): "k_ab3_fused_steps64_bf16_turbulent16M.md",
+    ("fused_fp32", 64): "k_ab3_fused_steps64_f32_turbulent16M.md",
 }
 
 
