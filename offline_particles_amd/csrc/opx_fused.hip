@@ -189,6 +189,46 @@ __device__ __forceinline__ uint32_t pack_cell(const Axis &z, const Axis &y, cons
            axis_field(x, g.xbits);
 }
 
+// An index rounded to a multiple of 2^-32 the way encode_axis rounds it, kept
+// in FP64 between the steps of one launch.  For v in [-2, N+2) it equals the
+// decoded stored form of v exactly: encode_axis rounds (v - floor v) 2^32 to an
+// integer (ties to even), v 2^32 differs from that by the even integer
+// floor(v) 2^32, so rint lands on the same multiple; both scalings by 2^32 are
+// exact.  The steps therefore need no encode/decode until the launch stores the
+// particle.  Outside that range the particle is out of the domain and the value
+// is left unclamped (the store clamps it); non-finite stays non-finite.
+__device__ __forceinline__ double round32(double v) { return ldexp(rint(ldexp(v, 32)), -32); }
+
+// One coordinate of a particle about to be sampled: c = floor(v), the exact
+// offset d = v - c (d * 2^32 is the stored offset q) and its FP32 rounding f
+// (the same value as (float)q * 2^-32).
+struct Coord {
+    int c;
+    double d;
+    float f;
+};
+
+__device__ __forceinline__ Coord coord(double v)
+{
+    const double fl = floor(v);
+    const double d = v - fl;
+    return {(int)fl, d, (float)d};
+}
+
+__device__ __forceinline__ Coord coord(const Axis &a)
+{
+    const double d = (double)a.q * (1.0 / kQ);
+    return {a.c, d, a.frac()};
+}
+
+// FP64 weight of the upper node for base b and stagger offset o at index v:
+// v - b + o, exact for a decoded stored axis or a round32 value in the grid.
+__device__ __forceinline__ double w64(double v, int b, double o) { return (v - (double)b) + o; }
+
+// The base of the 2-D corners at a position just stepped to: floor(v) clamped
+// to [0, N-2] in FP64, so that any finite v converts safely.
+__device__ __forceinline__ int base2(double v, int N) { return (int)fmin(fmax(floor(v), 0.0), (double)(N - 2)); }
+
 // One interpolation axis: base index and weight of the upper node.
 struct Ax {
     int b;
@@ -197,30 +237,31 @@ struct Ax {
 
 // CENTER stagger (offset 0, size N): truncate(idx) clamped to [0, N-2]
 // (kernels/_interpolation/linear.pxd:3-9), weight unclamped (extrapolation).
-__device__ __forceinline__ Ax ax_center(const Axis &a, int N)
+__device__ __forceinline__ Ax ax_center(const Coord &a, int N)
 {
     int b = max(0, min(a.c, N - 2));
-    return {b, a.frac() + (float)(a.c - b)};
+    return {b, a.f + (float)(a.c - b)};
 }
 
 // INNER stagger (offset -0.5, size N-1): truncate(idx - 0.5) clamped to [0, N-3].
-__device__ __forceinline__ Ax ax_inner(const Axis &a, int N)
+__device__ __forceinline__ Ax ax_inner(const Coord &a, int N)
 {
-    int b = a.c - 1 + (a.upper_half() ? 1 : 0);
+    int b = a.c - 1 + (a.d >= 0.5 ? 1 : 0);
     b = max(0, min(b, N - 3));
-    return {b, a.frac() + ((float)(a.c - b) - 0.5f)};
+    return {b, a.f + ((float)(a.c - b) - 0.5f)};
 }
 
 // OUTER stagger (offset +0.5, size N+1): truncate(idx + 0.5) clamped to [0, N-1].
-__device__ __forceinline__ Ax ax_outer(const Axis &a, int N)
+__device__ __forceinline__ Ax ax_outer(const Coord &a, int N)
 {
-    int b = a.c + (a.upper_half() ? 1 : 0);
+    int b = a.c + (a.d >= 0.5 ? 1 : 0);
     b = max(0, min(b, N - 1));
-    return {b, a.frac() + ((float)(a.c - b) + 0.5f)};
+    return {b, a.f + ((float)(a.c - b) + 0.5f)};
 }
 
-// FP64 weight of the upper node for base b and stagger offset o.
-__device__ __forceinline__ double w64(const Axis &a, int b, double o) { return ((double)(a.c - b) + o) + (double)a.q * (1.0 / kQ); }
+__device__ __forceinline__ Ax ax_center(const Axis &a, int N) { return ax_center(coord(a), N); }
+__device__ __forceinline__ Ax ax_inner(const Axis &a, int N) { return ax_inner(coord(a), N); }
+__device__ __forceinline__ Ax ax_outer(const Axis &a, int N) { return ax_outer(coord(a), N); }
 
 // The time blend of one stencil value, p*gt + q*ft with each operation rounded
 // in FP32 (no contraction): the same arithmetic as the reference's FP32
@@ -449,7 +490,6 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
 // multiply-add: one rounding fewer than two products and a sum, one
 // instruction fewer, and the same result wherever the helper is inlined.
 __device__ __forceinline__ float lerp32(float a, float b, float g, float f) { return fmaf(b, f, a * g); }
-__device__ __forceinline__ double lerp64(double a, double b, double g, double f) { return fma(b, f, a * g); }
 
 // a / b for the fused steps' index rates (velocity over grid spacing): the
 // hardware reciprocal estimate of b, the quotient, and one Newton-Raphson
@@ -472,16 +512,8 @@ __device__ __forceinline__ float tri32(const float v[8], float f0, float f1, flo
     return lerp32(lerp32(c00, c01, g1, f1), lerp32(c10, c11, g1, f1), g0, f0);
 }
 
-__device__ __forceinline__ double tri64(const float v[8], double f0, double f1, double f2)
-{
-    const double g0 = 1.0 - f0, g1 = 1.0 - f1, g2 = 1.0 - f2;
-    const double c00 = lerp64(v[0], v[1], g2, f2), c01 = lerp64(v[2], v[3], g2, f2);
-    const double c10 = lerp64(v[4], v[5], g2, f2), c11 = lerp64(v[6], v[7], g2, f2);
-    return lerp64(lerp64(c00, c01, g1, f1), lerp64(c10, c11, g1, f1), g0, f0);
-}
-
-// tri64 with every product and sum rounded on its own (no fused multiply-add),
-// for the density of the isopycnal rate: the rate takes the difference of this
+// The FP64 trilinear of FP32 stencil values with every product and sum rounded
+// on its own (no fused multiply-add), for the density of the isopycnal rate: the rate takes the difference of this
 // density and a target close to it, which magnifies any change in the last
 // bits of the density, so the density keeps this rounding.
 __device__ __forceinline__ double tri64_rn(const float v[8], double f0, double f1, double f2)
@@ -783,11 +815,11 @@ __device__ __forceinline__ uint8_t ab3_next_status(uint8_t st)
 
 // z at the particle (ab3_w_advection.pyx:144-165; Vtransform 2,
 // _vertical_coordinate.pxd:19-35) in FP64, C = Cs_w at w-levels (offset +0.5).
-__device__ __forceinline__ double z_at(const Axis &az, int ob, double h, double zeta, double hc, const double *Cw)
+__device__ __forceinline__ double z_at(double zi, int ob, double h, double zeta, double hc, const double *Cw)
 {
-    const double wC = w64(az, ob, 0.5);
+    const double wC = w64(zi, ob, 0.5);
     const double Cv = (1.0 - wC) * Cw[ob] + wC * Cw[ob + 1];
-    const double sigma = (az.pos() + 0.5) * Cw[kInvNZ] - 1.0;
+    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
     const double S0 = div64(hc * sigma + h * Cv, hc + h);
     return zeta + (zeta + h) * S0;
 }
@@ -930,8 +962,8 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const Step &rs = p.rs;
     const uint8_t st = p.st;
-    Axis az = p.az, ay = p.ay, ax = p.ax;
-    const double yi = ay.pos(), xi = ax.pos();
+    const Axis az = p.az, ay = p.ay, ax = p.ax;
+    const double zi = az.pos(), yi = ay.pos(), xi = ax.pos();
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
     const float gt = rs.gt, ft = rs.ft;
@@ -975,7 +1007,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
         load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
                    c2);
     double h0, zeta0;
-    const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
+    const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
     eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
@@ -987,19 +1019,20 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     ab3_startup(st, dz0, [&] { return s.hz1[i]; }, [&] { return s.hz2[i]; }, dz1, dz2);
     const uint8_t st_new = ab3_next_status(st);
 
-    // ---- z at x_n, t_n, then the AB3 update from the FP32 rates
+    // ---- z at x_n, t_n, then the AB3 update from the FP32 rates; the new
+    // position rounded as stored (the several-step kernels keep this form)
     const double hc = g.hc, dt12 = dt * kInv12;
-    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
-    ay = encode_axis(ab3_advance(yi, dt12, dy0, dy1, dy2), NY);
-    ax = encode_axis(ab3_advance(xi, dt12, dx0, dx1, dx2), NX);
+    const double z = ab3_advance(z_at(zi, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    const double yn = round32(ab3_advance(yi, dt12, dy0, dy1, dy2));
+    const double xn = round32(ab3_advance(xi, dt12, dx0, dx1, dx2));
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
-    Axis azn;
-    if (ay.ok && ax.ok) {
+    double zn = __builtin_nan(""); // zidx of a non-finite position is non-finite too
+    if (__builtin_isfinite(yn) && __builtin_isfinite(xn)) {
         // h, zeta at t_{n+1} at the new position; most particles stay in their
         // cell and the zeta snapshot pair does not change, so the corners read
         // above are reused
-        const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
+        const int by1 = base2(yn, NY), bx1 = base2(xn, NX);
         const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
         const bool same_pair = packed0 ? (j1 == j) : (j1 < 0 && rs.s0p == rs.s0 && rs.s1p == rs.s1);
         if (!(same_pair && by1 == c2.by && bx1 == c2.bx)) {
@@ -1013,17 +1046,16 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
                            bx1, false, c2);
         }
         double h1, zeta1;
-        eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
-        azn = encode_axis(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b), NZ);
-    } else {
-        azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
+        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1, 0.0), w64(xn, bx1, 0.0), h1, zeta1);
+        zn = round32(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b));
     }
 
     // ---- store: position, history shift (ab3_w_advection.pyx:173-178), status
-    s.cell[i] = (int32_t)pack_cell(azn, ay, ax, g);
+    const Axis azn = encode_axis(zn, NZ), ayn = encode_axis(yn, NY), axn = encode_axis(xn, NX);
+    s.cell[i] = (int32_t)pack_cell(azn, ayn, axn, g);
     s.fz[i] = azn.q;
-    s.fy[i] = ay.q;
-    s.fx[i] = ax.q;
+    s.fy[i] = ayn.q;
+    s.fx[i] = axn.q;
     s.hz2[i] = dz1;
     s.hy2[i] = dy1;
     s.hx2[i] = dx1;
@@ -1048,7 +1080,7 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
 
 // A particle's state while one launch steps it in registers.
 struct PState {
-    Axis az, ay, ax;
+    double zi, yi, xi; // the indices in the form round32 gives (non-finite where the stored axis is)
     float hz1, hy1, hx1, hz2, hy2, hx2; // derivative history f_{n-1}, f_{n-2}
     uint8_t st;
 };
@@ -1171,14 +1203,13 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
                                          IsoRate iso = IsoRate{}, double rho_t = 0.0)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
-    const Axis az = p.az;
-    Axis ay = p.ay, ax = p.ax;
-    const double yi = ay.pos(), xi = ax.pos();
+    const double zi = p.zi, yi = p.yi, xi = p.xi;
+    const Coord kz = coord(zi), ky = coord(yi), kx = coord(xi);
 
     // ---- f0 = (u/dx, v/dy, w) at (x_n, t_n)  (ab3_w_advection.pyx:86-119)
     const float gt = rs.gt, ft = rs.ft;
-    const Ax cz = ax_center(az, NZ), cy = ax_center(ay, NY), cx = ax_center(ax, NX);
-    const Ax ixu = ax_inner(ax, NX), iyv = ax_inner(ay, NY), ow = ax_outer(az, NZ);
+    const Ax cz = ax_center(kz, NZ), cy = ax_center(ky, NY), cx = ax_center(kx, NX);
+    const Ax ixu = ax_inner(kx, NX), iyv = ax_inner(ky, NY), ow = ax_outer(kz, NZ);
     float vu[8], vv[8], vw[8];
     const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1);
     const int j1 = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
@@ -1194,7 +1225,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     // weights and separately rounded terms (the rate magnifies its last bits)
     double w;
     if constexpr (ISO) {
-        const double rho = tri64_rn(vw, w64(az, cz.b, 0.0), w64(ay, cy.b, 0.0), w64(ax, cx.b, 0.0));
+        const double rho = tri64_rn(vw, w64(zi, cz.b, 0.0), w64(yi, cy.b, 0.0), w64(xi, cx.b, 0.0));
         const double drho = rho - rho_t;
         w = iso.w_max * drho / (iso.rho_scale + fabs(drho));
     } else {
@@ -1203,7 +1234,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     Corners2 &c2 = c.c2;
     sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
     double h0, zeta0;
-    const double wy0 = w64(ay, cy.b, 0.0), wx0 = w64(ax, cx.b, 0.0);
+    const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
     // the last post step evaluated h, zeta here at this time (same corners and
     // weights); kept across steps for BF16 only, where the registers allow
     constexpr bool kKeepHz = sizeof(T) == 2;
@@ -1226,36 +1257,35 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
 
     // ---- z at x_n, t_n, then the AB3 update (as ab3_body)
     const double hc = g.hc, dt12 = dt * kInv12;
-    const double z = ab3_advance(z_at(az, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
-    ay = encode_axis(ab3_advance(yi, dt12, dy0, dy1, dy2), NY);
-    ax = encode_axis(ab3_advance(xi, dt12, dx0, dx1, dx2), NX);
+    const double z = ab3_advance(z_at(zi, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    const double yn = round32(ab3_advance(yi, dt12, dy0, dy1, dy2));
+    const double xn = round32(ab3_advance(xi, dt12, dx0, dx1, dx2));
 
     // ---- post step: zidx from z with h, zeta at t_{n+1} (ab3_w_advection.pyx:214-229)
-    Axis azn;
-    if (ay.ok && ax.ok) {
+    double zn = __builtin_nan(""); // zidx of a non-finite position is non-finite too
+    if (__builtin_isfinite(yn) && __builtin_isfinite(xn)) {
         // h, zeta at t_{n+1} at the new position: most particles stay in their
         // cell and the zeta snapshot pair does not change, so the corners read
         // above are reused
-        const int by1 = max(0, min(ay.c, NY - 2)), bx1 = max(0, min(ax.c, NX - 2));
+        const int by1 = base2(yn, NY), bx1 = base2(xn, NX);
         sample2<LAYOUT, false>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
-        eval2(c2, rs.gt1, rs.ft1, w64(ay, by1, 0.0), w64(ax, bx1, 0.0), h1, zeta1);
+        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1, 0.0), w64(xn, bx1, 0.0), h1, zeta1);
         if constexpr (kKeepHz) {
             c.h = h1;
             c.zeta = zeta1;
             c.ft_hz = rs.ft1;
             c.s0_hz = rs.s0p;
         }
-        azn = encode_axis(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b), NZ);
+        zn = round32(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b));
     } else {
-        azn = encode_axis(__builtin_nan(""), NZ); // zidx of a non-finite position is non-finite too
         c.s0_hz = -1;
     }
 
     // ---- history shift (ab3_w_advection.pyx:173-178)
-    p.az = azn;
-    p.ay = ay;
-    p.ax = ax;
+    p.zi = zn;
+    p.yi = yn;
+    p.xi = xn;
     p.hz2 = dz1;
     p.hy2 = dy1;
     p.hx2 = dx1;
@@ -1301,14 +1331,18 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
     // state's pointers)
     {
         const uint32_t cell = (uint32_t)s.cell[i];
-        p.az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
-        p.ay = decode_axis(cell, g.xbits, g.ybits);
-        p.ax = decode_axis(cell, 0, g.xbits);
-        if (p.az.ok && p.ay.ok && p.ax.ok) {
-            p.az.q = s.fz[i];
-            p.ay.q = s.fy[i];
-            p.ax.q = s.fx[i];
+        Axis az = decode_axis(cell, g.xbits + g.ybits, g.zbits);
+        Axis ay = decode_axis(cell, g.xbits, g.ybits);
+        Axis ax = decode_axis(cell, 0, g.xbits);
+        if (az.ok && ay.ok && ax.ok) {
+            az.q = s.fz[i];
+            ay.q = s.fy[i];
+            ax.q = s.fx[i];
         }
+        const double nan = __builtin_nan("");
+        p.zi = az.ok ? az.pos() : nan;
+        p.yi = ay.ok ? ay.pos() : nan;
+        p.xi = ax.ok ? ax.pos() : nan;
         if constexpr (SCHEME != kRK2) {
             if (p.st != OPX_MULTISTEP_1) {
                 p.hz1 = s.hz1[i];
@@ -1329,11 +1363,11 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         const Step rs = load_release(table + (int64_t)k * tstride, r, win.nslots);
         if (!rs.active)
             continue;
-        if (!(p.az.ok && p.ay.ok && p.ax.ok)) {
+        const double zi = p.zi, yi = p.yi, xi = p.xi;
+        if (!(__builtin_isfinite(zi) && __builtin_isfinite(yi) && __builtin_isfinite(xi))) {
             p.st = OPX_NONFINITE;
             break;
         }
-        const double zi = p.az.pos(), yi = p.ay.pos(), xi = p.ax.pos();
         if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
               xi <= g.xidx_max)) {
             p.st = OPX_OUT_OF_DOMAIN;
@@ -1357,10 +1391,11 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
     if (moved) {
         if (s.z)
             s.z[i] = zlast;
-        s.cell[i] = (int32_t)pack_cell(p.az, p.ay, p.ax, g);
-        s.fz[i] = p.az.q;
-        s.fy[i] = p.ay.q;
-        s.fx[i] = p.ax.q;
+        const Axis az = encode_axis(p.zi, g.NZ), ay = encode_axis(p.yi, g.NY), ax = encode_axis(p.xi, g.NX);
+        s.cell[i] = (int32_t)pack_cell(az, ay, ax, g);
+        s.fz[i] = az.q;
+        s.fy[i] = ay.q;
+        s.fx[i] = ax.q;
         if constexpr (SCHEME != kRK2) {
             s.hz2[i] = p.hz2;
             s.hy2[i] = p.hy2;
@@ -1869,7 +1904,7 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
                                          float *zout)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
-    const double zi = p.az.pos(), yi = p.ay.pos(), xi = p.ax.pos();
+    const double zi = p.zi, yi = p.yi, xi = p.xi;
     const double hc = g.hc;
     const int j = packed_index<LAYOUT>(win, rs.s0, rs.s1), jm = packed_index<LAYOUT>(win, rs.s0m, rs.s1m);
     const int jp = packed_index<LAYOUT>(win, rs.s0p, rs.s1p);
@@ -1907,21 +1942,19 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const double zn = z + b1 * dt * w1 + b2 * dt * w2;
     const double yn = yi + b1 * dt * ky1 + b2 * dt * ky2;
     const double xn = xi + b1 * dt * kx1 + b2 * dt * kx2;
-    const Axis ay = encode_axis(yn, NY), ax = encode_axis(xn, NX);
-    Axis azn;
-    if (ay.ok && ax.ok) {
+    const double yr = round32(yn), xr = round32(xn);
+    double zr = __builtin_nan("");
+    if (__builtin_isfinite(yr) && __builtin_isfinite(xr)) {
         double h1, zeta1;
         float unused0, unused1;
-        rk2_2d<LAYOUT, RING_OK, false>(win, jp, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, ay.pos(), ax.pos(), c, h1,
-                                       zeta1, unused0, unused1);
+        rk2_2d<LAYOUT, RING_OK, false>(win, jp, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, yr, xr, c, h1, zeta1, unused0,
+                                       unused1);
         const int k0 = (int)floor(zis + 0.5);
-        azn = encode_axis(zidx_from_z(zn, h1, zeta1, hc, NZ, Cw, k0), NZ);
-    } else {
-        azn = encode_axis(__builtin_nan(""), NZ);
+        zr = round32(zidx_from_z(zn, h1, zeta1, hc, NZ, Cw, k0));
     }
-    p.az = azn;
-    p.ay = ay;
-    p.ax = ax;
+    p.zi = zr;
+    p.yi = yr;
+    p.xi = xr;
     if (zout)
         *zout = (float)zn;
     return true;
