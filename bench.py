@@ -532,13 +532,13 @@ def field_bytes_touched_fused(wl, pos, storage_bytes_3d, fields_3d=("u", "v", "w
 
 
 # fused AB3 kernel: per active particle it reads status (1), release id (2),
-# cell (4), 3 offsets (12) and 6 history values (24) and writes cell, offsets
-# and history (40 B).
-FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 40
-# fused RK2 kernel: reads status, release id, cell, 3 offsets; writes cell and
-# offsets (no history).  Its field term is counted at the step's start
-# position only (a lower bound: stage 2 reads around x*).
-FUSED_RK2_STATE_BYTES = 1 + 2 + 4 + 12 + 16
+# cell (4), 3 offsets (12), 6 history values (24) and z (8) and writes cell,
+# offsets, history and z (48 B).
+FUSED_STATE_BYTES = 1 + 2 + 4 + 12 + 24 + 8 + 48
+# fused RK2 kernel: reads status, release id, cell, 3 offsets and z; writes
+# cell, offsets and z (no history).  Its field term is counted at the step's
+# start position only (a lower bound: stage 2 reads around x*).
+FUSED_RK2_STATE_BYTES = 1 + 2 + 4 + 12 + 8 + 24
 # fused isopycnal-following AB3 kernel: the AB3 state plus the FP64 target density
 FUSED_ISO_STATE_BYTES = FUSED_STATE_BYTES + 8
 

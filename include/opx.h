@@ -208,10 +208,14 @@ int opx_gather(int64_t n, const int32_t *perm, const void *src, void *dst, int32
  * (c_z + 2) << (xbits + ybits) | (c_y + 2) << xbits | (c_x + 2); an axis field
  * of all ones marks a non-finite index (NaN/inf: flagged NONFINITE by the next
  * validation, exported as NaN).  hz1..hx2 are the FP32 AB3 derivative history
- * f_{n-1}, f_{n-2} (the reference's _dz1.._dxidx2).  `rel` (release id per
- * particle) may be NULL (all release 0); `z` (FP32 output) may be NULL; `pid`
- * is permuted with the state by opx_rebin and may be NULL when rebinning is
- * not used.  `rho_t` (FP64 target density of the isopycnal-following step,
+ * f_{n-1}, f_{n-2} (the reference's _dz1.._dxidx2).  `z` is each particle's
+ * depth z (m, FP64) after its last fused step, which the next step starts from
+ * instead of recomputing it from zidx (the two differ by the rounding of that
+ * round trip); NaN where not known yet: opx_split_from_f64 writes NaN and the
+ * first step computes z from zidx (compute_z).  The step entry points require
+ * it; opx_rebin permutes it.  `rel` (release id per particle) may be NULL (all
+ * release 0); `pid` is permuted with the state by opx_rebin and may be NULL
+ * when rebinning is not used.  `rho_t` (FP64 target density of the isopycnal-following step,
  * the reference's particle field rho_t) is NULL for w advection and is
  * permuted with the state by opx_rebin. */
 typedef struct opx_split_state {
@@ -222,7 +226,7 @@ typedef struct opx_split_state {
     float *hz1, *hy1, *hx1, *hz2, *hy2, *hx2;
     int32_t *pid;
     uint16_t *rel;
-    float *z;
+    double *z;
     double *rho_t;
 } opx_split_state;
 

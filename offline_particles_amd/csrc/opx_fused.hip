@@ -1006,9 +1006,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     else
         load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
                    c2);
-    double h0, zeta0;
     const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
-    eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
 
@@ -1019,10 +1017,17 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     ab3_startup(st, dz0, [&] { return s.hz1[i]; }, [&] { return s.hz2[i]; }, dz1, dz2);
     const uint8_t st_new = ab3_next_status(st);
 
-    // ---- z at x_n, t_n, then the AB3 update from the FP32 rates; the new
-    // position rounded as stored (the several-step kernels keep this form)
+    // ---- z at x_n, t_n (kept from the last step, else compute_z), then the
+    // AB3 update from the FP32 rates; the new position rounded as stored (the
+    // several-step kernels keep this form)
     const double hc = g.hc, dt12 = dt * kInv12;
-    const double z = ab3_advance(z_at(zi, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    double z0 = s.z[i];
+    if (__builtin_isnan(z0)) {
+        double h0, zeta0;
+        eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+        z0 = z_at(zi, ow.b, h0, zeta0, hc, Cw);
+    }
+    const double z = ab3_advance(z0, dt12, dz0, dz1, dz2);
     const double yn = round32(ab3_advance(yi, dt12, dy0, dy1, dy2));
     const double xn = round32(ab3_advance(xi, dt12, dx0, dx1, dx2));
 
@@ -1064,8 +1069,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     s.hx1[i] = dx0;
     if (st_new != st)
         s.status[i] = st_new;
-    if (s.z)
-        s.z[i] = (float)z;
+    s.z[i] = z;
 }
 
 template <typename T, int LAYOUT>
@@ -1081,6 +1085,7 @@ __device__ __forceinline__ void ab3_one(int64_t i, int64_t n, const opx_split_st
 // A particle's state while one launch steps it in registers.
 struct PState {
     double zi, yi, xi; // the indices in the form round32 gives (non-finite where the stored axis is)
+    double z;          // z (m) after the last step; NaN: not known yet (compute_z from zi)
     float hz1, hy1, hx1, hz2, hy2, hx2; // derivative history f_{n-1}, f_{n-2}
     uint8_t st;
 };
@@ -1100,12 +1105,6 @@ struct StencilCache {
     int s0;              // -1: no u, v, w nodes cached
     int s0_2d;           // -1: no 2-D corners cached
     Corners2 c2;
-    // h and zeta at the particle from the last post step, at the time with
-    // weight ft_hz of the pair with first slot s0_hz (-1: none): the next
-    // step's sample at the same position and time
-    double h, zeta;
-    float ft_hz;
-    int s0_hz;
 };
 
 // A field's stencil base (b0, b1, b2) packed like a cell: one compare per field.
@@ -1199,7 +1198,7 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
 // iso.w_max * drho / (iso.rho_scale + |drho|), drho = rho - rho_t.
 template <typename T, int LAYOUT, bool ISO = false>
 __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
-                                         const opx_window &win, double dt, const double *Cw, float *zout,
+                                         const opx_window &win, double dt, const double *Cw,
                                          IsoRate iso = IsoRate{}, double rho_t = 0.0)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
@@ -1233,17 +1232,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     }
     Corners2 &c2 = c.c2;
     sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
-    double h0, zeta0;
     const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
-    // the last post step evaluated h, zeta here at this time (same corners and
-    // weights); kept across steps for BF16 only, where the registers allow
-    constexpr bool kKeepHz = sizeof(T) == 2;
-    if (kKeepHz && c.s0_hz == rs.s0 && c.ft_hz == ft) {
-        h0 = c.h;
-        zeta0 = c.zeta;
-    } else {
-        eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
-    }
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
 
@@ -1255,9 +1244,16 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     ab3_startup(st, dz0, [&] { return p.hz1; }, [&] { return p.hz2; }, dz1, dz2);
     p.st = ab3_next_status(st);
 
-    // ---- z at x_n, t_n, then the AB3 update (as ab3_body)
+    // ---- z at x_n, t_n (kept from the last step, else compute_z), then the
+    // AB3 update (as ab3_body)
     const double hc = g.hc, dt12 = dt * kInv12;
-    const double z = ab3_advance(z_at(zi, ow.b, h0, zeta0, hc, Cw), dt12, dz0, dz1, dz2);
+    double z0 = p.z;
+    if (__builtin_isnan(z0)) {
+        double h0, zeta0;
+        eval2(c2, gt, ft, wy0, wx0, h0, zeta0);
+        z0 = z_at(zi, ow.b, h0, zeta0, hc, Cw);
+    }
+    const double z = ab3_advance(z0, dt12, dz0, dz1, dz2);
     const double yn = round32(ab3_advance(yi, dt12, dy0, dy1, dy2));
     const double xn = round32(ab3_advance(xi, dt12, dx0, dx1, dx2));
 
@@ -1271,29 +1267,20 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         sample2<LAYOUT, false>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
         eval2(c2, rs.gt1, rs.ft1, w64(yn, by1, 0.0), w64(xn, bx1, 0.0), h1, zeta1);
-        if constexpr (kKeepHz) {
-            c.h = h1;
-            c.zeta = zeta1;
-            c.ft_hz = rs.ft1;
-            c.s0_hz = rs.s0p;
-        }
         zn = round32(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b));
-    } else {
-        c.s0_hz = -1;
     }
 
     // ---- history shift (ab3_w_advection.pyx:173-178)
     p.zi = zn;
     p.yi = yn;
     p.xi = xn;
+    p.z = z;
     p.hz2 = dz1;
     p.hy2 = dy1;
     p.hx2 = dx1;
     p.hz1 = dz0;
     p.hy1 = dy0;
     p.hx1 = dx0;
-    if (zout)
-        *zout = (float)z;
     return true;
 }
 
@@ -1301,8 +1288,7 @@ enum Scheme { kAB3 = 0, kRK2 = 1, kAB3ISO = 2 };
 
 template <typename T, int LAYOUT, bool RING_OK>
 __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
-                                         const opx_window &win, double dt, double alpha, const double *Cw,
-                                         float *zout);
+                                         const opx_window &win, double dt, double alpha, const double *Cw);
 
 // `nsteps` fused steps of particle i (AB3 or RK2); step k reads its release's
 // weights from table[k * tstride + release].  The particle's state is loaded
@@ -1322,7 +1308,6 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
     StencilCache<T, LAYOUT> c;
     c.s0 = -1;
     c.s0_2d = -1;
-    c.s0_hz = -1;
     // the target density, read once per launch (ab3_isopycnal_following.pyx:131)
     double rho_t = 0.0;
     if constexpr (SCHEME == kAB3ISO)
@@ -1343,6 +1328,7 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         p.zi = az.ok ? az.pos() : nan;
         p.yi = ay.ok ? ay.pos() : nan;
         p.xi = ax.ok ? ax.pos() : nan;
+        p.z = s.z[i];
         if constexpr (SCHEME != kRK2) {
             if (p.st != OPX_MULTISTEP_1) {
                 p.hz1 = s.hz1[i];
@@ -1357,7 +1343,6 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         }
     }
     bool moved = false;
-    float zlast = 0.0f;
 #pragma unroll 1
     for (int k = 0; k < nsteps; ++k) {
         const Step rs = load_release(table + (int64_t)k * tstride, r, win.nslots);
@@ -1373,14 +1358,13 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
             p.st = OPX_OUT_OF_DOMAIN;
             break;
         }
-        float *zout = &zlast; // the last step's z, stored after the loop
         bool ok;
         if constexpr (SCHEME == kAB3)
-            ok = ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw, zout);
+            ok = ab3_step<T, LAYOUT>(p, c, rs, g, win, dt, Cw);
         else if constexpr (SCHEME == kAB3ISO)
-            ok = ab3_step<T, LAYOUT, true>(p, c, rs, g, win, dt, Cw, zout, iso, rho_t);
+            ok = ab3_step<T, LAYOUT, true>(p, c, rs, g, win, dt, Cw, iso, rho_t);
         else
-            ok = rk2_step<T, LAYOUT, RING_OK>(p, c, rs, g, win, dt, alpha, Cw, zout);
+            ok = rk2_step<T, LAYOUT, RING_OK>(p, c, rs, g, win, dt, alpha, Cw);
         if (!ok) {
             if (err)
                 *err = OPX_FLAG_UNPACKED_PAIR;
@@ -1389,8 +1373,7 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         moved = true;
     }
     if (moved) {
-        if (s.z)
-            s.z[i] = zlast;
+        s.z[i] = p.z;
         const Axis az = encode_axis(p.zi, g.NZ), ay = encode_axis(p.yi, g.NY), ax = encode_axis(p.xi, g.NX);
         s.cell[i] = (int32_t)pack_cell(az, ay, ax, g);
         s.fz[i] = az.q;
@@ -1745,6 +1728,8 @@ __global__ void k_split_from_f64(opx_split_state s, opx_grid g, const double *zi
     s.fz[i] = z.q;
     s.fy[i] = y.q;
     s.fx[i] = x.q;
+    if (s.z)
+        s.z[i] = __builtin_nan(""); // the first step computes z from zidx
 }
 
 __global__ void k_split_to_f64(opx_split_state s, opx_grid g, double *zidx, double *yidx, double *xidx)
@@ -1876,16 +1861,18 @@ __device__ __forceinline__ void rk2_uvw(const opx_window &win, const opx_grid &g
     w = (double)tri32(vw, (float)oz.w, (float)cy.w, (float)cx.w);
 }
 
-// h and zeta (FP64 weights; zeta blended in FP32) and, with DXDY, dx and dy
-// (FP32) at an FP64 horizontal position, through the 2-D corner cache.
+// h and zeta (FP64 weights; zeta blended in FP32) where `hz` is set and,
+// with DXDY, dx and dy (FP32) at an FP64 horizontal position, through the 2-D
+// corner cache.
 template <int LAYOUT, bool RING_OK, bool DXDY, typename Cache>
 __device__ __forceinline__ void rk2_2d(const opx_window &win, int j, int s0, int s1, float gt, float ft, int NY,
-                                       int NX, double yi, double xi, Cache &c, double &h, double &zeta, float &dxv,
-                                       float &dyv)
+                                       int NX, double yi, double xi, Cache &c, bool hz, double &h, double &zeta,
+                                       float &dxv, float &dyv)
 {
     const AxD cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
     sample2<LAYOUT, RING_OK>(win, j, s0, s1, NX, cy.b, cx.b, c.c2, c.s0_2d);
-    eval2(c.c2, gt, ft, cy.w, cx.w, h, zeta);
+    if (hz)
+        eval2(c.c2, gt, ft, cy.w, cx.w, h, zeta);
     if constexpr (DXDY) {
         dxv = bil32(c.c2.dx, (float)cy.w, (float)cx.w);
         dyv = bil32(c.c2.dy, (float)cy.w, (float)cx.w);
@@ -1900,8 +1887,7 @@ __device__ __forceinline__ void rk2_2d(const opx_window &win, int j, int s0, int
 // unchanged, if one of the step's pairs is not packed.
 template <typename T, int LAYOUT, bool RING_OK>
 __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, const Step &rs, const opx_grid &g,
-                                         const opx_window &win, double dt, double alpha, const double *Cw,
-                                         float *zout)
+                                         const opx_window &win, double dt, double alpha, const double *Cw)
 {
     const int NZ = g.NZ, NY = g.NY, NX = g.NX;
     const double zi = p.zi, yi = p.yi, xi = p.xi;
@@ -1911,15 +1897,21 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     if (LAYOUT != OPX_LAYOUT_RING && !RING_OK && (j < 0 || jm < 0 || jp < 0))
         return false;
 
-    // ---- stage 1 (rk2_w_advection.pyx:31-153): z at x_n and k1 = (u/dx, v/dy, w)
+    // ---- stage 1 (rk2_w_advection.pyx:31-153): z at x_n (kept from the last
+    // step, else compute_z) and k1 = (u/dx, v/dy, w)
     double h0, zeta0;
     float dxv, dyv;
-    rk2_2d<LAYOUT, RING_OK, true>(win, j, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, c, h0, zeta0, dxv, dyv);
+    double z = p.z;
+    const bool need_z = __builtin_isnan(z);
+    rk2_2d<LAYOUT, RING_OK, true>(win, j, rs.s0, rs.s1, rs.gt, rs.ft, NY, NX, yi, xi, c, need_z, h0, zeta0, dxv,
+                                  dyv);
     const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
-    const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
-    const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
-    const double S0 = div64(hc * sigma + h0 * Cv, hc + h0);
-    const double z = zeta0 + (zeta0 + h0) * S0;
+    if (need_z) {
+        const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
+        const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
+        const double S0 = div64(hc * sigma + h0 * Cv, hc + h0);
+        z = zeta0 + (zeta0 + h0) * S0;
+    }
     float u1, v1;
     double w1;
     rk2_uvw<T, LAYOUT, RING_OK>(win, g, j, rs.s0, rs.s1, rs.gt, rs.ft, zi, yi, xi, c, u1, v1, w1);
@@ -1930,7 +1922,8 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const double zs = z + adt * w1, ys = yi + adt * ky1, xs = xi + adt * kx1;
     double hs, zetas;
     float dxs, dys;
-    rk2_2d<LAYOUT, RING_OK, true>(win, jm, rs.s0m, rs.s1m, rs.gtm, rs.ftm, NY, NX, ys, xs, c, hs, zetas, dxs, dys);
+    rk2_2d<LAYOUT, RING_OK, true>(win, jm, rs.s0m, rs.s1m, rs.gtm, rs.ftm, NY, NX, ys, xs, c, true, hs, zetas, dxs,
+                                  dys);
     const double zis = zidx_from_z(zs, hs, zetas, hc, NZ, Cw, oz.b);
     float u2, v2;
     double w2;
@@ -1947,16 +1940,15 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     if (__builtin_isfinite(yr) && __builtin_isfinite(xr)) {
         double h1, zeta1;
         float unused0, unused1;
-        rk2_2d<LAYOUT, RING_OK, false>(win, jp, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, yr, xr, c, h1, zeta1, unused0,
-                                       unused1);
+        rk2_2d<LAYOUT, RING_OK, false>(win, jp, rs.s0p, rs.s1p, rs.gt1, rs.ft1, NY, NX, yr, xr, c, true, h1, zeta1,
+                                       unused0, unused1);
         const int k0 = (int)floor(zis + 0.5);
         zr = round32(zidx_from_z(zn, h1, zeta1, hc, NZ, Cw, k0));
     }
     p.zi = zr;
     p.yi = yr;
     p.xi = xr;
-    if (zout)
-        *zout = (float)zn;
+    p.z = zn;
     return true;
 }
 
@@ -2124,6 +2116,15 @@ int check_state(const opx_split_state *s, const char *what, bool need_hist = tru
     OPX_REQUIRE(s->status && s->cell && s->fz && s->fy && s->fx, "%s: null status/cell/offset arrays", what);
     OPX_REQUIRE(!need_hist || (s->hz1 && s->hy1 && s->hx1 && s->hz2 && s->hy2 && s->hx2), "%s: null history arrays",
                 what);
+    return OPX_OK;
+}
+
+// The step entry points also keep each particle's z in the state.
+int check_step_state(const opx_split_state *s, const char *what, bool need_hist = true)
+{
+    int rc;
+    if ((rc = check_state(s, what, need_hist))) return rc;
+    OPX_REQUIRE(s->n == 0 || s->z, "%s: null z array", what);
     return OPX_OK;
 }
 
@@ -2477,7 +2478,7 @@ int opx_ab3_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_w
                         const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_ab3_fused"))) return rc;
+    if ((rc = check_step_state(s, "opx_ab3_fused"))) return rc;
     if ((rc = check_grid(g))) return rc;
     if ((rc = check_window(win, g))) return rc;
     OPX_REQUIRE(table != nullptr, "null release table");
@@ -2504,7 +2505,7 @@ int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const o
                             double dt, const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_ab3_fused_dev"))) return rc;
+    if ((rc = check_step_state(s, "opx_ab3_fused_dev"))) return rc;
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
@@ -2550,7 +2551,7 @@ int opx_ab3_iso_fused_steps(const opx_split_state *s, const opx_grid *g, const o
                             int32_t *err_flag, void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_ab3_iso_fused"))) return rc;
+    if ((rc = check_step_state(s, "opx_ab3_iso_fused"))) return rc;
     if ((rc = check_grid(g))) return rc;
     if ((rc = check_window(win, g, true))) return rc;
     if ((rc = check_iso_args(s, table, table_stride, nsteps, w_max, rho_scale, &block))) return rc;
@@ -2567,7 +2568,7 @@ int opx_ab3_iso_fused_steps_dev(const opx_split_state *s, const opx_grid *g, con
                                 int32_t *err_flag, void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_ab3_iso_fused_dev"))) return rc;
+    if ((rc = check_step_state(s, "opx_ab3_iso_fused_dev"))) return rc;
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr, "null window");
     OPX_REQUIRE(layout >= OPX_LAYOUT_RING && layout <= OPX_LAYOUT_YPAIR, "unknown layout %d", layout);
@@ -2584,7 +2585,7 @@ int opx_rk2_fused_steps(const opx_split_state *s, const opx_grid *g, const opx_w
                         const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_rk2_fused", false))) return rc;
+    if ((rc = check_step_state(s, "opx_rk2_fused", false))) return rc;
     if ((rc = check_grid(g))) return rc;
     if ((rc = check_window(win, g))) return rc;
     OPX_REQUIRE(table != nullptr, "null release table");
@@ -2614,7 +2615,7 @@ int opx_rk2_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const o
                             void *stream)
 {
     int rc;
-    if ((rc = check_state(s, "opx_rk2_fused_dev", false))) return rc;
+    if ((rc = check_step_state(s, "opx_rk2_fused_dev", false))) return rc;
     if ((rc = check_grid(g))) return rc;
     OPX_REQUIRE(win_dev != nullptr && table != nullptr, "null window or release table");
     OPX_REQUIRE(dtype3 == OPX_F32 || dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");

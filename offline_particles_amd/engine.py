@@ -96,10 +96,10 @@ _NP = {torch.float64: np.float64, torch.uint8: np.uint8}
 class SplitState:
     """Device SoA of the split-position state (opx_split_state)."""
 
-    # status 1 + cell 4 + offsets 3 x 4 + FP32 history 6 x 4 + pid 4 + rel 2
-    BYTES_PER_PARTICLE = 47
+    # status 1 + cell 4 + offsets 3 x 4 + FP32 history 6 x 4 + pid 4 + rel 2 + z 8
+    BYTES_PER_PARTICLE = 55
 
-    def __init__(self, capacity: int, device, *, with_z: bool = False, with_rho_t: bool = False):
+    def __init__(self, capacity: int, device, *, with_rho_t: bool = False):
         n = int(capacity)
         self.capacity = n
         self.status = torch.full((n,), ParticleStatus.INACTIVE, dtype=torch.uint8, device=device)
@@ -110,7 +110,8 @@ class SplitState:
             setattr(self, name, torch.zeros(n, dtype=dt, device=device))
         self.pid = torch.zeros(n, dtype=torch.int32, device=device)
         self.rel = torch.zeros(n, dtype=torch.int16, device=device)  # read as uint16
-        self.z = torch.zeros(n, dtype=torch.float32, device=device) if with_z else None
+        # z (m) after each particle's last step, the next step's start (NaN: from zidx)
+        self.z = torch.full((n,), float("nan"), dtype=torch.float64, device=device)
         # target density of the isopycnal-following step (FP64, 8 B per particle)
         self.rho_t = torch.zeros(n, dtype=torch.float64, device=device) if with_rho_t else None
 
@@ -127,7 +128,7 @@ class SplitState:
             setattr(s, name, p(getattr(self, name), 4))
         s.pid = p(self.pid, 4)
         s.rel = p(self.rel, 2)
-        s.z = p(self.z, 4)
+        s.z = p(self.z, 8)
         s.rho_t = p(self.rho_t, 8)
         return s
 
@@ -136,7 +137,7 @@ class AB3Engine:
     """Fused AB3 stepping of one or many particle releases on one device."""
 
     def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
-                 capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2, with_z: bool = False,
+                 capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2,
                  max_releases: int = 256, stream_slots: int | None = None, scheme: str = "ab3",
                  alpha: float = 2 / 3):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -172,7 +173,6 @@ class AB3Engine:
         if self._stream_slots is not None and self._stream_slots < 3:
             raise ValueError("a streamed window needs at least 3 slots")
         self._build_window(fieldset)
-        self._with_z = with_z
         self._with_rho_t = scheme == "ab3_iso"
         self._state = self._new_state(capacity)
         self._spare = None
@@ -324,7 +324,7 @@ class AB3Engine:
         self.window = w
 
     def _new_state(self, capacity):
-        return SplitState(capacity, self.device, with_z=self._with_z, with_rho_t=self._with_rho_t)
+        return SplitState(capacity, self.device, with_rho_t=self._with_rho_t)
 
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
@@ -359,8 +359,7 @@ class AB3Engine:
         cap = n if exact else max(n, 2 * self._state.capacity)
         new = self._new_state(cap)
         old = self._state
-        for name in (("status", "cell", "pid", "rel") + _STATE_F32 + (("z",) if self._with_z else ())
-                     + (("rho_t",) if self._with_rho_t else ())):
+        for name in (("status", "cell", "pid", "rel", "z") + _STATE_F32 + (("rho_t",) if self._with_rho_t else ())):
             getattr(new, name)[: self.n].copy_(getattr(old, name)[: self.n])
         self._state = new
         self._spare = None

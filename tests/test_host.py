@@ -195,7 +195,7 @@ def test_iso_fused_entry_point_rejects_bad_arguments(lib):
     p = ctypes.addressof(buf)
     s = _lib.opx_split_state()
     s.n = 1
-    for name in ("status", "cell", "fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2", "rho_t"):
+    for name in ("status", "cell", "fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2", "z", "rho_t"):
         setattr(s, name, p)
     w = _lib.opx_window()
     w.u = w.v = w.w = w.zeta = w.hdd = w.C = p

@@ -42,8 +42,8 @@ def test_admission_rejects_oversized():
 def test_slot_bytes_match_state_arrays():
     s = SplitState(10, "cpu")
     per = sum(t.element_size() for t in (s.status, s.cell, s.fz, s.fy, s.fx, s.hz1, s.hy1, s.hx1, s.hz2, s.hy2,
-                                         s.hx2, s.pid, s.rel))
-    assert per == SplitState.BYTES_PER_PARTICLE == 47
+                                         s.hx2, s.pid, s.rel, s.z))
+    assert per == SplitState.BYTES_PER_PARTICLE == 55
     assert SLOT_BYTES == 2 * per + 16 + 8
     assert s.status.dtype == torch.uint8 and bool((s.status == 128).all())  # empty slots are INACTIVE
 
