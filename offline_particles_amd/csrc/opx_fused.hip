@@ -275,6 +275,17 @@ __device__ __forceinline__ float blend_rn(float p, float q, float gt, float ft)
     return p * gt + q * ft;
 }
 
+// a * b + c as the three-operand v_fma_f32, the instruction fmaf compiles to
+// (the same result).  Left to itself the compiler picks the two-operand
+// v_fmac_f32, which overwrites c; where c is a stencil node cached across the
+// steps of a launch it then copies every cached node once per step to keep it.
+__device__ __forceinline__ float fma3(float a, float b, float c)
+{
+    float r;
+    asm("v_fma_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // The fused forms' time blend of velocities and the free surface: p + (q - p)*ft
 // as one difference and one fused multiply-add (two instructions instead of
 // three; within an FP32 rounding or two of blend_rn).  RN selects blend_rn.
@@ -285,7 +296,7 @@ __device__ __forceinline__ float blend_t(float p, float q, float gt, float ft)
     if constexpr (RN)
         return blend_rn(p, q, gt, ft);
     else
-        return fmaf(q - p, ft, p);
+        return fma3(q - p, ft, p);
 }
 
 template <typename T, bool RN = false>
@@ -1236,13 +1247,24 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
 
-    // ---- Euler -> AB2 -> AB3 start-up, history from registers
+    // ---- Euler -> AB2 -> AB3 start-up, history from registers; a wave whose
+    // particles are all past the start-up (every step after a release's second)
+    // takes the history as it is, without the selects
     const uint8_t st = p.st;
     float dx1, dy1, dz1, dx2, dy2, dz2;
-    ab3_startup(st, dx0, [&] { return p.hx1; }, [&] { return p.hx2; }, dx1, dx2);
-    ab3_startup(st, dy0, [&] { return p.hy1; }, [&] { return p.hy2; }, dy1, dy2);
-    ab3_startup(st, dz0, [&] { return p.hz1; }, [&] { return p.hz2; }, dz1, dz2);
-    p.st = ab3_next_status(st);
+    if (__all(st == OPX_NORMAL)) {
+        dx1 = p.hx1;
+        dx2 = p.hx2;
+        dy1 = p.hy1;
+        dy2 = p.hy2;
+        dz1 = p.hz1;
+        dz2 = p.hz2;
+    } else {
+        ab3_startup(st, dx0, [&] { return p.hx1; }, [&] { return p.hx2; }, dx1, dx2);
+        ab3_startup(st, dy0, [&] { return p.hy1; }, [&] { return p.hy2; }, dy1, dy2);
+        ab3_startup(st, dz0, [&] { return p.hz1; }, [&] { return p.hz2; }, dz1, dz2);
+        p.st = ab3_next_status(st);
+    }
 
     // ---- z at x_n, t_n (kept from the last step, else compute_z), then the
     // AB3 update (as ab3_body)
