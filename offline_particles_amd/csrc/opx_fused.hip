@@ -118,6 +118,23 @@ __device__ __forceinline__ void gload_pair(const V *p, uint32_t i, V &a, V &b)
     }
 }
 
+// The element (or pair of elements) of a packed window at byte offset `off`.
+// A packed window is at most 4 GiB (opx_packed_bytes refuses larger ones), so
+// the offset fits 32 bits: where the window base is the same in every lane it
+// stays in scalar registers and each load takes the 32-bit offset alone (the
+// scalar-base form of global loads) instead of a 64-bit address per lane.
+template <typename V>
+__device__ __forceinline__ V gload_at(const void *base, uint32_t off)
+{
+    return gload(reinterpret_cast<const V *>(static_cast<const char *>(base) + off), 0u);
+}
+
+template <typename V>
+__device__ __forceinline__ void gload_pair_at(const void *base, uint32_t off, V &a, V &b)
+{
+    gload_pair(reinterpret_cast<const V *>(static_cast<const char *>(base) + off), 0u, a, b);
+}
+
 __device__ __forceinline__ float ld3(const float *__restrict__ p, uint32_t i) { return gbl(p)[i]; }
 __device__ __forceinline__ float ld3(const uint16_t *__restrict__ p, uint32_t i)
 {
@@ -368,28 +385,36 @@ struct Raw3<T, OPX_LAYOUT_YPAIR> {
 template <int LAYOUT>
 constexpr int kRawUnits = (LAYOUT == OPX_LAYOUT_OCT || LAYOUT == OPX_LAYOUT_YPAIR) ? 2 : 4;
 
+// Bytes of one packed node of a layout.
+template <typename T, int LAYOUT>
+constexpr uint32_t kNodeBytes =
+    LAYOUT == OPX_LAYOUT_PAIR   ? sizeof(typename Packed<T>::pair)
+    : LAYOUT == OPX_LAYOUT_QUAD ? sizeof(typename Packed<T>::quad)
+    : LAYOUT == OPX_LAYOUT_OCT  ? sizeof(typename Packed<T>::oct)
+                                : sizeof(typename Packed<T>::ypair);
+
 // Loads unit k of the raw nodes of a field whose stencil rows (axis0, axis1)
-// start at packed nodes rows[0..3].
+// start at byte offsets rows[0..3] of packed window pp.
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void load_unit(const void *__restrict__ pp, const uint32_t rows[4], int k,
                                           Raw3<T, LAYOUT> &r)
 {
     if constexpr (LAYOUT == OPX_LAYOUT_PAIR) {
         // the x-neighbours (i, i+1) in one load
-        gload_pair(static_cast<const typename Packed<T>::pair *>(pp), rows[k], r.n[2 * k], r.n[2 * k + 1]);
+        gload_pair_at(pp, rows[k], r.n[2 * k], r.n[2 * k + 1]);
     } else if constexpr (LAYOUT == OPX_LAYOUT_QUAD) {
-        r.n[k] = gload(static_cast<const typename Packed<T>::quad *>(pp), rows[k]);
+        r.n[k] = gload_at<typename Packed<T>::quad>(pp, rows[k]);
     } else if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
         // one node per z-plane holds that plane's four (y, x) corners of both snapshots
-        r.n[k] = gload(static_cast<const typename Packed<T>::oct *>(pp), rows[2 * k]);
+        r.n[k] = gload_at<typename Packed<T>::oct>(pp, rows[2 * k]);
     } else {
         // YPAIR: nodes i, i+1 of a z-plane: (y, x), (y+1, x) and (y, x+1), (y+1, x+1)
-        const auto *P = static_cast<const typename Packed<T>::ypair *>(pp);
+        using E = typename Packed<T>::ypair;
         if constexpr (sizeof(T) == 4) {
-            r.n[2 * k] = gload(P, rows[2 * k]);
-            r.n[2 * k + 1] = gload(P, rows[2 * k] + 1);
+            r.n[2 * k] = gload_at<E>(pp, rows[2 * k]);
+            r.n[2 * k + 1] = gload_at<E>(pp, rows[2 * k] + (uint32_t)sizeof(E));
         } else {
-            gload_pair(P, rows[2 * k], r.n[2 * k], r.n[2 * k + 1]); // one 16-byte load per z-plane
+            gload_pair_at(pp, rows[2 * k], r.n[2 * k], r.n[2 * k + 1]); // one 16-byte load per z-plane
         }
     }
 }
@@ -439,12 +464,16 @@ __device__ __forceinline__ void blend_unit(const Raw3<T, LAYOUT> &r, int k, floa
     }
 }
 
-// First packed node of each stencil row (axis0, axis1) of a field with dims
-// (., n1, n2) at base (b0, b1, b2).
-__device__ __forceinline__ void stencil_rows(int n1, int n2, int b0, int b1, int b2, uint32_t rows[4])
+// First node of each stencil row (axis0, axis1) of a field with dims
+// (., n1, n2) at base (b0, b1, b2), as element indices times `esize` (1: the
+// index; a packed node's bytes: its byte offset in the window).
+__device__ __forceinline__ void stencil_rows(int n1, int n2, int b0, int b1, int b2, uint32_t rows[4],
+                                             uint32_t esize = 1)
 {
-    const uint32_t s1 = (uint32_t)n2, s0 = (uint32_t)n1 * (uint32_t)n2;
-    const uint32_t b = ((uint32_t)b0 * (uint32_t)n1 + (uint32_t)b1) * (uint32_t)n2 + (uint32_t)b2;
+    const uint32_t s1 = (uint32_t)n2 * esize, s0 = (uint32_t)n1 * s1;
+    // 24-bit multiplies (full rate): check_grid keeps (NZ + 1) NY and NX below
+    // 2^24, and the node index itself below 2^31
+    const uint32_t b = (__umul24(__umul24(b0, n1) + b1, n2) + b2) * esize;
     rows[0] = b;
     rows[1] = b + s1;
     rows[2] = b + s0;
@@ -457,7 +486,7 @@ __device__ __forceinline__ void load_raw(const void *__restrict__ pp, int n1, in
                                          Raw3<T, LAYOUT> &r)
 {
     uint32_t rows[4];
-    stencil_rows(n1, n2, b0, b1, b2, rows);
+    stencil_rows(n1, n2, b0, b1, b2, rows, kNodeBytes<T, LAYOUT>);
 #pragma unroll
     for (int k = 0; k < kRawUnits<LAYOUT>; ++k)
         load_unit<T, LAYOUT>(pp, rows, k, r);
@@ -479,8 +508,8 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
                                         int n1, int n2, int b0, int b1, int b2, float v[8])
 {
     uint32_t rows[4];
-    stencil_rows(n1, n2, b0, b1, b2, rows);
     if constexpr (LAYOUT == OPX_LAYOUT_RING) {
+        stencil_rows(n1, n2, b0, b1, b2, rows);
         const T *p = static_cast<const T *>(pp), *q = static_cast<const T *>(qq);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -488,6 +517,7 @@ __device__ __forceinline__ void corners(const void *__restrict__ pp, const void 
             v[2 * r + 1] = blend<T, RN>(p, q, rows[r] + 1, gt, ft);
         }
     } else {
+        stencil_rows(n1, n2, b0, b1, b2, rows, kNodeBytes<T, LAYOUT>);
         Raw3<T, LAYOUT> r;
 #pragma unroll
         for (int k = 0; k < kRawUnits<LAYOUT>; ++k) {
@@ -553,19 +583,20 @@ struct Corners2 {
 __device__ __forceinline__ void load2_packed(const float4 *__restrict__ P, const float *__restrict__ dy_, int NX,
                                              int by, int bx, bool want_dxdy, Corners2 &c)
 {
-    const uint32_t b = (uint32_t)by * (uint32_t)NX + (uint32_t)bx;
-    const uint32_t o[4] = {b, b + 1, b + NX, b + NX + 1};
+    // byte offsets: 16-byte nodes of the packed window, 4-byte dy values
+    const uint32_t b = __umul24(by, NX) + bx;
+    const uint32_t o[4] = {16 * b, 16 * (b + 1), 16 * (b + NX), 16 * (b + NX + 1)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float4 m = gload(P, o[k]);
+        float4 m = gload_at<float4>(P, o[k]);
         c.h[k] = m.x;
         c.z0[k] = m.y;
         c.z1[k] = m.z;
         c.dx[k] = m.w;
     }
     if (want_dxdy) {
-        gload_pair(dy_, o[0], c.dy[0], c.dy[1]);
-        gload_pair(dy_, o[2], c.dy[2], c.dy[3]);
+        gload_pair_at(dy_, 4 * b, c.dy[0], c.dy[1]);
+        gload_pair_at(dy_, 4 * (b + NX), c.dy[2], c.dy[3]);
     }
     c.by = by;
     c.bx = bx;
@@ -628,6 +659,7 @@ struct IsoRate {
 
 constexpr int kMaxSlots = 1023; // slot fields of opx_release_step are 10 bits
 constexpr int kMaxLaunchSteps = 64; // steps one fused launch may take
+constexpr uint64_t kMaxWindowBytes = uint64_t(1) << 32; // one packed window (32-bit byte offsets)
 
 __device__ __forceinline__ int next_slot(int s, int nslots) { return s + 1 == nslots ? 0 : s + 1; }
 
@@ -2158,6 +2190,9 @@ int check_grid(const opx_grid *g)
     OPX_REQUIRE(g != nullptr, "null grid");
     OPX_REQUIRE(g->NZ >= 2 && g->NY >= 3 && g->NX >= 3, "grid %dx%dx%d too small", g->NZ, g->NY, g->NX);
     OPX_REQUIRE(g->NZ + 1 <= kMaxC, "NZ = %d above the %d levels the fused step keeps in LDS", g->NZ, kMaxC - 1);
+    OPX_REQUIRE((int64_t)(g->NZ + 1) * g->NY < (1 << 24) && g->NX < (1 << 24),
+                "grid %dx%dx%d: the fused steps index nodes with 24-bit multiplies ((NZ + 1) NY and NX below 2^24)",
+                g->NZ, g->NY, g->NX);
     OPX_REQUIRE(g->xbits > 0 && g->ybits > 0 && g->zbits > 0 && g->xbits + g->ybits + g->zbits <= 31 &&
                     (1 << g->xbits) >= g->NX + 5 && (1 << g->ybits) >= g->NY + 5 && (1 << g->zbits) >= g->NZ + 5,
                 "cell bit widths (%d,%d,%d) must hold N+5 values per axis in 31 bits", g->zbits, g->ybits,
@@ -2695,6 +2730,10 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
     *bytes_u = e * (uint64_t)g->NZ * g->NY * (g->NX - 1);
     *bytes_v = e * (uint64_t)g->NZ * (g->NY - 1) * g->NX;
     *bytes_w = e * (uint64_t)(g->NZ + 1) * g->NY * g->NX;
+    // the step kernels address a packed window with 32-bit byte offsets
+    OPX_REQUIRE(*bytes_w <= kMaxWindowBytes && *bytes_u <= kMaxWindowBytes && *bytes_v <= kMaxWindowBytes,
+                "a packed window of this grid in layout %d takes %llu bytes, above the 4 GiB of one packed window",
+                layout, (unsigned long long)*bytes_w);
     return OPX_OK;
 }
 

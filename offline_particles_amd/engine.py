@@ -343,15 +343,29 @@ class AB3Engine:
         self._work = torch.empty(int(need.value) + (1 << 20), dtype=torch.uint8, device=self.device)
         self._work_n = int(n)
 
-    def packed_window_bytes(self):
-        """Bytes of one packed stencil window (u, v, w, 2-D) in the current layout."""
-        key = self.layout
+    def packed_window_bytes(self, layout=None):
+        """Bytes of one packed stencil window (u, v, w, 2-D) in `layout` (default: the current one).
+
+        Raises OpxInvalidArgument where one window would exceed 4 GiB (the step
+        kernels address a window with 32-bit byte offsets)."""
+        key = self.layout if layout is None else int(layout)
         if key not in self._packed_bytes:
             bu, bv, bw, b2 = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-            _lib.call("opx_packed_bytes", ctypes.byref(self.grid), self.window.dtype3, self.layout,
+            _lib.call("opx_packed_bytes", ctypes.byref(self.grid), self.window.dtype3, key,
                       ctypes.byref(bu), ctypes.byref(bv), ctypes.byref(bw), ctypes.byref(b2))
             self._packed_bytes[key] = (bu.value, bv.value, bw.value, b2.value)
         return self._packed_bytes[key]
+
+    def layout_fits(self, layout: int) -> bool:
+        """Whether `layout` can be used on this grid: RING always, a packed layout
+        when one window of it is at most 4 GiB."""
+        if int(layout) == _lib.OPX_LAYOUT_RING:
+            return True
+        try:
+            self.packed_window_bytes(layout)
+        except _lib.OpxInvalidArgument:
+            return False
+        return True
 
     def _ensure_capacity(self, n, exact=False):
         if n <= self._state.capacity:
@@ -701,6 +715,8 @@ class AB3Engine:
 
     def set_layout(self, layout: int) -> None:
         """Choose the stencil layout (OPX_LAYOUT_RING, _PAIR, _QUAD, _PAIR_LDS, _OCT or _YPAIR)."""
+        if int(layout) != _lib.OPX_LAYOUT_RING:
+            self.packed_window_bytes(layout)  # refuses a window above 4 GiB
         if layout != self.layout:
             self.layout = int(layout)
             self._pair_bufs = None
@@ -833,6 +849,7 @@ class AB3Engine:
         if layouts is None:
             layouts = (_lib.OPX_LAYOUT_RING, _lib.OPX_LAYOUT_PAIR, _lib.OPX_LAYOUT_QUAD, _lib.OPX_LAYOUT_PAIR_LDS,
                        _lib.OPX_LAYOUT_OCT, _lib.OPX_LAYOUT_YPAIR)
+            layouts = tuple(lay for lay in layouts if self.layout_fits(lay))
         if launch_steps is None:
             launch_steps = (1, 8, 16, 32)
         saved_every, self.rebin_every = self.rebin_every, 0

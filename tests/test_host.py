@@ -283,3 +283,23 @@ def test_rk2_startup_timestepper_declarations():
     assert type(roms.ab3_w_advection_timestepper(ta, np.float64(120.0))) is ABTimestepper
     with pytest.raises(ValueError):
         roms.ab3_w_advection_timestepper(ta, np.float64(120.0), startup="euler")
+
+
+def test_packed_window_limited_to_4GiB(lib):
+    """The step kernels address a packed stencil window with 32-bit byte
+    offsets, so opx_packed_bytes refuses a layout whose window would exceed
+    4 GiB (OCT with FP32 nodes on a 2048 x 1536 x 60 grid: 32 B x 61 levels x
+    2048 x 1536 = 6.1 GB) and sizes the others (PAIR FP32: 8 B per node)."""
+    g = _lib.opx_grid()
+    g.NZ, g.NY, g.NX = 60, 1536, 2048
+    g.xbits, g.ybits, g.zbits = 12, 11, 7
+    g.hc = 10.0
+    g.zidx_max, g.yidx_max, g.xidx_max = g.NZ - 1, g.NY - 1, g.NX - 1
+    out = [ctypes.c_uint64() for _ in range(4)]
+    args = [ctypes.byref(o) for o in out]
+    assert lib.opx_packed_bytes(ctypes.byref(g), _lib.OPX_F32, _lib.OPX_LAYOUT_OCT, *args) == _lib.OPX_ERR_INVALID
+    assert b"4 GiB" in lib.opx_last_error()
+    assert lib.opx_packed_bytes(ctypes.byref(g), _lib.OPX_BF16, _lib.OPX_LAYOUT_OCT, *args) == _lib.OPX_OK
+    assert out[2].value == 16 * 61 * 1536 * 2048 <= 1 << 32
+    assert lib.opx_packed_bytes(ctypes.byref(g), _lib.OPX_F32, _lib.OPX_LAYOUT_PAIR, *args) == _lib.OPX_OK
+    assert out[2].value == 8 * 61 * 1536 * 2048
