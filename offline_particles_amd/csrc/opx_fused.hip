@@ -814,27 +814,30 @@ __device__ __forceinline__ double div64(double a, double b)
 // _vertical_coordinate.pxd:41-103): S = (z - zeta) / (zeta + h), the bracket k
 // with S_k <= S < S_{k+1} on the w-levels clamped to [0, NZ-1] (S_k increases
 // with k, so a walk from the hint k0 finds the reference's binary-search k),
-// and the linear inverse k - 1/2 + (S - S_k) / (S_{k+1} - S_k).
+// and the linear inverse k - 1/2 + (S - S_k) / (S_{k+1} - S_k).  The fused
+// forms assume a water column (zeta + h > 0).
 __device__ __forceinline__ double zidx_from_z(double z, double h, double zeta, double hc, int NZ, const double *Cw,
                                               int k0)
 {
     // S and the level values S_k = (hc sigma_k + h C_k) / (hc + h), all scaled
-    // by hc + h > 0 (order and ratios unchanged): two divisions instead of three
-    const double S = div64((z - zeta) * (hc + h), zeta + h);
-    auto S_at = [&](int k) { return hc * Cw[kMaxC + k] + h * Cw[k]; };
+    // by (hc + h)(zeta + h) > 0 (a water column: order and ratios unchanged):
+    // the bracket search compares without dividing and the inverse takes one
+    // division instead of three
+    const double A = (z - zeta) * (hc + h), D = zeta + h;
+    auto T_at = [&](int k) { return (hc * Cw[kMaxC + k] + h * Cw[k]) * D; };
     int k = max(0, min(k0, NZ - 1));
-    double Sk = S_at(k);
-    while (k > 0 && S < Sk) {
+    double Tk = T_at(k);
+    while (k > 0 && A < Tk) {
         --k;
-        Sk = S_at(k);
+        Tk = T_at(k);
     }
-    double Sk1 = S_at(k + 1);
-    while (k < NZ - 1 && S >= Sk1) {
+    double Tk1 = T_at(k + 1);
+    while (k < NZ - 1 && A >= Tk1) {
         ++k;
-        Sk = Sk1;
-        Sk1 = S_at(k + 1);
+        Tk = Tk1;
+        Tk1 = T_at(k + 1);
     }
-    return ((double)k - 0.5) + div64(S - Sk, Sk1 - Sk);
+    return ((double)k - 0.5) + div64(A - Tk, Tk1 - Tk);
 }
 
 // Euler -> AB2 -> AB3 start-up of one coordinate (ab3_w_advection.pyx:121-138):
