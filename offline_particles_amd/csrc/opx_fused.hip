@@ -1183,10 +1183,8 @@ __device__ __forceinline__ void sample2(const opx_window &win, int j, int s0, in
 // when its bases or the pair changed; with RING_OK a pair that is not packed
 // (j < 0) reads the rings and leaves nothing cached, without it the caller has
 // checked that j >= 0.  RN3: the third field (the isopycnal-following
-// step's density) is blended with blend_rn.  CACHE_W: the third field's nodes
-// are cached like u and v (else re-read every step, for a kernel whose
-// registers would not hold them).
-template <typename T, int LAYOUT, bool RING_OK, bool RN3 = false, bool CACHE_W = true>
+// step's density) is blended with blend_rn.
+template <typename T, int LAYOUT, bool RING_OK, bool RN3 = false>
 __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid &g, int j, int s0, int s1, float gt,
                                           float ft, int bz, int by, int bx, int bxu, int byv, int bzw,
                                           StencilCache<T, LAYOUT> &c, float vu[8], float vv[8], float vw[8])
@@ -1203,8 +1201,7 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
                 load_raw<T, LAYOUT>(window(win.pu, win.pu_stride, j), NY, NX - 1, bz, by, bxu, c.u);
             if (!(same && c.kv == kv))
                 load_raw<T, LAYOUT>(window(win.pv, win.pv_stride, j), NY - 1, NX, bz, byv, bx, c.v);
-            constexpr bool kCacheW = CACHE_W;
-            if (kCacheW && !(same && c.kw == kw))
+            if (!(same && c.kw == kw))
                 load_raw<T, LAYOUT>(window(win.pw, win.pw_stride, j), NY, NX, bzw, by, bx, c.w);
             c.s0 = s0;
             c.ku = ku;
@@ -1212,13 +1209,7 @@ __device__ __forceinline__ void blend_uvw(const opx_window &win, const opx_grid 
             c.kw = kw;
             blend_raw<T, LAYOUT>(c.u, gt, ft, vu);
             blend_raw<T, LAYOUT>(c.v, gt, ft, vv);
-            if constexpr (kCacheW) {
-                blend_raw<T, LAYOUT, RN3>(c.w, gt, ft, vw);
-            } else {
-                Raw3<T, LAYOUT> rw;
-                load_raw<T, LAYOUT>(window(win.pw, win.pw_stride, j), NY, NX, bzw, by, bx, rw);
-                blend_raw<T, LAYOUT, RN3>(rw, gt, ft, vw);
-            }
+            blend_raw<T, LAYOUT, RN3>(c.w, gt, ft, vw);
         }
     }
     if constexpr (LAYOUT == OPX_LAYOUT_RING || RING_OK) {
@@ -1913,9 +1904,7 @@ __device__ __forceinline__ void rk2_uvw(const opx_window &win, const opx_grid &g
     const AxD cz = axd(zi, 0.0, NZ), cy = axd(yi, 0.0, NY), cx = axd(xi, 0.0, NX);
     const AxD ix = axd(xi, -0.5, NX - 1), iy = axd(yi, -0.5, NY - 1), oz = axd(zi, 0.5, NZ + 1);
     float vu[8], vv[8], vw[8];
-    // FP32 RK2 (two samples per step) re-reads w: caching it would spill
-    blend_uvw<T, LAYOUT, RING_OK, false, sizeof(T) == 2>(win, g, j, s0, s1, gt, ft, cz.b, cy.b, cx.b, ix.b, iy.b, oz.b,
-                                                        c, vu, vv, vw);
+    blend_uvw<T, LAYOUT, RING_OK>(win, g, j, s0, s1, gt, ft, cz.b, cy.b, cx.b, ix.b, iy.b, oz.b, c, vu, vv, vw);
     u = tri32(vu, (float)cz.w, (float)cy.w, (float)ix.w);
     v = tri32(vv, (float)cz.w, (float)iy.w, (float)cx.w);
     w = (double)tri32(vw, (float)oz.w, (float)cy.w, (float)cx.w);
