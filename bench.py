@@ -27,8 +27,9 @@ Forms measured in the same run:
 - serving (``--serve SECONDS``, default 5): Poisson arrivals at 20/s for that
   many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 grid
   (FP32 and BF16 storage, the same requests) streamed through a 3-snapshot
-  window, admission by free HBM; every request runs to completion; p50/p99
-  latency under ``serving``.
+  window of which each slot receives only the 32 x 32-column tiles the
+  particles can reach (``--serve-tiles``), admission by free HBM; every
+  request runs to completion; p50/p99 latency under ``serving``.
 - isopycnal (``--extras iso``): the isopycnal-following AB3 on the same
   workload with a stratified, drifting density field, as the drop-in
   (validation, ab3_isopycnal_following, post step) and as the fused engine
@@ -641,10 +642,12 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
 
 def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_size=2_000_000, graph_steps=8,
-                tune=True, max_particles=256_000_000, storage=None, wl=None, reqs=None):
+                tune=True, max_particles=256_000_000, storage=None, wl=None, reqs=None, tiles=32, tile_copy="auto"):
     """Serving form: Poisson arrivals (``rate``/s for ``seconds`` s), sizes log-uniform on
     [1k, max_size], 1-10 simulated days (720-7200 steps of 120 s), BF16 fields streamed
-    through a 3-slot window, admission by free HBM; every request runs to completion."""
+    through a 3-slot window, admission by free HBM; every request runs to completion.
+    ``tiles``: each snapshot slot receives only the tiles of that many rho columns
+    square that the particles can reach (0: whole snapshots)."""
     import torch
 
     from offline_particles_amd.engine import AB3Engine
@@ -656,7 +659,10 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
     t = time.perf_counter()
     days = 10
     if wl is None:
-        wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1, steps=int(days * 86400 / 120) * 2 + 40,
+        # snapshots cycle, so the time array can cover any service clock: room for
+        # the longest request after arrivals over `seconds` at up to 10,000 steps/s
+        wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1,
+                                steps=int(days * 86400 / 120) + int(10_000 * seconds) + 40,
                                 nsnap_unique=nsnap, dtype=np.float32)
     if reqs is None:
         reqs = poisson_requests(wl, rate=rate, duration=seconds, size_range=(1_000, max_size),
@@ -674,7 +680,8 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
         # autotune on a probe: the 40 largest requests started together
         probe = sorted(reqs, key=lambda r: -r.size)[:40]
         eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
-                        capacity=sum(r.size for r in probe), max_releases=64, stream_slots=3)
+                        capacity=sum(r.size for r in probe), max_releases=64, stream_slots=3,
+                        stream_tiles=tiles or None)
         for r in probe:
             eng.add_release(r.zidx, r.yidx, r.xidx, nsteps=400)
         eng.rebin()
@@ -691,7 +698,9 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
     srv = ReleaseServer(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                         stream_slots=3, max_releases=4096, rebin_every=rebin_every, xlow=xlow, layout=layout,
                         block=block, graph_steps=graph_steps, max_particles=max_particles,
-                        launch_steps=launch_steps)
+                        launch_steps=launch_steps, stream_tiles=tiles or None)
+    if tiles:
+        srv.engine.tile_copy = tile_copy
     log(f"serving: server ready in {time.perf_counter() - t:.1f}s, capacity {srv.capacity} particles")
     srv.run(reqs)
     res = dict(srv.stats)
@@ -1237,6 +1246,9 @@ def main():
     ap.add_argument("--batched-max-size", type=int, default=1_000_000)
     ap.add_argument("--mix-storage", default="fp32,bf16",
                     help="field storage of the batching and serving forms (comma-separated: fp32, bf16)")
+    ap.add_argument("--serve-tiles", type=int, default=32, metavar="CELLS",
+                    help="serving form: stream only the tiles (CELLS rho columns square) of each snapshot that "
+                         "the particles can reach (0: whole snapshots)")
     ap.add_argument("--serve", type=float, default=5.0, metavar="SECONDS",
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
@@ -1347,7 +1359,8 @@ def main():
         wl_s = reqs = None
         for st in args.mix_storage.split(","):
             res, wl_s, reqs = run_serving(device, args.serve, grid=sg, graph_steps=args.graph_steps,
-                                          tune=not args.no_tune, storage=storages[st], wl=wl_s, reqs=reqs)
+                                          tune=not args.no_tune, storage=storages[st], wl=wl_s, reqs=reqs,
+                                          tiles=args.serve_tiles)
             serving[res["form"]] = res
             log("serving", res)
         del wl_s, reqs

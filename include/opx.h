@@ -290,6 +290,44 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
  * and p2. */
 int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
                     void *pv, void *pw, void *p2, void *stream);
+/* opx_pack_window for the rho rows [y0, y1) only (each field's rows in that
+ * range, every z-level and column; 0 <= y0 < y1 <= NY): refreshes the part of
+ * a packed window whose slots received new data there. */
+int opx_pack_window_rows(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1,
+                         int32_t y0, int32_t y1, void *pu, void *pv, void *pw, void *p2, void *stream);
+
+/* Sparse snapshot streaming: where the particles are and which parts of a
+ * snapshot to copy to the device.  Counterpart of the reference's lazy
+ * chunked field (ChunkedDaskArray.get_data_subset, spatial_arrays.py:260-295,
+ * which loads only the chunks inside the particles' padded bounding box from
+ * Launcher.construct_bbox, launcher.py:122-148); here the region is a set of
+ * horizontal tiles rather than one box, so separate clouds of particles do not
+ * pull in the space between them.
+ *
+ * opx_tile_occupancy: map[ty * ntx + tx] = 1 (else 0) for every tile of
+ * tile x tile rho columns, nty = ceil(NY / tile) by ntx = ceil(NX / tile),
+ * that holds a particle i < s->n whose status is not INACTIVE and whose y and
+ * x cells are finite (a cell outside the grid counts in the nearest edge
+ * tile).  `map` is device memory of nty * ntx bytes; the call clears it with a
+ * kernel (no memset), so it may be captured in a graph. */
+int opx_tile_occupancy(const opx_split_state *s, const opx_grid *g, int32_t tile, uint8_t *map, int32_t nty,
+                       int32_t ntx, void *stream);
+/* Copy boxes of a C-contiguous (nz, ny, nx) host array of elem_bytes elements
+ * to the same places of a device array of the same shape: box b = boxes[4b ..
+ * 4b + 3] = (y0, y1, x0, x1) covers rows [y0, y1) and columns [x0, x1) of
+ * every z-level, one rectangular DMA copy each (hipMemcpy3DAsync) on `stream`.
+ * `boxes` is host memory; `src` should be pinned for the copies to be
+ * asynchronous. */
+int opx_copy_boxes_h2d(void *dst, const void *src, int64_t nz, int64_t ny, int64_t nx, int32_t elem_bytes,
+                       const int32_t *boxes, int32_t nboxes, void *stream);
+/* The same boxes copied by a kernel on `stream` that reads the host array
+ * through its device mapping (hipHostGetDevicePointer): `src` must be pinned
+ * host memory mapped to the device over its whole extent (else
+ * OPX_ERR_INVALID, nothing launched).  For 1- and 2-byte elements, whose
+ * rectangular DMA copies run far below the link rate (rows not 4-byte aligned,
+ * or narrow). */
+int opx_gather_boxes_h2d(void *dst, const void *src, int64_t nz, int64_t ny, int64_t nx, int32_t elem_bytes,
+                         const int32_t *boxes, int32_t nboxes, void *stream);
 
 /* Per-release FP64 clock kept on the device.  A release advances at global
  * steps >= start_step while steps_left > 0. */

@@ -180,6 +180,14 @@ _SIGNATURES.update({
                          _P(ctypes.c_uint64), _P(ctypes.c_uint64)],
     "opx_pack_window": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp,
                         _vp, _vp],
+    "opx_pack_window_rows": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                             ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp],
+    "opx_tile_occupancy": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
+                           _vp],
+    "opx_copy_boxes_h2d": [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _vp,
+                           ctypes.c_int32, _vp],
+    "opx_gather_boxes_h2d": [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _vp,
+                             ctypes.c_int32, _vp],
     "opx_rebin": [_P(opx_split_state), _P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, _vp,
                   ctypes.c_uint64, _vp, _vp, _vp],
     "opx_spatial_order_workspace_bytes": [ctypes.c_int64, _P(ctypes.c_uint64)],

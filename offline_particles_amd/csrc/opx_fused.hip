@@ -1693,13 +1693,20 @@ __global__ void __launch_bounds__(64)
 
 // ---------------------------------------------------------------- packed windows
 
+// Packs the nodes of rows [row0, row0 + nrows) of every z-level (n rows of n2
+// nodes per level, n1 rows per level): thread j of the launch packs node
+// (z, row0 + r, x) with j = (z nrows + r) n2 + x, so a launch over all rows
+// packs node j.  The window limit (4 GiB) keeps node indices below 2^32.
 template <typename T, int LAYOUT>
-__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, int64_t n, int n1,
-                       int n2)
+__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, uint32_t n, int n1,
+                       int n2, int row0, int nrows)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
         return;
+    const uint32_t band = (uint32_t)nrows * (uint32_t)n2;
+    const uint32_t zl = j / band, r = j - zl * band;
+    const int64_t i = ((int64_t)zl * n1 + row0) * n2 + r;
     if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
         // neighbours beyond the last row / column are never stencil nodes of a base here
         const int64_t x1 = (i % n2) + 1 < n2 ? 1 : 0;
@@ -1753,15 +1760,68 @@ __global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *_
     }
 }
 
-// p2[i] = {h, zeta[s0], zeta[s1], dx} per rho point
+// p2[i] = {h, zeta[s0], zeta[s1], dx} per rho point, for points i0 .. i0 + n - 1
 __global__ void k_pack2(const float4 *__restrict__ hdd, const float *__restrict__ zp, const float *__restrict__ zq,
-                        float4 *__restrict__ out, int64_t n)
+                        float4 *__restrict__ out, int64_t i0, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    i += i0;
     const float4 m = hdd[i];
     out[i] = make_float4(m.x, zp[i], zq[i], m.y);
+}
+
+// ---------------------------------------------------------------- sparse snapshot streaming
+
+__global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// map[ty * ntx + tx] = 1 for every tile of tile x tile rho columns holding a
+// particle whose status is not INACTIVE and whose horizontal cells are finite
+// (cells outside the grid count in the nearest edge tile).  Threads that mark
+// one tile store the same byte.
+__global__ void k_tile_occupancy(opx_split_state s, opx_grid g, int tile, uint8_t *__restrict__ map, int ntx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n || (s.status[i] & OPX_INACTIVE))
+        return;
+    const uint32_t cell = (uint32_t)s.cell[i];
+    const Axis y = decode_axis(cell, g.xbits, g.ybits), x = decode_axis(cell, 0, g.xbits);
+    if (!(y.ok && x.ok))
+        return;
+    const int yy = min(max(y.c, 0), g.NY - 1), xx = min(max(x.c, 0), g.NX - 1);
+    map[(yy / tile) * ntx + xx / tile] = 1;
+}
+
+// Boxes of rows copied from mapped host memory: up to kGatherBoxes boxes per
+// launch, passed by value; block b copies one row (box, z-level, y) of its box,
+// found from the prefix counts of rows.
+constexpr int kGatherBoxes = 64;
+struct GatherBoxes {
+    int32_t box[kGatherBoxes][4]; // (y0, y1, x0, x1)
+    int64_t first_row[kGatherBoxes + 1];
+    int32_t n;
+};
+
+template <typename E>
+__global__ void k_gather_rows(E *__restrict__ dst, const E *__restrict__ src, int64_t nz, int64_t ny, int64_t nx,
+                              GatherBoxes B)
+{
+    const int64_t row = blockIdx.x;
+    int b = 0;
+    while (b + 1 < B.n && B.first_row[b + 1] <= row)
+        ++b;
+    const int32_t *x = B.box[b];
+    const int64_t k = row - B.first_row[b], h = x[1] - x[0];
+    const int64_t z = k / h, y = x[0] + (k - z * h);
+    const int64_t base = (z * ny + y) * nx;
+    for (int64_t c = x[2] + threadIdx.x; c < x[3]; c += blockDim.x)
+        dst[base + c] = src[base + c];
 }
 
 // ---------------------------------------------------------------- conversions
@@ -2729,8 +2789,8 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
     return OPX_OK;
 }
 
-int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
-                    void *pv, void *pw, void *p2, void *stream)
+int opx_pack_window_rows(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1,
+                         int32_t y0, int32_t y1, void *pu, void *pv, void *pw, void *p2, void *stream)
 {
     int rc;
     if ((rc = check_grid(g))) return rc;
@@ -2743,30 +2803,38 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
                 "layout must be PAIR, QUAD, OCT, YPAIR or PAIR_LDS");
     OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
                 s1, win->nslots);
+    OPX_REQUIRE(0 <= y0 && y0 < y1 && y1 <= g->NY, "rows [%d, %d) outside [0, %d)", y0, y1, g->NY);
     OPX_REQUIRE(pu && pv && pw && p2, "null output window");
     OPX_REQUIRE(win->hdd && win->zeta, "window without hdd/zeta");
+    uint64_t bu, bv, bw, b2;
+    if ((rc = opx_packed_bytes(g, win->dtype3, layout, &bu, &bv, &bw, &b2))) return rc;
     hipStream_t st = as_stream(stream);
-    const int64_t n2d = (int64_t)g->NY * g->NX;
+    const int64_t n2d = (int64_t)(y1 - y0) * g->NX;
     hipLaunchKernelGGL(k_pack2, dim3(grid_for(n2d)), dim3(kBlock), 0, st, reinterpret_cast<const float4 *>(win->hdd),
                        win->zeta + s0 * win->zeta_stride, win->zeta + s1 * win->zeta_stride,
-                       static_cast<float4 *>(p2), n2d);
+                       static_cast<float4 *>(p2), (int64_t)y0 * g->NX, n2d);
     if ((rc = check_launch("opx_pack_window 2d"))) return rc;
-    const int64_t nu = (int64_t)g->NZ * g->NY * (g->NX - 1), nv = (int64_t)g->NZ * (g->NY - 1) * g->NX;
     const int wl = w_levels(win, g);
     OPX_REQUIRE(wl == g->NZ || wl == g->NZ + 1, "third field levels %d not NZ or NZ + 1", wl);
-    const int64_t nw = (int64_t)wl * g->NY * g->NX;
     struct F {
         const void *base;
-        int64_t stride, n;
-        int n1, n2;
+        int64_t stride;
+        int nz, n1, n2;
         void *out;
-    } f[3] = {{win->u, win->u_stride, nu, g->NY, g->NX - 1, pu}, {win->v, win->v_stride, nv, g->NY - 1, g->NX, pv},
-              {win->w, win->w_stride, nw, g->NY, g->NX, pw}};
+    } f[3] = {{win->u, win->u_stride, g->NZ, g->NY, g->NX - 1, pu}, {win->v, win->v_stride, g->NZ, g->NY - 1, g->NX, pv},
+              {win->w, win->w_stride, wl, g->NY, g->NX, pw}};
     for (auto &x : f) {
-        const unsigned grid = grid_for(x.n);
+        // the field's rows in [y0, y1) (v has NY - 1 rows)
+        const int r0 = y0, r1 = min(y1, x.n1);
+        if (r0 >= r1)
+            continue;
+        const int64_t n = (int64_t)x.nz * (r1 - r0) * x.n2;
+        // a packed window of at most 4 GiB holds fewer than 2^32 nodes
+        OPX_REQUIRE(n < (int64_t(1) << 32), "window of %lld nodes", (long long)n);
+        const unsigned grid = grid_for(n);
 #define OPX_PACK(T, L)                                                                                      \
     hipLaunchKernelGGL((k_pack<T, L>), dim3(grid), dim3(kBlock), 0, st, b + s0 * x.stride, b + s1 * x.stride, \
-                       x.out, x.n, x.n1, x.n2)
+                       x.out, (uint32_t)n, x.n1, x.n2, r0, r1 - r0)
         if (win->dtype3 == OPX_F32) {
             const float *b = static_cast<const float *>(x.base);
             if (layout == OPX_LAYOUT_PAIR)
@@ -2790,6 +2858,132 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
         }
 #undef OPX_PACK
         if ((rc = check_launch("opx_pack_window"))) return rc;
+    }
+    return OPX_OK;
+}
+
+int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
+                    void *pv, void *pw, void *p2, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    return opx_pack_window_rows(win, g, layout, s0, s1, 0, g->NY, pu, pv, pw, p2, stream);
+}
+
+int opx_tile_occupancy(const opx_split_state *s, const opx_grid *g, int32_t tile, uint8_t *map, int32_t nty,
+                       int32_t ntx, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(s != nullptr && map != nullptr, "null state or map");
+    OPX_REQUIRE(tile >= 1, "tile %d < 1", tile);
+    OPX_REQUIRE(nty == (g->NY + tile - 1) / tile && ntx == (g->NX + tile - 1) / tile,
+                "map of %d x %d tiles, the grid needs %d x %d", nty, ntx, (g->NY + tile - 1) / tile,
+                (g->NX + tile - 1) / tile);
+    OPX_REQUIRE(s->n == 0 || (s->status && s->cell), "state without status or cells");
+    hipStream_t st = as_stream(stream);
+    const int64_t nm = (int64_t)nty * ntx;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(nm)), dim3(kBlock), 0, st, map, nm, (uint8_t)0);
+    if ((rc = check_launch("opx_tile_occupancy fill"))) return rc;
+    if (s->n > 0) {
+        hipLaunchKernelGGL(k_tile_occupancy, dim3(grid_for(s->n)), dim3(kBlock), 0, st, *s, *g, (int)tile, map,
+                           (int)ntx);
+        if ((rc = check_launch("opx_tile_occupancy"))) return rc;
+    }
+    return OPX_OK;
+}
+
+int opx_gather_boxes_h2d(void *dst, const void *src, int64_t nz, int64_t ny, int64_t nx, int32_t elem_bytes,
+                         const int32_t *boxes, int32_t nboxes, void *stream)
+{
+    int rc;
+    OPX_REQUIRE(dst != nullptr && src != nullptr, "null array");
+    OPX_REQUIRE(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8, "element size %d",
+                elem_bytes);
+    OPX_REQUIRE(nz >= 1 && ny >= 1 && nx >= 1, "array shape (%lld, %lld, %lld)", (long long)nz, (long long)ny,
+                (long long)nx);
+    OPX_REQUIRE(nboxes >= 0 && (nboxes == 0 || boxes != nullptr), "%d boxes", nboxes);
+    for (int32_t b = 0; b < nboxes; ++b) {
+        const int32_t *x = boxes + 4 * b;
+        OPX_REQUIRE(0 <= x[0] && x[0] < x[1] && x[1] <= ny && 0 <= x[2] && x[2] < x[3] && x[3] <= nx,
+                    "box %d (rows [%d, %d), columns [%d, %d)) outside (%lld, %lld)", b, x[0], x[1], x[2], x[3],
+                    (long long)ny, (long long)nx);
+    }
+    if (nboxes == 0)
+        return OPX_OK;
+    // the kernel reads the host array through its device mapping: both ends of
+    // the array must be mapped pinned memory of one contiguous mapping
+    const size_t total = (size_t)nz * ny * nx * elem_bytes;
+    void *d0 = nullptr, *d1 = nullptr;
+    const char *h0 = static_cast<const char *>(src);
+    if (hipHostGetDevicePointer(&d0, const_cast<char *>(h0), 0) != hipSuccess ||
+        hipHostGetDevicePointer(&d1, const_cast<char *>(h0 + total - 1), 0) != hipSuccess ||
+        static_cast<char *>(d1) - static_cast<char *>(d0) != (ptrdiff_t)(total - 1)) {
+        (void)hipGetLastError();
+        set_error("opx_gather_boxes_h2d: the source is not mapped pinned host memory");
+        return OPX_ERR_INVALID;
+    }
+    hipStream_t st = as_stream(stream);
+    for (int32_t b0 = 0; b0 < nboxes; b0 += kGatherBoxes) {
+        GatherBoxes B = {};
+        B.n = min(kGatherBoxes, nboxes - b0);
+        int64_t rows = 0;
+        for (int i = 0; i < B.n; ++i) {
+            const int32_t *x = boxes + 4 * (b0 + i);
+            for (int j = 0; j < 4; ++j)
+                B.box[i][j] = x[j];
+            B.first_row[i] = rows;
+            rows += nz * (int64_t)(x[1] - x[0]);
+        }
+        B.first_row[B.n] = rows;
+        OPX_REQUIRE(rows < (int64_t(1) << 31), "%lld rows in one launch", (long long)rows);
+        const dim3 grid((unsigned)rows), block(256);
+        switch (elem_bytes) {
+        case 1: hipLaunchKernelGGL(k_gather_rows<uint8_t>, grid, block, 0, st, static_cast<uint8_t *>(dst),
+                                   static_cast<const uint8_t *>(d0), nz, ny, nx, B); break;
+        case 2: hipLaunchKernelGGL(k_gather_rows<uint16_t>, grid, block, 0, st, static_cast<uint16_t *>(dst),
+                                   static_cast<const uint16_t *>(d0), nz, ny, nx, B); break;
+        case 4: hipLaunchKernelGGL(k_gather_rows<uint32_t>, grid, block, 0, st, static_cast<uint32_t *>(dst),
+                                   static_cast<const uint32_t *>(d0), nz, ny, nx, B); break;
+        default: hipLaunchKernelGGL(k_gather_rows<uint64_t>, grid, block, 0, st, static_cast<uint64_t *>(dst),
+                                    static_cast<const uint64_t *>(d0), nz, ny, nx, B); break;
+        }
+        if ((rc = check_launch("opx_gather_boxes_h2d"))) return rc;
+    }
+    return OPX_OK;
+}
+
+int opx_copy_boxes_h2d(void *dst, const void *src, int64_t nz, int64_t ny, int64_t nx, int32_t elem_bytes,
+                       const int32_t *boxes, int32_t nboxes, void *stream)
+{
+    OPX_REQUIRE(dst != nullptr && src != nullptr, "null array");
+    OPX_REQUIRE(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8, "element size %d",
+                elem_bytes);
+    OPX_REQUIRE(nz >= 1 && ny >= 1 && nx >= 1, "array shape (%lld, %lld, %lld)", (long long)nz, (long long)ny,
+                (long long)nx);
+    OPX_REQUIRE(nboxes >= 0 && (nboxes == 0 || boxes != nullptr), "%d boxes", nboxes);
+    for (int32_t b = 0; b < nboxes; ++b) {
+        const int32_t *x = boxes + 4 * b;
+        OPX_REQUIRE(0 <= x[0] && x[0] < x[1] && x[1] <= ny && 0 <= x[2] && x[2] < x[3] && x[3] <= nx,
+                    "box %d (rows [%d, %d), columns [%d, %d)) outside (%lld, %lld)", b, x[0], x[1], x[2], x[3],
+                    (long long)ny, (long long)nx);
+    }
+    hipStream_t st = as_stream(stream);
+    const size_t eb = (size_t)elem_bytes;
+    for (int32_t b = 0; b < nboxes; ++b) {
+        const int32_t *x = boxes + 4 * b;
+        hipMemcpy3DParms p = {};
+        p.srcPtr = make_hipPitchedPtr(const_cast<void *>(src), (size_t)nx * eb, (size_t)nx, (size_t)ny);
+        p.dstPtr = make_hipPitchedPtr(dst, (size_t)nx * eb, (size_t)nx, (size_t)ny);
+        p.srcPos = make_hipPos((size_t)x[2] * eb, (size_t)x[0], 0);
+        p.dstPos = p.srcPos;
+        p.extent = make_hipExtent((size_t)(x[3] - x[2]) * eb, (size_t)(x[1] - x[0]), (size_t)nz);
+        p.kind = hipMemcpyHostToDevice;
+        const hipError_t e = hipMemcpy3DAsync(&p, st);
+        if (e != hipSuccess) {
+            set_error("opx_copy_boxes_h2d: box %d: %s", b, hipGetErrorString(e));
+            return OPX_ERR_HIP;
+        }
     }
     return OPX_OK;
 }

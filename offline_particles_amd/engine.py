@@ -35,7 +35,25 @@ the same buffers (``offline_particles_amd.serving``).
 Snapshot windows: by default every snapshot of u, v, w, zeta is resident.
 With ``stream_slots`` the snapshots stay in pinned host memory and a ring of
 that many device slots is filled on demand (a side stream copies the next
-snapshot while the current ones are in use).
+snapshot while the current ones are in use).  With ``stream_tiles`` as well,
+a slot receives only the horizontal tiles (``stream_tiles`` rho columns
+square, every level) that the particles can reach while it is read: the
+reference's lazy chunked field loads only the chunks inside the particles'
+padded bounding box (``ChunkedDaskArray.get_data_subset``,
+spatial_arrays.py:260-295, with ``Launcher.construct_bbox``,
+launcher.py:122-148); here the region is the set of tiles holding particles,
+widened by a bound on how far they move, so separate clouds of particles do
+not pull in the ocean between them.  The bound: every step moves a particle
+by at most dt x (44/12 for AB3, max(alpha, |b1| + |b2|) for RK2) x R cells,
+R the largest |u| / dx or |v| / dy of any snapshot loaded so far (the
+interpolation weights of a particle inside the grid are convex, so the
+interpolated rate never exceeds R).  A tile map of the particles is computed
+on the device after every run of steps and read back asynchronously; before
+each run of steps every slot it reads must hold the tiles of the latest map
+read back (and of releases added since), widened by the bound times the
+steps since that map plus two cells of stencil reach.  Missing tiles are
+copied (rectangular DMA copies straight into the slot) and the packed windows'
+rows they touch are packed again.
 """
 
 from __future__ import annotations
@@ -72,6 +90,64 @@ LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib
                 _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct", _lib.OPX_LAYOUT_YPAIR: "ypair"}
 
 
+def dilate_tiles(mask: np.ndarray, d: int) -> np.ndarray:
+    """The tiles within d tiles (in y and in x) of a set tile of the 2-D bool `mask`."""
+    mask = np.asarray(mask, dtype=bool)
+    if d <= 0:
+        return mask.copy()
+    rows = mask.copy()
+    for s in range(1, min(d, mask.shape[0] - 1) + 1):
+        rows[s:] |= mask[:-s]
+        rows[:-s] |= mask[s:]
+    out = rows.copy()
+    for s in range(1, min(d, mask.shape[1] - 1) + 1):
+        out[:, s:] |= rows[:, :-s]
+        out[:, :-s] |= rows[:, s:]
+    return out
+
+
+def tile_boxes(mask: np.ndarray, tile: int, ny: int, nx: int, gap: int = 0):
+    """Boxes covering the set tiles of `mask` (tile rows x tile columns of `tile`
+    cells over an ny x nx grid) for rectangular copies.
+
+    In each tile row, runs of set tiles separated by at most `gap` unset tiles
+    form one run (longer rows copy faster); consecutive tile rows with the same
+    run are stacked into one box.  Returns (boxes, covered): int32 (k, 4) boxes
+    (y0, y1, x0, x1) in cells, clipped to the grid, and the bool mask of the
+    tiles they cover (the set tiles and the bridged gaps)."""
+    mask = np.asarray(mask, dtype=bool)
+    nty, ntx = mask.shape
+    covered = np.zeros_like(mask)
+    rows = []
+    for ty in range(nty):
+        idx = np.flatnonzero(mask[ty])
+        runs = []
+        if idx.size:
+            start = prev = int(idx[0])
+            for t in idx[1:]:
+                t = int(t)
+                if t - prev - 1 <= gap:
+                    prev = t
+                else:
+                    runs.append((start, prev + 1))
+                    start = prev = t
+            runs.append((start, prev + 1))
+        for a, b in runs:
+            covered[ty, a:b] = True
+        rows.append(runs)
+    boxes = []
+    open_runs: dict = {}
+    for ty in range(nty + 1):
+        cur = set(rows[ty]) if ty < nty else set()
+        for run in [r for r in open_runs if r not in cur]:
+            ty0 = open_runs.pop(run)
+            boxes.append((ty0 * tile, min(ty * tile, ny), run[0] * tile, min(run[1] * tile, nx)))
+        for run in cur:
+            open_runs.setdefault(run, ty)
+    out = np.array(sorted(boxes), dtype=np.int32).reshape(-1, 4)
+    return out, covered
+
+
 def _bits(n: int) -> int:
     return max(1, int(math.ceil(math.log2(max(n, 2)))))
 
@@ -91,6 +167,13 @@ def _h2d(a, dtype, device):
 
 
 _NP = {torch.float64: np.float64, torch.uint8: np.uint8}
+
+
+def _to_host(a) -> np.ndarray:
+    """A numpy view or copy of an array or tensor (device tensors are copied back)."""
+    if torch.is_tensor(a):
+        return a.detach().to("cpu", torch.float64).numpy()
+    return np.asarray(a, dtype=np.float64)
 
 
 class SplitState:
@@ -139,7 +222,7 @@ class AB3Engine:
     def __init__(self, fieldset, time_array, dt, *, storage_dtype=torch.float32, device=None,
                  capacity: int = 1 << 20, rebin_every: int = 25, xlow: int = 2,
                  max_releases: int = 256, stream_slots: int | None = None, scheme: str = "ab3",
-                 alpha: float = 2 / 3):
+                 alpha: float = 2 / 3, stream_tiles: int | None = None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if storage_dtype not in (torch.float32, torch.bfloat16):
             raise TypeError("storage_dtype must be torch.float32 or torch.bfloat16")
@@ -172,6 +255,12 @@ class AB3Engine:
         self._stream_slots = None if stream_slots is None else int(stream_slots)
         if self._stream_slots is not None and self._stream_slots < 3:
             raise ValueError("a streamed window needs at least 3 slots")
+        self._tile = None if stream_tiles is None else int(stream_tiles)
+        if self._tile is not None:
+            if self._stream_slots is None:
+                raise ValueError("stream_tiles needs a streamed window (stream_slots)")
+            if self._tile < 4:
+                raise ValueError(f"stream_tiles must be at least 4 cells, not {self._tile}")
         self._build_window(fieldset)
         self._with_rho_t = scheme == "ab3_iso"
         self._state = self._new_state(capacity)
@@ -294,6 +383,10 @@ class AB3Engine:
             self._copy_streams = [torch.cuda.Stream(self.device) for _ in range(3)]
             self.snapshot_loads = 0
             self.snapshot_bytes = 0
+            # bytes whole-snapshot loads would have copied (for the sparse mode's ratio)
+            self.snapshot_full_bytes = 0
+            if self._tile is not None:
+                self._init_tiles(fs, NY, NX)
         h = fs["h"].data.data.to(self.device, torch.float32)
         dx = fs["dx"].data.data.to(self.device, torch.float32)
         dy = fs["dy"].data.data.to(self.device, torch.float32)
@@ -445,6 +538,10 @@ class AB3Engine:
         self._clocks[r * cs:(r + 1) * cs].copy_(_h2d(clk, torch.uint8, self.device))
         self._nrel_dev.fill_(len(self._clocks_host))
         self._structs = None
+        if self._tile is not None and m:
+            # the release's tiles count until a map that includes it is read back
+            self._occ_seq += 1
+            self._occ_adds.append((self._occ_seq, self._tile_mask_of(_to_host(yidx), _to_host(xidx))))
         return r
 
     def steps_left(self, r: int) -> int:
@@ -585,11 +682,20 @@ class AB3Engine:
     # ---------------------------------------------------------------- streamed snapshots
 
     def _load_slot(self, It: int) -> None:
-        """Copy snapshot It into ring slot It % nslots on the side stream, after the
-        work already queued on the compute stream (which may still read the slot)."""
+        """Copy snapshot It into ring slot It % nslots on the side streams, after the
+        work already queued on the compute stream (which may still read the slot).
+        With stream_tiles only the tiles the particles can reach are copied (zeta,
+        2-D and small, is copied whole)."""
         slot = It % self.nslots
         src = It % self.nsnap
         compute = torch.cuda.current_stream(self.device)
+        hosts = {name: self._host_snapshot(name, src) for name in ("u", "v", "w", "zeta")}
+        boxes = None
+        if self._tile is not None:
+            self._rate = max(self._rate, self._snapshot_rate(src, hosts))
+            region = self._tile_region(self.global_step + self.tile_margin_steps)
+            boxes, covered = tile_boxes(region, self._tile, self.NY, self.NX, self.tile_gap)
+            self._slot_tiles[slot] = covered
         # one side stream per 3-D field: the copies run on separate DMA queues
         events = []
         nbytes = 0
@@ -599,18 +705,219 @@ class AB3Engine:
             with torch.cuda.stream(cs):
                 for name in names:
                     ring = self._zeta if name == "zeta" else dst
-                    host = self._host_snapshot(name, src)
-                    ring[slot].copy_(host, non_blocking=True)
-                    nbytes += host.nbytes
+                    host = hosts[name]
+                    self.snapshot_full_bytes += host.nbytes
+                    if boxes is None or name == "zeta":
+                        ring[slot].copy_(host, non_blocking=True)
+                        nbytes += host.nbytes
+                    else:
+                        if self._poison_unloaded:
+                            # a checking aid: any read of a tile not loaded gives NaN
+                            ring[slot].fill_(float("nan"))
+                        nbytes += self._copy_boxes(ring[slot], host, name, boxes, cs.cuda_stream)
                 ev = torch.cuda.Event()
                 ev.record(cs)
             events.append(ev)
+        if self._tile is not None:
+            # the host copies of the slot's snapshot serve its top-ups; the ones it
+            # held before stay referenced until their copies are done
+            if self._slot_host[slot] is not None:
+                self._host_keep.append(self._slot_host[slot])
+            self._slot_host[slot] = (hosts, events)
+            self._host_keep = [(h, e) for h, e in self._host_keep if not all(x.query() for x in e)]
         self._ring_It[slot] = It
         self._ring_ready[slot] = events
         self.snapshot_loads += 1
         self.snapshot_bytes += nbytes
         # pair-ring windows packed from the slot's old contents are repacked when
         # next read (their packed snapshot pair no longer matches the slots)
+
+    # ---------------------------------------------------------------- sparse (tiled) streaming
+
+    def _init_tiles(self, fs, NY, NX):
+        T = self._tile
+        self._nty, self._ntx = -(-NY // T), -(-NX // T)
+        self._slot_tiles = [np.zeros((self._nty, self._ntx), dtype=bool) for _ in range(self.nslots)]
+        self._slot_host = [None] * self.nslots
+        self._host_keep: list = []
+        dxm = float(torch.as_tensor(fs["dx"].data.data).min())
+        dym = float(torch.as_tensor(fs["dy"].data.data).min())
+        if not (dxm > 0 and dym > 0):
+            raise ValueError("stream_tiles needs positive grid spacings dx, dy")
+        self._inv_dx, self._inv_dy = 1.0 / dxm, 1.0 / dym
+        # largest displacement of one step in units of dt x R (see the module docstring)
+        if self.scheme == "rk2":
+            b1 = 1.0 / (2.0 * self.alpha)
+            self._reach = max(abs(self.alpha), abs(b1) + abs(1.0 - b1))
+        else:
+            self._reach = 44.0 / 12.0
+        self._snap_rate: dict = {}
+        self._rate = 0.0
+        self._occ_dev = torch.zeros(self._nty * self._ntx, dtype=torch.uint8, device=self.device)
+        self._occ_pending: list = []  # (seq, step, pinned map, event), oldest first
+        self._occ_done = None  # (seq, step, bool map): the latest map read back
+        self._occ_adds: list = []  # (seq, bool map) of releases added after it
+        self._occ_seq = 0
+        # steps of movement a load or top-up covers beyond the run of steps at hand
+        self.tile_margin_steps = 16
+        # unset tiles bridged between two runs of a tile row (longer copies)
+        self.tile_gap = 1
+        # wait for a newer map rather than widen the region past this many steps
+        self.max_occupancy_lag = 64
+        self.tile_topups = 0
+        self._poison_unloaded = False
+        self._gather_ok = True
+        # how tiles reach a slot: "dma" (rectangular DMA copies), "gather" (a
+        # kernel reading the pinned snapshot through its device mapping) or
+        # "auto" (gather for 1- and 2-byte fields, DMA otherwise)
+        self.tile_copy = "auto"
+
+    def _snapshot_rate(self, src: int, hosts: dict) -> float:
+        """Largest |u| / dx, |v| / dy of snapshot src (index units per time unit)."""
+        r = self._snap_rate.get(src)
+        if r is None:
+            r = 0.0
+            for name, inv in (("u", self._inv_dx), ("v", self._inv_dy)):
+                lo, hi = torch.aminmax(hosts[name].reshape(-1))
+                r = max(r, max(abs(float(lo)), abs(float(hi))) * inv)
+            if not math.isfinite(r):
+                r = math.inf
+            self._snap_rate[src] = r
+        return r
+
+    def _request_occupancy(self):
+        """Tile map of the particles as of the work queued so far, read back asynchronously."""
+        _lib.call("opx_tile_occupancy", ctypes.byref(self._state.struct(self.n)), ctypes.byref(self.grid),
+                  self._tile, self._occ_dev.data_ptr(), self._nty, self._ntx, self._stream())
+        buf = torch.empty(self._nty * self._ntx, dtype=torch.uint8, pin_memory=True)
+        buf.copy_(self._occ_dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._occ_seq += 1
+        self._occ_pending.append((self._occ_seq, self.global_step, buf, ev))
+
+    def _fold_occupancy(self):
+        seq, step, buf, ev = self._occ_pending.pop(0)
+        self._occ_done = (seq, step, buf.numpy().reshape(self._nty, self._ntx) != 0)
+
+    def _occupancy(self, horizon: int):
+        """(step S, tile mask): the latest map read back (positions after S steps) with
+        the releases added since; waits for a newer map when S lags `horizon` by more
+        than max_occupancy_lag steps."""
+        while self._occ_pending and self._occ_pending[0][3].query():
+            self._fold_occupancy()
+        while self._occ_pending and (self._occ_done is None
+                                     or horizon - self._occ_done[1] > self.max_occupancy_lag):
+            self._occ_pending[0][3].synchronize()
+            self._fold_occupancy()
+        if self._occ_done is None:
+            self._request_occupancy()
+            self._occ_pending[0][3].synchronize()
+            self._fold_occupancy()
+        seq, S, occ = self._occ_done
+        mask = occ.copy()
+        self._occ_adds = [(q, a) for q, a in self._occ_adds if q > seq]
+        for _, a in self._occ_adds:
+            mask |= a
+        return S, mask
+
+    def _tile_region(self, horizon: int) -> np.ndarray:
+        """Tiles a particle can read up to step `horizon`: the tiles of the latest map
+        widened by the steps since it times the bound on one step's displacement,
+        plus two cells of stencil reach."""
+        S, occ = self._occupancy(horizon)
+        r = max(0, horizon - S) * abs(self.dt) * self._rate * self._reach + 2.0
+        if not math.isfinite(r) or r >= self._tile * max(self._nty, self._ntx):
+            return np.ones_like(occ)
+        return dilate_tiles(occ, int(math.ceil(r / self._tile)))
+
+    def _tile_mask_of(self, y, x) -> np.ndarray:
+        """Tiles holding the finite positions (y, x) (cells outside the grid in the
+        nearest edge tile), as the occupancy kernel counts them."""
+        y = np.asarray(y, dtype=np.float64).reshape(-1)
+        x = np.asarray(x, dtype=np.float64).reshape(-1)
+        ok = np.isfinite(y) & np.isfinite(x)
+        ty = np.clip(np.floor(y[ok]), 0, self.NY - 1).astype(np.int64) // self._tile
+        tx = np.clip(np.floor(x[ok]), 0, self.NX - 1).astype(np.int64) // self._tile
+        mask = np.zeros((self._nty, self._ntx), dtype=bool)
+        mask[ty, tx] = True
+        return mask
+
+    def _copy_boxes(self, dst, host, name, boxes, stream) -> int:
+        """Copy the rho-cell boxes (clipped to the field's own shape) of `host` into
+        `dst`; returns the bytes copied."""
+        nz, ny, nx = self._ring_shape[name]
+        b = boxes.copy()
+        b[:, 1] = np.minimum(b[:, 1], ny)
+        b[:, 3] = np.minimum(b[:, 3], nx)
+        b = np.ascontiguousarray(b[(b[:, 0] < b[:, 1]) & (b[:, 2] < b[:, 3])], dtype=np.int32)
+        if not len(b):
+            return 0
+        if not (host.is_contiguous() and dst.is_contiguous()):
+            raise ValueError("tile copies need contiguous snapshots")
+        args = (dst.data_ptr(), host.data_ptr(), nz, ny, nx, host.element_size(), b.ctypes.data, len(b), stream)
+        gather = self.tile_copy == "gather" or (self.tile_copy == "auto" and host.element_size() < 4)
+        if gather and self._gather_ok:
+            # rectangular DMA copies of 2-byte rows run far below the link rate:
+            # a kernel reads the boxes through the pinned memory's device mapping
+            try:
+                _lib.call("opx_gather_boxes_h2d", *args)
+                return self._box_bytes(b, nz, host.element_size())
+            except _lib.OpxInvalidArgument:
+                self._gather_ok = False  # not mapped: DMA copies from now on
+        _lib.call("opx_copy_boxes_h2d", *args)
+        return self._box_bytes(b, nz, host.element_size())
+
+    @staticmethod
+    def _box_bytes(b, nz, eb) -> int:
+        return int(((b[:, 1] - b[:, 0]).astype(np.int64) * (b[:, 3] - b[:, 2])).sum()) * nz * eb
+
+    def _cover(self, need) -> None:
+        """Make every slot of snapshots `need` hold the tiles the particles can reach
+        by the end of the queued steps: copy the missing ones (on the compute
+        stream, so the next steps see them) and pack their rows again."""
+        req = self._tile_region(self.global_step)
+        grow = None
+        stream = self._stream()
+        for It in need:
+            slot = It % self.nslots
+            miss = req & ~self._slot_tiles[slot]
+            if not miss.any():
+                continue
+            if grow is None:
+                grow = self._tile_region(self.global_step + self.tile_margin_steps) | req
+            boxes, covered = tile_boxes(grow & ~self._slot_tiles[slot], self._tile, self.NY, self.NX, self.tile_gap)
+            hosts = self._slot_host[slot][0]
+            nbytes = 0
+            for dst, name in ((self._u, "u"), (self._v, "v"), (self._w, "w")):
+                nbytes += self._copy_boxes(dst[slot], hosts[name], name, boxes, stream)
+            self._slot_tiles[slot] |= covered
+            self.tile_topups += 1
+            self.snapshot_bytes += nbytes
+            self._repack_rows(slot, boxes)
+
+    def _repack_rows(self, slot: int, boxes) -> None:
+        """Pack again the rows of `boxes` (and the row above each, whose y + 1
+        neighbours they are) in the pair-ring windows packed from this slot's
+        current contents."""
+        if self.layout == _lib.OPX_LAYOUT_RING or self._pair_bufs is None or not len(boxes):
+            return
+        rows = sorted((max(0, int(y0) - 1), int(y1)) for y0, y1 in boxes[:, :2])
+        merged = []
+        for a, b in rows:
+            if merged and a <= merged[-1][1]:
+                merged[-1][1] = max(merged[-1][1], b)
+            else:
+                merged.append([a, b])
+        w = self.window
+        for s0 in {(slot - 1) % self.nslots, slot}:
+            key = self._pair_contents(s0)
+            if self._pair_key[s0] != key or None in key:
+                continue  # packed in full when next read
+            bufs = [b.data_ptr() + s0 * st for b, st in zip(self._pair_bufs, self._pair_strides)]
+            for a, b in merged:
+                _lib.call("opx_pack_window_rows", ctypes.byref(w), ctypes.byref(self.grid), self.layout, s0,
+                          (s0 + 1) % self.nslots, a, b, *bufs, self._stream())
 
     def _stage(self, name: str, src: int):
         """Start reading snapshot `src` of a lazily read field into pinned staging
@@ -667,6 +974,8 @@ class AB3Engine:
                 for ev in evs:
                     compute.wait_event(ev)
                 self._ring_ready[slot] = None
+        if self._tile is not None:
+            self._cover(need)
         nxt = need[-1] + 1
         slot = nxt % self.nslots
         cur = self._ring_It[slot]
@@ -808,6 +1117,8 @@ class AB3Engine:
                         e1 = torch.cuda.Event(enable_timing=True)
                         e1.record()
                         self.timer.append((e0, e1, m))
+            if self._tile is not None:
+                self._request_occupancy()
             self._since_rebin += k
             left -= k
 

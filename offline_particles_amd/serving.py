@@ -117,13 +117,13 @@ class ReleaseServer:
                  stream_slots: int | None = 3, hbm_fraction: float = 0.8, max_particles: int | None = None,
                  max_releases: int = 1024, rebin_every: int = 40, xlow: int = 4, graph_steps: int = 8,
                  layout: int | None = None, block: int = 128, start_time: float | None = None,
-                 max_inflight_chunks: int = 3, launch_steps: int = 1):
+                 max_inflight_chunks: int = 3, launch_steps: int = 1, stream_tiles: int | None = None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         # the window (rings and packed stencil windows) is allocated first, then
         # the particle buffers from what is left
         self.engine = AB3Engine(fieldset, time_array, dt, storage_dtype=storage_dtype, device=self.device,
                                 capacity=1, rebin_every=rebin_every, xlow=xlow, max_releases=max_releases,
-                                stream_slots=stream_slots)
+                                stream_slots=stream_slots, stream_tiles=stream_tiles)
         eng = self.engine
         if layout is not None:
             eng.set_layout(layout)
@@ -262,6 +262,10 @@ class ReleaseServer:
             "compactions": compactions,
             "snapshot_loads": getattr(eng, "snapshot_loads", 0),
             "snapshot_h2d_GBs": getattr(eng, "snapshot_bytes", 0) / elapsed / 1e9 if elapsed > 0 else 0.0,
+            "snapshot_bytes": int(getattr(eng, "snapshot_bytes", 0)),
+            "snapshot_full_bytes": int(getattr(eng, "snapshot_full_bytes", 0)),
+            "stream_tiles": eng._tile,
+            "tile_topups": int(getattr(eng, "tile_topups", 0)),
         }
         return done
 
