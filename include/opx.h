@@ -65,7 +65,18 @@ enum opx_error {
  * shape[0..ndim-1] >= 2 each; `offsets` are the stagger offsets that map
  * rho-centred particle indices to array indices (spatial_arrays.py:29-40).
  * ft/gt carry the storage type's rounded weights (FP32 values for F32/BF16).
+ *
+ * `packed` (optional, NULL for none) holds the same values arranged so one
+ * load brings a node's stencil corners in (y, x), built by opx_pack_field
+ * from prev (and next): OPX_PACK_QUAD for a static field, the corners
+ * (y, x), (y, x+1), (y+1, x), (y+1, x+1) of every node; OPX_PACK_OCT for a
+ * time-dependent field, those corners of prev then of next.  F32 and BF16
+ * fields of 2 or 3 dimensions only.  The interpolants read the corners from
+ * it and blend them with the same arithmetic, so results do not change; a
+ * stencil then touches 2 cache lines of a 3-D field instead of 8.
  */
+enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2 };
+
 typedef struct opx_field {
     const void *prev;
     const void *next;
@@ -75,6 +86,9 @@ typedef struct opx_field {
     double offsets[3];
     double ft;
     double gt;
+    const void *packed;
+    int32_t packed_kind; /* opx_pack_kind */
+    int32_t reserved;
 } opx_field;
 
 /* The eight simulation fields the ROMS w-advection kernels declare
@@ -290,6 +304,17 @@ int opx_packed_bytes(const opx_grid *g, int32_t dtype3, int32_t layout, uint64_t
  * and p2. */
 int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1, void *pu,
                     void *pv, void *pw, void *p2, void *stream);
+/* Packed stencil copies of a drop-in field (opx_field.packed): OPX_PACK_QUAD
+ * of `prev` when next is NULL, else OPX_PACK_OCT of (prev, next), for a
+ * C-contiguous (nz, ny, nx) F32 or BF16 array (nz = 1 for a 2-D field; ny,
+ * nx >= 2).  `out` is device memory of opx_packed_field_bytes bytes (QUAD:
+ * 4 values per node, OCT: 8).  No counterpart in the reference: its kernels
+ * read the blended FP64 array (fields.py:394-430); the values here are the
+ * same, stored so that each stencil row is not its own cache line. */
+int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, int64_t nx, uint64_t *bytes);
+int opx_pack_field(const void *prev, const void *next, int32_t dtype, int64_t nz, int64_t ny, int64_t nx, void *out,
+                   void *stream);
+
 /* opx_pack_window for the rho rows [y0, y1) only (each field's rows in that
  * range, every z-level and column; 0 <= y0 < y1 <= NY): refreshes the part of
  * a packed window whose slots received new data there. */

@@ -36,6 +36,9 @@ class OpxInvalidArgument(OpxError, ValueError):
 _vp = ctypes.c_void_p
 
 
+OPX_PACK_NONE, OPX_PACK_QUAD, OPX_PACK_OCT = 0, 1, 2
+
+
 class opx_field(ctypes.Structure):
     _fields_ = [
         ("prev", _vp), ("next", _vp),
@@ -43,6 +46,7 @@ class opx_field(ctypes.Structure):
         ("shape", ctypes.c_int64 * 3),
         ("offsets", ctypes.c_double * 3),
         ("ft", ctypes.c_double), ("gt", ctypes.c_double),
+        ("packed", _vp), ("packed_kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -180,6 +184,9 @@ _SIGNATURES.update({
                          _P(ctypes.c_uint64), _P(ctypes.c_uint64)],
     "opx_pack_window": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp,
                         _vp, _vp],
+    "opx_packed_field_bytes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                               _P(ctypes.c_uint64)],
+    "opx_pack_field": [_vp, _vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp],
     "opx_pack_window_rows": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp],
     "opx_tile_occupancy": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,

@@ -1772,6 +1772,29 @@ __global__ void k_pack2(const float4 *__restrict__ hdd, const float *__restrict_
     out[i] = make_float4(m.x, zp[i], zq[i], m.y);
 }
 
+// The QUAD copy of a static field for the drop-in interpolants: node i holds
+// its corners (y, x), (y, x+1), (y+1, x), (y+1, x+1); neighbours beyond the
+// last row / column repeat the node (never corners of a base, which the
+// interpolants clamp to size - 2).
+template <typename T>
+__global__ void k_pack_quad(const T *__restrict__ p, void *__restrict__ out, uint32_t n, int n1, int n2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t x1 = (i % (uint32_t)n2) + 1 < (uint32_t)n2 ? 1u : 0u;
+    const uint32_t y1 = ((i / (uint32_t)n2) % (uint32_t)n1) + 1 < (uint32_t)n1 ? (uint32_t)n2 : 0u;
+    const uint32_t c[4] = {i, i + x1, i + y1, i + y1 + x1};
+    if constexpr (sizeof(T) == 4) {
+        const float *P = (const float *)p;
+        static_cast<float4 *>(out)[i] = make_float4(P[c[0]], P[c[1]], P[c[2]], P[c[3]]);
+    } else {
+        const uint16_t *P = (const uint16_t *)p;
+        static_cast<uint2 *>(out)[i] =
+            make_uint2((uint32_t)P[c[0]] | ((uint32_t)P[c[1]] << 16), (uint32_t)P[c[2]] | ((uint32_t)P[c[3]] << 16));
+    }
+}
+
 // ---------------------------------------------------------------- sparse snapshot streaming
 
 __global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
@@ -2868,6 +2891,48 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
     int rc;
     if ((rc = check_grid(g))) return rc;
     return opx_pack_window_rows(win, g, layout, s0, s1, 0, g->NY, pu, pv, pw, p2, stream);
+}
+
+int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, int64_t nx, uint64_t *bytes)
+{
+    OPX_REQUIRE(bytes != nullptr, "null output");
+    OPX_REQUIRE(dtype == OPX_F32 || dtype == OPX_BF16, "packed stencils need F32 or BF16, not dtype %d", dtype);
+    OPX_REQUIRE(kind == OPX_PACK_QUAD || kind == OPX_PACK_OCT, "pack kind %d", kind);
+    OPX_REQUIRE(nz >= 1 && ny >= 2 && nx >= 2, "array shape (%lld, %lld, %lld)", (long long)nz, (long long)ny,
+                (long long)nx);
+    *bytes = (uint64_t)nz * ny * nx * (kind == OPX_PACK_QUAD ? 4 : 8) * (dtype == OPX_F32 ? 4 : 2);
+    return OPX_OK;
+}
+
+int opx_pack_field(const void *prev, const void *next, int32_t dtype, int64_t nz, int64_t ny, int64_t nx, void *out,
+                   void *stream)
+{
+    int rc;
+    uint64_t bytes;
+    if ((rc = opx_packed_field_bytes(dtype, next ? OPX_PACK_OCT : OPX_PACK_QUAD, nz, ny, nx, &bytes))) return rc;
+    OPX_REQUIRE(prev != nullptr && out != nullptr, "null array");
+    const int64_t n = nz * ny * nx;
+    OPX_REQUIRE(n < (int64_t(1) << 32), "field of %lld nodes", (long long)n);
+    hipStream_t st = as_stream(stream);
+    const unsigned grid = grid_for(n);
+    if (next) {
+        if (dtype == OPX_F32)
+            hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_OCT>), dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const float *>(prev), static_cast<const float *>(next), out, (uint32_t)n,
+                               (int)ny, (int)nx, 0, (int)ny);
+        else
+            hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_OCT>), dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const uint16_t *>(prev), static_cast<const uint16_t *>(next), out,
+                               (uint32_t)n, (int)ny, (int)nx, 0, (int)ny);
+    } else {
+        if (dtype == OPX_F32)
+            hipLaunchKernelGGL(k_pack_quad<float>, dim3(grid), dim3(kBlock), 0, st, static_cast<const float *>(prev),
+                               out, (uint32_t)n, (int)ny, (int)nx);
+        else
+            hipLaunchKernelGGL(k_pack_quad<uint16_t>, dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const uint16_t *>(prev), out, (uint32_t)n, (int)ny, (int)nx);
+    }
+    return check_launch("opx_pack_field");
 }
 
 int opx_tile_occupancy(const opx_split_state *s, const opx_grid *g, int32_t tile, uint8_t *map, int32_t nty,

@@ -120,6 +120,54 @@ __device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, doubl
     }
 }
 
+// The stencil corners (y, x), (y, x+1), (y+1, x), (y+1, x+1) of `node` from a
+// field's packed copy (opx_field.packed), blended and widened exactly as
+// load_blend does: one 16-byte (QUAD F32, OCT BF16), 32-byte (OCT F32) or
+// 8-byte (QUAD BF16) load instead of two rows per snapshot.
+__device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node, double v[4])
+{
+    if (f.dtype == OPX_F32) {
+        const float4 *e = static_cast<const float4 *>(f.packed);
+        if (f.packed_kind == OPX_PACK_QUAD) {
+            const float4 p = e[node];
+            v[0] = (double)p.x;
+            v[1] = (double)p.y;
+            v[2] = (double)p.z;
+            v[3] = (double)p.w;
+        } else {
+            const float4 p = e[2 * node], q = e[2 * node + 1];
+            const float gt = (float)f.gt, ft = (float)f.ft;
+            float a0 = p.x * gt, b0 = q.x * ft;
+            float a1 = p.y * gt, b1 = q.y * ft;
+            float a2 = p.z * gt, b2 = q.z * ft;
+            float a3 = p.w * gt, b3 = q.w * ft;
+            v[0] = (double)(a0 + b0);
+            v[1] = (double)(a1 + b1);
+            v[2] = (double)(a2 + b2);
+            v[3] = (double)(a3 + b3);
+        }
+    } else {
+        if (f.packed_kind == OPX_PACK_QUAD) {
+            const uint2 p = static_cast<const uint2 *>(f.packed)[node];
+            v[0] = (double)bf16_bits_to_f32((uint16_t)(p.x & 0xffffu));
+            v[1] = (double)bf16_bits_to_f32((uint16_t)(p.x >> 16));
+            v[2] = (double)bf16_bits_to_f32((uint16_t)(p.y & 0xffffu));
+            v[3] = (double)bf16_bits_to_f32((uint16_t)(p.y >> 16));
+        } else {
+            const uint4 e = static_cast<const uint4 *>(f.packed)[node];
+            const uint32_t P[2] = {e.x, e.y}, Q[2] = {e.z, e.w};
+            const float gt = (float)f.gt, ft = (float)f.ft;
+            for (int c = 0; c < 4; ++c) {
+                const uint16_t pb = (uint16_t)(c & 1 ? P[c >> 1] >> 16 : P[c >> 1] & 0xffffu);
+                const uint16_t qb = (uint16_t)(c & 1 ? Q[c >> 1] >> 16 : Q[c >> 1] & 0xffffu);
+                float a = bf16_bits_to_f32(pb) * gt;
+                float b = bf16_bits_to_f32(qb) * ft;
+                v[c] = (double)(a + b);
+            }
+        }
+    }
+}
+
 // linear_interpolation (linear.pxd:11-17) on a 1-D field at rho index idx.
 __device__ __forceinline__ double linear_f64(const opx_field &f, double idx)
 {
@@ -146,8 +194,17 @@ __device__ __forceinline__ double bilinear_f64(const opx_field &f, double y, dou
     double g1 = 1.0 - f1;
     int64_t b = I0 * n1 + I1;
     double v00, v01, v10, v11;
-    load_blend2(f, b, v00, v01);
-    load_blend2(f, b + n1, v10, v11);
+    if (f.packed) {
+        double c[4];
+        packed_corners(f, b, c);
+        v00 = c[0];
+        v01 = c[1];
+        v10 = c[2];
+        v11 = c[3];
+    } else {
+        load_blend2(f, b, v00, v01);
+        load_blend2(f, b + n1, v10, v11);
+    }
     return g0 * g1 * v00 + g0 * f1 * v01 + f0 * g1 * v10 + f0 * f1 * v11;
 }
 
@@ -172,10 +229,24 @@ __device__ __forceinline__ double trilinear_f64(const opx_field &f, double z, do
     int64_t b10 = b00 + n1 * n2;
     int64_t b11 = b10 + n2;
     double v000, v001, v010, v011, v100, v101, v110, v111;
-    load_blend2(f, b00, v000, v001);
-    load_blend2(f, b01, v010, v011);
-    load_blend2(f, b10, v100, v101);
-    load_blend2(f, b11, v110, v111);
+    if (f.packed) {
+        double c0[4], c1[4];
+        packed_corners(f, b00, c0);
+        packed_corners(f, b10, c1);
+        v000 = c0[0];
+        v001 = c0[1];
+        v010 = c0[2];
+        v011 = c0[3];
+        v100 = c1[0];
+        v101 = c1[1];
+        v110 = c1[2];
+        v111 = c1[3];
+    } else {
+        load_blend2(f, b00, v000, v001);
+        load_blend2(f, b01, v010, v011);
+        load_blend2(f, b10, v100, v101);
+        load_blend2(f, b11, v110, v111);
+    }
     return (g0 * g1 * g2 * v000 +
             g0 * g1 * f2 * v001 +
             g0 * f1 * g2 * v010 +

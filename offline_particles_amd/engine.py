@@ -66,6 +66,7 @@ import torch
 
 from . import _lib
 from .fields import StaticField, TimeDependentField
+from .fields import unpacked as fields_unpacked
 from .kernels.status import ParticleStatus
 from .snapshots import SnapshotSource
 from .spatial_arrays import Stagger
@@ -1280,8 +1281,11 @@ class AB3Engine:
                 raise ValueError("Time is out of bounds of the time array.")
             idx = np.searchsorted(ta, t, side="right") - 1
             tidx = idx + (t - ta[idx]) / (ta[idx + 1] - ta[idx])
-            for k, name in enumerate(names):
-                rows[r * len(names) + k] = field_struct(fieldset[name].get_field_data(tidx, _NO_BBOX), name)
+            # the particles are sorted by cell: the plain layouts serve them better
+            # than the drop-in's packed copies
+            with fields_unpacked():
+                for k, name in enumerate(names):
+                    rows[r * len(names) + k] = field_struct(fieldset[name].get_field_data(tidx, _NO_BBOX), name)
         index = {name: k for k, name in enumerate(names)}
         kinds = {1: _lib.OPX_SAMPLE_LINEAR, 2: _lib.OPX_SAMPLE_BILINEAR, 3: _lib.OPX_SAMPLE_TRILINEAR}
         outs, samples = [], []

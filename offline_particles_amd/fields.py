@@ -26,7 +26,63 @@ from ._lib import OPX_BF16, OPX_F32, OPX_F64, storage_code
 from .snapshots import SnapshotSource, ZarrSnapshots
 from .spatial_arrays import BBox, DeviceArray, SpatialArray, Stagger, as_device_tensor, default_device, to_torch_dtype
 
-FieldData = collections.namedtuple("FieldData", ["array", "offsets"])
+_FieldDataTuple = collections.namedtuple("FieldData", ["array", "offsets"])
+
+
+class FieldData(_FieldDataTuple):
+    """The reference's FieldData (array, offsets) (fields.py:12).
+
+    ``packed`` (an attribute, not a tuple item, so ``array, offsets = fd``
+    still works) is None or ``(tensor, kind)``: a copy of the same values in
+    which one load brings a node's stencil corners in (y, x)
+    (``opx_pack_field``; OPX_PACK_QUAD for a static field, OPX_PACK_OCT for a
+    snapshot pair).  The kernels read it when present; the values they
+    interpolate are unchanged."""
+
+    packed = None
+
+
+# Packed stencil copies for the drop-in kernels (F32 and BF16 device fields of
+# 2 or 3 dimensions): a 3-D stencil then touches 2 cache lines per field
+# instead of 8.  They cost 4 (static) or 8 (snapshot pair) values per node of
+# device memory; False turns them off.
+PACK_STENCILS = True
+
+
+class unpacked:
+    """Context in which get_field_data makes no packed copies (for kernels whose
+    particles are sorted by cell, where neighbouring threads already share cache
+    lines and the copies only add footprint)."""
+
+    def __enter__(self):
+        global PACK_STENCILS
+        self._saved, PACK_STENCILS = PACK_STENCILS, False
+        return self
+
+    def __exit__(self, *exc):
+        global PACK_STENCILS
+        PACK_STENCILS = self._saved
+        return False
+
+
+def _packable(t) -> bool:
+    return (PACK_STENCILS and isinstance(t, torch.Tensor) and t.is_cuda and t.dim() in (2, 3)
+            and t.dtype in (torch.float32, torch.bfloat16) and t.is_contiguous() and min(t.shape[-2:]) >= 2)
+
+
+def _pack(prev: torch.Tensor, next_: torch.Tensor | None) -> tuple:
+    """(packed tensor, kind) of prev (QUAD) or of the pair (prev, next_) (OCT)."""
+    from . import _lib
+
+    dims = (1,) * (3 - prev.dim()) + tuple(prev.shape)
+    kind = _lib.OPX_PACK_QUAD if next_ is None else _lib.OPX_PACK_OCT
+    code = storage_code(prev.dtype)
+    nbytes = _lib.ctypes.c_uint64()
+    _lib.call("opx_packed_field_bytes", code, kind, *dims, _lib.ctypes.byref(nbytes))
+    out = torch.empty(int(nbytes.value), dtype=torch.uint8, device=prev.device)
+    _lib.call("opx_pack_field", prev.data_ptr(), None if next_ is None else next_.data_ptr(), code, *dims,
+              out.data_ptr(), torch.cuda.current_stream(prev.device).cuda_stream)
+    return out, kind
 
 
 class BlendWindow:
@@ -161,7 +217,14 @@ class StaticField(Field):
 
     def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
         array, offsets = self._data.get_data_subset(bbox)
-        return FieldData(array, offsets)
+        fd = FieldData(array, offsets)
+        if _packable(array):
+            key = (array.data_ptr(), tuple(array.shape), array.dtype)
+            cached = getattr(self, "_packed", None)
+            if cached is None or cached[0] != key:
+                cached = self._packed = (key, _pack(array, None))
+            fd.packed = cached[1]
+        return fd
 
     @classmethod
     def from_numpy(cls, data, z_stagger, y_stagger, x_stagger, *, device=None) -> "StaticField":
@@ -307,11 +370,31 @@ class TimeDependentField(Field):
             raise IndexError(f"Valid range of time indices is 0,...,{self._num_timesteps - 2}, got {It}.")
         self._It = It
 
+    # packed snapshot pairs kept on the device (an RK2 step near a snapshot
+    # boundary reads two pairs)
+    PACKED_PAIRS = 2
+
     def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
         It, ft, gt = blend_weights(time_index, self.dtype)
         self.set_time_index(It)
-        window = BlendWindow(self.snapshot(It), self.snapshot(It + 1), ft, gt, It)
-        return FieldData(window, self.active_offsets)
+        prev, nxt = self.snapshot(It), self.snapshot(It + 1)
+        fd = FieldData(BlendWindow(prev, nxt, ft, gt, It), self.active_offsets)
+        if _packable(prev):
+            # the snapshot indices name the contents (a lazily read field may hand
+            # out a freed snapshot's address for another one), the pointers the storage
+            K = self._num_timesteps
+            idx = (It % K, (It + 1) % K) if self._cyclic else (It, It + 1)
+            key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype)
+            cache = self.__dict__.setdefault("_packed_pairs", collections.OrderedDict())
+            packed = cache.get(key)
+            if packed is None:
+                packed = cache[key] = _pack(prev, nxt)
+                while len(cache) > self.PACKED_PAIRS:
+                    cache.popitem(last=False)
+            else:
+                cache.move_to_end(key)
+            fd.packed = packed
+        return fd
 
     @classmethod
     def from_numpy(cls, data, z_stagger, y_stagger, x_stagger, **kw) -> "TimeDependentField":

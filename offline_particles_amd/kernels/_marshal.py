@@ -92,6 +92,10 @@ def field_struct(fd, name: str) -> _lib.opx_field:
     for d in range(t.dim()):
         f.shape[d] = int(t.shape[d])
         f.offsets[d] = float(offsets[d])
+    packed = getattr(fd, "packed", None)
+    if packed is not None:
+        f.packed = _device_data(packed[0], name)
+        f.packed_kind = int(packed[1])
     return f
 
 

@@ -1,0 +1,81 @@
+"""Packed stencil copies of the drop-in's fields (opx_field.packed).
+
+Gate: the drop-in gives the same bits with the packed copies as without
+(they hold the same values; the interpolants blend them with the same
+arithmetic), for AB3, RK2 and AB3 with an RK2 start-up, FP32 and BF16
+storage, spatially ordered or not; and opx_pack_field's layout equals the
+corners numpy picks.  The parity tests against the oracle run with the
+packed copies on (the default).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from offline_particles_amd import _lib, fields
+from offline_particles_amd.workloads import build_fieldset, build_simulation, turbulent_workload
+
+from ._helpers import gpu_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _wl():
+    # hourly snapshots, dt = 120 s: 60 steps cross two snapshot boundaries
+    return turbulent_workload(NX=96, NY=80, NZ=12, nparticles=20_000, steps=80, nsnap_unique=4)
+
+
+@pytest.mark.parametrize("order", [0, 9])
+@pytest.mark.parametrize("storage", [None, torch.bfloat16])
+@pytest.mark.parametrize("scheme", ["ab3", "rk2", "ab3_rk2"])
+def test_packed_stencils_change_nothing(device, scheme, storage, order):
+    wl = _wl()
+    res = {}
+    try:
+        for pack in (False, True):
+            fields.PACK_STENCILS = pack
+            sim = build_simulation(wl, scheme, device=device, storage_dtype=storage, spatial_order_every=order)
+            for _ in range(60):
+                sim.step()
+            torch.cuda.synchronize()
+            res[pack] = gpu_state(sim)
+            if pack:
+                fd = sim.fieldset["u"].get_field_data(0.25, None)
+                assert fd.packed is not None and fd.packed[1] == _lib.OPX_PACK_OCT
+                assert sim.fieldset["h"].get_field_data(0.25, None).packed[1] == _lib.OPX_PACK_QUAD
+    finally:
+        fields.PACK_STENCILS = True
+    for k, a in res[False].items():
+        assert np.array_equal(a, res[True][k], equal_nan=True), (scheme, storage, k)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 6, 11)])
+@pytest.mark.parametrize("pair", [False, True])
+def test_pack_field_matches_numpy(device, dtype, shape, pair):
+    rng = np.random.default_rng(3)
+    nz, ny, nx = shape
+    p = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    q = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    kind = _lib.OPX_PACK_OCT if pair else _lib.OPX_PACK_QUAD
+    code = _lib.storage_code(dtype)
+    nbytes = ctypes.c_uint64()
+    _lib.call("opx_packed_field_bytes", code, kind, nz, ny, nx, ctypes.byref(nbytes))
+    per = (8 if pair else 4) * p.element_size()
+    assert nbytes.value == nz * ny * nx * per
+    out = torch.zeros(nbytes.value, dtype=torch.uint8, device=device)
+    _lib.call("opx_pack_field", p.data_ptr(), q.data_ptr() if pair else None, code, nz, ny, nx, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    raw = {2: np.uint16, 4: np.uint32}[p.element_size()]
+    P = p.cpu().view(torch.int16 if dtype == torch.bfloat16 else torch.int32).numpy().view(raw)
+    Q = q.cpu().view(torch.int16 if dtype == torch.bfloat16 else torch.int32).numpy().view(raw)
+    got = out.cpu().numpy().view(raw).reshape(nz, ny, nx, -1)
+    # corners (y, x), (y, x+1), (y+1, x), (y+1, x+1); the last row / column repeats the node
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    y1, x1 = np.minimum(yy + 1, ny - 1), np.minimum(xx + 1, nx - 1)
+    corners = [(yy, xx), (yy, x1), (y1, xx), (y1, x1)]
+    # (two BF16 values per 32-bit word, the first in the low half: the same order as uint16)
+    want = np.stack([A[:, a, b] for A in ((P, Q) if pair else (P,)) for a, b in corners], axis=-1)
+    assert np.array_equal(got, want)
