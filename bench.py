@@ -27,7 +27,7 @@ Forms measured in the same run:
 - serving (``--serve SECONDS``, default 5): Poisson arrivals at 20/s for that
   many seconds, sizes 1k-2M, 1-10 simulated days, on a 2048 x 1536 x 60 grid
   (FP32 and BF16 storage, the same requests) streamed through a 3-snapshot
-  window of which each slot receives only the 32 x 32-column tiles the
+  window of which each slot receives only the 16 x 16-column tiles the
   particles can reach (``--serve-tiles``), admission by free HBM; every
   request runs to completion; p50/p99 latency under ``serving``.
 - isopycnal (``--extras iso``): the isopycnal-following AB3 on the same
@@ -642,7 +642,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
 
 
 def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_size=2_000_000, graph_steps=8,
-                tune=True, max_particles=256_000_000, storage=None, wl=None, reqs=None, tiles=32, tile_copy="auto"):
+                tune=True, max_particles=256_000_000, storage=None, wl=None, reqs=None, tiles=16, tile_copy="auto"):
     """Serving form: Poisson arrivals (``rate``/s for ``seconds`` s), sizes log-uniform on
     [1k, max_size], 1-10 simulated days (720-7200 steps of 120 s), BF16 fields streamed
     through a 3-slot window, admission by free HBM; every request runs to completion.
@@ -1246,7 +1246,7 @@ def main():
     ap.add_argument("--batched-max-size", type=int, default=1_000_000)
     ap.add_argument("--mix-storage", default="fp32,bf16",
                     help="field storage of the batching and serving forms (comma-separated: fp32, bf16)")
-    ap.add_argument("--serve-tiles", type=int, default=32, metavar="CELLS",
+    ap.add_argument("--serve-tiles", type=int, default=16, metavar="CELLS",
                     help="serving form: stream only the tiles (CELLS rho columns square) of each snapshot that "
                          "the particles can reach (0: whole snapshots)")
     ap.add_argument("--serve", type=float, default=5.0, metavar="SECONDS",

@@ -1821,30 +1821,39 @@ __global__ void k_tile_occupancy(opx_split_state s, opx_grid g, int tile, uint8_
     map[(yy / tile) * ntx + xx / tile] = 1;
 }
 
-// Boxes of rows copied from mapped host memory: up to kGatherBoxes boxes per
-// launch, passed by value; block b copies one row (box, z-level, y) of its box,
-// found from the prefix counts of rows.
-constexpr int kGatherBoxes = 64;
+// Boxes copied from mapped host memory: up to kGatherBoxes boxes per launch,
+// passed by value with the prefix counts of their elements; thread e of the
+// grid-stride loop copies element e of the launch (its box by binary search,
+// then x fastest, so the lanes of a wave read consecutive addresses of a row).
+constexpr int kGatherBoxes = 128;
 struct GatherBoxes {
     int32_t box[kGatherBoxes][4]; // (y0, y1, x0, x1)
-    int64_t first_row[kGatherBoxes + 1];
+    int64_t first[kGatherBoxes + 1];
     int32_t n;
 };
 
 template <typename E>
-__global__ void k_gather_rows(E *__restrict__ dst, const E *__restrict__ src, int64_t nz, int64_t ny, int64_t nx,
-                              GatherBoxes B)
+__global__ void k_gather_boxes(E *__restrict__ dst, const E *__restrict__ src, int64_t ny, int64_t nx, GatherBoxes B)
 {
-    const int64_t row = blockIdx.x;
-    int b = 0;
-    while (b + 1 < B.n && B.first_row[b + 1] <= row)
-        ++b;
-    const int32_t *x = B.box[b];
-    const int64_t k = row - B.first_row[b], h = x[1] - x[0];
-    const int64_t z = k / h, y = x[0] + (k - z * h);
-    const int64_t base = (z * ny + y) * nx;
-    for (int64_t c = x[2] + threadIdx.x; c < x[3]; c += blockDim.x)
-        dst[base + c] = src[base + c];
+    const int64_t total = B.first[B.n];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = B.n - 1; // the box b with first[b] <= e < first[b + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (B.first[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int32_t *x = B.box[lo];
+        const uint32_t w = (uint32_t)(x[3] - x[2]), h = (uint32_t)(x[1] - x[0]);
+        uint32_t k = (uint32_t)(e - B.first[lo]); // below 2^31: one box of at most (nz, ny, nx) < 2^32 elements
+        const uint32_t xx = k % w;
+        k /= w;
+        const uint32_t yy = k % h, z = k / h;
+        const int64_t idx = ((int64_t)z * ny + x[0] + yy) * nx + x[2] + xx;
+        dst[idx] = src[idx];
+    }
 }
 
 // ---------------------------------------------------------------- conversions
@@ -2988,30 +2997,30 @@ int opx_gather_boxes_h2d(void *dst, const void *src, int64_t nz, int64_t ny, int
         set_error("opx_gather_boxes_h2d: the source is not mapped pinned host memory");
         return OPX_ERR_INVALID;
     }
+    OPX_REQUIRE(nz * ny * nx < (int64_t(1) << 31), "array of %lld elements", (long long)(nz * ny * nx));
     hipStream_t st = as_stream(stream);
     for (int32_t b0 = 0; b0 < nboxes; b0 += kGatherBoxes) {
         GatherBoxes B = {};
         B.n = min(kGatherBoxes, nboxes - b0);
-        int64_t rows = 0;
+        int64_t total = 0;
         for (int i = 0; i < B.n; ++i) {
             const int32_t *x = boxes + 4 * (b0 + i);
             for (int j = 0; j < 4; ++j)
                 B.box[i][j] = x[j];
-            B.first_row[i] = rows;
-            rows += nz * (int64_t)(x[1] - x[0]);
+            B.first[i] = total;
+            total += nz * (int64_t)(x[1] - x[0]) * (x[3] - x[2]);
         }
-        B.first_row[B.n] = rows;
-        OPX_REQUIRE(rows < (int64_t(1) << 31), "%lld rows in one launch", (long long)rows);
-        const dim3 grid((unsigned)rows), block(256);
+        B.first[B.n] = total;
+        const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
         switch (elem_bytes) {
-        case 1: hipLaunchKernelGGL(k_gather_rows<uint8_t>, grid, block, 0, st, static_cast<uint8_t *>(dst),
-                                   static_cast<const uint8_t *>(d0), nz, ny, nx, B); break;
-        case 2: hipLaunchKernelGGL(k_gather_rows<uint16_t>, grid, block, 0, st, static_cast<uint16_t *>(dst),
-                                   static_cast<const uint16_t *>(d0), nz, ny, nx, B); break;
-        case 4: hipLaunchKernelGGL(k_gather_rows<uint32_t>, grid, block, 0, st, static_cast<uint32_t *>(dst),
-                                   static_cast<const uint32_t *>(d0), nz, ny, nx, B); break;
-        default: hipLaunchKernelGGL(k_gather_rows<uint64_t>, grid, block, 0, st, static_cast<uint64_t *>(dst),
-                                    static_cast<const uint64_t *>(d0), nz, ny, nx, B); break;
+        case 1: hipLaunchKernelGGL(k_gather_boxes<uint8_t>, dim3(grid), dim3(256), 0, st, static_cast<uint8_t *>(dst),
+                                   static_cast<const uint8_t *>(d0), ny, nx, B); break;
+        case 2: hipLaunchKernelGGL(k_gather_boxes<uint16_t>, dim3(grid), dim3(256), 0, st,
+                                   static_cast<uint16_t *>(dst), static_cast<const uint16_t *>(d0), ny, nx, B); break;
+        case 4: hipLaunchKernelGGL(k_gather_boxes<uint32_t>, dim3(grid), dim3(256), 0, st,
+                                   static_cast<uint32_t *>(dst), static_cast<const uint32_t *>(d0), ny, nx, B); break;
+        default: hipLaunchKernelGGL(k_gather_boxes<uint64_t>, dim3(grid), dim3(256), 0, st,
+                                    static_cast<uint64_t *>(dst), static_cast<const uint64_t *>(d0), ny, nx, B); break;
         }
         if ((rc = check_launch("opx_gather_boxes_h2d"))) return rc;
     }

@@ -760,9 +760,9 @@ class AB3Engine:
         self._occ_adds: list = []  # (seq, bool map) of releases added after it
         self._occ_seq = 0
         # steps of movement a load or top-up covers beyond the run of steps at hand
-        self.tile_margin_steps = 16
+        self.tile_margin_steps = 8
         # unset tiles bridged between two runs of a tile row (longer copies)
-        self.tile_gap = 1
+        self.tile_gap = 2
         # wait for a newer map rather than widen the region past this many steps
         self.max_occupancy_lag = 64
         self.tile_topups = 0
@@ -770,7 +770,8 @@ class AB3Engine:
         self._gather_ok = True
         # how tiles reach a slot: "dma" (rectangular DMA copies), "gather" (a
         # kernel reading the pinned snapshot through its device mapping) or
-        # "auto" (gather for 1- and 2-byte fields, DMA otherwise)
+        # "auto" (gather for 1- and 2-byte fields and for tiles narrower than
+        # 32 cells, whose rows DMA copies move slowly; DMA otherwise)
         self.tile_copy = "auto"
 
     def _snapshot_rate(self, src: int, hosts: dict) -> float:
@@ -857,7 +858,8 @@ class AB3Engine:
         if not (host.is_contiguous() and dst.is_contiguous()):
             raise ValueError("tile copies need contiguous snapshots")
         args = (dst.data_ptr(), host.data_ptr(), nz, ny, nx, host.element_size(), b.ctypes.data, len(b), stream)
-        gather = self.tile_copy == "gather" or (self.tile_copy == "auto" and host.element_size() < 4)
+        gather = self.tile_copy == "gather" or (self.tile_copy == "auto"
+                                                 and (host.element_size() < 4 or self._tile < 32))
         if gather and self._gather_ok:
             # rectangular DMA copies of 2-byte rows run far below the link rate:
             # a kernel reads the boxes through the pinned memory's device mapping

@@ -193,9 +193,9 @@ def test_copy_boxes_matches_numpy(device, dtype, fn):
     nz, ny, nx = 5, 37, 53
     host = torch.from_numpy(rng.standard_normal((nz, ny, nx)).astype(np.float32)).to(dtype).pin_memory()
     dev = torch.zeros((nz, ny, nx), dtype=dtype, device=device)
-    # 70 boxes: more than one launch of the gather kernel (64 boxes each)
+    # 200 boxes: more than one launch of the gather kernel (128 boxes each)
     boxes = [[0, 4, 0, 53], [10, 37, 5, 6], [20, 30, 40, 53], [36, 37, 0, 1]]
-    boxes += [[5 + i % 5, 6 + i % 5, i % 50, i % 50 + 3] for i in range(66)]
+    boxes += [[5 + i % 5, 6 + i % 5, i % 50, i % 50 + 3] for i in range(196)]
     boxes = np.array(boxes, dtype=np.int32)
     _lib.call(fn, dev.data_ptr(), host.data_ptr(), nz, ny, nx, host.element_size(), boxes.ctypes.data, len(boxes),
               torch.cuda.current_stream().cuda_stream)
