@@ -1398,13 +1398,18 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         if (!rs.active)
             continue;
         const double zi = p.zi, yi = p.yi, xi = p.xi;
-        if (!(__builtin_isfinite(zi) && __builtin_isfinite(yi) && __builtin_isfinite(xi))) {
-            p.st = OPX_NONFINITE;
-            break;
-        }
-        if (!(g.zidx_min <= zi && zi <= g.zidx_max && g.yidx_min <= yi && yi <= g.yidx_max && g.xidx_min <= xi &&
-              xi <= g.xidx_max)) {
-            p.st = OPX_OUT_OF_DOMAIN;
+        // a non-finite index fails the inclusive bounds too (comparisons with
+        // NaN are false, infinities lie outside), so the finiteness test that
+        // tells NONFINITE from OUT_OF_DOMAIN is needed only on that rare path;
+        // non-short-circuit & keeps the six compares as lane masks
+        const uint64_t in_mask =
+            __builtin_amdgcn_ballot_w64(g.zidx_min <= zi) & __builtin_amdgcn_ballot_w64(zi <= g.zidx_max) &
+            __builtin_amdgcn_ballot_w64(g.yidx_min <= yi) & __builtin_amdgcn_ballot_w64(yi <= g.yidx_max) &
+            __builtin_amdgcn_ballot_w64(g.xidx_min <= xi) & __builtin_amdgcn_ballot_w64(xi <= g.xidx_max);
+        const bool inb = (in_mask >> __lane_id()) & 1u;
+        if (!inb) {
+            const bool fin = __builtin_isfinite(zi) && __builtin_isfinite(yi) && __builtin_isfinite(xi);
+            p.st = fin ? OPX_OUT_OF_DOMAIN : OPX_NONFINITE;
             break;
         }
         bool ok;
