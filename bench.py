@@ -571,7 +571,7 @@ def run_fused(wl, device, steps, warmup, storage, rebin_every, xlow, tune=False,
     eng.step(warmup)
     if tune:
         # steps per launch from 1, 8, 16, 32 and 64 (1, 32 and 64 are the profiled launch sizes)
-        log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32, 64)))
+        log("autotune:", eng.autotune(launch_steps=(1, 8, 16, 32, 64, 128)))
     torch.cuda.synchronize()
     a0 = eng.active_count()
     z, y, x = eng.positions()
@@ -1253,7 +1253,7 @@ def main():
                     help="serving form: Poisson arrivals at 20/s for SECONDS on the 2048x1536x60 BF16 grid")
     ap.add_argument("--serve-grid", default="2048x1536x60", help="NXxNYxNZ of the serving form")
     ap.add_argument("--graph-steps", type=int, default=8, help="steps per captured HIP graph (0: no graphs)")
-    ap.add_argument("--launch-steps", type=int, default=1, help="fused steps per launch (1..64)")
+    ap.add_argument("--launch-steps", type=int, default=1, help="fused steps per launch (1..128)")
     ap.add_argument("--extras", default="iso,output",
                     help="also measure the isopycnal-following AB3 (iso), the output events (output) and, "
                          "on request, the Zarr snapshot ingest and trajectory writer on local disk (io) at the "

@@ -396,7 +396,7 @@ int opx_clock_advance(opx_release_clock *clocks, opx_release_step *table, int32_
                       const double *time_array, int32_t ntime, int32_t nslots, int32_t global_step,
                       int32_t *step_dev, double dt, double alpha, int32_t *err_flag, void *stream);
 
-/* opx_clock_advance over `nsteps` (1..64) consecutive global steps: step k's
+/* opx_clock_advance over `nsteps` (1..128) consecutive global steps: step k's
  * entries go to table[k * table_stride + r] (table_stride >= nrel), and the
  * device step counter advances by nsteps.  The same per-step arithmetic as
  * nsteps calls of opx_clock_advance. */
@@ -419,7 +419,7 @@ int opx_ab3_fused_dev(const opx_split_state *s, const opx_grid *g, const opx_win
                       int32_t layout, const opx_release_step *table, double dt, const int32_t *n_active, int32_t block,
                       void *stream);
 
-/* `nsteps` (1..64) fused AB3 steps in one launch, with step k's release
+/* `nsteps` (1..128) fused AB3 steps in one launch, with step k's release
  * weights at table[k * table_stride + release] (from opx_clock_advance_steps).
  * Each particle's state stays in registers between the steps, and the u, v, w
  * and 2-D stencil nodes it read are blended again at the next step's weights
@@ -440,7 +440,7 @@ int opx_ab3_fused_steps_dev(const opx_split_state *s, const opx_grid *g, const o
                             int32_t layout, const opx_release_step *table, int32_t table_stride, int32_t nsteps,
                             double dt, const int32_t *n_active, int32_t block, int32_t *err_flag, void *stream);
 
-/* `nsteps` (1..64) fused isopycnal-following AB3 steps in one launch: the
+/* `nsteps` (1..128) fused isopycnal-following AB3 steps in one launch: the
  * steps of opx_ab3_fused_steps with the vertical rate
  * w = w_max * drho / (rho_scale + |drho|), drho = rho(x_n, t_n) - rho_t[i]
  * (ab3_isopycnal_following.pyx:26-191, post step :193-242; launch order of
@@ -473,7 +473,7 @@ int opx_rk2_fused(const opx_split_state *s, const opx_grid *g, const opx_window 
                   const opx_release_step *table, double dt, double alpha, const int32_t *n_active, int32_t block,
                   void *stream);
 
-/* `nsteps` (1..64) fused RK2 steps in one launch, step k's release weights at
+/* `nsteps` (1..128) fused RK2 steps in one launch, step k's release weights at
  * table[k * table_stride + release] (from opx_clock_advance_steps with this
  * alpha); the state stays in registers and stage 2 and the update blend the
  * stencil nodes stage 1 read when they fall in the same cells and pair.  The

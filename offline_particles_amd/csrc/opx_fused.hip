@@ -658,7 +658,7 @@ struct IsoRate {
 };
 
 constexpr int kMaxSlots = 1023; // slot fields of opx_release_step are 10 bits
-constexpr int kMaxLaunchSteps = 64; // steps one fused launch may take
+constexpr int kMaxLaunchSteps = 128; // steps one fused launch may take
 constexpr uint64_t kMaxWindowBytes = uint64_t(1) << 32; // one packed window (32-bit byte offsets)
 
 __device__ __forceinline__ int next_slot(int s, int nslots) { return s + 1 == nslots ? 0 : s + 1; }

@@ -85,7 +85,7 @@ _ROMS_STAGGERS = {
 _STATE_F32 = ("fz", "fy", "fx", "hz1", "hy1", "hx1", "hz2", "hy2", "hx2")
 
 # steps one fused launch may take (the release table holds one row per step)
-MAX_LAUNCH_STEPS = 64
+MAX_LAUNCH_STEPS = 128
 
 LAYOUT_NAMES = {_lib.OPX_LAYOUT_RING: "ring", _lib.OPX_LAYOUT_PAIR: "pair", _lib.OPX_LAYOUT_QUAD: "quad",
                 _lib.OPX_LAYOUT_PAIR_LDS: "pair_lds", _lib.OPX_LAYOUT_OCT: "oct", _lib.OPX_LAYOUT_YPAIR: "ypair"}
