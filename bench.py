@@ -716,8 +716,9 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
 
     The bin shape, layout and workgroup size come from the autotuner run on a
     probe engine holding a sample of the releases; the main run then steps
-    every release to completion, adding each at its start step, with runs of
-    steps replayed from captured HIP graphs.  Particle-steps are the scheduled
+    every release to completion in runs of one tuned launch length, adding the
+    releases that start within a run before it (each still starts at its own
+    step), with runs of steps replayed from captured HIP graphs.  Particle-steps are the scheduled
     ones (size x steps of each release); the fraction of particles that left
     the domain (and stopped early) is reported beside it.
     """
@@ -738,9 +739,9 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     eng = AB3Engine(fs, wl.time_array, np.float64(wl.dt), storage_dtype=storage, device=device,
                     capacity=sum(r.size for r in probe), max_releases=64)
     for r in probe:
-        eng.add_release(*r.positions, nsteps=400)
+        eng.add_release(*r.positions, nsteps=1200)
     eng.rebin()
-    tuning = eng.autotune(xlows=(3, 4, 5), blocks=(128,), cadence_probe=30)
+    tuning = eng.autotune(xlows=(3, 4, 5), blocks=(128,), cadence_probe=30, launch_steps=(1, 8, 16, 32, 64, 128))
     log("batched autotune:", {k: v for k, v in tuning.items() if k != "trials"})
     del eng
     torch.cuda.empty_cache()
@@ -761,14 +762,17 @@ def run_batched(device, nrel, max_size, storage, graph_steps, grid=(1024, 768, 5
     nxt = 0
     g = 0
     rel_ids = {}
+    # releases starting within the next run of steps are added ahead of their
+    # start step (their particles wait, skipped, until it), so each run is one
+    # tuned launch length rather than the few steps between two starts
+    run = max(eng.launch_steps, graph_steps, 1)
     while g < last:
-        while nxt < nrel and start_step[order[nxt]] <= g:
+        k = min(run, last - g)
+        while nxt < nrel and start_step[order[nxt]] < g + k:
             r = rels[order[nxt]]
             rel_ids[int(order[nxt])] = eng.add_release(*r.positions, start_time=r.start_time, nsteps=r.nsteps,
                                                        start_step=start_step[order[nxt]])
             nxt += 1
-        upcoming = start_step[order[nxt]] if nxt < nrel else last
-        k = max(1, min(upcoming, last) - g)
         eng.step(k)
         g += k
     torch.cuda.synchronize()
