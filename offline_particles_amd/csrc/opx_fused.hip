@@ -53,6 +53,17 @@ __device__ __forceinline__ void stage_levels(double *L, const double *C, int NZ,
         L[kInvNZ] = 1.0 / NZ;
 }
 
+// L[kInvNZ] read where it is used: the index passes through an empty asm
+// statement, so the LDS read is not hoisted out of a step loop (where the
+// value would hold two VGPRs for the whole launch, for a branch that is
+// rarely taken).
+__device__ __forceinline__ double inv_nz_here(const double *L)
+{
+    int k = kInvNZ;
+    asm volatile("" : "+v"(k));
+    return L[k];
+}
+
 // ---------------------------------------------------------------- helpers
 
 // Pointers read from the window descriptor are generic; loads through them
@@ -1426,6 +1437,10 @@ __device__ __forceinline__ void fused_loop(int64_t i, const opx_split_state &s, 
         }
         moved = true;
     }
+    // the state's addresses computed afresh from the index (through an empty
+    // asm statement, so they are not kept from the loads above across the
+    // steps, where they would hold VGPRs)
+    asm volatile("" : "+v"(i));
     if (moved) {
         s.z[i] = p.z;
         const Axis az = encode_axis(p.zi, g.NZ), ay = encode_axis(p.yi, g.NY), ax = encode_axis(p.xi, g.NX);
@@ -2054,7 +2069,7 @@ __device__ __forceinline__ bool rk2_step(PState &p, StencilCache<T, LAYOUT> &c, 
     const AxD oz = axd(zi, 0.5, NZ + 1); // C = Cs_w at w-levels, offset +0.5
     if (need_z) {
         const double Cv = (1.0 - oz.w) * Cw[oz.b] + oz.w * Cw[oz.b + 1];
-        const double sigma = (zi + 0.5) * Cw[kInvNZ] - 1.0;
+        const double sigma = (zi + 0.5) * inv_nz_here(Cw) - 1.0;
         const double S0 = div64(hc * sigma + h0 * Cv, hc + h0);
         z = zeta0 + (zeta0 + h0) * S0;
     }
