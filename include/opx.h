@@ -74,8 +74,13 @@ enum opx_error {
  * fields of 2 or 3 dimensions only.  The interpolants read the corners from
  * it and blend them with the same arithmetic, so results do not change; a
  * stencil then touches 2 cache lines of a 3-D field instead of 8.
+ * OPX_PACK_TPAIR (time-dependent fields) interleaves the snapshots, prev[i]
+ * then next[i] for every node: no value is stored twice, and one load brings
+ * a stencil row's x-neighbours of both snapshots.  It suits particles sorted
+ * by cell, whose neighbours in a wavefront share rows: the 4 copies of every
+ * node in an OCT copy would then only add footprint.
  */
-enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2 };
+enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2, OPX_PACK_TPAIR = 3 };
 
 typedef struct opx_field {
     const void *prev;
@@ -308,12 +313,18 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
  * of `prev` when next is NULL, else OPX_PACK_OCT of (prev, next), for a
  * C-contiguous (nz, ny, nx) F32 or BF16 array (nz = 1 for a 2-D field; ny,
  * nx >= 2).  `out` is device memory of opx_packed_field_bytes bytes (QUAD:
- * 4 values per node, OCT: 8).  No counterpart in the reference: its kernels
+ * 4 values per node, OCT: 8, TPAIR: 2).  No counterpart in the reference: its kernels
  * read the blended FP64 array (fields.py:394-430); the values here are the
  * same, stored so that each stencil row is not its own cache line. */
 int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, int64_t nx, uint64_t *bytes);
 int opx_pack_field(const void *prev, const void *next, int32_t dtype, int64_t nz, int64_t ny, int64_t nx, void *out,
                    void *stream);
+
+/* opx_pack_field with the kind named: OPX_PACK_QUAD (next NULL),
+ * OPX_PACK_OCT or OPX_PACK_TPAIR (next given; TPAIR: 2 values per node).
+ * No counterpart in the reference, as opx_pack_field. */
+int opx_pack_field_as(const void *prev, const void *next, int32_t dtype, int32_t kind, int64_t nz, int64_t ny,
+                      int64_t nx, void *out, void *stream);
 
 /* opx_pack_window for the rho rows [y0, y1) only (each field's rows in that
  * range, every z-level and column; 0 <= y0 < y1 <= NY): refreshes the part of

@@ -36,7 +36,7 @@ class OpxInvalidArgument(OpxError, ValueError):
 _vp = ctypes.c_void_p
 
 
-OPX_PACK_NONE, OPX_PACK_QUAD, OPX_PACK_OCT = 0, 1, 2
+OPX_PACK_NONE, OPX_PACK_QUAD, OPX_PACK_OCT, OPX_PACK_TPAIR = 0, 1, 2, 3
 
 
 class opx_field(ctypes.Structure):
@@ -187,6 +187,8 @@ _SIGNATURES.update({
     "opx_packed_field_bytes": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                _P(ctypes.c_uint64)],
     "opx_pack_field": [_vp, _vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp],
+    "opx_pack_field_as": [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                          _vp, _vp],
     "opx_pack_window_rows": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp],
     "opx_tile_occupancy": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,

@@ -2926,25 +2926,44 @@ int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, 
 {
     OPX_REQUIRE(bytes != nullptr, "null output");
     OPX_REQUIRE(dtype == OPX_F32 || dtype == OPX_BF16, "packed stencils need F32 or BF16, not dtype %d", dtype);
-    OPX_REQUIRE(kind == OPX_PACK_QUAD || kind == OPX_PACK_OCT, "pack kind %d", kind);
+    OPX_REQUIRE(kind == OPX_PACK_QUAD || kind == OPX_PACK_OCT || kind == OPX_PACK_TPAIR, "pack kind %d", kind);
     OPX_REQUIRE(nz >= 1 && ny >= 2 && nx >= 2, "array shape (%lld, %lld, %lld)", (long long)nz, (long long)ny,
                 (long long)nx);
-    *bytes = (uint64_t)nz * ny * nx * (kind == OPX_PACK_QUAD ? 4 : 8) * (dtype == OPX_F32 ? 4 : 2);
+    const int per = kind == OPX_PACK_QUAD ? 4 : kind == OPX_PACK_OCT ? 8 : 2;
+    *bytes = (uint64_t)nz * ny * nx * per * (dtype == OPX_F32 ? 4 : 2);
     return OPX_OK;
 }
 
 int opx_pack_field(const void *prev, const void *next, int32_t dtype, int64_t nz, int64_t ny, int64_t nx, void *out,
                    void *stream)
 {
+    return opx_pack_field_as(prev, next, dtype, next ? OPX_PACK_OCT : OPX_PACK_QUAD, nz, ny, nx, out, stream);
+}
+
+int opx_pack_field_as(const void *prev, const void *next, int32_t dtype, int32_t kind, int64_t nz, int64_t ny,
+                      int64_t nx, void *out, void *stream)
+{
     int rc;
     uint64_t bytes;
-    if ((rc = opx_packed_field_bytes(dtype, next ? OPX_PACK_OCT : OPX_PACK_QUAD, nz, ny, nx, &bytes))) return rc;
+    if ((rc = opx_packed_field_bytes(dtype, kind, nz, ny, nx, &bytes))) return rc;
     OPX_REQUIRE(prev != nullptr && out != nullptr, "null array");
+    OPX_REQUIRE((next != nullptr) == (kind != OPX_PACK_QUAD), "pack kind %d %s a next snapshot", kind,
+                kind == OPX_PACK_QUAD ? "takes no" : "needs");
     const int64_t n = nz * ny * nx;
     OPX_REQUIRE(n < (int64_t(1) << 32), "field of %lld nodes", (long long)n);
     hipStream_t st = as_stream(stream);
     const unsigned grid = grid_for(n);
-    if (next) {
+    if (kind == OPX_PACK_TPAIR) {
+        // the fused windows' PAIR layout: (prev[i], next[i]) per node
+        if (dtype == OPX_F32)
+            hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const float *>(prev), static_cast<const float *>(next), out, (uint32_t)n,
+                               (int)ny, (int)nx, 0, (int)ny);
+        else
+            hipLaunchKernelGGL((k_pack<uint16_t, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const uint16_t *>(prev), static_cast<const uint16_t *>(next), out,
+                               (uint32_t)n, (int)ny, (int)nx, 0, (int)ny);
+    } else if (next) {
         if (dtype == OPX_F32)
             hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_OCT>), dim3(grid), dim3(kBlock), 0, st,
                                static_cast<const float *>(prev), static_cast<const float *>(next), out, (uint32_t)n,

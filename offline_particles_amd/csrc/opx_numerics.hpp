@@ -80,11 +80,44 @@ __device__ __forceinline__ void load_pair(const double *p, double &a, double &b)
     b = __hiloint2double((int)y1, (int)y0);
 }
 
+// load_blend of stencil values i and i + 1 from a TPAIR copy (prev[i],
+// next[i], prev[i + 1], next[i + 1] adjacent): one 16-byte load at 8-byte
+// alignment (FP32) or one 8-byte load at 4-byte alignment (BF16), blended in
+// the same arithmetic.
+__device__ __forceinline__ void tpair_blend2(const opx_field &f, int64_t i, double &va, double &vb)
+{
+    const float gt = (float)f.gt, ft = (float)f.ft;
+    float p0, q0, p1, q1;
+    if (f.dtype == OPX_F32) {
+        const num_u32x4_a8 t = *reinterpret_cast<const num_u32x4_a8 *>(static_cast<const float *>(f.packed) + 2 * i);
+        const uint32_t x0 = t.x, y0 = t.y, x1 = t.z, y1 = t.w;
+        p0 = __uint_as_float(x0);
+        q0 = __uint_as_float(y0);
+        p1 = __uint_as_float(x1);
+        q1 = __uint_as_float(y1);
+    } else {
+        const num_u32x2_a4 t = *reinterpret_cast<const num_u32x2_a4 *>(static_cast<const uint32_t *>(f.packed) + i);
+        const uint32_t e0 = t.x, e1 = t.y;
+        p0 = bf16_bits_to_f32((uint16_t)(e0 & 0xffffu));
+        q0 = bf16_bits_to_f32((uint16_t)(e0 >> 16));
+        p1 = bf16_bits_to_f32((uint16_t)(e1 & 0xffffu));
+        q1 = bf16_bits_to_f32((uint16_t)(e1 >> 16));
+    }
+    float a0 = p0 * gt, b0 = q0 * ft;
+    float a1 = p1 * gt, b1 = q1 * ft;
+    va = (double)(a0 + b0);
+    vb = (double)(a1 + b1);
+}
+
 // load_blend of stencil values i and i + 1 (the same values, in the same
 // arithmetic), reading each snapshot's neighbours with one load for FP32 and
-// FP64 fields.
+// FP64 fields, or both snapshots' with one load from a TPAIR copy.
 __device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, double &va, double &vb)
 {
+    if (f.packed) {
+        tpair_blend2(f, i, va, vb);
+        return;
+    }
     if (f.dtype == OPX_F64) {
         double p0, p1;
         load_pair(static_cast<const double *>(f.prev) + i, p0, p1);
@@ -121,7 +154,7 @@ __device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, doubl
 }
 
 // The stencil corners (y, x), (y, x+1), (y+1, x), (y+1, x+1) of `node` from a
-// field's packed copy (opx_field.packed), blended and widened exactly as
+// field's QUAD or OCT copy (opx_field.packed), blended and widened exactly as
 // load_blend does: one 16-byte (QUAD F32, OCT BF16), 32-byte (OCT F32) or
 // 8-byte (QUAD BF16) load instead of two rows per snapshot.
 __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node, double v[4])
@@ -194,7 +227,7 @@ __device__ __forceinline__ double bilinear_f64(const opx_field &f, double y, dou
     double g1 = 1.0 - f1;
     int64_t b = I0 * n1 + I1;
     double v00, v01, v10, v11;
-    if (f.packed) {
+    if (f.packed && f.packed_kind != OPX_PACK_TPAIR) {
         double c[4];
         packed_corners(f, b, c);
         v00 = c[0];
@@ -229,7 +262,7 @@ __device__ __forceinline__ double trilinear_f64(const opx_field &f, double z, do
     int64_t b10 = b00 + n1 * n2;
     int64_t b11 = b10 + n2;
     double v000, v001, v010, v011, v100, v101, v110, v111;
-    if (f.packed) {
+    if (f.packed && f.packed_kind != OPX_PACK_TPAIR) {
         double c0[4], c1[4];
         packed_corners(f, b00, c0);
         packed_corners(f, b10, c1);

@@ -56,7 +56,8 @@ inline int check_field(const opx_field &f, int ndim, const char *name, bool allo
     if (f.packed != nullptr) {
         OPX_REQUIRE(f.dtype == OPX_F32 || f.dtype == OPX_BF16, "field '%s': packed stencils need F32 or BF16", name);
         OPX_REQUIRE(ndim >= 2, "field '%s': packed stencils need 2 or 3 dimensions", name);
-        OPX_REQUIRE(f.packed_kind == (f.next == nullptr ? OPX_PACK_QUAD : OPX_PACK_OCT),
+        OPX_REQUIRE(f.next == nullptr ? f.packed_kind == OPX_PACK_QUAD
+                                      : (f.packed_kind == OPX_PACK_OCT || f.packed_kind == OPX_PACK_TPAIR),
                     "field '%s': packed kind %d does not match a %s field", name, f.packed_kind,
                     f.next == nullptr ? "static" : "time-dependent");
     }

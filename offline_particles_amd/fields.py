@@ -35,9 +35,10 @@ class FieldData(_FieldDataTuple):
     ``packed`` (an attribute, not a tuple item, so ``array, offsets = fd``
     still works) is None or ``(tensor, kind)``: a copy of the same values in
     which one load brings a node's stencil corners in (y, x)
-    (``opx_pack_field``; OPX_PACK_QUAD for a static field, OPX_PACK_OCT for a
-    snapshot pair).  The kernels read it when present; the values they
-    interpolate are unchanged."""
+    (``opx_pack_field_as``; OPX_PACK_QUAD for a static field, OPX_PACK_OCT for
+    a snapshot pair) or, inside ``sorted_stencils``, a stencil row's
+    x-neighbours of both snapshots (OPX_PACK_TPAIR for a snapshot pair).  The
+    kernels read it when present; the values they interpolate are unchanged."""
 
     packed = None
 
@@ -65,22 +66,53 @@ class unpacked:
         return False
 
 
+# Inside sorted_stencils, snapshot pairs are packed as OPX_PACK_TPAIR.
+_SORTED = False
+
+
+class sorted_stencils:
+    """Context for kernels whose particles are sorted by cell (a Simulation with
+    spatial ordering): snapshot pairs get the TPAIR copy, the two snapshots
+    interleaved node by node, instead of the OCT copy.  Neighbouring threads
+    then share stencil rows, so the OCT copy's four copies of every node only
+    add footprint; TPAIR stores each value once and still reads a row of both
+    snapshots with one load."""
+
+    def __enter__(self):
+        global _SORTED
+        self._saved, _SORTED = _SORTED, True
+        return self
+
+    def __exit__(self, *exc):
+        global _SORTED
+        _SORTED = self._saved
+        return False
+
+
 def _packable(t) -> bool:
     return (PACK_STENCILS and isinstance(t, torch.Tensor) and t.is_cuda and t.dim() in (2, 3)
             and t.dtype in (torch.float32, torch.bfloat16) and t.is_contiguous() and min(t.shape[-2:]) >= 2)
 
 
-def _pack(prev: torch.Tensor, next_: torch.Tensor | None) -> tuple:
-    """(packed tensor, kind) of prev (QUAD) or of the pair (prev, next_) (OCT)."""
+def _pair_kind() -> int:
+    from . import _lib
+
+    return _lib.OPX_PACK_TPAIR if _SORTED else _lib.OPX_PACK_OCT
+
+
+def _pack(prev: torch.Tensor, next_: torch.Tensor | None, kind: int | None = None) -> tuple:
+    """(packed tensor, kind) of prev (QUAD) or of the pair (prev, next_) (OCT,
+    or the kind given)."""
     from . import _lib
 
     dims = (1,) * (3 - prev.dim()) + tuple(prev.shape)
-    kind = _lib.OPX_PACK_QUAD if next_ is None else _lib.OPX_PACK_OCT
+    if kind is None:
+        kind = _lib.OPX_PACK_QUAD if next_ is None else _lib.OPX_PACK_OCT
     code = storage_code(prev.dtype)
     nbytes = _lib.ctypes.c_uint64()
     _lib.call("opx_packed_field_bytes", code, kind, *dims, _lib.ctypes.byref(nbytes))
     out = torch.empty(int(nbytes.value), dtype=torch.uint8, device=prev.device)
-    _lib.call("opx_pack_field", prev.data_ptr(), None if next_ is None else next_.data_ptr(), code, *dims,
+    _lib.call("opx_pack_field_as", prev.data_ptr(), None if next_ is None else next_.data_ptr(), code, kind, *dims,
               out.data_ptr(), torch.cuda.current_stream(prev.device).cuda_stream)
     return out, kind
 
@@ -384,11 +416,12 @@ class TimeDependentField(Field):
             # out a freed snapshot's address for another one), the pointers the storage
             K = self._num_timesteps
             idx = (It % K, (It + 1) % K) if self._cyclic else (It, It + 1)
-            key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype)
+            kind = _pair_kind()
+            key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype, kind)
             cache = self.__dict__.setdefault("_packed_pairs", collections.OrderedDict())
             packed = cache.get(key)
             if packed is None:
-                packed = cache[key] = _pack(prev, nxt)
+                packed = cache[key] = _pack(prev, nxt, kind)
                 while len(cache) > self.PACKED_PAIRS:
                     cache.popitem(last=False)
             else:

@@ -18,6 +18,7 @@ from typing import Callable
 import numpy as np
 import torch
 
+from .fields import sorted_stencils
 from .kernels import merge_particle_fields
 from .launcher import Launcher
 from .particles import Particles, ParticlesView
@@ -246,8 +247,15 @@ class Simulation:
 
     def step(self) -> None:
         """Advance the particles by one timestep, then run scheduled events."""
-        if self._order_every and self._since_order % self._order_every == 0:
-            self.reorder()
+        if self._order_every:
+            if self._since_order % self._order_every == 0:
+                self.reorder()
+            # the particles are sorted by cell: snapshot pairs as TPAIR copies
+            with sorted_stencils():
+                self._timestepper.timestep_particles(self._particles, self._launcher)
+                self._since_order += 1
+                self._invoke_events()
+            return
         self._timestepper.timestep_particles(self._particles, self._launcher)
         self._since_order += 1
         self._invoke_events()

@@ -3,9 +3,9 @@
 Gate: the drop-in gives the same bits with the packed copies as without
 (they hold the same values; the interpolants blend them with the same
 arithmetic), for AB3, RK2 and AB3 with an RK2 start-up, FP32 and BF16
-storage, spatially ordered or not; and opx_pack_field's layout equals the
-corners numpy picks.  The parity tests against the oracle run with the
-packed copies on (the default).
+storage, spatially ordered (TPAIR copies of the snapshot pairs) or not (OCT
+copies); and opx_pack_field_as's layouts equal the values numpy picks.  The
+parity tests against the oracle run with the packed copies on (the default).
 """
 
 import ctypes
@@ -42,6 +42,9 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
             torch.cuda.synchronize()
             res[pack] = gpu_state(sim)
             if pack:
+                # the kinds the steps packed: TPAIR when the particles are sorted
+                kinds = {k[-1] for k in sim.fieldset["u"]._packed_pairs}
+                assert kinds == {_lib.OPX_PACK_TPAIR if order else _lib.OPX_PACK_OCT}, kinds
                 fd = sim.fieldset["u"].get_field_data(0.25, None)
                 assert fd.packed is not None and fd.packed[1] == _lib.OPX_PACK_OCT
                 assert sim.fieldset["h"].get_field_data(0.25, None).packed[1] == _lib.OPX_PACK_QUAD
@@ -49,6 +52,31 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
         fields.PACK_STENCILS = True
     for k, a in res[False].items():
         assert np.array_equal(a, res[True][k], equal_nan=True), (scheme, storage, k)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape", [(5, 7, 9), (1, 6, 11)])
+def test_pack_tpair_matches_numpy(device, dtype, shape):
+    rng = np.random.default_rng(4)
+    nz, ny, nx = shape
+    p = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    q = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    code = _lib.storage_code(dtype)
+    nbytes = ctypes.c_uint64()
+    _lib.call("opx_packed_field_bytes", code, _lib.OPX_PACK_TPAIR, nz, ny, nx, ctypes.byref(nbytes))
+    assert nbytes.value == nz * ny * nx * 2 * p.element_size()
+    out = torch.zeros(nbytes.value, dtype=torch.uint8, device=device)
+    _lib.call("opx_pack_field_as", p.data_ptr(), q.data_ptr(), code, _lib.OPX_PACK_TPAIR, nz, ny, nx, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    raw = {2: np.uint16, 4: np.uint32}[p.element_size()]
+    view = torch.int16 if dtype == torch.bfloat16 else torch.int32
+    P, Q = (t.cpu().view(view).numpy().view(raw) for t in (p, q))
+    got = out.cpu().numpy().view(raw).reshape(nz, ny, nx, 2)
+    assert np.array_equal(got, np.stack([P, Q], axis=-1))
+    # a static field has no TPAIR copy
+    with pytest.raises(RuntimeError):
+        _lib.call("opx_pack_field_as", p.data_ptr(), None, code, _lib.OPX_PACK_TPAIR, nz, ny, nx, out.data_ptr(),
+                  torch.cuda.current_stream().cuda_stream)
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
