@@ -11,11 +11,11 @@ from .fieldset import Fieldset
 from .models import roms
 from .particles import Particles, ParticlesView
 from .simulation import Event, Simulation, SimulationBuilder, SimulationState
-from .spatial_arrays import BBox, DeviceArray, Stagger
+from .spatial_arrays import BBox, ChunkedArray, DeviceArray, Stagger
 from .timesteppers import ABTimestepper, RK2Timestepper, Timestepper
 
 __all__ = [
     "TimeDependentField", "StaticField", "FieldData", "BlendWindow", "Fieldset", "Simulation",
     "SimulationBuilder", "Event", "SimulationState", "RK2Timestepper", "ABTimestepper", "Timestepper",
-    "Particles", "ParticlesView", "Stagger", "BBox", "DeviceArray", "kernels", "output", "roms",
+    "Particles", "ParticlesView", "Stagger", "BBox", "DeviceArray", "ChunkedArray", "kernels", "output", "roms",
 ]

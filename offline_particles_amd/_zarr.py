@@ -143,6 +143,10 @@ class ZarrArray:
 
     # ---------------------------------------------------------------- chunks
 
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
     def chunk_path(self, idx: Sequence[int]) -> str:
         return os.path.join(self.path, self._prefix + self._sep.join(str(int(i)) for i in idx))
 

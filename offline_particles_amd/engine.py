@@ -1260,7 +1260,6 @@ class AB3Engine:
         tensor (each particle's outputs written together).
         """
         from .kernels._marshal import field_struct
-        from .launcher import _NO_BBOX
 
         if self._compacted:
             raise RuntimeError("releases were retired: sample() follows positions(), which needs every release")
@@ -1287,7 +1286,7 @@ class AB3Engine:
             # than the drop-in's packed copies
             with fields_unpacked():
                 for k, name in enumerate(names):
-                    rows[r * len(names) + k] = field_struct(fieldset[name].get_field_data(tidx, _NO_BBOX), name)
+                    rows[r * len(names) + k] = field_struct(fieldset[name].get_field_data(tidx, None), name)
         index = {name: k for k, name in enumerate(names)}
         kinds = {1: _lib.OPX_SAMPLE_LINEAR, 2: _lib.OPX_SAMPLE_BILINEAR, 3: _lib.OPX_SAMPLE_TRILINEAR}
         outs, samples = [], []

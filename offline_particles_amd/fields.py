@@ -18,13 +18,16 @@ when a caller needs it.
 from __future__ import annotations
 
 import collections
+import os
 
 import numpy as np
 import torch
 
 from ._lib import OPX_BF16, OPX_F32, OPX_F64, storage_code
-from .snapshots import SnapshotSource, ZarrSnapshots
-from .spatial_arrays import BBox, DeviceArray, SpatialArray, Stagger, as_device_tensor, default_device, to_torch_dtype
+from ._zarr import ZarrArray
+from .snapshots import ArraySnapshots, SnapshotSource, ZarrSnapshots
+from .spatial_arrays import (BBox, ChunkedArray, DeviceArray, SpatialArray, Stagger, as_device_tensor, default_device,
+                             to_torch_dtype)
 
 _FieldDataTuple = collections.namedtuple("FieldData", ["array", "offsets"])
 
@@ -210,6 +213,10 @@ class Field:
     def nspatial_dims(self) -> int:
         return len(self.spatial_shape)
 
+    # True when get_field_data reads only what the particles' bounding box
+    # covers (chunked fields), so the launcher computes the box for it
+    needs_bbox = False
+
     def _expected_spatial(self, simulation_shape):
         staggered = (
             self.z_stagger.expected_size(simulation_shape[1]),
@@ -229,6 +236,10 @@ class StaticField(Field):
     @property
     def data(self) -> SpatialArray:
         return self._data
+
+    @property
+    def needs_bbox(self) -> bool:
+        return bool(getattr(self._data, "needs_bbox", False))
 
     @property
     def dtype(self):
@@ -251,7 +262,8 @@ class StaticField(Field):
         array, offsets = self._data.get_data_subset(bbox)
         fd = FieldData(array, offsets)
         if _packable(array):
-            key = (array.data_ptr(), tuple(array.shape), array.dtype)
+            # a chunked array's version names the loaded chunks (storage may be reused)
+            key = (array.data_ptr(), tuple(array.shape), array.dtype, getattr(self._data, "version", 0))
             cached = getattr(self, "_packed", None)
             if cached is None or cached[0] != key:
                 cached = self._packed = (key, _pack(array, None))
@@ -264,6 +276,15 @@ class StaticField(Field):
 
     from_array = from_numpy
 
+    @classmethod
+    def from_dask(cls, data, z_stagger, y_stagger, x_stagger, *, device=None, storage_dtype=None) -> "StaticField":
+        """A field read chunk by chunk from a chunked array (fields.py:216-231):
+        each launch holds on the device only the chunks that cover the
+        particles' bounding box (``ChunkedArray``).  ``data`` is a dask array or
+        any array with ``chunks`` and basic slicing (e.g. a ``ZarrArray``)."""
+        return cls(ChunkedArray(data, Stagger(z_stagger), Stagger(y_stagger), Stagger(x_stagger), device=device,
+                                storage_dtype=storage_dtype))
+
 
 class TimeDependentField(Field):
     """A field given as snapshots on a time axis, resident in HBM (fields.py:237-458).
@@ -272,7 +293,11 @@ class TimeDependentField(Field):
     (e.g. ``ZarrSnapshots``) read lazily: then only the snapshots in use stay
     on the device (the last few, the next one decoded ahead on a host thread),
     as the reference's dask-backed fields load what a launch needs
-    (fields.py:443-458).  With ``cyclic=True`` snapshot ``It`` is
+    (fields.py:443-458).  A chunked host array (a dask array, a ``ZarrArray``)
+    is read lazily the same way, a snapshot at a time; with
+    ``spatial_array_factory=ChunkedArray`` (``from_dask``) each launch reads
+    only the chunks of its two snapshots that cover the particles' bounding
+    box (the reference's ChunkedDaskArray slices).  With ``cyclic=True`` snapshot ``It`` is
     ``data[It % T]`` so a short set of periodic snapshots serves an arbitrarily
     long time axis; otherwise ``It`` must lie in ``[0, T-2]`` as in the
     reference.
@@ -288,6 +313,18 @@ class TimeDependentField(Field):
             raise ValueError("TimeDependentField requires at least 2 dimensions (time + spatial).")
         if data.shape[0] < 2 and not cyclic:
             raise ValueError("TimeDependentField requires at least 2 time steps.")
+        # chunked: the host array, sliced per launch by the bounding box
+        self._chunked = None
+        if spatial_array_factory is ChunkedArray:
+            if isinstance(data, (SnapshotSource, torch.Tensor)):
+                raise TypeError("a chunked field needs a host array with chunks (a dask array, a ZarrArray)")
+            self._chunked = data
+            self._chunk_slices = collections.OrderedDict()
+            data = ArraySnapshots(data)
+        elif (hasattr(data, "chunks") and not isinstance(data, (np.ndarray, torch.Tensor, SnapshotSource))):
+            # a lazily read chunked array: a snapshot at a time, as the reference
+            # slices data[It] (fields.py:340-392)
+            data = ArraySnapshots(data)
         if isinstance(data, SnapshotSource):
             self._source = data
             self._snapshots = None
@@ -312,6 +349,45 @@ class TimeDependentField(Field):
         src = ZarrSnapshots(store, name, workers=workers)
         data = src.read_all() if preload else src
         return cls(data, z_stagger, y_stagger, x_stagger, cyclic=cyclic, device=device, storage_dtype=storage_dtype)
+
+    @classmethod
+    def from_zarr_chunked(cls, store, name: str, z_stagger, y_stagger, x_stagger, *, cyclic: bool = False,
+                          device=None, storage_dtype=None) -> "TimeDependentField":
+        """A field over a (T, ...) Zarr v3 array of a local store whose launches
+        read only the chunks covering the particles' bounding box (the Zarr
+        form of ``from_dask``)."""
+        return cls.from_dask(ZarrArray(os.fspath(store), name), z_stagger, y_stagger, x_stagger, cyclic=cyclic,
+                             device=device, storage_dtype=storage_dtype)
+
+    @classmethod
+    def from_dask(cls, data, z_stagger, y_stagger, x_stagger, *, preload_space: bool = False, cyclic: bool = False,
+                  device=None, storage_dtype=None) -> "TimeDependentField":
+        """A field over a chunked (T, ...) array (fields.py:443-458): with
+        ``preload_space`` each snapshot in use is read whole, otherwise each
+        launch reads the chunks covering the particles' bounding box
+        (``ChunkedArray``).  ``data`` is a dask array or any array with
+        ``chunks`` and basic slicing (e.g. a ``ZarrArray``)."""
+        factory = DeviceArray if preload_space else ChunkedArray
+        return cls(data, z_stagger, y_stagger, x_stagger, factory, cyclic=cyclic, device=device,
+                   storage_dtype=storage_dtype)
+
+    @property
+    def needs_bbox(self) -> bool:
+        return self._chunked is not None
+
+    def _chunk_slice(self, It: int) -> ChunkedArray:
+        """Snapshot It as a ChunkedArray (the last three kept, so the window's
+        snapshots keep their loaded chunks as it moves, fields.py:340-366)."""
+        idx = It % self._num_timesteps if self._cyclic else It
+        a = self._chunk_slices.get(idx)
+        if a is None:
+            a = self._chunk_slices[idx] = ChunkedArray(self._chunked, self.z_stagger, self.y_stagger, self.x_stagger,
+                                                       index=(idx,), device=self._device, storage_dtype=self._storage)
+            while len(self._chunk_slices) > 3:
+                self._chunk_slices.popitem(last=False)
+        else:
+            self._chunk_slices.move_to_end(idx)
+        return a
 
     @property
     def source(self):
@@ -409,15 +485,25 @@ class TimeDependentField(Field):
     def get_field_data(self, time_index: float, bbox: BBox) -> FieldData:
         It, ft, gt = blend_weights(time_index, self.dtype)
         self.set_time_index(It)
-        prev, nxt = self.snapshot(It), self.snapshot(It + 1)
-        fd = FieldData(BlendWindow(prev, nxt, ft, gt, It), self.active_offsets)
+        versions = ()
+        if self._chunked is not None:
+            # the covering chunks of both snapshots (fields.py:414-416); the
+            # versions name what was loaded
+            pa, na = self._chunk_slice(It), self._chunk_slice(It + 1)
+            prev, offsets = pa.get_data_subset(bbox)
+            nxt, _ = na.get_data_subset(bbox)
+            versions = (pa.version, na.version)
+        else:
+            prev, nxt = self.snapshot(It), self.snapshot(It + 1)
+            offsets = self.active_offsets
+        fd = FieldData(BlendWindow(prev, nxt, ft, gt, It), offsets)
         if _packable(prev):
             # the snapshot indices name the contents (a lazily read field may hand
             # out a freed snapshot's address for another one), the pointers the storage
             K = self._num_timesteps
             idx = (It % K, (It + 1) % K) if self._cyclic else (It, It + 1)
             kind = _pair_kind()
-            key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype, kind)
+            key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype, kind) + versions
             cache = self.__dict__.setdefault("_packed_pairs", collections.OrderedDict())
             packed = cache.get(key)
             if packed is None:

@@ -80,6 +80,24 @@ class SnapshotSource:
         return np.stack([self.read(i) for i in range(len(self))])
 
 
+class ArraySnapshots(SnapshotSource):
+    """Snapshots of any (T, ...) array with basic slicing (a dask array, a
+    ``ZarrArray``, a memory map): snapshot i is ``data[i]``, computed when the
+    slice has a ``compute()`` method (dask)."""
+
+    def __init__(self, data, *, workers: int = 2) -> None:
+        super().__init__(data.shape, data.dtype, workers=workers)
+        if len(self.shape) < 2:
+            raise ValueError("a snapshot array needs a time axis and a spatial one")
+        self._data = data
+
+    def _read_into(self, i: int, out: np.ndarray) -> None:
+        block = self._data[i]
+        if hasattr(block, "compute"):
+            block = block.compute()
+        out[...] = np.asarray(block)
+
+
 class ZarrSnapshots(SnapshotSource):
     """Snapshots of a Zarr v3 array (T, ...) in a local directory store; each
     snapshot is decoded chunk by chunk on ``workers`` threads."""

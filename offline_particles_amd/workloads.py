@@ -391,7 +391,7 @@ def build_fieldset(wl: Workload, *, device=None, storage_dtype=None, cyclic: boo
 
 
 def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=None, positions=None,
-                     status=None, ab_kernel=None, spatial_order_every: int = 0):
+                     status=None, ab_kernel=None, spatial_order_every: int = 0, fieldset=None):
     """A drop-in Simulation for the workload with particles placed.
 
     ``scheme`` is "rk2", "ab3" or "ab3_rk2" (AB3 whose first two steps are RK2
@@ -400,7 +400,8 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
     ``spatial_order_every`` > 0 keeps the particle arrays in the Morton order
     of the particles' cells, refreshed every that many steps (results and what
     the user sees are unchanged).  ``ab_kernel`` replaces the AB3
-    kernel (same declarations), e.g. to time it.
+    kernel (same declarations), e.g. to time it.  ``fieldset`` replaces the
+    one built from the workload (e.g. one of chunked fields).
     """
     from .kernels.roms import ab3_post_w_advection_kernel, ab3_w_advection_kernel
     from .kernels.validation import validation_kernel
@@ -409,7 +410,7 @@ def build_simulation(wl: Workload, scheme: str, *, device=None, storage_dtype=No
     from .timesteppers import ABTimestepper
 
     pos = wl.positions if positions is None else positions
-    fs = build_fieldset(wl, device=device, storage_dtype=storage_dtype)
+    fs = fieldset if fieldset is not None else build_fieldset(wl, device=device, storage_dtype=storage_dtype)
     dt = np.float64(wl.dt)
     if scheme == "rk2":
         ts = roms.rk2_w_advection_timestepper(wl.time_array, dt)
