@@ -5,7 +5,7 @@ run as hand-written HIP kernels in ``libopx.so`` (include/opx.h) on particle
 and field data resident in device memory.
 """
 
-from . import kernels, output
+from . import events, kernels, output
 from .fields import BlendWindow, FieldData, StaticField, TimeDependentField
 from .fieldset import Fieldset
 from .models import roms
@@ -17,5 +17,6 @@ from .timesteppers import ABTimestepper, RK2Timestepper, Timestepper
 __all__ = [
     "TimeDependentField", "StaticField", "FieldData", "BlendWindow", "Fieldset", "Simulation",
     "SimulationBuilder", "Event", "SimulationState", "RK2Timestepper", "ABTimestepper", "Timestepper",
-    "Particles", "ParticlesView", "Stagger", "BBox", "DeviceArray", "ChunkedArray", "kernels", "output", "roms",
+    "Particles", "ParticlesView", "Stagger", "BBox", "DeviceArray", "ChunkedArray", "events", "kernels", "output",
+    "roms",
 ]

@@ -303,3 +303,44 @@ def test_packed_window_limited_to_4GiB(lib):
     assert out[2].value == 16 * 61 * 1536 * 2048 <= 1 << 32
     assert lib.opx_packed_bytes(ctypes.byref(g), _lib.OPX_F32, _lib.OPX_LAYOUT_PAIR, *args) == _lib.OPX_OK
     assert out[2].value == 8 * 61 * 1536 * 2048
+
+
+# The reference's public names by module (its __init__.py files: __init__.py:1-25,
+# events/__init__.py:1-11, kernels/__init__.py, kernels/output/__init__.py,
+# kernels/roms/__init__.py, models/roms/__init__.py:8-20, output/__init__.py,
+# fields.py, spatial_arrays.py, timesteppers.py): a script's imports resolve after
+# the package name is switched.
+REFERENCE_PUBLIC = {
+    "": ["TimeDependentField", "StaticField", "Fieldset", "Simulation", "SimulationBuilder", "Event",
+         "SimulationState", "RK2Timestepper", "Timestepper", "kernels", "output", "roms"],
+    "events": ["Event", "SimulationState", "IterationScheduler", "TimeScheduler"],
+    "kernels": ["ParticleKernel", "KernelFunction", "ParticleStatus", "merge_particle_fields", "merge_scalars",
+                "merge_simulation_fields"],
+    "kernels.output": ["linear_interpolation_kernel", "bilinear_interpolation_kernel",
+                       "trilinear_interpolation_kernel", "vertical_interpolation_kernel",
+                       "horizontal_interpolation_kernel"],
+    "kernels.roms": ["compute_z_kernel", "ab3_w_advection_kernel", "ab3_post_w_advection_kernel",
+                     "ab3_isopycnal_following_kernel", "ab3_post_isopycnal_following_kernel",
+                     "rk2_w_advection_step_1_kernel", "rk2_w_advection_step_2_kernel",
+                     "rk2_w_advection_update_kernel"],
+    "models.roms": ["compute_z_kernel", "ab3_w_advection_kernel", "ab3_post_w_advection_kernel",
+                    "ab3_w_advection_timestepper", "rk2_w_advection_step_1_kernel", "rk2_w_advection_step_2_kernel",
+                    "rk2_w_advection_update_kernel", "rk2_w_advection_timestepper"],
+    "output": ["AbstractOutputWriter", "AbstractOutputWriterBuilder", "Output", "ZarrOutputWriter",
+               "ZarrOutputBuilder", "linearly_interpolate_fields"],
+    "fields": ["Field", "StaticField", "TimeDependentField", "FieldData"],
+    "spatial_arrays": ["BBox", "Stagger", "SpatialArray", "NumpyArray", "ChunkedDaskArray",
+                       "compute_new_lower_bound", "compute_new_upper_bound"],
+    "timesteppers": ["Timestepper", "RK2Timestepper", "ABTimestepper"],
+    "launcher": ["Launcher", "ScalarSource"],
+    "particles": ["Particles", "ParticlesView"],
+}
+
+
+@pytest.mark.parametrize("module", sorted(REFERENCE_PUBLIC))
+def test_reference_public_names_resolve(module):
+    import importlib
+
+    mod = importlib.import_module("offline_particles_amd" + ("." + module if module else ""))
+    missing = [n for n in REFERENCE_PUBLIC[module] if not hasattr(mod, n)]
+    assert not missing, (module, missing)
