@@ -332,6 +332,16 @@ int opx_pack_field_as(const void *prev, const void *next, int32_t dtype, int32_t
 int opx_pack_window_rows(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1,
                          int32_t y0, int32_t y1, void *pu, void *pv, void *pw, void *p2, void *stream);
 
+/* opx_pack_window for the nodes of `boxes` only (nboxes x (y0, y1, x0, x1)
+ * in rho rows and columns, all levels; each box widened by the row above and
+ * the column to its left, whose packed elements hold its nodes as y + 1 and
+ * x + 1 neighbours, and clipped to each field's shape), and the 2-D window
+ * whole: the repack of a pair whose slots hold only the tiles the particles
+ * can reach.  No counterpart in the reference; the tiled window is the
+ * counterpart of its bounding-box chunk loading (spatial_arrays.py:260-295). */
+int opx_pack_window_boxes(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1,
+                          const int32_t *boxes, int32_t nboxes, void *pu, void *pv, void *pw, void *p2, void *stream);
+
 /* Sparse snapshot streaming: where the particles are and which parts of a
  * snapshot to copy to the device.  Counterpart of the reference's lazy
  * chunked field (ChunkedDaskArray.get_data_subset, spatial_arrays.py:260-295,

@@ -189,6 +189,8 @@ _SIGNATURES.update({
     "opx_pack_field": [_vp, _vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp],
     "opx_pack_field_as": [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                           _vp, _vp],
+    "opx_pack_window_boxes": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+                              ctypes.c_int32, _vp, _vp, _vp, _vp, _vp],
     "opx_pack_window_rows": [_P(opx_window), _P(opx_grid), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp],
     "opx_tile_occupancy": [_P(opx_split_state), _P(opx_grid), ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,

@@ -721,9 +721,24 @@ __device__ __forceinline__ double add_mul_rn(double a, double b, double c)
 // reference raises (time outside the array, or t == t_last).
 __device__ bool time_index(const double *__restrict__ ta, int n, double t, double &tidx)
 {
-    if (!(t >= ta[0] && t <= ta[n - 1]))
+    if (n < 2 || !(t >= ta[0] && t <= ta[n - 1]))
         return false;
-    int lo = 0, hi = n; // searchsorted(side="right"): count of entries <= t
+    // searchsorted(side="right"): the count of entries <= t.  First the entry g
+    // where a uniformly spaced array would hold t: if ta[g] <= t < ta[g + 1]
+    // the count is g + 1 (two loads, where each step's time lies for a
+    // uniform array); otherwise the binary search runs on the side of g that
+    // holds the count, so the result is the search's over any sorted array
+    int g = 0;
+    if (n > 2 && ta[n - 1] > ta[0])
+        g = max(0, min(n - 2, (int)((t - ta[0]) / (ta[n - 1] - ta[0]) * (double)(n - 1))));
+    int lo, hi;
+    if (ta[g] <= t) {
+        lo = g + 1;
+        hi = ta[g + 1] <= t ? n : g + 1;
+    } else {
+        lo = 0;
+        hi = g;
+    }
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
         if (ta[mid] <= t)
@@ -1713,20 +1728,11 @@ __global__ void __launch_bounds__(64)
 
 // ---------------------------------------------------------------- packed windows
 
-// Packs the nodes of rows [row0, row0 + nrows) of every z-level (n rows of n2
-// nodes per level, n1 rows per level): thread j of the launch packs node
-// (z, row0 + r, x) with j = (z nrows + r) n2 + x, so a launch over all rows
-// packs node j.  The window limit (4 GiB) keeps node indices below 2^32.
+// Packs node i of a field of n1 rows of n2 nodes per level from slots p, q.
 template <typename T, int LAYOUT>
-__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, uint32_t n, int n1,
-                       int n2, int row0, int nrows)
+__device__ __forceinline__ void pack_node(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out,
+                                          int64_t i, int n1, int n2)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n)
-        return;
-    const uint32_t band = (uint32_t)nrows * (uint32_t)n2;
-    const uint32_t zl = j / band, r = j - zl * band;
-    const int64_t i = ((int64_t)zl * n1 + row0) * n2 + r;
     if constexpr (LAYOUT == OPX_LAYOUT_OCT) {
         // neighbours beyond the last row / column are never stencil nodes of a base here
         const int64_t x1 = (i % n2) + 1 < n2 ? 1 : 0;
@@ -1777,6 +1783,58 @@ __global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *_
             static_cast<uint2 *>(out)[i] = make_uint2((uint32_t)P[i] | ((uint32_t)P[i1] << 16),
                                                       (uint32_t)Q[i] | ((uint32_t)Q[i1] << 16));
         }
+    }
+}
+
+// Packs the nodes of rows [row0, row0 + nrows) of every z-level (n rows of n2
+// nodes per level, n1 rows per level): thread j of the launch packs node
+// (z, row0 + r, x) with j = (z nrows + r) n2 + x, so a launch over all rows
+// packs node j.  The window limit (4 GiB) keeps node indices below 2^32.
+template <typename T, int LAYOUT>
+__global__ void k_pack(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, uint32_t n, int n1,
+                       int n2, int row0, int nrows)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    const uint32_t band = (uint32_t)nrows * (uint32_t)n2;
+    const uint32_t zl = j / band, r = j - zl * band;
+    pack_node<T, LAYOUT>(p, q, out, ((int64_t)zl * n1 + row0) * n2 + r, n1, n2);
+}
+
+// Boxes of (y0, y1, x0, x1) of one field's nodes over every level, up to 128
+// per launch, with the element prefix sums of the boxes (as for the tile
+// gathers).
+constexpr int kPackBoxes = 128;
+struct PackBoxes {
+    int32_t box[kPackBoxes][4];
+    int64_t first[kPackBoxes + 1];
+    int32_t n;
+};
+
+// Packs every node of the boxes (all levels): element e of the launch is node
+// (z, y0 + yy, x0 + xx) of its box, found by binary search, x fastest.
+template <typename T, int LAYOUT>
+__global__ void k_pack_boxes(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, int n1, int n2,
+                             PackBoxes B)
+{
+    const int64_t total = B.first[B.n];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = B.n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (B.first[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int32_t *x = B.box[lo];
+        const uint32_t w = (uint32_t)(x[3] - x[2]), h = (uint32_t)(x[1] - x[0]);
+        uint32_t k = (uint32_t)(e - B.first[lo]); // a box of one window: fewer than 2^32 nodes
+        const uint32_t xx = k % w;
+        k /= w;
+        const uint32_t yy = k % h, z = k / h;
+        pack_node<T, LAYOUT>(p, q, out, ((int64_t)z * n1 + x[0] + yy) * n2 + x[2] + xx, n1, n2);
     }
 }
 
@@ -2910,6 +2968,105 @@ int opx_pack_window_rows(const opx_window *win, const opx_grid *g, int32_t layou
         }
 #undef OPX_PACK
         if ((rc = check_launch("opx_pack_window"))) return rc;
+    }
+    return OPX_OK;
+}
+
+int opx_pack_window_boxes(const opx_window *win, const opx_grid *g, int32_t layout, int32_t s0, int32_t s1,
+                          const int32_t *boxes, int32_t nboxes, void *pu, void *pv, void *pw, void *p2, void *stream)
+{
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    OPX_REQUIRE(win && win->u && win->v && win->w, "null window");
+    OPX_REQUIRE(win->dtype3 == OPX_F32 || win->dtype3 == OPX_BF16, "dtype3 must be F32 or BF16");
+    if (layout == OPX_LAYOUT_PAIR_LDS) // same packed data as PAIR
+        layout = OPX_LAYOUT_PAIR;
+    OPX_REQUIRE(layout == OPX_LAYOUT_PAIR || layout == OPX_LAYOUT_QUAD || layout == OPX_LAYOUT_OCT ||
+                    layout == OPX_LAYOUT_YPAIR,
+                "layout must be PAIR, QUAD, OCT, YPAIR or PAIR_LDS");
+    OPX_REQUIRE(s0 >= 0 && s0 < win->nslots && s1 >= 0 && s1 < win->nslots, "slots (%d, %d) outside [0, %d)", s0,
+                s1, win->nslots);
+    OPX_REQUIRE(nboxes >= 0 && (nboxes == 0 || boxes != nullptr), "%d boxes", nboxes);
+    for (int32_t b = 0; b < nboxes; ++b) {
+        const int32_t *x = boxes + 4 * b;
+        OPX_REQUIRE(0 <= x[0] && x[0] < x[1] && x[1] <= g->NY && 0 <= x[2] && x[2] < x[3] && x[3] <= g->NX,
+                    "box %d (rows [%d, %d), columns [%d, %d)) outside (%d, %d)", b, x[0], x[1], x[2], x[3], g->NY,
+                    g->NX);
+    }
+    OPX_REQUIRE(pu && pv && pw && p2, "null output window");
+    OPX_REQUIRE(win->hdd && win->zeta, "window without hdd/zeta");
+    uint64_t bu, bv, bw, b2;
+    if ((rc = opx_packed_bytes(g, win->dtype3, layout, &bu, &bv, &bw, &b2))) return rc;
+    hipStream_t st = as_stream(stream);
+    // the 2-D window whole (NY x NX points, small)
+    const int64_t n2d = (int64_t)g->NY * g->NX;
+    hipLaunchKernelGGL(k_pack2, dim3(grid_for(n2d)), dim3(kBlock), 0, st, reinterpret_cast<const float4 *>(win->hdd),
+                       win->zeta + s0 * win->zeta_stride, win->zeta + s1 * win->zeta_stride,
+                       static_cast<float4 *>(p2), (int64_t)0, n2d);
+    if ((rc = check_launch("opx_pack_window_boxes 2d"))) return rc;
+    if (nboxes == 0)
+        return OPX_OK;
+    const int wl = w_levels(win, g);
+    OPX_REQUIRE(wl == g->NZ || wl == g->NZ + 1, "third field levels %d not NZ or NZ + 1", wl);
+    struct F {
+        const void *base;
+        int64_t stride;
+        int nz, n1, n2;
+        void *out;
+    } f[3] = {{win->u, win->u_stride, g->NZ, g->NY, g->NX - 1, pu}, {win->v, win->v_stride, g->NZ, g->NY - 1, g->NX, pv},
+              {win->w, win->w_stride, wl, g->NY, g->NX, pw}};
+    for (auto &x : f) {
+        for (int32_t b0 = 0; b0 < nboxes; b0 += kPackBoxes) {
+            // each box widened by the row above and the column to its left (whose
+            // packed elements hold the box's nodes as y + 1 / x + 1 neighbours)
+            // and clipped to the field's rows and columns
+            PackBoxes B = {};
+            int64_t total = 0;
+            for (int32_t i = b0; i < min(nboxes, b0 + kPackBoxes); ++i) {
+                const int32_t *bx = boxes + 4 * i;
+                const int32_t y0 = max(0, bx[0] - 1), y1 = min(bx[1], x.n1);
+                const int32_t c0 = max(0, bx[2] - 1), c1 = min(bx[3], x.n2);
+                if (y0 >= y1 || c0 >= c1)
+                    continue;
+                B.box[B.n][0] = y0;
+                B.box[B.n][1] = y1;
+                B.box[B.n][2] = c0;
+                B.box[B.n][3] = c1;
+                B.first[B.n] = total;
+                total += (int64_t)x.nz * (y1 - y0) * (c1 - c0);
+                ++B.n;
+            }
+            if (B.n == 0)
+                continue;
+            B.first[B.n] = total;
+            const unsigned grid = (unsigned)std::min<int64_t>((total + kBlock - 1) / kBlock, 8192);
+#define OPX_PACKB(T, L)                                                                                            \
+    hipLaunchKernelGGL((k_pack_boxes<T, L>), dim3(grid), dim3(kBlock), 0, st, b + s0 * x.stride, b + s1 * x.stride, \
+                       x.out, x.n1, x.n2, B)
+            if (win->dtype3 == OPX_F32) {
+                const float *b = static_cast<const float *>(x.base);
+                if (layout == OPX_LAYOUT_PAIR)
+                    OPX_PACKB(float, OPX_LAYOUT_PAIR);
+                else if (layout == OPX_LAYOUT_QUAD)
+                    OPX_PACKB(float, OPX_LAYOUT_QUAD);
+                else if (layout == OPX_LAYOUT_YPAIR)
+                    OPX_PACKB(float, OPX_LAYOUT_YPAIR);
+                else
+                    OPX_PACKB(float, OPX_LAYOUT_OCT);
+            } else {
+                const uint16_t *b = static_cast<const uint16_t *>(x.base);
+                if (layout == OPX_LAYOUT_PAIR)
+                    OPX_PACKB(uint16_t, OPX_LAYOUT_PAIR);
+                else if (layout == OPX_LAYOUT_QUAD)
+                    OPX_PACKB(uint16_t, OPX_LAYOUT_QUAD);
+                else if (layout == OPX_LAYOUT_YPAIR)
+                    OPX_PACKB(uint16_t, OPX_LAYOUT_YPAIR);
+                else
+                    OPX_PACKB(uint16_t, OPX_LAYOUT_OCT);
+            }
+#undef OPX_PACKB
+            if ((rc = check_launch("opx_pack_window_boxes"))) return rc;
+        }
     }
     return OPX_OK;
 }

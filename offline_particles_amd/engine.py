@@ -900,27 +900,23 @@ class AB3Engine:
             self._repack_rows(slot, boxes)
 
     def _repack_rows(self, slot: int, boxes) -> None:
-        """Pack again the rows of `boxes` (and the row above each, whose y + 1
-        neighbours they are) in the pair-ring windows packed from this slot's
-        current contents."""
+        """Pack again the nodes of `boxes` (and of the row above and the column
+        left of each, whose y + 1 / x + 1 neighbours they are) in the pair-ring
+        windows packed from this slot's current contents."""
         if self.layout == _lib.OPX_LAYOUT_RING or self._pair_bufs is None or not len(boxes):
             return
-        rows = sorted((max(0, int(y0) - 1), int(y1)) for y0, y1 in boxes[:, :2])
-        merged = []
-        for a, b in rows:
-            if merged and a <= merged[-1][1]:
-                merged[-1][1] = max(merged[-1][1], b)
-            else:
-                merged.append([a, b])
-        w = self.window
         for s0 in {(slot - 1) % self.nslots, slot}:
             key = self._pair_contents(s0)
             if self._pair_key[s0] != key or None in key:
-                continue  # packed in full when next read
-            bufs = [b.data_ptr() + s0 * st for b, st in zip(self._pair_bufs, self._pair_strides)]
-            for a, b in merged:
-                _lib.call("opx_pack_window_rows", ctypes.byref(w), ctypes.byref(self.grid), self.layout, s0,
-                          (s0 + 1) % self.nslots, a, b, *bufs, self._stream())
+                continue  # packed when next read
+            self._pack_boxes(s0, boxes)
+
+    def _pack_boxes(self, s0: int, boxes) -> None:
+        """opx_pack_window_boxes of pair-ring window s0 over `boxes` (int32 (k, 4))."""
+        b = np.ascontiguousarray(boxes, dtype=np.int32)
+        bufs = [p.data_ptr() + s0 * st for p, st in zip(self._pair_bufs, self._pair_strides)]
+        _lib.call("opx_pack_window_boxes", ctypes.byref(self.window), ctypes.byref(self.grid), self.layout, s0,
+                  (s0 + 1) % self.nslots, b.ctypes.data, len(b), *bufs, self._stream())
 
     def _stage(self, name: str, src: int):
         """Start reading snapshot `src` of a lazily read field into pinned staging
@@ -1020,9 +1016,16 @@ class AB3Engine:
             s0 = a % self.nslots
             key = self._pair_contents(s0)
             if self._pair_key[s0] != key:
-                bufs = [b.data_ptr() + s0 * st for b, st in zip(self._pair_bufs, self._pair_strides)]
-                _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, s0,
-                          (s0 + 1) % self.nslots, *bufs, self._stream())
+                if self._tile is not None:
+                    # the slots hold only their loaded tiles: pack those (top-ups
+                    # pack what they add)
+                    s1 = (s0 + 1) % self.nslots
+                    boxes, _ = tile_boxes(self._slot_tiles[s0] | self._slot_tiles[s1], self._tile, self.NY, self.NX)
+                    self._pack_boxes(s0, boxes)
+                else:
+                    bufs = [b.data_ptr() + s0 * st for b, st in zip(self._pair_bufs, self._pair_strides)]
+                    _lib.call("opx_pack_window", ctypes.byref(w), ctypes.byref(self.grid), self.layout, s0,
+                              (s0 + 1) % self.nslots, *bufs, self._stream())
                 self._pair_key[s0] = key
 
     def set_layout(self, layout: int) -> None:
