@@ -170,6 +170,42 @@ def _h2d(a, dtype, device):
 _NP = {torch.float64: np.float64, torch.uint8: np.uint8}
 
 
+class _PinnedStaging:
+    """Pinned host buffers for host->device copies of release arrays, reused
+    once the copy that last read them has finished (an event per buffer): the
+    arrays are written once, straight into pinned memory, and the steady state
+    allocates no pinned memory (an allocation of it can wait for the device)."""
+
+    def __init__(self) -> None:
+        self._bufs: list = []  # [pinned uint8 tensor, event of its last copy or None]
+
+    def to_device(self, arrays, dtype, device) -> list:
+        """The host arrays as device tensors of `dtype` (views of one copy, stream-ordered)."""
+        npdt = np.dtype(_NP[dtype])
+        flat = [np.asarray(a).reshape(-1) for a in arrays]
+        sizes = [a.size for a in flat]
+        nbytes = sum(sizes) * npdt.itemsize
+        slot = next((b for b in self._bufs if b[0].numel() >= nbytes and (b[1] is None or b[1].query())), None)
+        if slot is None:
+            slot = [torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, pin_memory=True), None]
+            self._bufs.append(slot)
+        host = slot[0][:nbytes].view(dtype)
+        hn = host.numpy()
+        off = 0
+        for a, n in zip(flat, sizes):
+            np.copyto(hn[off:off + n], a, casting="unsafe")
+            off += n
+        dev = host.to(device, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        slot[1] = ev
+        out, off = [], 0
+        for n in sizes:
+            out.append(dev[off:off + n])
+            off += n
+        return out
+
+
 def _to_host(a) -> np.ndarray:
     """A numpy view or copy of an array or tensor (device tensors are copied back)."""
     if torch.is_tensor(a):
@@ -288,12 +324,14 @@ class AB3Engine:
         self._retired: list = []
         self._retired_particles = 0
         self._next_pid = 0
+        self._staging = _PinnedStaging()
         self._compacted = False
         self.global_step = 0
         self._since_rebin = 0
         self._work = None
         self._structs = None
         self._graphs = {}
+        self._capture_stream = None
         self.graph_steps = 0  # > 0: replay captured graphs of this many steps
         # steps per fused launch (1..MAX_LAUNCH_STEPS): the state stays in
         # registers between them and unchanged stencils are not read again
@@ -493,7 +531,10 @@ class AB3Engine:
         r = self._free_ids.pop() if self._free_ids else len(self._clocks_host)
         if r >= 1 << 15:
             raise ValueError("too many releases")
-        z, y, x = (_h2d(a, torch.float64, self.device) for a in (zidx, yidx, xidx))
+        if any(torch.is_tensor(a) and a.device.type != "cpu" for a in (zidx, yidx, xidx)):
+            z, y, x = (_h2d(a, torch.float64, self.device) for a in (zidx, yidx, xidx))
+        else:
+            z, y, x = self._staging.to_device((zidx, yidx, xidx), torch.float64, self.device)
         m = int(z.numel())
         if not (y.numel() == m and x.numel() == m):
             raise ValueError("zidx, yidx, xidx must have the same length")
@@ -678,6 +719,45 @@ class AB3Engine:
         self._left[idx[ok]] -= 1
         self._left[idx[~ok]] = 0
         its = np.unique(np.concatenate([p[ok] for p in parts]))
+        return {(int(i), int(i) + 1) for i in its}
+
+    def _advance_host_clocks_run(self, k: int):
+        """k calls of _advance_host_clocks (global steps global_step ..
+        global_step + k - 1) at once: the same clocks afterwards and the union
+        of the pairs.  A release's times over the run are a cumulative sum
+        along its row, the sequential additions (the rounding) of k repeated
+        t += dt; it runs from its start step while it has steps left, up to
+        the first step whose times leave the array (which stops it)."""
+        live = np.nonzero(self._left > 0)[0]
+        if live.size == 0 or k <= 0:
+            return set()
+        j0 = np.maximum(self._start[live] - self.global_step, 0)
+        n = np.minimum(np.maximum(k - j0, 0), self._left[live])
+        keep = n > 0
+        live, n = live[keep], n[keep]
+        if live.size == 0:
+            return set()
+        R, m = live.size, int(n.max())
+        T = np.empty((R, m + 1))
+        T[:, 0] = self._t[live]
+        T[:, 1:] = self.dt
+        T = np.cumsum(T, axis=1)  # T[:, i]: the time after i steps
+        it, ok = (a.reshape(R, m + 1) for a in self._time_index(T.ravel()))
+        step_ok = ok[:, :-1] & ok[:, 1:]
+        parts = [it[:, :-1], it[:, 1:]]
+        if self.scheme == "rk2":
+            # intermediate time of stage 2: time + alpha * dt (timesteppers.py:250)
+            im, okm = (a.reshape(R, m) for a in self._time_index((T[:, :-1] + self.alpha * self.dt).ravel()))
+            step_ok &= okm
+            parts.append(im)
+        steps = np.arange(m)
+        fail = (steps[None, :] < n[:, None]) & ~step_ok
+        done = np.where(fail.any(axis=1), fail.argmax(axis=1), n)
+        ran = steps[None, :] < done[:, None]
+        self._t[live] = T[np.arange(R), done]
+        self._left[live] -= done
+        self._left[live[done < n]] = 0
+        its = np.unique(np.concatenate([p[ran] for p in parts]))
         return {(int(i), int(i) + 1) for i in its}
 
     # ---------------------------------------------------------------- streamed snapshots
@@ -1076,11 +1156,23 @@ class AB3Engine:
         key = (self._state.cell.data_ptr(), self._launch_span(), k, self.layout, self.block, self.launch_steps)
         g = self._graphs.get(key)
         if g is None:
+            # captured on a side stream ordered after the compute stream's work,
+            # without torch.cuda.graph's device synchronisation and cache release
+            # on entry (each capture would drain the device and re-allocate)
+            if self._capture_stream is None:
+                self._capture_stream = torch.cuda.Stream(device=self.device)
+            cap = self._capture_stream
+            cur = torch.cuda.current_stream(self.device)
+            cap.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                stream = torch.cuda.current_stream(self.device).cuda_stream
-                for m in self._launch_sizes(k):
-                    self._launch_steps(stream, m)
+            with torch.cuda.stream(cap):
+                g.capture_begin()
+                try:
+                    for m in self._launch_sizes(k):
+                        self._launch_steps(cap.cuda_stream, m)
+                finally:
+                    g.capture_end()
+            cur.wait_stream(cap)
             self._graphs[key] = g
         return g
 
@@ -1104,10 +1196,8 @@ class AB3Engine:
                 k = min(max(self.graph_steps, k), left)
             if self.rebin_every > 0:
                 k = max(1, min(k, self.rebin_every - self._since_rebin))
-            pairs = set()
-            for _ in range(k):
-                pairs |= self._advance_host_clocks()
-                self.global_step += 1
+            pairs = self._advance_host_clocks_run(k)
+            self.global_step += k
             self._ensure_snapshots(pairs)
             self._ensure_windows(pairs)
             self._upload_window()

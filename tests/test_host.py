@@ -344,3 +344,43 @@ def test_reference_public_names_resolve(module):
     mod = importlib.import_module("offline_particles_amd" + ("." + module if module else ""))
     missing = [n for n in REFERENCE_PUBLIC[module] if not hasattr(mod, n)]
     assert not missing, (module, missing)
+
+
+@pytest.mark.parametrize("scheme", ["ab3", "rk2"])
+@pytest.mark.parametrize("k", [1, 7, 128])
+@pytest.mark.parametrize("uneven", [False, True])
+def test_host_clock_run_equals_stepwise(scheme, k, uneven):
+    """The engine's host clock mirror over a run of k steps at once
+    (_advance_host_clocks_run) against k calls of the per-step mirror: the
+    same snapshot pairs, times (bit for bit) and steps left."""
+    import types
+
+    from offline_particles_amd.engine import AB3Engine
+
+    rng = np.random.default_rng(k + 10 * uneven + (scheme == "rk2"))
+    n = 200
+    ta = (np.concatenate([[0.0], np.cumsum(rng.uniform(0.4, 2.5, n - 1) * 3600.0)]) if uneven
+          else np.arange(n, dtype=np.float64) * 3600.0)
+
+    def engine():
+        R = 300
+        e = types.SimpleNamespace(time_array=ta, dt=np.float64(120.0), alpha=2.0 / 3.0, scheme=scheme,
+                                  global_step=1000)
+        e._t = rng.uniform(ta[0], ta[-1], R)
+        e._t[:30] = ta[-1] - rng.uniform(0, 200 * 120.0, 30)  # leave the array within the run
+        e._t[30:40] = ta[rng.integers(0, n, 10)]
+        e._start = 1000 + rng.integers(-5, k + 5, R)
+        e._left = rng.integers(-1, 2 * k + 2, R)
+        e._time_index = lambda t: AB3Engine._time_index(e, t)
+        return e
+
+    a = engine()
+    b = types.SimpleNamespace(**{kk: (v.copy() if isinstance(v, np.ndarray) else v) for kk, v in vars(a).items()})
+    b._time_index = lambda t: AB3Engine._time_index(b, t)
+    want = set()
+    for _ in range(k):
+        want |= AB3Engine._advance_host_clocks(a)
+        a.global_step += 1
+    got = AB3Engine._advance_host_clocks_run(b, k)
+    assert got == want
+    assert np.array_equal(a._t, b._t) and np.array_equal(a._left, b._left)
