@@ -252,6 +252,9 @@ __device__ __forceinline__ Coord coord(const Axis &a)
 // FP64 weight of the upper node for base b and stagger offset o at index v:
 // v - b + o, exact for a decoded stored axis or a round32 value in the grid.
 __device__ __forceinline__ double w64(double v, int b, double o) { return (v - (double)b) + o; }
+// w64 of a rho-centred axis (offset 0): v - b (the add of +0.0 that w64 would
+// keep, for a -0.0 it cannot rule out, changes the sign of a zero weight only)
+__device__ __forceinline__ double w64(double v, int b) { return v - (double)b; }
 
 // The base of the 2-D corners at a position just stepped to: floor(v) clamped
 // to [0, N-2] in FP64, so that any finite v converts safely.
@@ -1078,7 +1081,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
     else
         load2_ring(hdd, win.zeta + rs.s0 * win.zeta_stride, win.zeta + rs.s1 * win.zeta_stride, NX, cy.b, cx.b, true,
                    c2);
-    const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
+    const double wy0 = w64(yi, cy.b), wx0 = w64(xi, cx.b);
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
 
@@ -1123,7 +1126,7 @@ __device__ __forceinline__ void ab3_body(int64_t i, const Pre &p, const opx_spli
                            bx1, false, c2);
         }
         double h1, zeta1;
-        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1, 0.0), w64(xn, bx1, 0.0), h1, zeta1);
+        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1), w64(xn, bx1), h1, zeta1);
         zn = round32(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b));
     }
 
@@ -1288,7 +1291,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     // weights and separately rounded terms (the rate magnifies its last bits)
     double w;
     if constexpr (ISO) {
-        const double rho = tri64_rn(vw, w64(zi, cz.b, 0.0), w64(yi, cy.b, 0.0), w64(xi, cx.b, 0.0));
+        const double rho = tri64_rn(vw, w64(zi, cz.b), w64(yi, cy.b), w64(xi, cx.b));
         const double drho = rho - rho_t;
         w = iso.w_max * drho / (iso.rho_scale + fabs(drho));
     } else {
@@ -1296,7 +1299,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
     }
     Corners2 &c2 = c.c2;
     sample2<LAYOUT, false>(win, j, rs.s0, rs.s1, NX, cy.b, cx.b, c2, c.s0_2d);
-    const double wy0 = w64(yi, cy.b, 0.0), wx0 = w64(xi, cx.b, 0.0);
+    const double wy0 = w64(yi, cy.b), wx0 = w64(xi, cx.b);
     const float dxv = bil32(c2.dx, (float)wy0, (float)wx0), dyv = bil32(c2.dy, (float)wy0, (float)wx0);
     const float dx0 = div_nr(u, dxv), dy0 = div_nr(v, dyv), dz0 = (float)w;
 
@@ -1341,7 +1344,7 @@ __device__ __forceinline__ bool ab3_step(PState &p, StencilCache<T, LAYOUT> &c, 
         const int by1 = base2(yn, NY), bx1 = base2(xn, NX);
         sample2<LAYOUT, false>(win, j1, rs.s0p, rs.s1p, NX, by1, bx1, c2, c.s0_2d);
         double h1, zeta1;
-        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1, 0.0), w64(xn, bx1, 0.0), h1, zeta1);
+        eval2(c2, rs.gt1, rs.ft1, w64(yn, by1), w64(xn, bx1), h1, zeta1);
         zn = round32(zidx_from_z(z, h1, zeta1, hc, NZ, Cw, ow.b));
     }
 
