@@ -78,9 +78,13 @@ enum opx_error {
  * then next[i] for every node: no value is stored twice, and one load brings
  * a stencil row's x-neighbours of both snapshots.  It suits particles sorted
  * by cell, whose neighbours in a wavefront share rows: the 4 copies of every
- * node in an OCT copy would then only add footprint.
+ * node in an OCT copy would then only add footprint.  OPX_PACK_HEX
+ * (time-dependent 3-D fields) holds the 8 corners (z, z+1) x (y, y+1) x
+ * (x, x+1) of both snapshots of every node, prev's then next's, corner
+ * (dz, dy, dx) at (dz * 2 + dy) * 2 + dx: 16 values, 64 bytes (FP32) or 32
+ * (BF16), so a trilinear stencil is one cache line.
  */
-enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2, OPX_PACK_TPAIR = 3 };
+enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2, OPX_PACK_TPAIR = 3, OPX_PACK_HEX = 4 };
 
 typedef struct opx_field {
     const void *prev;
@@ -313,7 +317,7 @@ int opx_pack_window(const opx_window *win, const opx_grid *g, int32_t layout, in
  * of `prev` when next is NULL, else OPX_PACK_OCT of (prev, next), for a
  * C-contiguous (nz, ny, nx) F32 or BF16 array (nz = 1 for a 2-D field; ny,
  * nx >= 2).  `out` is device memory of opx_packed_field_bytes bytes (QUAD:
- * 4 values per node, OCT: 8, TPAIR: 2).  No counterpart in the reference: its kernels
+ * 4 values per node, OCT: 8, TPAIR: 2, HEX: 16).  No counterpart in the reference: its kernels
  * read the blended FP64 array (fields.py:394-430); the values here are the
  * same, stored so that each stencil row is not its own cache line. */
 int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, int64_t nx, uint64_t *bytes);
@@ -321,8 +325,9 @@ int opx_pack_field(const void *prev, const void *next, int32_t dtype, int64_t nz
                    void *stream);
 
 /* opx_pack_field with the kind named: OPX_PACK_QUAD (next NULL),
- * OPX_PACK_OCT or OPX_PACK_TPAIR (next given; TPAIR: 2 values per node).
- * No counterpart in the reference, as opx_pack_field. */
+ * OPX_PACK_OCT, OPX_PACK_TPAIR or OPX_PACK_HEX (next given; TPAIR: 2 values
+ * per node, HEX: 16, nz >= 2).  No counterpart in the reference, as
+ * opx_pack_field. */
 int opx_pack_field_as(const void *prev, const void *next, int32_t dtype, int32_t kind, int64_t nz, int64_t ny,
                       int64_t nx, void *out, void *stream);
 

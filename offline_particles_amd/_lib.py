@@ -36,7 +36,7 @@ class OpxInvalidArgument(OpxError, ValueError):
 _vp = ctypes.c_void_p
 
 
-OPX_PACK_NONE, OPX_PACK_QUAD, OPX_PACK_OCT, OPX_PACK_TPAIR = 0, 1, 2, 3
+OPX_PACK_NONE, OPX_PACK_QUAD, OPX_PACK_OCT, OPX_PACK_TPAIR, OPX_PACK_HEX = 0, 1, 2, 3, 4
 
 
 class opx_field(ctypes.Structure):

@@ -1876,6 +1876,39 @@ __global__ void k_pack_quad(const T *__restrict__ p, void *__restrict__ out, uin
     }
 }
 
+// The HEX copy of a snapshot pair for the drop-in interpolants: node i holds
+// the corners (z, z+1) x (y, y+1) x (x, x+1) of p then of q, corner (dz, dy,
+// dx) at (dz * 2 + dy) * 2 + dx; neighbours beyond the last level / row /
+// column repeat the node (never corners of a base, which the interpolants
+// clamp to size - 2).
+template <typename T>
+__global__ void k_pack_hex(const T *__restrict__ p, const T *__restrict__ q, void *__restrict__ out, uint32_t n, int n0,
+                           int n1, int n2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t plane = (uint32_t)n1 * (uint32_t)n2;
+    const uint32_t x1 = (i % (uint32_t)n2) + 1 < (uint32_t)n2 ? 1u : 0u;
+    const uint32_t y1 = ((i / (uint32_t)n2) % (uint32_t)n1) + 1 < (uint32_t)n1 ? (uint32_t)n2 : 0u;
+    const uint32_t z1 = i / plane + 1 < (uint32_t)n0 ? plane : 0u;
+    const uint32_t c[8] = {i, i + x1, i + y1, i + y1 + x1, i + z1, i + z1 + x1, i + z1 + y1, i + z1 + y1 + x1};
+    if constexpr (sizeof(T) == 4) {
+        const float *P = (const float *)p, *Q = (const float *)q;
+        float4 *o = static_cast<float4 *>(out) + 4 * (size_t)i;
+        o[0] = make_float4(P[c[0]], P[c[1]], P[c[2]], P[c[3]]);
+        o[1] = make_float4(P[c[4]], P[c[5]], P[c[6]], P[c[7]]);
+        o[2] = make_float4(Q[c[0]], Q[c[1]], Q[c[2]], Q[c[3]]);
+        o[3] = make_float4(Q[c[4]], Q[c[5]], Q[c[6]], Q[c[7]]);
+    } else {
+        const uint16_t *P = (const uint16_t *)p, *Q = (const uint16_t *)q;
+        auto two = [](uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); };
+        uint4 *o = static_cast<uint4 *>(out) + 2 * (size_t)i;
+        o[0] = make_uint4(two(P[c[0]], P[c[1]]), two(P[c[2]], P[c[3]]), two(P[c[4]], P[c[5]]), two(P[c[6]], P[c[7]]));
+        o[1] = make_uint4(two(Q[c[0]], Q[c[1]]), two(Q[c[2]], Q[c[3]]), two(Q[c[4]], Q[c[5]]), two(Q[c[6]], Q[c[7]]));
+    }
+}
+
 // ---------------------------------------------------------------- sparse snapshot streaming
 
 __global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
@@ -3086,10 +3119,12 @@ int opx_packed_field_bytes(int32_t dtype, int32_t kind, int64_t nz, int64_t ny, 
 {
     OPX_REQUIRE(bytes != nullptr, "null output");
     OPX_REQUIRE(dtype == OPX_F32 || dtype == OPX_BF16, "packed stencils need F32 or BF16, not dtype %d", dtype);
-    OPX_REQUIRE(kind == OPX_PACK_QUAD || kind == OPX_PACK_OCT || kind == OPX_PACK_TPAIR, "pack kind %d", kind);
+    OPX_REQUIRE(kind == OPX_PACK_QUAD || kind == OPX_PACK_OCT || kind == OPX_PACK_TPAIR || kind == OPX_PACK_HEX,
+                "pack kind %d", kind);
     OPX_REQUIRE(nz >= 1 && ny >= 2 && nx >= 2, "array shape (%lld, %lld, %lld)", (long long)nz, (long long)ny,
                 (long long)nx);
-    const int per = kind == OPX_PACK_QUAD ? 4 : kind == OPX_PACK_OCT ? 8 : 2;
+    OPX_REQUIRE(kind != OPX_PACK_HEX || nz >= 2, "a HEX copy needs a 3-D field (nz = %lld)", (long long)nz);
+    const int per = kind == OPX_PACK_QUAD ? 4 : kind == OPX_PACK_OCT ? 8 : kind == OPX_PACK_HEX ? 16 : 2;
     *bytes = (uint64_t)nz * ny * nx * per * (dtype == OPX_F32 ? 4 : 2);
     return OPX_OK;
 }
@@ -3113,7 +3148,15 @@ int opx_pack_field_as(const void *prev, const void *next, int32_t dtype, int32_t
     OPX_REQUIRE(n < (int64_t(1) << 32), "field of %lld nodes", (long long)n);
     hipStream_t st = as_stream(stream);
     const unsigned grid = grid_for(n);
-    if (kind == OPX_PACK_TPAIR) {
+    if (kind == OPX_PACK_HEX) {
+        if (dtype == OPX_F32)
+            hipLaunchKernelGGL(k_pack_hex<float>, dim3(grid), dim3(kBlock), 0, st, static_cast<const float *>(prev),
+                               static_cast<const float *>(next), out, (uint32_t)n, (int)nz, (int)ny, (int)nx);
+        else
+            hipLaunchKernelGGL(k_pack_hex<uint16_t>, dim3(grid), dim3(kBlock), 0, st,
+                               static_cast<const uint16_t *>(prev), static_cast<const uint16_t *>(next), out,
+                               (uint32_t)n, (int)nz, (int)ny, (int)nx);
+    } else if (kind == OPX_PACK_TPAIR) {
         // the fused windows' PAIR layout: (prev[i], next[i]) per node
         if (dtype == OPX_F32)
             hipLaunchKernelGGL((k_pack<float, OPX_LAYOUT_PAIR>), dim3(grid), dim3(kBlock), 0, st,

@@ -201,6 +201,35 @@ __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node,
     }
 }
 
+// The 8 corners (dz, dy, dx) of `node` from a field's HEX copy (prev's then
+// next's in one 64-byte FP32 / 32-byte BF16 element), each blended and
+// widened exactly as load_blend does; corner (dz, dy, dx) at (dz*2+dy)*2+dx.
+__device__ __forceinline__ void hex_corners(const opx_field &f, int64_t node, double v[8])
+{
+    const float gt = (float)f.gt, ft = (float)f.ft;
+    if (f.dtype == OPX_F32) {
+        const float4 *e = static_cast<const float4 *>(f.packed) + 4 * node;
+        const float4 p0 = e[0], p1 = e[1], q0 = e[2], q1 = e[3];
+        const float P[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+        const float Q[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        for (int c = 0; c < 8; ++c) {
+            float a = P[c] * gt, b = Q[c] * ft;
+            v[c] = (double)(a + b);
+        }
+    } else {
+        const uint4 *e = static_cast<const uint4 *>(f.packed) + 2 * node;
+        const uint4 pe = e[0], qe = e[1];
+        const uint32_t P[4] = {pe.x, pe.y, pe.z, pe.w}, Q[4] = {qe.x, qe.y, qe.z, qe.w};
+        for (int c = 0; c < 8; ++c) {
+            const uint16_t pb = (uint16_t)(c & 1 ? P[c >> 1] >> 16 : P[c >> 1] & 0xffffu);
+            const uint16_t qb = (uint16_t)(c & 1 ? Q[c >> 1] >> 16 : Q[c >> 1] & 0xffffu);
+            float a = bf16_bits_to_f32(pb) * gt;
+            float b = bf16_bits_to_f32(qb) * ft;
+            v[c] = (double)(a + b);
+        }
+    }
+}
+
 // linear_interpolation (linear.pxd:11-17) on a 1-D field at rho index idx.
 __device__ __forceinline__ double linear_f64(const opx_field &f, double idx)
 {
@@ -262,7 +291,18 @@ __device__ __forceinline__ double trilinear_f64(const opx_field &f, double z, do
     int64_t b10 = b00 + n1 * n2;
     int64_t b11 = b10 + n2;
     double v000, v001, v010, v011, v100, v101, v110, v111;
-    if (f.packed && f.packed_kind != OPX_PACK_TPAIR) {
+    if (f.packed && f.packed_kind == OPX_PACK_HEX) {
+        double c[8];
+        hex_corners(f, b00, c);
+        v000 = c[0];
+        v001 = c[1];
+        v010 = c[2];
+        v011 = c[3];
+        v100 = c[4];
+        v101 = c[5];
+        v110 = c[6];
+        v111 = c[7];
+    } else if (f.packed && f.packed_kind != OPX_PACK_TPAIR) {
         double c0[4], c1[4];
         packed_corners(f, b00, c0);
         packed_corners(f, b10, c1);

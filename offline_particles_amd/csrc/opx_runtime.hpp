@@ -57,7 +57,8 @@ inline int check_field(const opx_field &f, int ndim, const char *name, bool allo
         OPX_REQUIRE(f.dtype == OPX_F32 || f.dtype == OPX_BF16, "field '%s': packed stencils need F32 or BF16", name);
         OPX_REQUIRE(ndim >= 2, "field '%s': packed stencils need 2 or 3 dimensions", name);
         OPX_REQUIRE(f.next == nullptr ? f.packed_kind == OPX_PACK_QUAD
-                                      : (f.packed_kind == OPX_PACK_OCT || f.packed_kind == OPX_PACK_TPAIR),
+                                      : (f.packed_kind == OPX_PACK_OCT || f.packed_kind == OPX_PACK_TPAIR ||
+                                         (f.packed_kind == OPX_PACK_HEX && ndim == 3)),
                     "field '%s': packed kind %d does not match a %s field", name, f.packed_kind,
                     f.next == nullptr ? "static" : "time-dependent");
     }

@@ -37,19 +37,21 @@ class FieldData(_FieldDataTuple):
 
     ``packed`` (an attribute, not a tuple item, so ``array, offsets = fd``
     still works) is None or ``(tensor, kind)``: a copy of the same values in
-    which one load brings a node's stencil corners in (y, x)
-    (``opx_pack_field_as``; OPX_PACK_QUAD for a static field, OPX_PACK_OCT for
-    a snapshot pair) or, inside ``sorted_stencils``, a stencil row's
-    x-neighbours of both snapshots (OPX_PACK_TPAIR for a snapshot pair).  The
-    kernels read it when present; the values they interpolate are unchanged."""
+    which one load brings a node's stencil corners (``opx_pack_field_as``;
+    OPX_PACK_QUAD for a static field: the (y, x) corners; OPX_PACK_HEX for a
+    3-D snapshot pair: all 8 corners of both snapshots; OPX_PACK_OCT for a 2-D
+    one: the (y, x) corners of both) or, inside ``sorted_stencils``, a stencil
+    row's x-neighbours of both snapshots (OPX_PACK_TPAIR).  The kernels read
+    it when present; the values they interpolate are unchanged."""
 
     packed = None
 
 
 # Packed stencil copies for the drop-in kernels (F32 and BF16 device fields of
-# 2 or 3 dimensions): a 3-D stencil then touches 2 cache lines per field
-# instead of 8.  They cost 4 (static) or 8 (snapshot pair) values per node of
-# device memory; False turns them off.
+# 2 or 3 dimensions): a 3-D stencil of a snapshot pair then touches 1 cache
+# line (HEX) instead of 8, a 2-D one 1 instead of 2.  They cost 16 (3-D
+# pair), 8 (2-D pair) or 4 (static) values per node of device memory; False
+# turns them off.
 PACK_STENCILS = True
 
 
@@ -97,10 +99,22 @@ def _packable(t) -> bool:
             and t.dtype in (torch.float32, torch.bfloat16) and t.is_contiguous() and min(t.shape[-2:]) >= 2)
 
 
-def _pair_kind() -> int:
+# Largest HEX copy made of one snapshot pair (16 values per node); larger
+# 3-D fields get the OCT copy (8 values per node).
+HEX_MAX_BYTES = 16 << 30
+
+
+def _pair_kind(t: torch.Tensor) -> int:
+    """The packed kind of a snapshot pair like `t`: TPAIR for sorted particles,
+    else HEX for a 3-D field (a trilinear stencil in one line) within
+    HEX_MAX_BYTES, else OCT."""
     from . import _lib
 
-    return _lib.OPX_PACK_TPAIR if _SORTED else _lib.OPX_PACK_OCT
+    if _SORTED:
+        return _lib.OPX_PACK_TPAIR
+    if t.dim() == 3 and t.shape[0] >= 2 and t.numel() * 16 * t.element_size() <= HEX_MAX_BYTES:
+        return _lib.OPX_PACK_HEX
+    return _lib.OPX_PACK_OCT
 
 
 def _pack(prev: torch.Tensor, next_: torch.Tensor | None, kind: int | None = None) -> tuple:
@@ -502,7 +516,7 @@ class TimeDependentField(Field):
             # out a freed snapshot's address for another one), the pointers the storage
             K = self._num_timesteps
             idx = (It % K, (It + 1) % K) if self._cyclic else (It, It + 1)
-            kind = _pair_kind()
+            kind = _pair_kind(prev)
             key = idx + (prev.data_ptr(), nxt.data_ptr(), tuple(prev.shape), prev.dtype, kind) + versions
             cache = self.__dict__.setdefault("_packed_pairs", collections.OrderedDict())
             packed = cache.get(key)

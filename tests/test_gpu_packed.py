@@ -3,8 +3,9 @@
 Gate: the drop-in gives the same bits with the packed copies as without
 (they hold the same values; the interpolants blend them with the same
 arithmetic), for AB3, RK2 and AB3 with an RK2 start-up, FP32 and BF16
-storage, spatially ordered (TPAIR copies of the snapshot pairs) or not (OCT
-copies); and opx_pack_field_as's layouts equal the values numpy picks.  The
+storage, spatially ordered (TPAIR copies of the snapshot pairs) or not (HEX
+copies of 3-D pairs, OCT of 2-D ones); and opx_pack_field_as's layouts equal
+the values numpy picks.  The
 parity tests against the oracle run with the packed copies on (the default).
 """
 
@@ -44,14 +45,46 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
             if pack:
                 # the kinds the steps packed: TPAIR when the particles are sorted
                 kinds = {k[-1] for k in sim.fieldset["u"]._packed_pairs}
+                assert kinds == {_lib.OPX_PACK_TPAIR if order else _lib.OPX_PACK_HEX}, kinds
+                kinds = {k[-1] for k in sim.fieldset["zeta"]._packed_pairs}
                 assert kinds == {_lib.OPX_PACK_TPAIR if order else _lib.OPX_PACK_OCT}, kinds
                 fd = sim.fieldset["u"].get_field_data(0.25, None)
-                assert fd.packed is not None and fd.packed[1] == _lib.OPX_PACK_OCT
+                assert fd.packed is not None and fd.packed[1] == _lib.OPX_PACK_HEX
                 assert sim.fieldset["h"].get_field_data(0.25, None).packed[1] == _lib.OPX_PACK_QUAD
     finally:
         fields.PACK_STENCILS = True
     for k, a in res[False].items():
         assert np.array_equal(a, res[True][k], equal_nan=True), (scheme, storage, k)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape", [(5, 7, 9), (2, 3, 2)])
+def test_pack_hex_matches_numpy(device, dtype, shape):
+    rng = np.random.default_rng(5)
+    nz, ny, nx = shape
+    p = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    q = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(device, dtype)
+    code = _lib.storage_code(dtype)
+    nbytes = ctypes.c_uint64()
+    _lib.call("opx_packed_field_bytes", code, _lib.OPX_PACK_HEX, nz, ny, nx, ctypes.byref(nbytes))
+    assert nbytes.value == nz * ny * nx * 16 * p.element_size()
+    out = torch.zeros(nbytes.value, dtype=torch.uint8, device=device)
+    _lib.call("opx_pack_field_as", p.data_ptr(), q.data_ptr(), code, _lib.OPX_PACK_HEX, nz, ny, nx, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    raw = {2: np.uint16, 4: np.uint32}[p.element_size()]
+    view = torch.int16 if dtype == torch.bfloat16 else torch.int32
+    P, Q = (t.cpu().view(view).numpy().view(raw) for t in (p, q))
+    got = out.cpu().numpy().view(raw).reshape(nz, ny, nx, 16)
+    # corner (dz, dy, dx) at (dz * 2 + dy) * 2 + dx, prev's then next's; the last
+    # level / row / column repeats the node
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    z1, y1, x1 = np.minimum(zz + 1, nz - 1), np.minimum(yy + 1, ny - 1), np.minimum(xx + 1, nx - 1)
+    corners = [(a, b, c) for a in (zz, z1) for b in (yy, y1) for c in (xx, x1)]
+    want = np.stack([A[a, b, c] for A in (P, Q) for a, b, c in corners], axis=-1)
+    assert np.array_equal(got, want)
+    # a 2-D field has no HEX copy
+    with pytest.raises(RuntimeError):
+        _lib.call("opx_packed_field_bytes", code, _lib.OPX_PACK_HEX, 1, ny, nx, ctypes.byref(nbytes))
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
