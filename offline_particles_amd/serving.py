@@ -252,6 +252,11 @@ class ReleaseServer:
             "particle_steps": int(work),
             "seconds": elapsed,
             "particle_steps_per_s": work / elapsed if elapsed > 0 else 0.0,
+            # the work the requests ask for per second of their arrival span: no
+            # server delivers more than this over the span (realtime mode)
+            "offered_particle_steps_per_s": (work / max(r.arrival for r in done)
+                                             if realtime and done and max(r.arrival for r in done) > 0 else None),
+            "last_arrival_s": float(max(r.arrival for r in done)) if realtime and done else None,
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_max_s": float(lat.max()),
