@@ -660,9 +660,10 @@ def run_serving(device, seconds, rate=20.0, grid=(2048, 1536, 60), nsnap=4, max_
     days = 10
     if wl is None:
         # snapshots cycle, so the time array can cover any service clock: room for
-        # the longest request after arrivals over `seconds` at up to 10,000 steps/s
+        # the longest request after arrivals over `seconds` at up to 40,000 steps/s
+        # (the service clock runs fastest while few particles are live)
         wl = turbulent_workload(NX=NX, NY=NY, NZ=NZ, nparticles=1,
-                                steps=int(days * 86400 / 120) + int(10_000 * seconds) + 40,
+                                steps=int(days * 86400 / 120) + int(40_000 * seconds) + 40,
                                 nsnap_unique=nsnap, dtype=np.float32)
     if reqs is None:
         reqs = poisson_requests(wl, rate=rate, duration=seconds, size_range=(1_000, max_size),

@@ -43,11 +43,18 @@ padded bounding box (``ChunkedDaskArray.get_data_subset``,
 spatial_arrays.py:260-295, with ``Launcher.construct_bbox``,
 launcher.py:122-148); here the region is the set of tiles holding particles,
 widened by a bound on how far they move, so separate clouds of particles do
-not pull in the ocean between them.  The bound: every step moves a particle
-by at most dt x (44/12 for AB3, max(alpha, |b1| + |b2|) for RK2) x R cells,
-R the largest |u| / dx or |v| / dy of any snapshot loaded so far (the
-interpolation weights of a particle inside the grid are convex, so the
-interpolated rate never exceeds R).  A tile map of the particles is computed
+not pull in the ocean between them.  The bound: m steps read the fields
+within dt x R x (m + 4) cells of where a particle was before them for AB3
+and within dt x R x (m s + |alpha| + s), s = |b1| + |b2|, for RK2, R the
+largest |u| / dx or |v| / dy of any snapshot loaded so far (the
+interpolation weights of a particle inside the grid are convex, so every
+interpolated rate is at most R).  For AB3 the m steps' increments
+dt (23 f[n] - 16 f[n-1] + 5 f[n-2]) / 12 sum to dt times a combination of the
+rates in which each rate inside the run has weight 1 (23 - 16 + 5 = 12) and
+the run's ends weights 23/12, 7/12, 11/12 and 5/12, so the total weight is
+m + 22/12 for m >= 2 and 44/12 for one step (the Euler and AB2 start-up steps
+weigh less); RK2 moves at most s dt R per step and its stage reads alpha dt R
+further.  A tile map of the particles is computed
 on the device after every run of steps and read back asynchronously; before
 each run of steps every slot it reads must hold the tiles of the latest map
 read back (and of releases added since), widened by the bound times the
@@ -107,6 +114,17 @@ def dilate_tiles(mask: np.ndarray, d: int) -> np.ndarray:
     return out
 
 
+def run_reach(scheme: str, alpha: float = 2 / 3):
+    """(s, c): m steps of `scheme` read the fields within dt x R x (m s + c) cells
+    (horizontally) of a particle's position before them, R the largest |u| / dx,
+    |v| / dy (the module docstring derives the constants)."""
+    if scheme == "rk2":
+        b1 = 1.0 / (2.0 * alpha)
+        s = abs(b1) + abs(1.0 - b1)
+        return s, abs(alpha) + s
+    return 1.0, 4.0
+
+
 def tile_boxes(mask: np.ndarray, tile: int, ny: int, nx: int, gap: int = 0):
     """Boxes covering the set tiles of `mask` (tile rows x tile columns of `tile`
     cells over an ny x nx grid) for rectangular copies.
@@ -114,39 +132,50 @@ def tile_boxes(mask: np.ndarray, tile: int, ny: int, nx: int, gap: int = 0):
     In each tile row, runs of set tiles separated by at most `gap` unset tiles
     form one run (longer rows copy faster); consecutive tile rows with the same
     run are stacked into one box.  Returns (boxes, covered): int32 (k, 4) boxes
-    (y0, y1, x0, x1) in cells, clipped to the grid, and the bool mask of the
-    tiles they cover (the set tiles and the bridged gaps)."""
+    (y0, y1, x0, x1) in cells, clipped to the grid, in ascending order, and the
+    bool mask of the tiles they cover (the set tiles and the bridged gaps).
+
+    Array operations throughout (no loop over rows or runs): the streaming
+    engine calls this for every snapshot load, top-up and window pack."""
     mask = np.asarray(mask, dtype=bool)
     nty, ntx = mask.shape
-    covered = np.zeros_like(mask)
-    rows = []
-    for ty in range(nty):
-        idx = np.flatnonzero(mask[ty])
-        runs = []
-        if idx.size:
-            start = prev = int(idx[0])
-            for t in idx[1:]:
-                t = int(t)
-                if t - prev - 1 <= gap:
-                    prev = t
-                else:
-                    runs.append((start, prev + 1))
-                    start = prev = t
-            runs.append((start, prev + 1))
-        for a, b in runs:
-            covered[ty, a:b] = True
-        rows.append(runs)
-    boxes = []
-    open_runs: dict = {}
-    for ty in range(nty + 1):
-        cur = set(rows[ty]) if ty < nty else set()
-        for run in [r for r in open_runs if r not in cur]:
-            ty0 = open_runs.pop(run)
-            boxes.append((ty0 * tile, min(ty * tile, ny), run[0] * tile, min(run[1] * tile, nx)))
-        for run in cur:
-            open_runs.setdefault(run, ty)
-    out = np.array(sorted(boxes), dtype=np.int32).reshape(-1, 4)
-    return out, covered
+    edge = np.zeros((nty, 1), dtype=np.int8)
+    d = np.diff(np.concatenate([edge, mask.view(np.int8), edge], axis=1), axis=1).ravel()
+    # runs of set tiles, row-major (each row's in order), and their ends (exclusive)
+    ty, a = np.divmod(np.flatnonzero(d == 1), ntx + 1)
+    b = np.flatnonzero(d == -1) % (ntx + 1)
+    if not ty.size:
+        return np.zeros((0, 4), dtype=np.int32), mask.copy()
+    covered = mask
+    if gap > 0:
+        # a run joins the one before it in its row when at most `gap` unset tiles
+        # lie between them
+        join = np.zeros(ty.size, dtype=bool)
+        join[1:] = (ty[1:] == ty[:-1]) & (a[1:] - b[:-1] <= gap)
+        if join.any():
+            keep = ~join
+            end = np.ones(ty.size, dtype=bool)
+            end[:-1] = keep[1:]
+            ty, a, b = ty[keep], a[keep], b[end]
+            marks = np.zeros((nty, ntx + 1), dtype=np.int8)
+            marks[ty, a] = 1
+            marks[ty, b] = -1
+            covered = np.cumsum(marks, axis=1, dtype=np.int8)[:, :ntx] > 0
+    covered = covered.copy() if covered is mask else covered
+    # a box is a run repeated in consecutive rows: group equal runs by row
+    key = a.astype(np.int64) * (ntx + 1) + b
+    o = np.lexsort((ty, key))
+    ks, ts = key[o], ty[o]
+    first = np.ones(ks.size, dtype=bool)
+    first[1:] = (ks[1:] != ks[:-1]) | (ts[1:] != ts[:-1] + 1)
+    last = np.ones(ks.size, dtype=bool)
+    last[:-1] = first[1:]
+    y0 = ts[first].astype(np.int64) * tile
+    y1 = np.minimum((ts[last].astype(np.int64) + 1) * tile, ny)
+    x0 = a[o][first].astype(np.int64) * tile
+    x1 = np.minimum(b[o][first].astype(np.int64) * tile, nx)
+    out = np.stack([y0, y1, x0, x1], axis=1)
+    return out[np.lexsort((x1, x0, y1, y0))].astype(np.int32), covered
 
 
 def _bits(n: int) -> int:
@@ -826,12 +855,7 @@ class AB3Engine:
         if not (dxm > 0 and dym > 0):
             raise ValueError("stream_tiles needs positive grid spacings dx, dy")
         self._inv_dx, self._inv_dy = 1.0 / dxm, 1.0 / dym
-        # largest displacement of one step in units of dt x R (see the module docstring)
-        if self.scheme == "rk2":
-            b1 = 1.0 / (2.0 * self.alpha)
-            self._reach = max(abs(self.alpha), abs(b1) + abs(1.0 - b1))
-        else:
-            self._reach = 44.0 / 12.0
+        self._reach, self._reach0 = run_reach(self.scheme, self.alpha)
         self._snap_rate: dict = {}
         self._rate = 0.0
         self._occ_dev = torch.zeros(self._nty * self._ntx, dtype=torch.uint8, device=self.device)
@@ -905,10 +929,11 @@ class AB3Engine:
 
     def _tile_region(self, horizon: int) -> np.ndarray:
         """Tiles a particle can read up to step `horizon`: the tiles of the latest map
-        widened by the steps since it times the bound on one step's displacement,
-        plus two cells of stencil reach."""
+        widened by the bound on the reach of the steps since it, plus two cells of
+        stencil reach."""
         S, occ = self._occupancy(horizon)
-        r = max(0, horizon - S) * abs(self.dt) * self._rate * self._reach + 2.0
+        m = max(0, horizon - S)
+        r = (m * self._reach + self._reach0) * abs(self.dt) * self._rate + 2.0
         if not math.isfinite(r) or r >= self._tile * max(self._nty, self._ntx):
             return np.ones_like(occ)
         return dilate_tiles(occ, int(math.ceil(r / self._tile)))

@@ -53,6 +53,65 @@ def test_tile_boxes_cover_exactly(seed, gap):
         assert (tx + 1 + right[0]) - left[-1] - 1 <= gap
 
 
+def _loop_tile_boxes(mask, tile, ny, nx, gap):
+    """Row by row: the runs of each tile row (bridging gaps of at most `gap`), then
+    each run stacked over the consecutive rows that hold the same run."""
+    runs_of = []
+    covered = np.zeros_like(mask)
+    for row in mask:
+        runs, j = [], 0
+        while j < row.size:
+            if not row[j]:
+                j += 1
+                continue
+            k = j
+            while k < row.size and row[k]:
+                k += 1
+            if runs and j - runs[-1][1] <= gap:
+                runs[-1] = (runs[-1][0], k)
+            else:
+                runs.append((j, k))
+            j = k
+        runs_of.append(runs)
+    boxes = []
+    for ty, runs in enumerate(runs_of):
+        for a, b in runs:
+            covered[ty, a:b] = True
+            if ty and (a, b) in runs_of[ty - 1]:
+                continue  # part of the box opened in an earlier row
+            end = ty + 1
+            while end < len(runs_of) and (a, b) in runs_of[end]:
+                end += 1
+            boxes.append((ty * tile, min(end * tile, ny), a * tile, min(b * tile, nx)))
+    return np.array(sorted(boxes), dtype=np.int32).reshape(-1, 4), covered
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("gap", [0, 1, 2, 4])
+def test_tile_boxes_equal_row_by_row_construction(seed, gap):
+    """The array form gives the row-by-row construction's boxes, in the same order,
+    and the same covered mask, on random masks and on clustered ones (patches of
+    tiles, as the particles of a request occupy them)."""
+    rng = np.random.default_rng(100 + seed)
+    nty, ntx = int(rng.integers(1, 60)), int(rng.integers(1, 70))
+    if seed % 2:
+        mask = rng.random((nty, ntx)) < rng.uniform(0.02, 0.7)
+    else:
+        mask = np.zeros((nty, ntx), bool)
+        for _ in range(int(rng.integers(0, 20))):
+            y, x = int(rng.integers(0, nty)), int(rng.integers(0, ntx))
+            mask[y:y + int(rng.integers(1, 5)), x:x + int(rng.integers(1, 5))] = True
+    T = int(rng.integers(2, 33))
+    ny = int(rng.integers((nty - 1) * T + 1, nty * T + 1))
+    nx = int(rng.integers((ntx - 1) * T + 1, ntx * T + 1))
+    boxes, covered = tile_boxes(mask, T, ny, nx, gap)
+    want_boxes, want_covered = _loop_tile_boxes(mask, T, ny, nx, gap)
+    assert boxes.dtype == np.int32
+    assert np.array_equal(boxes, want_boxes)
+    assert np.array_equal(covered, want_covered)
+    assert covered is not mask
+
+
 def test_tile_boxes_stack_rows_and_clip():
     m = np.zeros((5, 6), bool)
     m[1, 1] = m[2, 1] = True  # the same run in two rows: one box
@@ -95,3 +154,46 @@ def test_step_displacement_bound_holds_on_oracle_trajectories(oracle_lib, scheme
         worst = max(worst, float(d.max(initial=0.0)))
     assert worst > 0.05 * wl.dt * R  # the flow moves the particles
     assert worst <= wl.dt * reach * R, (worst, wl.dt * reach * R)
+
+
+@pytest.mark.parametrize("scheme", ["ab3", "rk2"])
+def test_run_displacement_bound_holds_on_oracle_trajectories(oracle_lib, scheme):
+    """The bound the tiled streaming widens the tile map by: m steps move a particle
+    at most dt x R x (m + 4) cells horizontally from where it was for AB3 (with its
+    Euler and AB2 start-up) and dt x R x (m s + alpha + s) for RK2 (s = |b1| + |b2| =
+    1 for alpha = 2/3), R the largest |u| / dx, |v| / dy of the snapshots.  Checked
+    over every run of m = 1..40 steps of FP64 oracle trajectories, and the engine
+    widens by exactly these constants."""
+    from oracle import OracleSimulation
+
+    from offline_particles_amd.workloads import turbulent_workload
+
+    from ._helpers import oracle_fields
+
+    wl = turbulent_workload(NX=48, NY=40, NZ=8, nparticles=2000, steps=80, nsnap_unique=3, dtype=np.float64)
+    f = wl.fields
+    R = max(np.abs(f["u"]).max() / f["dx"].min(), np.abs(f["v"]).max() / f["dy"].min())
+    sim = OracleSimulation(oracle_fields(wl), wl.time_array, wl.dt, scheme=scheme, nparticles=2000, hc=wl.hc,
+                           NZ=wl.NZ, bounds=wl.bounds)
+    for k, a in zip(("zidx", "yidx", "xidx"), wl.positions):
+        sim.p[k][:] = a
+    if scheme == "ab3":
+        sim.p["status"][:] = 10  # MULTISTEP_1: Euler, AB2, then AB3
+    ys, xs = [sim.p["yidx"].copy()], [sim.p["xidx"].copy()]
+    for _ in range(40):
+        sim.step()
+        ys.append(sim.p["yidx"].copy())
+        xs.append(sim.p["xidx"].copy())
+    live = (sim.p["status"] & 128) == 0
+    ys, xs = np.array(ys)[:, live], np.array(xs)[:, live]
+    s, c = (1.0, 4.0) if scheme == "ab3" else (1.0, 2.0 / 3.0 + 1.0)
+    worst = 0.0
+    for m in range(1, 41):
+        d = np.maximum(np.abs(ys[m:] - ys[:-m]), np.abs(xs[m:] - xs[:-m])).max()
+        assert d <= wl.dt * R * (m * s + c), (m, d, wl.dt * R * (m * s + c))
+        worst = max(worst, d / (wl.dt * R * (m * s + c)))
+    assert worst > 0.05  # the flow moves the particles
+
+    from offline_particles_amd.engine import run_reach
+
+    assert np.allclose(run_reach(scheme, 2.0 / 3.0), (s, c))
