@@ -46,7 +46,7 @@ class opx_field(ctypes.Structure):
         ("shape", ctypes.c_int64 * 3),
         ("offsets", ctypes.c_double * 3),
         ("ft", ctypes.c_double), ("gt", ctypes.c_double),
-        ("packed", _vp), ("packed_kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("packed", _vp), ("packed_kind", ctypes.c_int32), ("packed_stride", ctypes.c_int32),
     ]
 
 

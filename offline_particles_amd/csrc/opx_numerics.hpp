@@ -156,13 +156,15 @@ __device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, doubl
 // The stencil corners (y, x), (y, x+1), (y+1, x), (y+1, x+1) of `node` from a
 // field's QUAD or OCT copy (opx_field.packed), blended and widened exactly as
 // load_blend does: one 16-byte (QUAD F32, OCT BF16), 32-byte (OCT F32) or
-// 8-byte (QUAD BF16) load instead of two rows per snapshot.
+// 8-byte (QUAD BF16) load instead of two rows per snapshot.  QUAD elements
+// sit opx_field.packed_stride bytes apart (0: packed back to back).
 __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node, double v[4])
 {
     if (f.dtype == OPX_F32) {
         const float4 *e = static_cast<const float4 *>(f.packed);
         if (f.packed_kind == OPX_PACK_QUAD) {
-            const float4 p = e[node];
+            const int64_t stride = f.packed_stride ? f.packed_stride : 16;
+            const float4 p = *reinterpret_cast<const float4 *>(static_cast<const char *>(f.packed) + node * stride);
             v[0] = (double)p.x;
             v[1] = (double)p.y;
             v[2] = (double)p.z;
@@ -181,7 +183,8 @@ __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node,
         }
     } else {
         if (f.packed_kind == OPX_PACK_QUAD) {
-            const uint2 p = static_cast<const uint2 *>(f.packed)[node];
+            const int64_t stride = f.packed_stride ? f.packed_stride : 8;
+            const uint2 p = *reinterpret_cast<const uint2 *>(static_cast<const char *>(f.packed) + node * stride);
             v[0] = (double)bf16_bits_to_f32((uint16_t)(p.x & 0xffffu));
             v[1] = (double)bf16_bits_to_f32((uint16_t)(p.x >> 16));
             v[2] = (double)bf16_bits_to_f32((uint16_t)(p.y & 0xffffu));

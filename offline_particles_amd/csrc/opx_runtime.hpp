@@ -61,6 +61,15 @@ inline int check_field(const opx_field &f, int ndim, const char *name, bool allo
                                          (f.packed_kind == OPX_PACK_HEX && ndim == 3)),
                     "field '%s': packed kind %d does not match a %s field", name, f.packed_kind,
                     f.next == nullptr ? "static" : "time-dependent");
+        const int quad = f.dtype == OPX_F32 ? 16 : 8;
+        OPX_REQUIRE(f.packed_stride == 0 || (f.packed_kind == OPX_PACK_QUAD && f.packed_stride % quad == 0 &&
+                                             f.packed_stride <= 256),
+                    "field '%s': packed_stride %d is not 0 or a QUAD stride (a multiple of %d up to 256)", name,
+                    f.packed_stride, quad);
+        OPX_REQUIRE(f.packed_kind != OPX_PACK_QUAD || reinterpret_cast<uintptr_t>(f.packed) % quad == 0,
+                    "field '%s': QUAD copy not %d-byte aligned", name, quad);
+    } else {
+        OPX_REQUIRE(f.packed_stride == 0, "field '%s': packed_stride without a packed copy", name);
     }
     return OPX_OK;
 }

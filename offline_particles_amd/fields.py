@@ -99,6 +99,10 @@ def _packable(t) -> bool:
             and t.dtype in (torch.float32, torch.bfloat16) and t.is_contiguous() and min(t.shape[-2:]) >= 2)
 
 
+# The plain kernels read dx, dy and h's QUAD copies from one buffer holding the
+# three side by side (opx_field.packed_stride), when they share a dtype and shape.
+MERGE_STATIC_STENCILS = True
+
 # Largest HEX copy made of one snapshot pair (16 values per node); larger
 # 3-D fields get the OCT copy (8 values per node).
 HEX_MAX_BYTES = 16 << 30

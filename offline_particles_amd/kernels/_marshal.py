@@ -99,10 +99,55 @@ def field_struct(fd, name: str) -> _lib.opx_field:
     return f
 
 
+def merge_quads(quads, elem: int) -> torch.Tensor:
+    """One buffer of 4 x `elem` bytes per node holding the QUAD copies `quads`
+    (uint8 tensors of n x `elem` bytes, one per field) side by side, the rest
+    zero: read with opx_field.packed_stride = 4 x elem, field j at j x elem."""
+    n = quads[0].numel() // elem
+    if len(quads) > 4 or any(q.numel() != n * elem for q in quads):
+        raise ValueError("QUAD copies of different sizes")
+    out = torch.zeros((n, 4, elem), dtype=torch.uint8, device=quads[0].device)
+    for j, q in enumerate(quads):
+        out[:, j] = q.view(n, elem)
+    return out
+
+
+_STATIC = ("dx", "dy", "h")
+
+
+def _merge_static(s, fielddata) -> None:
+    """Point the dx, dy and h descriptors of `s` at one buffer holding their QUAD
+    copies side by side (64 bytes per node for FP32), when all three have QUAD
+    copies of one dtype and shape: the three static stencils of a step are then
+    one cache line instead of three.  The buffer is kept with dx's copy and made
+    again when any of the three copies changes."""
+    fs = [getattr(s, n) for n in _STATIC]
+    packs = [getattr(fielddata[n], "packed", None) for n in _STATIC]
+    if any(p is None or p[1] != _lib.OPX_PACK_QUAD for p in packs):
+        return
+    if len({(f.dtype, f.ndim, tuple(f.shape[:f.ndim])) for f in fs}) != 1:
+        return
+    tensors = [p[0] for p in packs]
+    elem = 16 if fs[0].dtype == _lib.OPX_F32 else 8
+    cached = getattr(tensors[0], "_opx_static_merge", None)
+    if cached is None or cached[0] is not tensors[1] or cached[1] is not tensors[2]:
+        # the cache holds dy's and h's copies, so their identity cannot be reused
+        cached = (tensors[1], tensors[2], merge_quads(tensors, elem))
+        tensors[0]._opx_static_merge = cached
+    base = cached[2].data_ptr()
+    for j, f in enumerate(fs):
+        f.packed = base + j * elem
+        f.packed_stride = 4 * elem
+
+
 def fields_struct(fielddata: dict, names) -> _lib.opx_roms_fields:
+    from ..fields import MERGE_STATIC_STENCILS
+
     s = _lib.opx_roms_fields()
     for name in names:
         setattr(s, name, field_struct(fielddata[name], name))
+    if MERGE_STATIC_STENCILS and all(n in names for n in _STATIC):
+        _merge_static(s, fielddata)
     return s
 
 

@@ -35,14 +35,19 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
     wl = _wl()
     res = {}
     try:
-        for pack in (False, True):
-            fields.PACK_STENCILS = pack
+        for pack in (False, True, "separate"):
+            # "separate": packed copies, but dx, dy and h each in its own buffer
+            fields.PACK_STENCILS = bool(pack)
+            fields.MERGE_STATIC_STENCILS = pack is True
             sim = build_simulation(wl, scheme, device=device, storage_dtype=storage, spatial_order_every=order)
             for _ in range(60):
                 sim.step()
             torch.cuda.synchronize()
             res[pack] = gpu_state(sim)
-            if pack:
+            if pack is True:
+                # dx, dy and h read from one buffer of their QUADs side by side
+                merged = getattr(sim.fieldset["dx"]._packed[1][0], "_opx_static_merge", None)
+                assert merged is not None and merged[2].numel() == 4 * sim.fieldset["dx"]._packed[1][0].numel()
                 # the kinds the steps packed: TPAIR when the particles are sorted
                 kinds = {k[-1] for k in sim.fieldset["u"]._packed_pairs}
                 assert kinds == {_lib.OPX_PACK_TPAIR if order else _lib.OPX_PACK_HEX}, kinds
@@ -53,8 +58,10 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
                 assert sim.fieldset["h"].get_field_data(0.25, None).packed[1] == _lib.OPX_PACK_QUAD
     finally:
         fields.PACK_STENCILS = True
+        fields.MERGE_STATIC_STENCILS = True
     for k, a in res[False].items():
         assert np.array_equal(a, res[True][k], equal_nan=True), (scheme, storage, k)
+        assert np.array_equal(a, res["separate"][k], equal_nan=True), (scheme, storage, k)
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
