@@ -84,12 +84,14 @@ enum opx_error {
  * (dz, dy, dx) at (dz * 2 + dy) * 2 + dx: 16 values, 64 bytes (FP32) or 32
  * (BF16), so a trilinear stencil is one cache line.
  *
- * `packed_stride` (QUAD only; 0 for the others) is the distance in bytes
- * between consecutive nodes' QUAD elements; 0 means the element size (16
- * bytes FP32, 8 BF16).  It lets several static fields of one shape share one
- * buffer, their QUADs of a node side by side: fields.py puts dx, dy and h in
- * one 64-byte (FP32) element per node, so the three static stencils a step
- * reads are one cache line.  A multiple of the element size, at most 256.
+ * `packed_stride` (QUAD and OCT; 0 for the others) is the distance in bytes
+ * between consecutive nodes' elements; 0 means the element size (QUAD 16
+ * bytes FP32, 8 BF16; OCT 32 and 16).  It lets fields of one shape share one
+ * buffer, their elements of a node side by side: the drop-in's host code puts
+ * dx, dy and h's QUADs in one 64-byte (FP32) element per node, and zeta's OCT
+ * with h's QUAD in another, so the static stencils of a step are one cache
+ * line and h with zeta another.  A multiple of the QUAD size (QUAD) or of 16
+ * bytes (OCT), at least the element size, at most 256.
  */
 enum opx_pack_kind { OPX_PACK_NONE = 0, OPX_PACK_QUAD = 1, OPX_PACK_OCT = 2, OPX_PACK_TPAIR = 3, OPX_PACK_HEX = 4 };
 
@@ -104,7 +106,7 @@ typedef struct opx_field {
     double gt;
     const void *packed;
     int32_t packed_kind;   /* opx_pack_kind */
-    int32_t packed_stride; /* bytes between nodes' QUAD elements; 0: the element size */
+    int32_t packed_stride; /* bytes between nodes' QUAD / OCT elements; 0: the element size */
 } opx_field;
 
 /* The eight simulation fields the ROMS w-advection kernels declare

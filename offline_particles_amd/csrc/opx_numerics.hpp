@@ -156,8 +156,8 @@ __device__ __forceinline__ void load_blend2(const opx_field &f, int64_t i, doubl
 // The stencil corners (y, x), (y, x+1), (y+1, x), (y+1, x+1) of `node` from a
 // field's QUAD or OCT copy (opx_field.packed), blended and widened exactly as
 // load_blend does: one 16-byte (QUAD F32, OCT BF16), 32-byte (OCT F32) or
-// 8-byte (QUAD BF16) load instead of two rows per snapshot.  QUAD elements
-// sit opx_field.packed_stride bytes apart (0: packed back to back).
+// 8-byte (QUAD BF16) load instead of two rows per snapshot.  QUAD and OCT
+// elements sit opx_field.packed_stride bytes apart (0: packed back to back).
 __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node, double v[4])
 {
     if (f.dtype == OPX_F32) {
@@ -170,7 +170,9 @@ __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node,
             v[2] = (double)p.z;
             v[3] = (double)p.w;
         } else {
-            const float4 p = e[2 * node], q = e[2 * node + 1];
+            const int64_t stride = f.packed_stride ? f.packed_stride : 32;
+            const float4 *o = reinterpret_cast<const float4 *>(static_cast<const char *>(f.packed) + node * stride);
+            const float4 p = o[0], q = o[1];
             const float gt = (float)f.gt, ft = (float)f.ft;
             float a0 = p.x * gt, b0 = q.x * ft;
             float a1 = p.y * gt, b1 = q.y * ft;
@@ -190,7 +192,8 @@ __device__ __forceinline__ void packed_corners(const opx_field &f, int64_t node,
             v[2] = (double)bf16_bits_to_f32((uint16_t)(p.y & 0xffffu));
             v[3] = (double)bf16_bits_to_f32((uint16_t)(p.y >> 16));
         } else {
-            const uint4 e = static_cast<const uint4 *>(f.packed)[node];
+            const int64_t stride = f.packed_stride ? f.packed_stride : 16;
+            const uint4 e = *reinterpret_cast<const uint4 *>(static_cast<const char *>(f.packed) + node * stride);
             const uint32_t P[2] = {e.x, e.y}, Q[2] = {e.z, e.w};
             const float gt = (float)f.gt, ft = (float)f.ft;
             for (int c = 0; c < 4; ++c) {

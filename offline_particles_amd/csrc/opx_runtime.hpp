@@ -61,13 +61,17 @@ inline int check_field(const opx_field &f, int ndim, const char *name, bool allo
                                          (f.packed_kind == OPX_PACK_HEX && ndim == 3)),
                     "field '%s': packed kind %d does not match a %s field", name, f.packed_kind,
                     f.next == nullptr ? "static" : "time-dependent");
+        // QUAD elements are 16 (F32) or 8 (BF16) bytes, OCT 32 or 16, read in 16-byte loads
         const int quad = f.dtype == OPX_F32 ? 16 : 8;
-        OPX_REQUIRE(f.packed_stride == 0 || (f.packed_kind == OPX_PACK_QUAD && f.packed_stride % quad == 0 &&
+        const int unit = f.packed_kind == OPX_PACK_QUAD ? quad : 16;
+        const int size = f.packed_kind == OPX_PACK_QUAD ? quad : 2 * quad;
+        const bool strided = f.packed_kind == OPX_PACK_QUAD || f.packed_kind == OPX_PACK_OCT;
+        OPX_REQUIRE(f.packed_stride == 0 || (strided && f.packed_stride % unit == 0 && f.packed_stride >= size &&
                                              f.packed_stride <= 256),
-                    "field '%s': packed_stride %d is not 0 or a QUAD stride (a multiple of %d up to 256)", name,
-                    f.packed_stride, quad);
-        OPX_REQUIRE(f.packed_kind != OPX_PACK_QUAD || reinterpret_cast<uintptr_t>(f.packed) % quad == 0,
-                    "field '%s': QUAD copy not %d-byte aligned", name, quad);
+                    "field '%s': packed_stride %d is not 0 or a QUAD / OCT stride (a multiple of %d from %d to 256)",
+                    name, f.packed_stride, unit, size);
+        OPX_REQUIRE(!strided || reinterpret_cast<uintptr_t>(f.packed) % unit == 0,
+                    "field '%s': packed copy not %d-byte aligned", name, unit);
     } else {
         OPX_REQUIRE(f.packed_stride == 0, "field '%s': packed_stride without a packed copy", name);
     }

@@ -100,7 +100,8 @@ def _packable(t) -> bool:
 
 
 # The plain kernels read dx, dy and h's QUAD copies from one buffer holding the
-# three side by side (opx_field.packed_stride), when they share a dtype and shape.
+# three side by side (opx_field.packed_stride), when they share a dtype and shape,
+# and zeta's OCT copy with h's QUAD from another.
 MERGE_STATIC_STENCILS = True
 
 # Largest HEX copy made of one snapshot pair (16 values per node); larger

@@ -112,6 +112,46 @@ def merge_quads(quads, elem: int) -> torch.Tensor:
     return out
 
 
+def merge_elements(parts, sizes, stride: int) -> torch.Tensor:
+    """One buffer of `stride` bytes per node holding the packed copies `parts`
+    (uint8 tensors of n x sizes[j] bytes) side by side, part j at
+    sum(sizes[:j]), the rest zero."""
+    n = parts[0].numel() // sizes[0]
+    if sum(sizes) > stride or any(p.numel() != n * e for p, e in zip(parts, sizes)):
+        raise ValueError("packed copies of different node counts, or wider than the stride")
+    out = torch.zeros((n, stride), dtype=torch.uint8, device=parts[0].device)
+    at = 0
+    for p, e in zip(parts, sizes):
+        out[:, at:at + e] = p.view(n, e)
+        at += e
+    return out
+
+
+def _merge_zeta_h(s, fielddata) -> None:
+    """Point the zeta and h descriptors of `s` at one buffer holding, per node,
+    zeta's OCT element (its snapshot pair) and then h's QUAD (64 bytes for FP32),
+    when zeta has an OCT copy and h a QUAD copy of the same shape: the post step
+    and compute_z then read h and zeta in one cache line.  The buffer is kept
+    with zeta's copy of the pair (made again for each pair)."""
+    fz, fh = s.zeta, s.h
+    pz, ph = getattr(fielddata["zeta"], "packed", None), getattr(fielddata["h"], "packed", None)
+    if pz is None or ph is None or pz[1] != _lib.OPX_PACK_OCT or ph[1] != _lib.OPX_PACK_QUAD:
+        return
+    if fz.ndim != 2 or fh.ndim != 2 or tuple(fz.shape[:2]) != tuple(fh.shape[:2]):
+        return
+    oct_ = 32 if fz.dtype == _lib.OPX_F32 else 16
+    quad = 16 if fh.dtype == _lib.OPX_F32 else 8
+    stride = 1 << (oct_ + quad - 1).bit_length()
+    cached = getattr(pz[0], "_opx_zeta_h_merge", None)
+    if cached is None or cached[0] is not ph[0]:
+        # the cache holds h's copy, so its identity cannot be reused
+        cached = (ph[0], merge_elements([pz[0], ph[0]], [oct_, quad], stride))
+        pz[0]._opx_zeta_h_merge = cached
+    base = cached[1].data_ptr()
+    fz.packed, fz.packed_stride = base, stride
+    fh.packed, fh.packed_stride = base + oct_, stride
+
+
 _STATIC = ("dx", "dy", "h")
 
 
@@ -148,6 +188,8 @@ def fields_struct(fielddata: dict, names) -> _lib.opx_roms_fields:
         setattr(s, name, field_struct(fielddata[name], name))
     if MERGE_STATIC_STENCILS and all(n in names for n in _STATIC):
         _merge_static(s, fielddata)
+    if MERGE_STATIC_STENCILS and "zeta" in names and "h" in names:
+        _merge_zeta_h(s, fielddata)  # h then reads from here, dx and dy from the static buffer
     return s
 
 

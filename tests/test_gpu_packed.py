@@ -48,6 +48,9 @@ def test_packed_stencils_change_nothing(device, scheme, storage, order):
                 # dx, dy and h read from one buffer of their QUADs side by side
                 merged = getattr(sim.fieldset["dx"]._packed[1][0], "_opx_static_merge", None)
                 assert merged is not None and merged[2].numel() == 4 * sim.fieldset["dx"]._packed[1][0].numel()
+                # and h with zeta's OCT copy of the pair (caller's order only)
+                zh = [getattr(v[0], "_opx_zeta_h_merge", None) for v in sim.fieldset["zeta"]._packed_pairs.values()]
+                assert any(x is not None for x in zh) == (order == 0), zh
                 # the kinds the steps packed: TPAIR when the particles are sorted
                 kinds = {k[-1] for k in sim.fieldset["u"]._packed_pairs}
                 assert kinds == {_lib.OPX_PACK_TPAIR if order else _lib.OPX_PACK_HEX}, kinds

@@ -78,3 +78,46 @@ def test_merge_static_leaves_mismatched_fields_alone():
     fd["dy"].packed = None
     _marshal._merge_static(s, fd)
     assert all(getattr(s, k).packed_stride == 0 for k in ("dx", "dy", "h"))
+
+
+def test_merge_zeta_h_layout_and_descriptors():
+    """zeta's OCT element and h's QUAD side by side per node (64 bytes for FP32,
+    32 for BF16 zeta with FP32 h), both descriptors pointing into the buffer."""
+    n = 30
+    for zdt, hdt, oct_, quad, stride in ((_lib.OPX_F32, _lib.OPX_F32, 32, 16, 64),
+                                         (_lib.OPX_BF16, _lib.OPX_F32, 16, 16, 32),
+                                         (_lib.OPX_BF16, _lib.OPX_BF16, 16, 8, 32)):
+        s = _lib.opx_roms_fields()
+        for name, dt in (("zeta", zdt), ("h", hdt)):
+            f = getattr(s, name)
+            f.dtype, f.ndim = dt, 2
+            f.shape[0], f.shape[1] = 5, 6
+        z, h = _quad(n, oct_, 3), _quad(n, quad, 4)
+        fd = {"zeta": types.SimpleNamespace(packed=(z, _lib.OPX_PACK_OCT)),
+              "h": types.SimpleNamespace(packed=(h, _lib.OPX_PACK_QUAD))}
+        _marshal._merge_zeta_h(s, fd)
+        buf = z._opx_zeta_h_merge[1]
+        assert buf.shape == (n, stride)
+        assert s.zeta.packed == buf.data_ptr() and s.zeta.packed_stride == stride
+        assert s.h.packed == buf.data_ptr() + oct_ and s.h.packed_stride == stride
+        b = buf.numpy()
+        assert np.array_equal(b[:, :oct_].reshape(-1), z.numpy())
+        assert np.array_equal(b[:, oct_:oct_ + quad].reshape(-1), h.numpy())
+        assert not b[:, oct_ + quad:].any()
+
+
+def test_merge_zeta_h_skips_tpair_and_other_shapes():
+    n = 30
+    s = _lib.opx_roms_fields()
+    for name in ("zeta", "h"):
+        f = getattr(s, name)
+        f.dtype, f.ndim = _lib.OPX_F32, 2
+        f.shape[0], f.shape[1] = 5, 6
+    fd = {"zeta": types.SimpleNamespace(packed=(_quad(n, 8, 1), _lib.OPX_PACK_TPAIR)),
+          "h": types.SimpleNamespace(packed=(_quad(n, 16, 2), _lib.OPX_PACK_QUAD))}
+    _marshal._merge_zeta_h(s, fd)  # sorted particles: the TPAIR copy stays alone
+    assert s.zeta.packed_stride == 0 and s.h.packed_stride == 0
+    s.h.shape[1] = 7
+    fd["zeta"].packed = (_quad(n, 32, 1), _lib.OPX_PACK_OCT)
+    _marshal._merge_zeta_h(s, fd)
+    assert s.zeta.packed_stride == 0 and s.h.packed_stride == 0
